@@ -1,0 +1,2021 @@
+/*
+ * Oracle: FP64 CPU restatement of aither's per-nonlinear-iteration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY -- see aither_oracle.h.  Only tests/, the smoke
+ * check and bench.py's cpu_baseline leg load this code, as the checker.
+ *
+ * Each function cites the reference file:line whose algorithm it restates.
+ * Operation order follows the reference expression by expression; the file is
+ * compiled with -ffp-contract=off (no fused multiply-add).
+ *
+ * Data layout mirrors the reference's multiArray3d: ghost-padded arrays with i
+ * fastest, AoS over equations (include/multiArray3d.hpp:104-113).
+ */
+#include "aither_oracle.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EPS 1.0e-30 /* include/macros.hpp.in:21 */
+#define NEQ 5       /* rho, u, v, w, p (single species, no turbulence) */
+#define MAXBC 64
+
+typedef struct { double x, y, z; } v3;
+typedef struct { v3 u; double m; } uv3; /* unitVec3dMag: include/vector3d.hpp:131 */
+
+static v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 vscl(double s, v3 a) { return V(a.x * s, a.y * s, a.z * s); }
+static v3 vdiv(v3 a, double s) { return V(a.x / s, a.y / s, a.z / s); }
+/* std::inner_product with T(0) init: include/vector3d.hpp:303-305 */
+static double vdot(v3 a, v3 b) {
+  double s = 0.0;
+  s = s + a.x * b.x;
+  s = s + a.y * b.y;
+  s = s + a.z * b.z;
+  return s;
+}
+static double vmag(v3 a) { return sqrt(vdot(a, a)); }
+/* include/vector3d.hpp:332-341 */
+static v3 vcross(v3 a, v3 b) {
+  return V(a.y * b.z - a.z * b.y, -1.0 * (a.x * b.z - a.z * b.x),
+           a.x * b.y - a.y * b.x);
+}
+static uv3 make_uv3(v3 a) {
+  uv3 r;
+  r.u = vdiv(a, vmag(a)); /* Normalize: include/vector3d.hpp:375-377 */
+  r.m = vmag(a);
+  return r;
+}
+static double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+/* ------------------------------------------------------------------------ */
+/* boundary surfaces and states                                             */
+
+typedef struct {
+  char type[32];
+  int d[7]; /* imin imax jmin jmax kmin kmax tag */
+} surf_t;
+
+typedef struct {
+  char kind[32];
+  int tag;
+  double p, rho;
+  v3 vel;
+  double p0, t0;
+  v3 dir;
+  int nonreflecting;
+  int has_mf;
+} bcstate_t;
+
+/* src/boundaryConditions.cpp:2424-2450 */
+static int surf_type(const surf_t *s) {
+  if (s->d[0] == s->d[1]) return (s->d[1] == 0) ? 1 : 2;
+  if (s->d[2] == s->d[3]) return (s->d[3] == 0) ? 3 : 4;
+  if (s->d[4] == s->d[5]) return (s->d[5] == 0) ? 5 : 6;
+  return 0;
+}
+
+/* boundarySurface::operator< : src/boundaryConditions.cpp:91-105 */
+static int surf_cmp(const void *pa, const void *pb) {
+  const surf_t *a = (const surf_t *)pa, *b = (const surf_t *)pb;
+  int ta = surf_type(a), tb = surf_type(b);
+  if (ta != tb) return ta < tb ? -1 : 1;
+  for (int q = 0; q < 7; ++q)
+    if (a->d[q] != b->d[q]) return a->d[q] < b->d[q] ? -1 : 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* block storage                                                            */
+
+typedef struct {
+  int ni, nj, nk, g;
+  int NI, NJ, NK;
+  double *vol;
+  v3 *ctr;
+  uv3 *fa[3];
+  v3 *fc[3];
+  double *cw[3];
+  double *W;      /* padded, NEQ per cell */
+  double *R;      /* physical, NEQ per cell */
+  double *sr;     /* physical */
+  double *dt;     /* physical */
+  double *A, *Ainv;
+  double *consN, *consNm1;
+  double *x;      /* padded, NEQ per cell */
+  int *order;     /* 3 per physical cell */
+  int nsurf;
+  surf_t *surf;
+  int ictag;
+} blk_t;
+
+struct orc_case {
+  char dir[1024];
+  char grid[512];
+  int iterations, nonlinear, sweeps;
+  double rRef, tRef, lRef, aRef;
+  double dtIn;
+  double cflStart, cflMax, cflStep, cfl;
+  double theta, zeta, relax, dualCFL;
+  double kappa;
+  int recon;   /* 0 constant, 1 MUSCL */
+  int limiter; /* 0 none, 1 vanAlbada, 2 minmod */
+  int flux;    /* 0 roe, 1 ausm */
+  int timeint; /* 0 explicitEuler 1 implicitEuler 2 crankNicholson 3 bdf2 */
+  int solver;  /* 0 lusgs, 1 dplur */
+  int implicit, multilevel, storeN, initMatrix;
+  orc_gas gas;
+  double gamma;
+  double fluidN, fluidM;
+  int nic;
+  struct {
+    int tag, fromFile;
+    char file[512];
+    double rho, p;
+    v3 vel;
+  } ic[16];
+  int nbc;
+  bcstate_t bcs[MAXBC];
+  int nb;
+  blk_t *blk;
+  double totalCells;
+  int matres_on;
+  double l2first[NEQ];
+};
+
+static long cidx(const blk_t *b, int i, int j, int k) {
+  return (long)(i + b->g) + (long)(j + b->g) * b->NI +
+         (long)(k + b->g) * b->NI * b->NJ;
+}
+static long fidx(const blk_t *b, int d, int i, int j, int k) {
+  int FI = b->NI + (d == 0), FJ = b->NJ + (d == 1);
+  return (long)(i + b->g) + (long)(j + b->g) * FI + (long)(k + b->g) * FI * FJ;
+}
+static long pidx(const blk_t *b, int i, int j, int k) {
+  return (long)i + (long)j * b->ni + (long)k * b->ni * b->nj;
+}
+static long ncells_padded(const blk_t *b) { return (long)b->NI * b->NJ * b->NK; }
+static long nfaces_padded(const blk_t *b, int d) {
+  return (long)(b->NI + (d == 0)) * (b->NJ + (d == 1)) * (b->NK + (d == 2));
+}
+static long ncells(const blk_t *b) { return (long)b->ni * b->nj * b->nk; }
+
+/* index remapping (dir, d1, d2, d3) -> (i, j, k):
+ * include/multiArray3d.hpp:231-243 (dir i: d1=j d2=k; j: d1=k d2=i; k: d1=i d2=j) */
+static void map3(int dir, int d3, int d1, int d2, int *i, int *j, int *k) {
+  if (dir == 0) { *i = d3; *j = d1; *k = d2; }
+  else if (dir == 1) { *i = d2; *j = d3; *k = d1; }
+  else { *i = d1; *j = d2; *k = d3; }
+}
+static int ndir(const blk_t *b, int d) { return d == 0 ? b->ni : (d == 1 ? b->nj : b->nk); }
+
+/* ------------------------------------------------------------------------ */
+/* thermodynamics: calorically perfect single species                       */
+/* eos: src/eos.cpp:40-115, thermo: include/thermodynamic.hpp:58-121,       */
+/* src/thermodynamic.cpp:63-113, helpers include/arrayView.hpp:384-444       */
+
+static double cps(const orc_gas *g) { return g->R * (g->n + 1.0); }
+static double cvs(const orc_gas *g) { return g->R * g->n; }
+double orc_gamma(const orc_gas *g) {
+  /* Cp(t,mf)/Cv(t,mf) with mf = {1.0}: 0.0 + 1.0*Cp_s */
+  double cp = 0.0 + 1.0 * cps(g);
+  double cv = 0.0 + 1.0 * cvs(g);
+  return cp / cv;
+}
+static double temperature(const orc_gas *g, const double *W) {
+  double rhoR = 0.0;
+  rhoR += W[0] * g->R;
+  return W[4] / rhoR;
+}
+double orc_sos(const orc_gas *g, const double *W) {
+  return sqrt(orc_gamma(g) * W[4] / W[0]);
+}
+static double velmag(const double *W) { return vmag(V(W[1], W[2], W[3])); }
+double orc_enthalpy(const orc_gas *g, const double *W) {
+  double t = temperature(g, W);
+  double vel = velmag(W);
+  double h = 0.0;
+  h += 1.0 * (0.0 + cps(g) * t); /* hf = 0 */
+  return h + 0.5 * vel * vel;
+}
+static double energy(const orc_gas *g, const double *W) {
+  double t = temperature(g, W);
+  double e = 0.0;
+  e += 1.0 * (0.0 + cvs(g) * t);
+  double vel = velmag(W);
+  return e + 0.5 * vel * vel;
+}
+/* PrimToCons: include/primitive.hpp:151-169 */
+void orc_prim_to_cons(const orc_gas *g, const double *W, double *U) {
+  double rho = W[0];
+  U[0] = W[0];
+  U[1] = rho * W[1];
+  U[2] = rho * W[2];
+  U[3] = rho * W[3];
+  U[4] = rho * energy(g, W);
+}
+/* primitive(cons, phys): include/primitive.hpp:129-148; PressFromEnergy
+ * src/eos.cpp:40-52; TemperatureFromSpecEnergy src/thermodynamic.cpp:107-113 */
+void orc_cons_to_prim(const orc_gas *g, const double *U, double *W) {
+  double rho = 0.0 + U[0];
+  W[0] = U[0];
+  W[1] = U[1] / rho;
+  W[2] = U[2] / rho;
+  W[3] = U[3] / rho;
+  double en = U[4] / rho;
+  double vel = velmag(W);
+  double spec = en - 0.5 * vel * vel;
+  double rhoSum = 0.0 + W[0];
+  double mf = W[0] / rhoSum;
+  double hf = 0.0 + 0.0 * mf;
+  double cv = 0.0 + mf * cvs(g);
+  double t = (spec - hf) / cv;
+  double p = 0.0;
+  p += W[0] * g->R * t;
+  W[4] = p;
+}
+/* UpdatePrimWithCons: include/primitive.hpp:206-231 (single species: the
+ * mass-fraction clip and renormalisation leave rho unchanged) */
+static void update_prim(const orc_gas *g, const double *W, const double *du,
+                        double *out) {
+  double U[NEQ];
+  orc_prim_to_cons(g, W, U);
+  for (int e = 0; e < NEQ; ++e) U[e] = U[e] + du[e];
+  double rho = 0.0 + U[0];
+  double mf = U[0] / rho;
+  mf = dmax(mf, 0.0);
+  double total = 0.0 + mf;
+  mf /= total;
+  U[0] = rho * mf;
+  orc_cons_to_prim(g, U, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fluxes                                                                   */
+
+/* inviscidFlux::ConstructFromPrim: include/inviscidFlux.hpp:128-159 */
+void orc_phys_flux(const orc_gas *g, const double *W, const double *n,
+                   double *F) {
+  v3 vel = V(W[1], W[2], W[3]);
+  v3 nn = V(n[0], n[1], n[2]);
+  double vn = vdot(vel, nn);
+  F[0] = W[0] * vn;
+  double rho = 0.0 + W[0];
+  F[1] = rho * vn * vel.x + W[4] * nn.x;
+  F[2] = rho * vn * vel.y + W[4] * nn.y;
+  F[3] = rho * vn * vel.z + W[4] * nn.z;
+  F[4] = rho * vn * orc_enthalpy(g, W);
+}
+
+/* RoeFlux: include/inviscidFlux.hpp:259-382, src/inviscidFlux.cpp:26-32;
+ * RoeAveragedState: include/primitive.hpp:245-280 */
+void orc_roe_flux(const orc_gas *g, const double *L, const double *Rt,
+                  const double *nv, double *F) {
+  v3 n = V(nv[0], nv[1], nv[2]);
+  double roe[NEQ];
+  double denRatio = sqrt(Rt[0] / L[0]);
+  roe[0] = L[0] * denRatio;
+  roe[1] = (L[1] + denRatio * Rt[1]) / (1.0 + denRatio);
+  roe[2] = (L[2] + denRatio * Rt[2]) / (1.0 + denRatio);
+  roe[3] = (L[3] + denRatio * Rt[3]) / (1.0 + denRatio);
+  roe[4] = (L[4] + denRatio * Rt[4]) / (1.0 + denRatio);
+
+  double hR = orc_enthalpy(g, roe);
+  double aR = orc_sos(g, roe);
+  double rhoR = 0.0 + roe[0];
+  v3 vroe = V(roe[1], roe[2], roe[3]);
+  double velNormR = vdot(vroe, n);
+  double mfR = roe[0] / rhoR;
+  double delta[NEQ];
+  for (int e = 0; e < NEQ; ++e) delta[e] = Rt[e] - L[e];
+  v3 dvel = V(delta[1], delta[2], delta[3]);
+  double normVelDiff = vdot(dvel, n);
+
+  double diss[NEQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const double entropyFix = 0.1;
+
+  /* left moving acoustic wave */
+  double waveSpeed = fabs(velNormR - aR);
+  if (waveSpeed < entropyFix)
+    waveSpeed = 0.5 * (waveSpeed * waveSpeed / entropyFix + entropyFix);
+  double waveStrength =
+      (delta[4] - rhoR * aR * normVelDiff) / (2.0 * aR * aR);
+  double wss = waveSpeed * waveStrength;
+  diss[0] += wss * mfR;
+  diss[1] += wss * (vroe.x - aR * n.x);
+  diss[2] += wss * (vroe.y - aR * n.y);
+  diss[3] += wss * (vroe.z - aR * n.z);
+  diss[4] += wss * (hR - aR * velNormR);
+
+  /* entropy wave */
+  waveSpeed = fabs(velNormR);
+  waveStrength = -delta[4] / (aR * aR);
+  wss = waveSpeed * waveStrength;
+  diss[0] += wss * mfR + waveSpeed * delta[0];
+  double drho = 0.0 + delta[0];
+  waveStrength = drho - delta[4] / (aR * aR);
+  wss = waveSpeed * waveStrength;
+  diss[1] += wss * vroe.x;
+  diss[2] += wss * vroe.y;
+  diss[3] += wss * vroe.z;
+  diss[4] += wss * 0.5 * vdot(vroe, vroe);
+
+  /* shear wave */
+  waveStrength = rhoR;
+  wss = waveSpeed * waveStrength;
+  diss[1] += wss * (dvel.x - normVelDiff * n.x);
+  diss[2] += wss * (dvel.y - normVelDiff * n.y);
+  diss[3] += wss * (dvel.z - normVelDiff * n.z);
+  diss[4] += wss * (vdot(vroe, dvel) - velNormR * normVelDiff);
+
+  /* right moving acoustic wave */
+  waveSpeed = fabs(velNormR + aR);
+  if (waveSpeed < entropyFix)
+    waveSpeed = 0.5 * (waveSpeed * waveSpeed / entropyFix + entropyFix);
+  waveStrength = (delta[4] + rhoR * aR * normVelDiff) / (2.0 * aR * aR);
+  wss = waveSpeed * waveStrength;
+  diss[0] += wss * mfR;
+  diss[1] += wss * (vroe.x + aR * n.x);
+  diss[2] += wss * (vroe.y + aR * n.y);
+  diss[3] += wss * (vroe.z + aR * n.z);
+  diss[4] += wss * (hR + aR * velNormR);
+
+  double FL[NEQ], FR[NEQ];
+  orc_phys_flux(g, L, nv, FL);
+  orc_phys_flux(g, Rt, nv, FR);
+  for (int e = 0; e < NEQ; ++e) {
+    double t = FL[e] + (FR[e] - diss[e]);
+    F[e] = t * 0.5;
+  }
+}
+
+static double sgn(double x) { return (x > 0.0) - (x < 0.0); } /* Sign: include/utility.hpp */
+
+/* AUSMFlux (AUSMPW+): include/inviscidFlux.hpp:161-209, 395-481 */
+void orc_ausm_flux(const orc_gas *g, const double *L, const double *Rt,
+                   const double *nv, double *F) {
+  v3 a = V(nv[0], nv[1], nv[2]);
+  double velNormL = vdot(V(L[1], L[2], L[3]), a);
+  double velNormR = vdot(V(Rt[1], Rt[2], Rt[3]), a);
+  double sosL = orc_sos(g, L);
+  double sosR = orc_sos(g, Rt);
+  double sosStar = sqrt(sosL * sosR);
+  double vel = 0.5 * (velNormL + velNormR);
+  double sos = sosStar;
+  if (vel < 0.0) {
+    sos = sosStar * sosStar / dmax(velNormR, sosStar);
+  } else if (vel > 0.0) {
+    sos = sosStar * sosStar / dmax(velNormL, sosStar);
+  }
+  double ml = velNormL / sos;
+  double mr = velNormR / sos;
+  double mPlusL = fabs(ml) <= 1.0 ? 0.25 * pow(ml + 1.0, 2.0)
+                                  : 0.5 * (ml + fabs(ml));
+  double mMinusR = fabs(mr) <= 1.0 ? -0.25 * pow(mr - 1.0, 2.0)
+                                   : 0.5 * (mr - fabs(mr));
+  double pPlus = fabs(ml) <= 1.0 ? 0.25 * pow(ml + 1.0, 2.0) * (2.0 - ml)
+                                 : 0.5 * (1.0 + sgn(ml));
+  double pMinus = fabs(mr) <= 1.0 ? 0.25 * pow(mr - 1.0, 2.0) * (2.0 + mr)
+                                  : 0.5 * (1.0 - sgn(mr));
+  double ps = pPlus * L[4] + pMinus * Rt[4];
+  double w = 1.0 - pow(dmin(L[4] / Rt[4], Rt[4] / L[4]), 3.0);
+  double fl = fabs(ml) < 1.0 ? L[4] / ps - 1.0 : 0.0;
+  double fr = fabs(mr) < 1.0 ? Rt[4] / ps - 1.0 : 0.0;
+  double mavg = mPlusL + mMinusR;
+  double mPlusLBar = mavg >= 0.0
+                         ? mPlusL + mMinusR * ((1.0 - w) * (1.0 + fr) - fl)
+                         : mPlusL * w * (1.0 + fl);
+  double mMinusRBar = mavg >= 0.0
+                          ? mMinusR * w * (1.0 + fr)
+                          : mMinusR + mPlusL * ((1.0 - w) * (1.0 + fl) - fr);
+  /* inviscidFlux::AUSMFlux member: include/inviscidFlux.hpp:161-209 */
+  double vl = mPlusLBar * sos;
+  F[0] = L[0] * vl;
+  double rhoL = 0.0 + L[0];
+  F[1] = rhoL * vl * L[1] + pPlus * L[4] * a.x;
+  F[2] = rhoL * vl * L[2] + pPlus * L[4] * a.y;
+  F[3] = rhoL * vl * L[3] + pPlus * L[4] * a.z;
+  F[4] = rhoL * vl * orc_enthalpy(g, L);
+  double vr = mMinusRBar * sos;
+  F[0] += Rt[0] * vr;
+  double rhoR = 0.0 + Rt[0];
+  F[1] += rhoR * vr * Rt[1] + pMinus * Rt[4] * a.x;
+  F[2] += rhoR * vr * Rt[2] + pMinus * Rt[4] * a.y;
+  F[3] += rhoR * vr * Rt[3] + pMinus * Rt[4] * a.z;
+  F[4] += rhoR * vr * orc_enthalpy(g, Rt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruction: include/reconstruction.hpp:109-154, src/limiter.cpp:24-53 */
+
+static double lim_vanalbada(double r) {
+  double r2 = r * r;
+  double l = (r + r2) / (1.0 + r2);
+  return dmax(0.0, l);
+}
+static double lim_minmod(double r) { return dmax(0.0, dmin(1.0, r)); }
+
+void orc_muscl(const double *u2, const double *u1, const double *d1,
+               double kappa, int limiter, double uw2, double uw, double dw,
+               double *out) {
+  double dPlus = (uw + uw) / (uw + dw);
+  double dMinus = (uw + uw) / (uw + uw2);
+  for (int e = 0; e < NEQ; ++e) {
+    double r = (EPS + (d1[e] - u1[e]) * dPlus) / (EPS + (u1[e] - u2[e]) * dMinus);
+    double lim, inv;
+    if (limiter == 1) {
+      lim = lim_vanalbada(r);
+      inv = lim_vanalbada(1.0 / r);
+    } else if (limiter == 2) {
+      lim = lim_minmod(r);
+      inv = lim_minmod(1.0 / r);
+    } else {
+      lim = 1.0;
+      inv = 1.0;
+    }
+    double a = 0.25 * ((u1[e] - u2[e]) * dMinus);
+    double b = (1.0 - kappa) * lim + (1.0 + kappa) * r * inv;
+    out[e] = u1[e] + a * b;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* spectral radii: include/spectralRadius.hpp:43-80                        */
+
+static double inv_cell_specrad(const orc_gas *g, const double *W, uv3 aL,
+                               uv3 aR) {
+  v3 s = vadd(aL.u, aR.u);
+  v3 avg = vscl(0.5, s);
+  v3 normAvg = vdiv(avg, vmag(avg));
+  double fMag = 0.5 * (aL.m + aR.m);
+  return (fabs(vdot(V(W[1], W[2], W[3]), normAvg)) + orc_sos(g, W)) * fMag;
+}
+double orc_inv_cell_specrad(const orc_gas *g, const double *W, const double *aL,
+                            const double *aR) {
+  uv3 l = {V(aL[0], aL[1], aL[2]), aL[3]};
+  uv3 r = {V(aR[0], aR[1], aR[2]), aR[3]};
+  return inv_cell_specrad(g, W, l, r);
+}
+static double inv_face_specrad(const orc_gas *g, const double *W, uv3 a) {
+  return 0.5 * a.m * (fabs(vdot(V(W[1], W[2], W[3]), a.u)) + orc_sos(g, W));
+}
+
+/* RusanovScalarOffDiagonal (inviscid, no turbulence):
+ * src/fluxJacobian.cpp:122-162; ConvectiveFluxUpdate
+ * include/inviscidFlux.hpp:543-562 */
+static void scalar_offdiag(const orc_gas *g, const double *Wnb,
+                           const double *du, uv3 a, int positive, double *out) {
+  double Wu[NEQ];
+  update_prim(g, Wnb, du, Wu);
+  double n[3] = {a.u.x, a.u.y, a.u.z};
+  double Fold[NEQ], Fnew[NEQ];
+  orc_phys_flux(g, Wnb, n, Fold);
+  orc_phys_flux(g, Wu, n, Fnew);
+  double half = 0.5 * a.m;
+  double lam = inv_face_specrad(g, Wnb, a);
+  for (int e = 0; e < NEQ; ++e) {
+    double fc = half * (Fnew[e] - Fold[e]);
+    double sd = du[e] * lam;
+    out[e] = positive ? fc + sd : fc - sd;
+  }
+}
+void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
+                        const double *area, int positive, double *out) {
+  uv3 a = {V(area[0], area[1], area[2]), area[3]};
+  scalar_offdiag(g, Wnb, du, a, positive, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry: src/plot3d.cpp:36-359, 490-498                                 */
+
+static v3 P(const v3 *c, int ni, int nj, int i, int j, int k) {
+  return c[(long)i + (long)j * (ni + 1) + (long)k * (ni + 1) * (nj + 1)];
+}
+
+double orc_pyramid_volume(const double *p, const double *a, const double *b,
+                          const double *c, const double *d) {
+  v3 vp = V(p[0], p[1], p[2]), va = V(a[0], a[1], a[2]),
+     vb = V(b[0], b[1], b[2]), vc = V(c[0], c[1], c[2]),
+     vd = V(d[0], d[1], d[2]);
+  v3 s = vadd(vadd(vadd(vsub(va, vp), vsub(vb, vp)), vsub(vc, vp)), vsub(vd, vp));
+  v3 xp = vscl(0.25, s);
+  v3 xac = vsub(vc, va);
+  v3 xbd = vsub(vd, vb);
+  return 1.0 / 6.0 * vdot(xp, vcross(xac, xbd));
+}
+static double pyr(v3 p, v3 a, v3 b, v3 c, v3 d) {
+  double pp[3] = {p.x, p.y, p.z}, aa[3] = {a.x, a.y, a.z},
+         bb[3] = {b.x, b.y, b.z}, cc[3] = {c.x, c.y, c.z},
+         dd[3] = {d.x, d.y, d.z};
+  return orc_pyramid_volume(pp, aa, bb, cc, dd);
+}
+
+static v3 centroid(const v3 *c, int ni, int nj, int i, int j, int k) {
+  v3 s = P(c, ni, nj, i, j, k);
+  s = vadd(s, P(c, ni, nj, i + 1, j, k));
+  s = vadd(s, P(c, ni, nj, i, j + 1, k));
+  s = vadd(s, P(c, ni, nj, i + 1, j + 1, k));
+  s = vadd(s, P(c, ni, nj, i, j, k + 1));
+  s = vadd(s, P(c, ni, nj, i + 1, j, k + 1));
+  s = vadd(s, P(c, ni, nj, i, j + 1, k + 1));
+  s = vadd(s, P(c, ni, nj, i + 1, j + 1, k + 1));
+  return vscl(0.125, s);
+}
+
+static int build_geometry(blk_t *b, const v3 *c) {
+  int ni = b->ni, nj = b->nj, nk = b->nk;
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        v3 ct = centroid(c, ni, nj, i, j, k);
+        double v = 0.0;
+        v += pyr(ct, P(c, ni, nj, i, j, k), P(c, ni, nj, i, j, k + 1),
+                 P(c, ni, nj, i, j + 1, k + 1), P(c, ni, nj, i, j + 1, k));
+        v += pyr(ct, P(c, ni, nj, i + 1, j, k), P(c, ni, nj, i + 1, j + 1, k),
+                 P(c, ni, nj, i + 1, j + 1, k + 1), P(c, ni, nj, i + 1, j, k + 1));
+        v += pyr(ct, P(c, ni, nj, i, j, k), P(c, ni, nj, i + 1, j, k),
+                 P(c, ni, nj, i + 1, j, k + 1), P(c, ni, nj, i, j, k + 1));
+        v += pyr(ct, P(c, ni, nj, i, j + 1, k), P(c, ni, nj, i, j + 1, k + 1),
+                 P(c, ni, nj, i + 1, j + 1, k + 1), P(c, ni, nj, i + 1, j + 1, k));
+        v += pyr(ct, P(c, ni, nj, i, j, k), P(c, ni, nj, i, j + 1, k),
+                 P(c, ni, nj, i + 1, j + 1, k), P(c, ni, nj, i + 1, j, k));
+        v += pyr(ct, P(c, ni, nj, i, j, k + 1), P(c, ni, nj, i + 1, j, k + 1),
+                 P(c, ni, nj, i + 1, j + 1, k + 1), P(c, ni, nj, i, j + 1, k + 1));
+        if (!(v > 0.0)) return -1;
+        b->vol[cidx(b, i, j, k)] = v;
+        b->ctr[cidx(b, i, j, k)] = ct;
+      }
+  /* i faces */
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i <= ni; ++i) {
+        v3 xac = vsub(P(c, ni, nj, i, j + 1, k + 1), P(c, ni, nj, i, j, k));
+        v3 xbd = vsub(P(c, ni, nj, i, j + 1, k), P(c, ni, nj, i, j, k + 1));
+        b->fa[0][fidx(b, 0, i, j, k)] = make_uv3(vscl(0.5, vcross(xbd, xac)));
+        v3 s = P(c, ni, nj, i, j, k);
+        s = vadd(s, P(c, ni, nj, i, j + 1, k));
+        s = vadd(s, P(c, ni, nj, i, j, k + 1));
+        s = vadd(s, P(c, ni, nj, i, j + 1, k + 1));
+        b->fc[0][fidx(b, 0, i, j, k)] = vscl(0.25, s);
+      }
+  /* j faces */
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j <= nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        v3 xac = vsub(P(c, ni, nj, i, j, k + 1), P(c, ni, nj, i + 1, j, k));
+        v3 xbd = vsub(P(c, ni, nj, i, j, k), P(c, ni, nj, i + 1, j, k + 1));
+        b->fa[1][fidx(b, 1, i, j, k)] = make_uv3(vscl(0.5, vcross(xbd, xac)));
+        v3 s = P(c, ni, nj, i, j, k);
+        s = vadd(s, P(c, ni, nj, i + 1, j, k));
+        s = vadd(s, P(c, ni, nj, i, j, k + 1));
+        s = vadd(s, P(c, ni, nj, i + 1, j, k + 1));
+        b->fc[1][fidx(b, 1, i, j, k)] = vscl(0.25, s);
+      }
+  /* k faces */
+  for (int k = 0; k <= nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        v3 xac = vsub(P(c, ni, nj, i, j + 1, k), P(c, ni, nj, i + 1, j, k));
+        v3 xbd = vsub(P(c, ni, nj, i + 1, j + 1, k), P(c, ni, nj, i, j, k));
+        b->fa[2][fidx(b, 2, i, j, k)] = make_uv3(vscl(0.5, vcross(xbd, xac)));
+        v3 s = P(c, ni, nj, i, j, k);
+        s = vadd(s, P(c, ni, nj, i + 1, j, k));
+        s = vadd(s, P(c, ni, nj, i, j + 1, k));
+        s = vadd(s, P(c, ni, nj, i + 1, j + 1, k));
+        b->fc[2][fidx(b, 2, i, j, k)] = vscl(0.25, s);
+      }
+  return 0;
+}
+
+/* face index along direction d for "cell" q with face array of direction d,
+ * lower/upper surface (SliceArray/InsertArray with id == dir:
+ * include/multiArray3d.hpp:536-580, 722-760) */
+static int face_of(int q, int upper) { return upper ? q + 1 : q; }
+
+/* procBlock::AssignGhostCellsGeom: src/procBlock.cpp:2160-2268 */
+static void ghost_geometry(blk_t *b) {
+  for (int layer = 1; layer <= b->g; ++layer) {
+    for (int s = 0; s < b->nsurf; ++s) {
+      const surf_t *sf = &b->surf[s];
+      if (strcmp(sf->type, "interblock") == 0) continue;
+      int st = surf_type(sf);
+      int dir = (st - 1) / 2;
+      int upper = (st % 2 == 0);
+      int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      int r1s = sf->d[2 * d1], r1e = sf->d[2 * d1 + 1];
+      int r2s = sf->d[2 * d2], r2e = sf->d[2 * d2 + 1];
+      int r3 = sf->d[2 * dir];
+      int n3 = ndir(b, dir);
+      int gC, iC, pC, piC, iF, piF;
+      if (upper) {
+        gC = r3 + layer - 1;
+        iC = r3 - layer;
+        pC = gC - 1;
+        if (iC < 0) iC = 0;
+        piC = iC + 1;
+        iF = r3 - layer;
+        if (iF < 0) iF = 0;
+        piF = iF + 1;
+      } else {
+        gC = r3 - layer;
+        iC = r3 + layer - 1;
+        pC = gC + 1;
+        if (iC >= n3) iC = n3 - 1;
+        piC = iC - 1;
+        iF = r3 + layer;
+        if (iF > n3) iF = n3;
+        piF = iF - 1;
+      }
+      /* distF2F and distC2C over the surface patch */
+      int n1 = r1e - r1s, n2 = r2e - r2s;
+      v3 *f2f = (v3 *)malloc(sizeof(v3) * (size_t)(n1 * n2));
+      v3 *c2c = (v3 *)malloc(sizeof(v3) * (size_t)(n1 * n2));
+      for (int q2 = 0; q2 < n2; ++q2)
+        for (int q1 = 0; q1 < n1; ++q1) {
+          int i, j, k, i2, j2, k2;
+          map3(dir, piF, r1s + q1, r2s + q2, &i, &j, &k);
+          map3(dir, iF, r1s + q1, r2s + q2, &i2, &j2, &k2);
+          f2f[q1 + q2 * n1] =
+              vsub(b->fc[dir][fidx(b, dir, i, j, k)], b->fc[dir][fidx(b, dir, i2, j2, k2)]);
+          if (layer > 1) {
+            map3(dir, piC, r1s + q1, r2s + q2, &i, &j, &k);
+            map3(dir, iC, r1s + q1, r2s + q2, &i2, &j2, &k2);
+            c2c[q1 + q2 * n1] = vsub(b->ctr[cidx(b, i, j, k)], b->ctr[cidx(b, i2, j2, k2)]);
+          } else {
+            c2c[q1 + q2 * n1] = f2f[q1 + q2 * n1];
+          }
+        }
+      /* volumes, normal-direction face data, centroids */
+      for (int q2 = 0; q2 < n2; ++q2)
+        for (int q1 = 0; q1 < n1; ++q1) {
+          int gi, gj, gk, ii, ij, ik, pi, pj, pk;
+          map3(dir, gC, r1s + q1, r2s + q2, &gi, &gj, &gk);
+          map3(dir, iC, r1s + q1, r2s + q2, &ii, &ij, &ik);
+          map3(dir, pC, r1s + q1, r2s + q2, &pi, &pj, &pk);
+          b->vol[cidx(b, gi, gj, gk)] = b->vol[cidx(b, ii, ij, ik)];
+          /* normal-direction face area and centre */
+          {
+            int fi, fj, fk, si, sj, sk, ti, tj, tk;
+            map3(dir, face_of(gC, upper), r1s + q1, r2s + q2, &fi, &fj, &fk);
+            map3(dir, face_of(iC, upper), r1s + q1, r2s + q2, &si, &sj, &sk);
+            map3(dir, face_of(pC, upper), r1s + q1, r2s + q2, &ti, &tj, &tk);
+            b->fa[dir][fidx(b, dir, fi, fj, fk)] = b->fa[dir][fidx(b, dir, si, sj, sk)];
+            b->fc[dir][fidx(b, dir, fi, fj, fk)] =
+                vadd(f2f[q1 + q2 * n1], b->fc[dir][fidx(b, dir, ti, tj, tk)]);
+          }
+          b->ctr[cidx(b, gi, gj, gk)] =
+              vadd(b->ctr[cidx(b, pi, pj, pk)], c2c[q1 + q2 * n1]);
+        }
+      /* transverse face data: range grown by one along the face direction;
+       * distC2C is duplicated at the grown end (GrowInI/J/K,
+       * include/multiArray3d.hpp:929-990) */
+      for (int t = 1; t <= 2; ++t) {
+        int fd = (dir + t) % 3;
+        int m1 = n1 + (t == 1), m2 = n2 + (t == 2);
+        for (int q2 = 0; q2 < m2; ++q2)
+          for (int q1 = 0; q1 < m1; ++q1) {
+            int gi, gj, gk, ii, ij, ik, pi, pj, pk;
+            map3(dir, gC, r1s + q1, r2s + q2, &gi, &gj, &gk);
+            map3(dir, iC, r1s + q1, r2s + q2, &ii, &ij, &ik);
+            map3(dir, pC, r1s + q1, r2s + q2, &pi, &pj, &pk);
+            int c1 = q1 < n1 ? q1 : n1 - 1, c2 = q2 < n2 ? q2 : n2 - 1;
+            b->fa[fd][fidx(b, fd, gi, gj, gk)] = b->fa[fd][fidx(b, fd, ii, ij, ik)];
+            b->fc[fd][fidx(b, fd, gi, gj, gk)] =
+                vadd(c2c[c1 + c2 * n1], b->fc[fd][fidx(b, fd, pi, pj, pk)]);
+          }
+      }
+      free(f2f);
+      free(c2c);
+    }
+  }
+}
+
+/* procBlock::CalcCellWidths: src/procBlock.cpp:6397-6411 (all cells incl.
+ * ghosts; edge/corner ghost cells hold unused values) */
+static void cell_widths(blk_t *b) {
+  int g = b->g;
+  for (int k = -g; k < b->nk + g; ++k)
+    for (int j = -g; j < b->nj + g; ++j)
+      for (int i = -g; i < b->ni + g; ++i) {
+        long c = cidx(b, i, j, k);
+        b->cw[0][c] = vmag(vsub(b->fc[0][fidx(b, 0, i, j, k)], b->fc[0][fidx(b, 0, i + 1, j, k)]));
+        b->cw[1][c] = vmag(vsub(b->fc[1][fidx(b, 1, i, j, k)], b->fc[1][fidx(b, 1, i, j + 1, k)]));
+        b->cw[2][c] = vmag(vsub(b->fc[2][fidx(b, 2, i, j, k)], b->fc[2][fidx(b, 2, i, j, k + 1)]));
+      }
+}
+
+/* HyperplaneReorder: src/utility.cpp:377-398 */
+void orc_hyperplane_order(int ni, int nj, int nk, int *out) {
+  int np = ni + nj + nk - 2;
+  long n = 0;
+  for (int p = 0; p < np; ++p)
+    for (int k = 0; k < nk; ++k)
+      for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i)
+          if (i + j + k == p) {
+            out[3 * n] = i;
+            out[3 * n + 1] = j;
+            out[3 * n + 2] = k;
+            ++n;
+          }
+}
+
+/* ------------------------------------------------------------------------ */
+/* input parsing (subset of src/input.cpp:167-643, src/inputStates.cpp)     */
+
+static void trim(char *s) {
+  /* Trim: strip whitespace, then cut at '#' (src/inputStates.cpp:231-252) */
+  char *p = s;
+  while (*p && isspace((unsigned char)*p)) ++p;
+  memmove(s, p, strlen(p) + 1);
+  size_t n = strlen(s);
+  while (n > 0 && isspace((unsigned char)s[n - 1])) s[--n] = 0;
+  char *h = strchr(s, '#');
+  if (h) *h = 0;
+  n = strlen(s);
+  while (n > 0 && isspace((unsigned char)s[n - 1])) s[--n] = 0;
+}
+
+static void seterr(char *err, int errlen, const char *fmt, ...) {
+  if (!err || errlen <= 0) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(err, (size_t)errlen, fmt, ap);
+  va_end(ap);
+}
+
+/* split a <...> list body into top-level entries "name(...)" */
+static int split_entries(const char *body, char out[][1024], int maxn) {
+  int n = 0, depth = 0;
+  const char *start = NULL;
+  for (const char *p = body; *p; ++p) {
+    if (*p == '(') {
+      if (depth == 0) {
+        const char *q = p;
+        while (q > body && (isalnum((unsigned char)q[-1]) || q[-1] == '_')) --q;
+        start = q;
+      }
+      ++depth;
+    } else if (*p == ')') {
+      --depth;
+      if (depth == 0 && start && n < maxn) {
+        size_t len = (size_t)(p - start + 1);
+        if (len > 1023) len = 1023;
+        memcpy(out[n], start, len);
+        out[n][len] = 0;
+        ++n;
+        start = NULL;
+      }
+    }
+  }
+  return n;
+}
+
+static int get_param(const char *entry, const char *key, char *val, int vlen) {
+  const char *lp = strchr(entry, '(');
+  if (!lp) return 0;
+  char buf[1024];
+  snprintf(buf, sizeof buf, "%s", lp + 1);
+  char *rp = strrchr(buf, ')');
+  if (rp) *rp = 0;
+  char *save = NULL;
+  for (char *tok = strtok_r(buf, ";", &save); tok; tok = strtok_r(NULL, ";", &save)) {
+    char t[512];
+    snprintf(t, sizeof t, "%s", tok);
+    trim(t);
+    char *eq = strchr(t, '=');
+    if (!eq) continue;
+    *eq = 0;
+    char k[128];
+    snprintf(k, sizeof k, "%s", t);
+    trim(k);
+    if (strcmp(k, key) == 0) {
+      snprintf(val, (size_t)vlen, "%s", eq + 1);
+      trim(val);
+      size_t L = strlen(val);
+      if (L > 0 && val[L - 1] == ',') val[L - 1] = 0;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+static v3 parse_vec(const char *s) {
+  double a = 0, b = 0, c = 0;
+  const char *p = strchr(s, '[');
+  if (p) sscanf(p + 1, "%lf , %lf , %lf", &a, &b, &c);
+  return V(a, b, c);
+}
+
+static void entry_name(const char *e, char *name, int n) {
+  const char *lp = strchr(e, '(');
+  int L = lp ? (int)(lp - e) : (int)strlen(e);
+  if (L >= n) L = n - 1;
+  memcpy(name, e, (size_t)L);
+  name[L] = 0;
+  trim(name);
+}
+
+/* fluid properties: read <name>.dat from the case directory (as
+ * src/fluid.cpp:105-118 does), else the built-in values of
+ * fluidDatabase/air.dat:14-15 and fluidDatabase/N2.dat */
+static int fluid_props(const char *dir, const char *name, double *n, double *M) {
+  char path[1200];
+  snprintf(path, sizeof path, "%s/%s.dat", dir, name);
+  FILE *f = fopen(path, "r");
+  if (f) {
+    char line[512];
+    int got = 0;
+    while (fgets(line, sizeof line, f)) {
+      trim(line);
+      char *c = strchr(line, ':');
+      if (!c) continue;
+      *c = 0;
+      char key[128];
+      snprintf(key, sizeof key, "%s", line);
+      trim(key);
+      if (strcmp(key, "n") == 0) { *n = strtod(c + 1, NULL); got |= 1; }
+      if (strcmp(key, "molarMass") == 0) { *M = strtod(c + 1, NULL) / 1000.; got |= 2; }
+    }
+    fclose(f);
+    if (got == 3) return 0;
+  }
+  if (strcmp(name, "air") == 0) { *n = 2.5; *M = 28.97 / 1000.; return 0; }
+  if (strcmp(name, "N2") == 0) { *n = 2.5; *M = 28.0134 / 1000.; return 0; }
+  return -1;
+}
+
+static int read_p3d(const char *path, double lref, int *nb, int **dims, v3 ***coords) {
+  /* ReadP3dGrid: src/plot3d.cpp:363-445 */
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  int n = 0;
+  if (fread(&n, sizeof(int), 1, f) != 1 || n <= 0 || n > 100000) { fclose(f); return -2; }
+  int *d = (int *)malloc(sizeof(int) * 3 * (size_t)n);
+  if (fread(d, sizeof(int), 3 * (size_t)n, f) != 3 * (size_t)n) { fclose(f); free(d); return -2; }
+  v3 **c = (v3 **)calloc((size_t)n, sizeof(v3 *));
+  for (int b = 0; b < n; ++b) {
+    long np = (long)d[3 * b] * d[3 * b + 1] * d[3 * b + 2];
+    c[b] = (v3 *)malloc(sizeof(v3) * (size_t)np);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)np);
+    for (int comp = 0; comp < 3; ++comp) {
+      if (fread(tmp, sizeof(double), (size_t)np, f) != (size_t)np) {
+        fclose(f); free(tmp); return -2;
+      }
+      for (long q = 0; q < np; ++q) {
+        double v = tmp[q] / lref;
+        if (comp == 0) c[b][q].x = v;
+        else if (comp == 1) c[b][q].y = v;
+        else c[b][q].z = v;
+      }
+    }
+    free(tmp);
+  }
+  fclose(f);
+  *nb = n;
+  *dims = d;
+  *coords = c;
+  return 0;
+}
+
+/* nearest neighbour over a point cloud (the reference uses a k-d tree,
+ * src/kdtree.cpp:29-225; any exact nearest-neighbour search gives the same
+ * match).  Uniform bucket grid. */
+typedef struct {
+  int np;
+  v3 *pts;
+  double lo[3], h[3];
+  int nb[3];
+  int *head, *next;
+} nn_t;
+
+static void nn_build(nn_t *t, v3 *pts, int np) {
+  t->np = np;
+  t->pts = pts;
+  double hi[3];
+  t->lo[0] = t->lo[1] = t->lo[2] = 1e300;
+  hi[0] = hi[1] = hi[2] = -1e300;
+  for (int q = 0; q < np; ++q) {
+    double c[3] = {pts[q].x, pts[q].y, pts[q].z};
+    for (int d = 0; d < 3; ++d) {
+      if (c[d] < t->lo[d]) t->lo[d] = c[d];
+      if (c[d] > hi[d]) hi[d] = c[d];
+    }
+  }
+  double m = cbrt((double)np);
+  for (int d = 0; d < 3; ++d) {
+    int nbd = (int)m + 1;
+    if (nbd > 256) nbd = 256;
+    t->nb[d] = nbd;
+    double ext = hi[d] - t->lo[d];
+    t->h[d] = ext > 0 ? ext / nbd * 1.000001 : 1.0;
+  }
+  long nbt = (long)t->nb[0] * t->nb[1] * t->nb[2];
+  t->head = (int *)malloc(sizeof(int) * (size_t)nbt);
+  t->next = (int *)malloc(sizeof(int) * (size_t)np);
+  for (long q = 0; q < nbt; ++q) t->head[q] = -1;
+  for (int q = 0; q < np; ++q) {
+    double c[3] = {pts[q].x, pts[q].y, pts[q].z};
+    int bi[3];
+    for (int d = 0; d < 3; ++d) {
+      bi[d] = (int)((c[d] - t->lo[d]) / t->h[d]);
+      if (bi[d] < 0) bi[d] = 0;
+      if (bi[d] >= t->nb[d]) bi[d] = t->nb[d] - 1;
+    }
+    long key = bi[0] + (long)bi[1] * t->nb[0] + (long)bi[2] * t->nb[0] * t->nb[1];
+    t->next[q] = t->head[key];
+    t->head[key] = q;
+  }
+}
+
+static int nn_query(const nn_t *t, v3 x) {
+  double c[3] = {x.x, x.y, x.z};
+  int bi[3];
+  for (int d = 0; d < 3; ++d) {
+    bi[d] = (int)floor((c[d] - t->lo[d]) / t->h[d]);
+    if (bi[d] < 0) bi[d] = 0;
+    if (bi[d] >= t->nb[d]) bi[d] = t->nb[d] - 1;
+  }
+  int best = -1;
+  double bd = 1e300;
+  int maxr = t->nb[0] + t->nb[1] + t->nb[2];
+  for (int r = 0; r <= maxr; ++r) {
+    for (int a = bi[0] - r; a <= bi[0] + r; ++a)
+      for (int bq = bi[1] - r; bq <= bi[1] + r; ++bq)
+        for (int cq = bi[2] - r; cq <= bi[2] + r; ++cq) {
+          if (a < 0 || bq < 0 || cq < 0 || a >= t->nb[0] || bq >= t->nb[1] || cq >= t->nb[2])
+            continue;
+          if (abs(a - bi[0]) != r && abs(bq - bi[1]) != r && abs(cq - bi[2]) != r) continue;
+          long key = a + (long)bq * t->nb[0] + (long)cq * t->nb[0] * t->nb[1];
+          for (int q = t->head[key]; q >= 0; q = t->next[q]) {
+            v3 dv = vsub(t->pts[q], x);
+            double dd = vdot(dv, dv);
+            if (dd < bd) { bd = dd; best = q; }
+          }
+        }
+    if (best >= 0) {
+      double hmin = t->h[0];
+      if (t->h[1] < hmin) hmin = t->h[1];
+      if (t->h[2] < hmin) hmin = t->h[2];
+      double reach = r * hmin;
+      if (reach * reach >= bd) break;
+    }
+  }
+  return best;
+}
+
+static void free_case(orc_case *c);
+
+static int alloc_block(blk_t *b) {
+  long nc = ncells_padded(b), np = ncells(b);
+  b->vol = (double *)calloc((size_t)nc, sizeof(double));
+  b->ctr = (v3 *)calloc((size_t)nc, sizeof(v3));
+  for (int d = 0; d < 3; ++d) {
+    b->fa[d] = (uv3 *)calloc((size_t)nfaces_padded(b, d), sizeof(uv3));
+    b->fc[d] = (v3 *)calloc((size_t)nfaces_padded(b, d), sizeof(v3));
+    b->cw[d] = (double *)calloc((size_t)nc, sizeof(double));
+  }
+  b->W = (double *)calloc((size_t)nc * NEQ, sizeof(double));
+  b->x = (double *)calloc((size_t)nc * NEQ, sizeof(double));
+  b->R = (double *)calloc((size_t)np * NEQ, sizeof(double));
+  b->sr = (double *)calloc((size_t)np, sizeof(double));
+  b->dt = (double *)calloc((size_t)np, sizeof(double));
+  b->A = (double *)calloc((size_t)np, sizeof(double));
+  b->Ainv = (double *)calloc((size_t)np, sizeof(double));
+  b->consN = (double *)calloc((size_t)np * NEQ, sizeof(double));
+  b->consNm1 = (double *)calloc((size_t)np * NEQ, sizeof(double));
+  b->order = (int *)malloc(sizeof(int) * 3 * (size_t)np);
+  return (b->vol && b->W && b->order) ? 0 : -1;
+}
+
+static const bcstate_t *find_bc(const orc_case *c, int tag) {
+  for (int q = 0; q < c->nbc; ++q)
+    if (c->bcs[q].tag == tag) return &c->bcs[q];
+  return NULL;
+}
+
+orc_case *orc_load(const char *inp_path, char *err, int errlen) {
+  FILE *f = fopen(inp_path, "r");
+  if (!f) {
+    seterr(err, errlen, "cannot open %s", inp_path);
+    return NULL;
+  }
+  orc_case *c = (orc_case *)calloc(1, sizeof(orc_case));
+  /* defaults: src/input.cpp:50-103 */
+  c->iterations = 1;
+  c->rRef = -1.0;
+  c->tRef = -1.0;
+  c->lRef = 1.0;
+  c->dtIn = -1.0;
+  c->cflMax = 1.0;
+  c->cflStep = 0.0;
+  c->cflStart = 1.0;
+  c->theta = 1.0;
+  c->zeta = 0.0;
+  c->relax = 1.0;
+  c->sweeps = 1;
+  c->nonlinear = 1;
+  c->dualCFL = -1.0;
+  c->kappa = -2.0;
+  c->recon = 0;
+  c->limiter = 0;
+  c->flux = 0;
+  c->timeint = 0;
+  c->solver = 0;
+  char fluidName[64] = "air";
+  {
+    const char *sl = strrchr(inp_path, '/');
+    if (sl) {
+      size_t L = (size_t)(sl - inp_path);
+      if (L >= sizeof c->dir) L = sizeof c->dir - 1;
+      memcpy(c->dir, inp_path, L);
+      c->dir[L] = 0;
+    } else {
+      strcpy(c->dir, ".");
+    }
+  }
+  char line[4096];
+  int readingBC = 0, nbcBlocks = 0, curBlk = 0, surfLeft = -1, surfIdx = 0;
+  int *nsurfPer = NULL;
+  surf_t **surfsPer = NULL;
+  while (fgets(line, sizeof line, f)) {
+    trim(line);
+    if (!line[0]) continue;
+    if (readingBC) {
+      if (surfLeft < 0) {
+        int a, b2, c3;
+        if (sscanf(line, "%d %d %d", &a, &b2, &c3) != 3) {
+          seterr(err, errlen, "bad BC surface count line: %s", line);
+          goto fail;
+        }
+        nsurfPer[curBlk] = a + b2 + c3;
+        surfsPer[curBlk] = (surf_t *)calloc((size_t)(a + b2 + c3), sizeof(surf_t));
+        surfLeft = a + b2 + c3;
+        surfIdx = 0;
+      } else {
+        surf_t *s = &surfsPer[curBlk][surfIdx];
+        if (sscanf(line, "%31s %d %d %d %d %d %d %d", s->type, &s->d[0], &s->d[1],
+                   &s->d[2], &s->d[3], &s->d[4], &s->d[5], &s->d[6]) != 8) {
+          seterr(err, errlen, "bad BC surface line: %s", line);
+          goto fail;
+        }
+        ++surfIdx;
+        --surfLeft;
+      }
+      if (surfLeft == 0) {
+        ++curBlk;
+        surfLeft = -1;
+        if (curBlk == nbcBlocks) readingBC = 0;
+      }
+      continue;
+    }
+    char *colon = strchr(line, ':');
+    if (!colon) continue;
+    *colon = 0;
+    char key[128];
+    snprintf(key, sizeof key, "%s", line);
+    trim(key);
+    char val[8192];
+    snprintf(val, sizeof val, "%s", colon + 1);
+    trim(val);
+    if (val[0] == '<' && !strchr(val, '>')) {
+      char more[4096];
+      while (fgets(more, sizeof more, f)) {
+        trim(more);
+        strncat(val, " ", sizeof val - strlen(val) - 1);
+        strncat(val, more, sizeof val - strlen(val) - 1);
+        if (strchr(more, '>')) break;
+      }
+    }
+    if (!strcmp(key, "gridName")) snprintf(c->grid, sizeof c->grid, "%s", val);
+    else if (!strcmp(key, "timeStep")) c->dtIn = strtod(val, NULL);
+    else if (!strcmp(key, "iterations")) c->iterations = atoi(val);
+    else if (!strcmp(key, "referenceDensity")) c->rRef = strtod(val, NULL);
+    else if (!strcmp(key, "referenceTemperature")) c->tRef = strtod(val, NULL);
+    else if (!strcmp(key, "referenceLength")) c->lRef = strtod(val, NULL);
+    else if (!strcmp(key, "timeIntegration")) {
+      if (!strcmp(val, "explicitEuler")) c->timeint = 0;
+      else if (!strcmp(val, "implicitEuler")) { c->timeint = 1; c->theta = 1.0; c->zeta = 0.0; }
+      else if (!strcmp(val, "crankNicholson")) { c->timeint = 2; c->theta = 0.5; c->zeta = 0.0; }
+      else if (!strcmp(val, "bdf2")) { c->timeint = 3; c->theta = 1.0; c->zeta = 0.5; }
+      else { seterr(err, errlen, "unsupported timeIntegration %s", val); goto fail; }
+    } else if (!strcmp(key, "faceReconstruction")) {
+      c->recon = 1;
+      if (!strcmp(val, "upwind")) c->kappa = -1.0;
+      else if (!strcmp(val, "fromm")) c->kappa = 0.0;
+      else if (!strcmp(val, "quick")) c->kappa = 0.5;
+      else if (!strcmp(val, "central")) c->kappa = 1.0;
+      else if (!strcmp(val, "thirdOrder")) c->kappa = 1.0 / 3.0;
+      else if (!strcmp(val, "constant")) c->recon = 0;
+      else { seterr(err, errlen, "unsupported faceReconstruction %s", val); goto fail; }
+    } else if (!strcmp(key, "limiter")) {
+      if (!strcmp(val, "none")) c->limiter = 0;
+      else if (!strcmp(val, "vanAlbada")) c->limiter = 1;
+      else if (!strcmp(val, "minmod")) c->limiter = 2;
+      else { seterr(err, errlen, "unsupported limiter %s", val); goto fail; }
+    } else if (!strcmp(key, "equationSet")) {
+      if (strcmp(val, "euler")) { seterr(err, errlen, "unsupported equationSet %s", val); goto fail; }
+    } else if (!strcmp(key, "matrixSolver")) {
+      if (!strcmp(val, "lusgs")) c->solver = 0;
+      else if (!strcmp(val, "dplur")) c->solver = 1;
+      else { seterr(err, errlen, "unsupported matrixSolver %s", val); goto fail; }
+    } else if (!strcmp(key, "matrixSweeps")) c->sweeps = atoi(val);
+    else if (!strcmp(key, "matrixRelaxation")) c->relax = strtod(val, NULL);
+    else if (!strcmp(key, "nonlinearIterations")) c->nonlinear = atoi(val);
+    else if (!strcmp(key, "cflMax")) c->cflMax = strtod(val, NULL);
+    else if (!strcmp(key, "cflStep")) c->cflStep = strtod(val, NULL);
+    else if (!strcmp(key, "cflStart")) c->cflStart = strtod(val, NULL);
+    else if (!strcmp(key, "dualTimeCFL")) c->dualCFL = strtod(val, NULL);
+    else if (!strcmp(key, "inviscidFlux")) {
+      if (!strcmp(val, "roe")) c->flux = 0;
+      else if (!strcmp(val, "ausm")) c->flux = 1;
+      else { seterr(err, errlen, "unsupported inviscidFlux %s", val); goto fail; }
+    } else if (!strcmp(key, "inviscidFluxJacobian")) {
+      if (strcmp(val, "rusanov")) { seterr(err, errlen, "unsupported inviscidFluxJacobian %s", val); goto fail; }
+    } else if (!strcmp(key, "multigridLevels")) {
+      if (atoi(val) != 1) { seterr(err, errlen, "multigrid is not supported"); goto fail; }
+    } else if (!strcmp(key, "fluids")) {
+      char ents[8][1024];
+      int ne = split_entries(val, ents, 8);
+      if (ne != 1) { seterr(err, errlen, "only single-species fluids are supported"); goto fail; }
+      char nm[64];
+      if (get_param(ents[0], "name", nm, sizeof nm)) snprintf(fluidName, sizeof fluidName, "%s", nm);
+    } else if (!strcmp(key, "initialConditions")) {
+      char ents[16][1024];
+      int ne = split_entries(val, ents, 16);
+      for (int q = 0; q < ne && c->nic < 16; ++q) {
+        char v[512];
+        c->ic[c->nic].tag = get_param(ents[q], "tag", v, sizeof v) ? atoi(v) : -1;
+        if (get_param(ents[q], "file", v, sizeof v)) {
+          c->ic[c->nic].fromFile = 1;
+          snprintf(c->ic[c->nic].file, sizeof c->ic[c->nic].file, "%s", v);
+        } else {
+          if (get_param(ents[q], "pressure", v, sizeof v)) c->ic[c->nic].p = strtod(v, NULL);
+          if (get_param(ents[q], "density", v, sizeof v)) c->ic[c->nic].rho = strtod(v, NULL);
+          if (get_param(ents[q], "velocity", v, sizeof v)) c->ic[c->nic].vel = parse_vec(v);
+        }
+        c->nic++;
+      }
+    } else if (!strcmp(key, "boundaryStates")) {
+      char ents[MAXBC][1024];
+      int ne = split_entries(val, ents, MAXBC);
+      for (int q = 0; q < ne && c->nbc < MAXBC; ++q) {
+        bcstate_t *s = &c->bcs[c->nbc++];
+        char v[512];
+        entry_name(ents[q], s->kind, sizeof s->kind);
+        s->tag = get_param(ents[q], "tag", v, sizeof v) ? atoi(v) : -1;
+        if (get_param(ents[q], "pressure", v, sizeof v)) s->p = strtod(v, NULL);
+        if (get_param(ents[q], "density", v, sizeof v)) s->rho = strtod(v, NULL);
+        if (get_param(ents[q], "velocity", v, sizeof v)) s->vel = parse_vec(v);
+        if (get_param(ents[q], "p0", v, sizeof v)) s->p0 = strtod(v, NULL);
+        if (get_param(ents[q], "t0", v, sizeof v)) s->t0 = strtod(v, NULL);
+        if (get_param(ents[q], "direction", v, sizeof v)) s->dir = parse_vec(v);
+        if (get_param(ents[q], "nonreflecting", v, sizeof v)) s->nonreflecting = !strcmp(v, "true");
+        if (s->nonreflecting) { seterr(err, errlen, "nonreflecting BCs are not supported"); goto fail; }
+      }
+    } else if (!strcmp(key, "boundaryConditions")) {
+      nbcBlocks = atoi(val);
+      nsurfPer = (int *)calloc((size_t)nbcBlocks, sizeof(int));
+      surfsPer = (surf_t **)calloc((size_t)nbcBlocks, sizeof(surf_t *));
+      readingBC = nbcBlocks > 0;
+      curBlk = 0;
+      surfLeft = -1;
+    }
+    /* other keys (output, decomposition, models at their defaults) are
+     * accepted and ignored */
+  }
+  fclose(f);
+  f = NULL;
+
+  if (c->rRef <= 0 || c->tRef <= 0) { seterr(err, errlen, "reference density/temperature missing"); goto fail2; }
+  if (fluid_props(c->dir, fluidName, &c->fluidN, &c->fluidM)) {
+    seterr(err, errlen, "unknown fluid %s", fluidName);
+    goto fail2;
+  }
+  c->implicit = (c->timeint >= 1);
+  c->multilevel = (c->timeint == 3);
+  c->storeN = c->implicit;
+  c->initMatrix = (c->solver == 1 || c->sweeps > 1);
+  if (c->timeint == 0) c->nonlinear = 1; /* src/input.cpp:928-934 */
+
+  /* reference speed of sound and nondimensional gas constant:
+   * src/input.cpp:616-621, src/fluid.cpp:89-103 */
+  {
+    double Ru = 8.3144598;
+    double Rdim = Ru / c->fluidM;
+    double gamma = (c->fluidN + 1) / c->fluidN;
+    double a = 0.0;
+    a += 1.0 * gamma * Rdim * c->tRef;
+    c->aRef = sqrt(a);
+    double RuNd = Ru / (c->aRef * c->aRef * c->rRef / (c->tRef * pow(c->lRef, 3.0)));
+    double Mnd = c->fluidM / (c->rRef / pow(c->lRef, 3.0));
+    c->gas.R = RuNd / Mnd;
+    c->gas.n = c->fluidN;
+    c->gamma = orc_gamma(&c->gas);
+  }
+  /* state data nondimensionalisation: src/inputStates.cpp:464-473, 590-599,
+   * 674-681 (stagnationInlet direction is left as given: Normalize() returns
+   * a copy, src/inputStates.cpp:594) */
+  for (int q = 0; q < c->nic; ++q) {
+    if (c->ic[q].fromFile) continue;
+    c->ic[q].vel = vdiv(c->ic[q].vel, c->aRef);
+    c->ic[q].rho /= c->rRef;
+    c->ic[q].p /= c->rRef * c->aRef * c->aRef;
+  }
+  for (int q = 0; q < c->nbc; ++q) {
+    bcstate_t *s = &c->bcs[q];
+    if (!strcmp(s->kind, "stagnationInlet")) {
+      s->p0 /= c->rRef * c->aRef * c->aRef;
+      s->t0 /= c->tRef;
+    } else if (!strcmp(s->kind, "pressureOutlet")) {
+      s->p /= c->rRef * c->aRef * c->aRef;
+    } else {
+      s->vel = vdiv(s->vel, c->aRef);
+      s->rho /= c->rRef;
+      s->p /= c->rRef * c->aRef * c->aRef;
+    }
+  }
+
+  /* grid */
+  {
+    char path[2048];
+    snprintf(path, sizeof path, "%s/%s.xyz", c->dir, c->grid);
+    int nbg = 0, *dims = NULL;
+    v3 **coords = NULL;
+    if (read_p3d(path, c->lRef, &nbg, &dims, &coords)) {
+      seterr(err, errlen, "cannot read grid %s", path);
+      goto fail2;
+    }
+    if (nbg != nbcBlocks) {
+      seterr(err, errlen, "grid has %d blocks but input has %d BC blocks", nbg, nbcBlocks);
+      goto fail2;
+    }
+    c->nb = nbg;
+    c->blk = (blk_t *)calloc((size_t)nbg, sizeof(blk_t));
+    int g = (c->recon == 0) ? 1 : 2; /* src/input.cpp:1127-1143 */
+    c->totalCells = 0;
+    for (int bq = 0; bq < nbg; ++bq) {
+      blk_t *b = &c->blk[bq];
+      b->ni = dims[3 * bq] - 1;
+      b->nj = dims[3 * bq + 1] - 1;
+      b->nk = dims[3 * bq + 2] - 1;
+      b->g = g;
+      b->NI = b->ni + 2 * g;
+      b->NJ = b->nj + 2 * g;
+      b->NK = b->nk + 2 * g;
+      c->totalCells += (double)ncells(b);
+      if (alloc_block(b)) { seterr(err, errlen, "out of memory"); goto fail2; }
+      b->nsurf = nsurfPer[bq];
+      b->surf = surfsPer[bq];
+      surfsPer[bq] = NULL;
+      for (int s = 0; s < b->nsurf; ++s) {
+        if (!strcmp(b->surf[s].type, "interblock") || !strcmp(b->surf[s].type, "periodic")) {
+          seterr(err, errlen, "interblock/periodic connections are not supported");
+          goto fail2;
+        }
+      }
+      qsort(b->surf, (size_t)b->nsurf, sizeof(surf_t), surf_cmp);
+      if (build_geometry(b, coords[bq])) { seterr(err, errlen, "negative volume in block %d", bq); goto fail2; }
+      ghost_geometry(b);
+      cell_widths(b);
+      orc_hyperplane_order(b->ni, b->nj, b->nk, b->order);
+      free(coords[bq]);
+    }
+    free(coords);
+    free(dims);
+  }
+
+  /* initial conditions: procBlock::InitializeStates, src/procBlock.cpp:280-355 */
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    int use = -1;
+    for (int q = 0; q < c->nic; ++q) {
+      if (c->ic[q].tag == -1 && use < 0) use = q;
+      if (c->ic[q].tag == bq) { use = q; break; }
+    }
+    if (use < 0) { seterr(err, errlen, "no initial condition for block %d", bq); goto fail2; }
+    if (!c->ic[use].fromFile) {
+      for (int k = 0; k < b->nk; ++k)
+        for (int j = 0; j < b->nj; ++j)
+          for (int i = 0; i < b->ni; ++i) {
+            double *W = &b->W[cidx(b, i, j, k) * NEQ];
+            W[0] = 1.0 * c->ic[use].rho;
+            W[1] = c->ic[use].vel.x;
+            W[2] = c->ic[use].vel.y;
+            W[3] = c->ic[use].vel.z;
+            W[4] = c->ic[use].p;
+          }
+    } else {
+      /* CalcTreeFromCloud: src/utility.cpp:513-605 */
+      char path[2048];
+      snprintf(path, sizeof path, "%s/%s", c->dir, c->ic[use].file);
+      FILE *cf = fopen(path, "r");
+      if (!cf) { seterr(err, errlen, "cannot open IC file %s", path); goto fail2; }
+      int np = 0;
+      char sp[256];
+      if (fscanf(cf, "%d", &np) != 1 || np <= 0) { fclose(cf); seterr(err, errlen, "bad IC file"); goto fail2; }
+      if (fscanf(cf, "%255s", sp) != 1) { fclose(cf); seterr(err, errlen, "bad IC file"); goto fail2; }
+      v3 *pts = (v3 *)malloc(sizeof(v3) * (size_t)np);
+      double *st = (double *)malloc(sizeof(double) * NEQ * (size_t)np);
+      for (int q = 0; q < np; ++q) {
+        double x, y, z, r, u, v, w, p, tk, om, mf;
+        if (fscanf(cf, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &x, &y, &z, &r, &u,
+                   &v, &w, &p, &tk, &om, &mf) != 11) {
+          fclose(cf); free(pts); free(st);
+          seterr(err, errlen, "bad IC line %d", q);
+          goto fail2;
+        }
+        pts[q] = V(x / c->lRef, y / c->lRef, z / c->lRef);
+        double rho = r / c->rRef;
+        st[NEQ * q + 0] = rho * mf;
+        st[NEQ * q + 1] = u / c->aRef;
+        st[NEQ * q + 2] = v / c->aRef;
+        st[NEQ * q + 3] = w / c->aRef;
+        st[NEQ * q + 4] = p / (c->rRef * c->aRef * c->aRef);
+      }
+      fclose(cf);
+      nn_t t;
+      nn_build(&t, pts, np);
+      for (int k = 0; k < b->nk; ++k)
+        for (int j = 0; j < b->nj; ++j)
+          for (int i = 0; i < b->ni; ++i) {
+            int id = nn_query(&t, b->ctr[cidx(b, i, j, k)]);
+            memcpy(&b->W[cidx(b, i, j, k) * NEQ], &st[NEQ * id], sizeof(double) * NEQ);
+          }
+      free(t.head);
+      free(t.next);
+      free(pts);
+      free(st);
+    }
+  }
+  free(nsurfPer);
+  free(surfsPer);
+  c->matres_on = 1;
+  return c;
+
+fail:
+  if (f) fclose(f);
+fail2:
+  if (surfsPer) {
+    for (int q = 0; q < nbcBlocks; ++q) free(surfsPer[q]);
+    free(surfsPer);
+  }
+  free(nsurfPer);
+  free_case(c);
+  return NULL;
+}
+
+static void free_case(orc_case *c) {
+  if (!c) return;
+  for (int bq = 0; bq < c->nb && c->blk; ++bq) {
+    blk_t *b = &c->blk[bq];
+    free(b->vol); free(b->ctr);
+    for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
+    free(b->W); free(b->x); free(b->R); free(b->sr); free(b->dt);
+    free(b->A); free(b->Ainv); free(b->consN); free(b->consNm1);
+    free(b->order); free(b->surf);
+  }
+  free(c->blk);
+  free(c);
+}
+void orc_free(orc_case *c) { free_case(c); }
+
+/* ------------------------------------------------------------------------ */
+/* boundary conditions                                                      */
+
+/* ExtrapolateHoldMixture: src/ghostStates.cpp:691-709 */
+static void extrap_hold(const double *bnd, double factor, const double *in,
+                        double *out) {
+  double bndRho = 0.0 + bnd[0];
+  double bndMf = bnd[0] / bndRho;
+  double intRho = 0.0 + in[0];
+  double ghostRho = factor * bndRho - intRho;
+  if (ghostRho <= 0.0) {
+    memcpy(out, bnd, sizeof(double) * NEQ);
+    return;
+  }
+  for (int e = 0; e < NEQ; ++e) out[e] = factor * bnd[e] - in[e];
+  out[0] = dmax(ghostRho * bndMf, 0.0);
+}
+
+/* GetGhostState: src/ghostStates.cpp:62-689 (inviscid branches) */
+static int ghost_state(const orc_case *c, const double *in, const char *bc,
+                       v3 area, int surf, int tag, int layer, double *ghost) {
+  const orc_gas *g = &c->gas;
+  memcpy(ghost, in, sizeof(double) * NEQ);
+  int isLower = (surf % 2 == 1);
+  v3 n = isLower ? vscl(-1.0, area) : area;
+  if (!strcmp(bc, "slipWall")) {
+    v3 vel = V(in[1], in[2], in[3]);
+    double vn = vdot(vel, n);
+    v3 gv = vsub(vel, vscl(vn, vscl(2.0, n)));
+    ghost[1] = gv.x;
+    ghost[2] = gv.y;
+    ghost[3] = gv.z;
+  } else if (!strcmp(bc, "characteristic")) {
+    const bcstate_t *s = find_bc(c, tag);
+    if (!s) return -1;
+    double fs[NEQ];
+    fs[0] = s->rho * 1.0;
+    fs[1] = s->vel.x;
+    fs[2] = s->vel.y;
+    fs[3] = s->vel.z;
+    fs[4] = s->p;
+    double velIntNorm = vdot(V(in[1], in[2], in[3]), n);
+    double sosInt = orc_sos(g, in);
+    double machInt = fabs(velIntNorm) / sosInt;
+    if (machInt >= 1.0 && velIntNorm < 0.0) {
+      memcpy(ghost, fs, sizeof fs);
+    } else if (machInt >= 1.0 && velIntNorm >= 0.0) {
+      /* interior values */
+    } else if (machInt < 1.0 && velIntNorm < 0.0) {
+      double rhoSoSInt = (0.0 + in[0]) * sosInt;
+      v3 velDiff = vsub(V(fs[1], fs[2], fs[3]), V(in[1], in[2], in[3]));
+      ghost[4] = 0.5 * (fs[4] + in[4] - rhoSoSInt * vdot(n, velDiff));
+      double dp = fs[4] - ghost[4];
+      double rho = (0.0 + fs[0]) - dp / (sosInt * sosInt);
+      ghost[0] = rho * (fs[0] / (0.0 + fs[0]));
+      ghost[1] = fs[1] - n.x * dp / rhoSoSInt;
+      ghost[2] = fs[2] - n.y * dp / rhoSoSInt;
+      ghost[3] = fs[3] - n.z * dp / rhoSoSInt;
+    } else if (machInt < 1.0 && velIntNorm >= 0.0) {
+      double rhoSoSInt = (0.0 + in[0]) * sosInt;
+      double dp = in[4] - fs[4];
+      double rho = (0.0 + in[0]) - dp / (sosInt * sosInt);
+      ghost[0] = rho * (in[0] / (0.0 + in[0]));
+      ghost[1] = in[1] + n.x * dp / rhoSoSInt;
+      ghost[2] = in[2] + n.y * dp / rhoSoSInt;
+      ghost[3] = in[3] + n.z * dp / rhoSoSInt;
+      ghost[4] = fs[4];
+    } else {
+      return -2;
+    }
+    double t1[NEQ];
+    extrap_hold(ghost, 2.0, in, t1);
+    memcpy(ghost, t1, sizeof t1);
+    if (layer > 1) {
+      extrap_hold(ghost, (double)layer, in, t1);
+      memcpy(ghost, t1, sizeof t1);
+    }
+  } else if (!strcmp(bc, "supersonicInflow")) {
+    const bcstate_t *s = find_bc(c, tag);
+    if (!s) return -1;
+    ghost[0] = 0.0;
+    ghost[0] = s->rho * 1.0;
+    ghost[1] = s->vel.x;
+    ghost[2] = s->vel.y;
+    ghost[3] = s->vel.z;
+    ghost[4] = s->p;
+  } else if (!strcmp(bc, "supersonicOutflow")) {
+    if (layer > 1)
+      for (int e = 0; e < NEQ; ++e) ghost[e] = (double)layer * ghost[e] - in[e];
+  } else if (!strcmp(bc, "stagnationInlet")) {
+    const bcstate_t *s = find_bc(c, tag);
+    if (!s) return -1;
+    double gm1 = c->gamma - 1.0;
+    v3 vel = V(in[1], in[2], in[3]);
+    double sosI = orc_sos(g, in);
+    double rNeg = vdot(vel, n) - 2.0 * sosI / gm1;
+    double cosTheta = -1.0 * vdot(vel, n) / vmag(vel);
+    double stagSoSsq = pow(sosI, 2.0) + 0.5 * gm1 * vdot(vel, vel);
+    double sosB = -1.0 * rNeg * gm1 / (gm1 * cosTheta * cosTheta + 2.0) *
+                  (1.0 + cosTheta * sqrt((gm1 * cosTheta * cosTheta + 2.0) *
+                                             stagSoSsq / (gm1 * rNeg * rNeg) -
+                                         0.5 * gm1));
+    double tb = s->t0 * (sosB * sosB / stagSoSsq);
+    double pb = s->p0 * pow(sosB * sosB / stagSoSsq, c->gamma / gm1);
+    double vbMag = sqrt(2.0 / gm1 * (s->t0 - tb));
+    /* DensityTP: src/eos.cpp:111-115 with mf = {1.0} */
+    double Rmix = 0.0 + 1.0 * g->R;
+    double rhoGhost = pb / (Rmix * tb);
+    ghost[0] = 0.0;
+    ghost[0] = rhoGhost * 1.0;
+    ghost[1] = vbMag * s->dir.x;
+    ghost[2] = vbMag * s->dir.y;
+    ghost[3] = vbMag * s->dir.z;
+    ghost[4] = pb;
+    double t1[NEQ];
+    extrap_hold(ghost, 2.0, in, t1);
+    memcpy(ghost, t1, sizeof t1);
+    if (layer > 1) {
+      extrap_hold(ghost, (double)layer, in, t1);
+      memcpy(ghost, t1, sizeof t1);
+    }
+  } else if (!strcmp(bc, "pressureOutlet")) {
+    const bcstate_t *s = find_bc(c, tag);
+    if (!s) return -1;
+    double pb = s->p;
+    double sosI = orc_sos(g, in);
+    double rhoSoSInt = (0.0 + in[0]) * sosI;
+    ghost[4] = pb;
+    double dp = in[4] - ghost[4];
+    double rho = (0.0 + in[0]) - dp / (sosI * sosI);
+    ghost[0] = rho * (in[0] / (0.0 + in[0]));
+    ghost[1] = in[1] + n.x * dp / rhoSoSInt;
+    ghost[2] = in[2] + n.y * dp / rhoSoSInt;
+    ghost[3] = in[3] + n.z * dp / rhoSoSInt;
+    if (vdot(V(ghost[1], ghost[2], ghost[3]), n) / orc_sos(g, ghost) >= 1.0)
+      memcpy(ghost, in, sizeof(double) * NEQ);
+    for (int e = 0; e < NEQ; ++e) ghost[e] = 2.0 * ghost[e] - in[e];
+    if (layer > 1)
+      for (int e = 0; e < NEQ; ++e) ghost[e] = (double)layer * ghost[e] - in[e];
+  } else {
+    return -3;
+  }
+  return 0;
+}
+
+/* procBlock::AssignInviscidGhostCells: src/procBlock.cpp:2449-2529 */
+static void assign_ghosts(orc_case *c, blk_t *b) {
+  for (int layer = 1; layer <= b->g; ++layer) {
+    for (int s = 0; s < b->nsurf; ++s) {
+      const surf_t *sf = &b->surf[s];
+      int st = surf_type(sf);
+      int dir = (st - 1) / 2;
+      int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      int r1s = sf->d[2 * d1], r1e = sf->d[2 * d1 + 1];
+      int r2s = sf->d[2 * d2], r2e = sf->d[2 * d2 + 1];
+      int r3 = sf->d[2 * dir];
+      int n3 = ndir(b, dir);
+      int gC, iC, aC;
+      if (st % 2 == 0) {
+        gC = r3 + layer - 1;
+        iC = r3 - layer;
+        aC = r3 - 1;
+        if (iC < 0) iC = 0;
+      } else {
+        gC = r3 - layer;
+        iC = r3 + layer - 1;
+        aC = r3;
+        if (iC >= n3) iC = n3 - 1;
+      }
+      char bcn[32];
+      snprintf(bcn, sizeof bcn, "%s", sf->type);
+      if (!strcmp(bcn, "viscousWall")) strcpy(bcn, "slipWall");
+      int isSlip = !strcmp(bcn, "slipWall");
+      for (int q2 = r2s; q2 < r2e; ++q2)
+        for (int q1 = r1s; q1 < r1e; ++q1) {
+          int fi, fj, fk, ci, cj, ck, gi, gj, gk;
+          map3(dir, r3, q1, q2, &fi, &fj, &fk);
+          map3(dir, isSlip ? iC : aC, q1, q2, &ci, &cj, &ck);
+          map3(dir, gC, q1, q2, &gi, &gj, &gk);
+          uv3 a = b->fa[dir][fidx(b, dir, fi, fj, fk)];
+          double gs[NEQ];
+          ghost_state(c, &b->W[cidx(b, ci, cj, ck) * NEQ], bcn, a.u, st, sf->d[6],
+                      layer, gs);
+          memcpy(&b->W[cidx(b, gi, gj, gk) * NEQ], gs, sizeof gs);
+        }
+    }
+  }
+}
+
+void orc_bc(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) assign_ghosts(c, &c->blk[bq]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* residual: procBlock::CalcResidualNoSource / CalcInvFluxI/J/K            */
+/* src/procBlock.cpp:6111-6147, 384-769                                     */
+
+static void inv_flux_dir(orc_case *c, blk_t *b, int d) {
+  int ei = (d == 0), ej = (d == 1), ek = (d == 2);
+  int n3 = ndir(b, d);
+  for (int k = 0; k < b->nk + ek; ++k)
+    for (int j = 0; j < b->nj + ej; ++j)
+      for (int i = 0; i < b->ni + ei; ++i) {
+        int f3 = d == 0 ? i : (d == 1 ? j : k);
+        const double *Wm1 = &b->W[cidx(b, i - ei, j - ej, k - ek) * NEQ];
+        const double *W0 = &b->W[cidx(b, i, j, k) * NEQ];
+        double WL[NEQ], WR[NEQ];
+        if (c->recon == 0) {
+          memcpy(WL, Wm1, sizeof WL);
+          memcpy(WR, W0, sizeof WR);
+        } else {
+          const double *Wm2 = &b->W[cidx(b, i - 2 * ei, j - 2 * ej, k - 2 * ek) * NEQ];
+          const double *Wp1 = &b->W[cidx(b, i + ei, j + ej, k + ek) * NEQ];
+          double cm2 = b->cw[d][cidx(b, i - 2 * ei, j - 2 * ej, k - 2 * ek)];
+          double cm1 = b->cw[d][cidx(b, i - ei, j - ej, k - ek)];
+          double c0 = b->cw[d][cidx(b, i, j, k)];
+          double cp1 = b->cw[d][cidx(b, i + ei, j + ej, k + ek)];
+          orc_muscl(Wm2, Wm1, W0, c->kappa, c->limiter, cm2, cm1, c0, WL);
+          orc_muscl(Wp1, W0, Wm1, c->kappa, c->limiter, cp1, c0, cm1, WR);
+        }
+        uv3 a = b->fa[d][fidx(b, d, i, j, k)];
+        double n[3] = {a.u.x, a.u.y, a.u.z};
+        double F[NEQ];
+        if (c->flux == 0) orc_roe_flux(&c->gas, WL, WR, n, F);
+        else orc_ausm_flux(&c->gas, WL, WR, n, F);
+        if (f3 > 0) {
+          double *R = &b->R[pidx(b, i - ei, j - ej, k - ek) * NEQ];
+          for (int e = 0; e < NEQ; ++e) R[e] += F[e] * a.m;
+        }
+        if (f3 < n3) {
+          double *R = &b->R[pidx(b, i, j, k) * NEQ];
+          for (int e = 0; e < NEQ; ++e) R[e] -= F[e] * a.m;
+          uv3 aU = b->fa[d][fidx(b, d, i + ei, j + ej, k + ek)];
+          double lam = inv_cell_specrad(&c->gas, W0, a, aU);
+          b->sr[pidx(b, i, j, k)] += lam;
+          if (c->implicit) b->A[pidx(b, i, j, k)] += lam;
+        }
+      }
+}
+
+void orc_residual(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    memset(b->R, 0, sizeof(double) * NEQ * (size_t)ncells(b));
+    memset(b->sr, 0, sizeof(double) * (size_t)ncells(b));
+    inv_flux_dir(c, b, 0);
+    inv_flux_dir(c, b, 1);
+    inv_flux_dir(c, b, 2);
+  }
+}
+
+/* CalcBlockTimeStep / CalcCellDt: src/procBlock.cpp:782-820 */
+void orc_time_step(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          long p = pidx(b, i, j, k);
+          if (c->dtIn > 0.0) b->dt[p] = c->dtIn * c->aRef / c->lRef;
+          else b->dt[p] = c->cfl * (b->vol[cidx(b, i, j, k)] / dmax(b->sr[p], 0.0));
+        }
+  }
+}
+
+/* AddDiagonalTerms + Invert: src/linearSolver.cpp:146-188, src/matrix.cpp:57-105 */
+void orc_invert_diagonal(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          long p = pidx(b, i, j, k);
+          double dvt = (b->vol[cidx(b, i, j, k)] * (1.0 + c->zeta)) / (b->dt[p] * c->theta);
+          if (c->dualCFL > 0.0) dvt += dmax(b->sr[p], 0.0) / c->dualCFL;
+          b->A[p] *= c->relax;
+          b->A[p] += dvt;
+          b->Ainv[p] = 1.0 * (1.0 / b->A[p]);
+        }
+  }
+}
+
+void orc_reset_diagonal(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq)
+    memset(c->blk[bq].A, 0, sizeof(double) * (size_t)ncells(&c->blk[bq]));
+}
+
+/* SolDeltaMmN / SolDeltaNm1: src/procBlock.cpp:1010-1034; the rhs
+ * b = -R/theta + forcing + Nm1 - MmN (forcing is zero on the single level) */
+static void rhs_b(const orc_case *c, const blk_t *b, int i, int j, int k,
+                  int withForcing, double *out) {
+  long p = pidx(b, i, j, k);
+  double vol = b->vol[cidx(b, i, j, k)];
+  double thetaInv = 1.0 / c->theta;
+  double coeff = (vol * (1.0 + c->zeta)) / (b->dt[p] * c->theta);
+  double U[NEQ];
+  orc_prim_to_cons(&c->gas, &b->W[cidx(b, i, j, k) * NEQ], U);
+  double nm1[NEQ] = {0, 0, 0, 0, 0};
+  if (c->multilevel) {
+    double c2 = (vol * c->zeta) / (b->dt[p] * c->theta);
+    for (int e = 0; e < NEQ; ++e)
+      nm1[e] = c2 * (b->consN[p * NEQ + e] - b->consNm1[p * NEQ + e]);
+  }
+  for (int e = 0; e < NEQ; ++e) {
+    double mmn = coeff * (U[e] - b->consN[p * NEQ + e]);
+    double v = -thetaInv * b->R[p * NEQ + e];
+    if (withForcing) v = v + 0.0;
+    out[e] = (v + nm1[e]) - mmn;
+  }
+}
+
+/* ImplicitLower / ImplicitUpper: src/procBlock.cpp:1056-1161 */
+static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
+                        const double *x, int upper, double *out) {
+  for (int e = 0; e < NEQ; ++e) out[e] = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    int ei = (d == 0), ej = (d == 1), ek = (d == 2);
+    int s = upper ? 1 : -1;
+    int ni = i + s * ei, nj = j + s * ej, nk = k + s * ek;
+    if (ni < 0 || nj < 0 || nk < 0 || ni >= b->ni || nj >= b->nj || nk >= b->nk) continue;
+    /* face between the cell and its neighbour */
+    int fi = upper ? ni : i, fj = upper ? nj : j, fk = upper ? nk : k;
+    uv3 a = b->fa[d][fidx(b, d, fi, fj, fk)];
+    double o[NEQ];
+    scalar_offdiag(&c->gas, &b->W[cidx(b, ni, nj, nk) * NEQ],
+                   &x[cidx(b, ni, nj, nk) * NEQ], a, !upper, o);
+    for (int e = 0; e < NEQ; ++e) out[e] += o[e];
+  }
+}
+
+/* InitializeMatrixUpdate: src/linearSolver.cpp:111-144 */
+void orc_init_update(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    memset(b->x, 0, sizeof(double) * NEQ * (size_t)ncells_padded(b));
+    if (!c->initMatrix) continue;
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          double rb[NEQ];
+          rhs_b(c, b, i, j, k, 0, rb);
+          long p = pidx(b, i, j, k);
+          for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = rb[e] * b->Ainv[p];
+        }
+  }
+}
+
+/* lusgs::LUSGS_Forward / LUSGS_Backward: src/linearSolver.cpp:341-428 */
+static void lusgs_forward(orc_case *c, blk_t *b, int sweep) {
+  long n = ncells(b);
+  for (long q = 0; q < n; ++q) {
+    int i = b->order[3 * q], j = b->order[3 * q + 1], k = b->order[3 * q + 2];
+    double L[NEQ], U[NEQ], rb[NEQ];
+    implicit_lu(c, b, i, j, k, b->x, 0, L);
+    if (sweep > 0 || c->initMatrix) {
+      implicit_lu(c, b, i, j, k, b->x, 1, U);
+      for (int e = 0; e < NEQ; ++e) L[e] -= U[e];
+    }
+    rhs_b(c, b, i, j, k, 1, rb);
+    long p = pidx(b, i, j, k);
+    for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = (rb[e] + L[e]) * b->Ainv[p];
+  }
+}
+
+static void lusgs_backward(orc_case *c, blk_t *b, int sweep) {
+  long n = ncells(b);
+  for (long q = n - 1; q >= 0; --q) {
+    int i = b->order[3 * q], j = b->order[3 * q + 1], k = b->order[3 * q + 2];
+    double U[NEQ];
+    implicit_lu(c, b, i, j, k, b->x, 1, U);
+    long p = pidx(b, i, j, k);
+    double *xc = &b->x[cidx(b, i, j, k) * NEQ];
+    if (sweep > 0 || c->initMatrix) {
+      double L[NEQ], rb[NEQ];
+      implicit_lu(c, b, i, j, k, b->x, 0, L);
+      rhs_b(c, b, i, j, k, 1, rb);
+      for (int e = 0; e < NEQ; ++e) xc[e] = ((rb[e] + L[e]) - U[e]) * b->Ainv[p];
+    } else {
+      for (int e = 0; e < NEQ; ++e) xc[e] = xc[e] - U[e] * b->Ainv[p];
+    }
+  }
+}
+
+/* dplur::DPLUR: src/linearSolver.cpp:473-507 */
+static void dplur_sweep(orc_case *c, blk_t *b) {
+  long npad = ncells_padded(b);
+  double *xold = (double *)malloc(sizeof(double) * NEQ * (size_t)npad);
+  memcpy(xold, b->x, sizeof(double) * NEQ * (size_t)npad);
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i) {
+        double L[NEQ], U[NEQ], rb[NEQ];
+        implicit_lu(c, b, i, j, k, xold, 0, L);
+        implicit_lu(c, b, i, j, k, xold, 1, U);
+        for (int e = 0; e < NEQ; ++e) L[e] -= U[e];
+        rhs_b(c, b, i, j, k, 0, rb);
+        long p = pidx(b, i, j, k);
+        for (int e = 0; e < NEQ; ++e)
+          b->x[cidx(b, i, j, k) * NEQ + e] = ((rb[e] + 0.0) + L[e]) * b->Ainv[p];
+      }
+  free(xold);
+}
+
+/* linearSolver::Residual / AXmB (src/linearSolver.cpp:58-109) and the
+ * CycleAtLevel reduction (src/mgSolution.cpp:197-206) */
+static double matrix_residual(orc_case *c) {
+  double l2 = 0.0;
+  long total = 0;
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    double part = 0.0;
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          double L[NEQ], U[NEQ], rb[NEQ];
+          implicit_lu(c, b, i, j, k, b->x, 0, L);
+          implicit_lu(c, b, i, j, k, b->x, 1, U);
+          for (int e = 0; e < NEQ; ++e) L[e] -= U[e];
+          rhs_b(c, b, i, j, k, 0, rb);
+          long p = pidx(b, i, j, k);
+          for (int e = 0; e < NEQ; ++e) {
+            double ax = b->x[cidx(b, i, j, k) * NEQ + e] * b->A[p];
+            double axmb = (ax - L[e]) - rb[e];
+            double r = 0.0 - axmb;
+            part += r * r;
+          }
+        }
+    l2 += part;
+    total += ncells_padded(b) * NEQ;
+  }
+  return l2 / (double)total;
+}
+
+double orc_relax(orc_case *c) {
+  for (int sw = 0; sw < c->sweeps; ++sw) {
+    for (int bq = 0; bq < c->nb; ++bq) {
+      if (c->solver == 0) lusgs_forward(c, &c->blk[bq], sw);
+      else dplur_sweep(c, &c->blk[bq]);
+    }
+    if (c->solver == 0)
+      for (int bq = 0; bq < c->nb; ++bq) lusgs_backward(c, &c->blk[bq], sw);
+  }
+  return c->matres_on ? matrix_residual(c) : 0.0;
+}
+
+/* UpdateBlock: src/procBlock.cpp:826-871 (+ ExplicitEulerTimeAdvance
+ * :880-896, ImplicitTimeAdvance :902-916) */
+void orc_update(orc_case *c, int mm, double *l2, double *linf, int *linfloc) {
+  for (int e = 0; e < NEQ; ++e) l2[e] = 0.0;
+  *linf = 0.0;
+  for (int q = 0; q < 5; ++q) linfloc[q] = 0;
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          long p = pidx(b, i, j, k);
+          double *W = &b->W[cidx(b, i, j, k) * NEQ];
+          double *R = &b->R[p * NEQ];
+          double Wn[NEQ];
+          if (c->timeint == 0) {
+            double U[NEQ];
+            orc_prim_to_cons(&c->gas, W, U);
+            double f = b->dt[p] / b->vol[cidx(b, i, j, k)];
+            for (int e = 0; e < NEQ; ++e) U[e] -= f * R[e];
+            orc_cons_to_prim(&c->gas, U, Wn);
+          } else {
+            update_prim(&c->gas, W, &b->x[cidx(b, i, j, k) * NEQ], Wn);
+          }
+          memcpy(W, Wn, sizeof Wn);
+          for (int e = 0; e < NEQ; ++e) l2[e] += R[e] * R[e];
+          for (int e = 0; e < NEQ; ++e)
+            if (R[e] > *linf) {
+              *linf = R[e];
+              linfloc[0] = bq;
+              linfloc[1] = i;
+              linfloc[2] = j;
+              linfloc[3] = k;
+              linfloc[4] = e + 1;
+            }
+        }
+    if (c->multilevel && mm == c->nonlinear - 1)
+      memcpy(b->consNm1, b->consN, sizeof(double) * NEQ * (size_t)ncells(b));
+  }
+}
+
+/* input::CalcCFL (src/input.cpp:647-649) + StoreOldSolution
+ * (src/mgSolution.cpp:103-114, src/procBlock.cpp:1037-1053) */
+void orc_begin_step(orc_case *c, int nn) {
+  c->cfl = dmin(c->cflStart + nn * c->cflStep, c->cflMax);
+  if (!c->storeN) return;
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i)
+          orc_prim_to_cons(&c->gas, &b->W[cidx(b, i, j, k) * NEQ],
+                           &b->consN[pidx(b, i, j, k) * NEQ]);
+    if (c->multilevel && nn == 0)
+      memcpy(b->consNm1, b->consN, sizeof(double) * NEQ * (size_t)ncells(b));
+  }
+}
+
+/* mgSolution::Iterate: src/mgSolution.cpp:246-269, ImplicitUpdate :209-244 */
+int orc_iterate(orc_case *c, int mm, double *l2, double *linf, int *linfloc,
+                double *matres) {
+  orc_bc(c);
+  orc_residual(c);
+  orc_time_step(c);
+  double mr = 0.0;
+  if (c->implicit) {
+    orc_invert_diagonal(c);
+    orc_init_update(c);
+    mr = orc_relax(c);
+    orc_update(c, mm, l2, linf, linfloc);
+    orc_reset_diagonal(c);
+  } else {
+    orc_update(c, mm, l2, linf, linfloc);
+  }
+  /* src/main.cpp:269-270 */
+  *matres = sqrt(mr / (c->totalCells * NEQ));
+  return 0;
+}
+
+/* PrintResiduals normalisation: src/output.cpp:1028-1063 */
+void orc_normalize(orc_case *c, int nn, int mm, const double *l2, double *out) {
+  if (nn == 0 && mm == 0) {
+    for (int e = 0; e < NEQ; ++e) c->l2first[e] = l2[e];
+  } else if (nn < 5 && mm == 0) {
+    if (0.0 + l2[0] > 0.0 + c->l2first[0]) c->l2first[0] = l2[0];
+    for (int e = 1; e < NEQ; ++e) c->l2first[e] = dmax(c->l2first[e], l2[e]);
+  }
+  out[0] = ((0.0 + l2[0]) + EPS) / ((0.0 + c->l2first[0]) + EPS);
+  for (int e = 1; e < NEQ; ++e) out[e] = (l2[e] + EPS) / (c->l2first[e] + EPS);
+}
+
+double orc_cfl(const orc_case *c) { return c->cfl; }
+
+/* ------------------------------------------------------------------------ */
+/* accessors                                                                */
+
+int orc_num_blocks(const orc_case *c) { return c->nb; }
+void orc_block_dims(const orc_case *c, int b, int *ni, int *nj, int *nk) {
+  *ni = c->blk[b].ni;
+  *nj = c->blk[b].nj;
+  *nk = c->blk[b].nk;
+}
+int orc_num_eq(const orc_case *c) { (void)c; return NEQ; }
+int orc_num_ghosts(const orc_case *c) { return c->nb ? c->blk[0].g : 0; }
+int orc_iterations(const orc_case *c) { return c->iterations; }
+int orc_nonlinear_iterations(const orc_case *c) { return c->nonlinear; }
+void orc_refs(const orc_case *c, double *o) {
+  o[0] = c->rRef; o[1] = c->tRef; o[2] = c->lRef; o[3] = c->aRef;
+}
+void orc_gas_of(const orc_case *c, orc_gas *g) { *g = c->gas; }
+void orc_set_matrix_residual(orc_case *c, int on) { c->matres_on = on; }
+
+void orc_get_state(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i)
+        memcpy(&out[pidx(b, i, j, k) * NEQ], &b->W[cidx(b, i, j, k) * NEQ], sizeof(double) * NEQ);
+}
+void orc_set_state(orc_case *c, int bq, const double *in) {
+  blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i)
+        memcpy(&b->W[cidx(b, i, j, k) * NEQ], &in[pidx(b, i, j, k) * NEQ], sizeof(double) * NEQ);
+}
+void orc_get_state_padded(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->W, sizeof(double) * NEQ * (size_t)ncells_padded(b));
+}
+void orc_get_residual(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->R, sizeof(double) * NEQ * (size_t)ncells(b));
+}
+void orc_get_dt(const orc_case *c, int bq, double *out) {
+  memcpy(out, c->blk[bq].dt, sizeof(double) * (size_t)ncells(&c->blk[bq]));
+}
+void orc_get_specrad(const orc_case *c, int bq, double *out) {
+  memcpy(out, c->blk[bq].sr, sizeof(double) * (size_t)ncells(&c->blk[bq]));
+}
+void orc_get_diag(const orc_case *c, int bq, double *out) {
+  memcpy(out, c->blk[bq].A, sizeof(double) * (size_t)ncells(&c->blk[bq]));
+}
+void orc_get_update(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i)
+        memcpy(&out[pidx(b, i, j, k) * NEQ], &b->x[cidx(b, i, j, k) * NEQ], sizeof(double) * NEQ);
+}
+void orc_set_update(orc_case *c, int bq, const double *in) {
+  blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i)
+        memcpy(&b->x[cidx(b, i, j, k) * NEQ], &in[pidx(b, i, j, k) * NEQ], sizeof(double) * NEQ);
+}
+void orc_get_volume(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i) out[pidx(b, i, j, k)] = b->vol[cidx(b, i, j, k)];
+}
+void orc_get_center(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+      for (int i = 0; i < b->ni; ++i) {
+        v3 v = b->ctr[cidx(b, i, j, k)];
+        long p = pidx(b, i, j, k);
+        out[3 * p] = v.x; out[3 * p + 1] = v.y; out[3 * p + 2] = v.z;
+      }
+}
+void orc_get_face_area(const orc_case *c, int bq, int d, double *out) {
+  const blk_t *b = &c->blk[bq];
+  long n = nfaces_padded(b, d);
+  for (long q = 0; q < n; ++q) {
+    out[4 * q] = b->fa[d][q].u.x;
+    out[4 * q + 1] = b->fa[d][q].u.y;
+    out[4 * q + 2] = b->fa[d][q].u.z;
+    out[4 * q + 3] = b->fa[d][q].m;
+  }
+}
+void orc_get_cell_width(const orc_case *c, int bq, int d, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->cw[d], sizeof(double) * (size_t)ncells_padded(b));
+}

@@ -1,0 +1,127 @@
+/*
+ * Oracle: FP64 CPU restatement of aither's per-nonlinear-iteration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library (aither_amd) links,
+ * loads or calls this code; it is used by tests/, by __graft_entry__.smoke() and
+ * by bench.py's cpu_baseline leg, always as the checker, never as the thing
+ * measured or shipped.
+ *
+ * Every function restates the reference algorithm it names (file:line in the
+ * mnucci32/aither sources) in plain, single-threaded C99, compiled with
+ * -ffp-contract=off so that each floating-point operation is evaluated in the
+ * order the reference writes it.
+ *
+ * Scope: single-species calorically perfect ideal gas, Euler equations,
+ * multi-block input without interblock connections.  Reconstruction: constant,
+ * MUSCL (kappa family) with none / vanAlbada / minmod limiters.  Inviscid flux:
+ * Roe, AUSMPW+.  Time integration: explicitEuler, implicitEuler,
+ * crankNicholson, bdf2 (dual time).  Implicit operators: scalar LU-SGS and
+ * scalar DP-LUR (Rusanov off-diagonals).  Boundary conditions: slipWall
+ * (viscousWall in the inviscid ghost pass), characteristic, supersonicInflow,
+ * supersonicOutflow, stagnationInlet, pressureOutlet (reflecting).
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - reference-held regression known answers (testCases/regressionTests.py)
+ *     for subsonicCylinder and supersonicWedge, run on the reference's own
+ *     grids and input files copied as fixtures under tests/golden/;
+ *   - closed forms and invariants for every kernel-level function.
+ */
+#ifndef AITHER_ORACLE_H
+#define AITHER_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_case orc_case;
+
+/* gas model: single species, calorically perfect, nondimensional */
+typedef struct {
+  double R;      /* nondimensional gas constant (src/fluid.cpp:89-103) */
+  double n;      /* Cv/R (fluidDatabase .dat key "n") */
+} orc_gas;
+
+/* ---- whole-case API (mirrors src/main.cpp:101-302 single rank) ---- */
+orc_case *orc_load(const char *inp_path, char *err, int errlen);
+void orc_free(orc_case *c);
+int orc_num_blocks(const orc_case *c);
+void orc_block_dims(const orc_case *c, int b, int *ni, int *nj, int *nk);
+int orc_num_eq(const orc_case *c);
+int orc_num_ghosts(const orc_case *c);
+int orc_iterations(const orc_case *c);
+int orc_nonlinear_iterations(const orc_case *c);
+void orc_refs(const orc_case *c, double *out4); /* rRef, tRef, lRef, aRef */
+void orc_gas_of(const orc_case *c, orc_gas *g);
+void orc_set_matrix_residual(orc_case *c, int on);
+
+/* physical cells, AoS nondimensional primitive [rho,u,v,w,p], i fastest */
+void orc_get_state(const orc_case *c, int b, double *out);
+void orc_set_state(orc_case *c, int b, const double *in);
+/* padded state (ghosts included), AoS, i fastest over (ni+2g)(nj+2g)(nk+2g) */
+void orc_get_state_padded(const orc_case *c, int b, double *out);
+
+/* one outer step: CalcCFL + StoreOldSolution (src/main.cpp:235-238) */
+void orc_begin_step(orc_case *c, int nn);
+/* one nonlinear iteration (mgSolution::Iterate, src/mgSolution.cpp:246-269)
+ * l2 (neq, sum of squares before sqrt), linf[1], linfloc[5] = blk,i,j,k,eqn,
+ * matrix residual (already normalised as src/main.cpp:269-270) */
+int orc_iterate(orc_case *c, int mm, double *l2, double *linf, int *linfloc,
+                double *matres);
+/* PrintResiduals normalisation (src/output.cpp:1028-1063): out = neq values,
+ * out[0] is the mass residual.  l2 is the per-equation L2 (after sqrt). */
+void orc_normalize(orc_case *c, int nn, int mm, const double *l2, double *out);
+double orc_cfl(const orc_case *c);
+
+/* granular steps, in the order Iterate runs them */
+void orc_bc(orc_case *c);             /* GetBoundaryConditions */
+void orc_residual(orc_case *c);       /* CalcResidual */
+void orc_time_step(orc_case *c);      /* CalcTimeStep */
+void orc_invert_diagonal(orc_case *c);/* AddDiagonalTerms + Invert */
+void orc_init_update(orc_case *c);    /* InitializeMatrixUpdate */
+double orc_relax(orc_case *c);        /* Relax(sweeps) -> CycleAtLevel value */
+void orc_update(orc_case *c, int mm, double *l2, double *linf, int *linfloc);
+void orc_reset_diagonal(orc_case *c);
+
+/* per-block field getters (physical cells, i fastest) */
+void orc_get_residual(const orc_case *c, int b, double *out);  /* neq/cell */
+void orc_get_dt(const orc_case *c, int b, double *out);
+void orc_get_specrad(const orc_case *c, int b, double *out);
+void orc_get_diag(const orc_case *c, int b, double *out);      /* A (scalar) */
+void orc_get_update(const orc_case *c, int b, double *out);    /* du, neq/cell */
+void orc_set_update(orc_case *c, int b, const double *in);
+void orc_get_volume(const orc_case *c, int b, double *out);
+void orc_get_center(const orc_case *c, int b, double *out);    /* 3/cell */
+/* padded face geometry: unit normal + magnitude, 4/face */
+void orc_get_face_area(const orc_case *c, int b, int dir, double *out);
+void orc_get_cell_width(const orc_case *c, int b, int dir, double *out); /* padded */
+
+/* ---- kernel-level functions for closed-form tests ---- */
+double orc_gamma(const orc_gas *g);
+double orc_sos(const orc_gas *g, const double *W);
+double orc_enthalpy(const orc_gas *g, const double *W);
+void orc_prim_to_cons(const orc_gas *g, const double *W, double *U);
+void orc_cons_to_prim(const orc_gas *g, const double *U, double *W);
+void orc_roe_flux(const orc_gas *g, const double *WL, const double *WR,
+                  const double *n, double *F);
+void orc_ausm_flux(const orc_gas *g, const double *WL, const double *WR,
+                   const double *n, double *F);
+void orc_phys_flux(const orc_gas *g, const double *W, const double *n,
+                   double *F);
+/* limiter: 0 none, 1 vanAlbada, 2 minmod */
+void orc_muscl(const double *u2, const double *u1, const double *d1,
+               double kappa, int limiter, double uw2, double uw, double dw,
+               double *out);
+double orc_pyramid_volume(const double *p, const double *a, const double *b,
+                          const double *c, const double *d);
+void orc_hyperplane_order(int ni, int nj, int nk, int *out /* 3 per cell */);
+double orc_inv_cell_specrad(const orc_gas *g, const double *W,
+                            const double *aL /*ux,uy,uz,mag*/,
+                            const double *aR);
+void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
+                        const double *area /*ux,uy,uz,mag*/, int positive,
+                        double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
