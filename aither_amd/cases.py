@@ -1,0 +1,197 @@
+"""Seeded case generators in the reference's own input formats.
+
+Each generator writes ``<name>.inp`` (key: value input file,
+src/input.cpp:167-643), ``<name>.xyz`` (binary multi-block Plot3D,
+src/plot3d.cpp:363-445) and, where the initial state is not uniform, a point
+cloud at the cell centres (src/utility.cpp:513-605) referenced by
+``icState(tag=-1; file=...)``.
+
+Workloads:
+  * ``vortex_case``: inviscid Euler, calorically perfect air, single Cartesian
+    block n^3 on [0,10]^3, isentropic vortex (strength 5, core radius 1, axis
+    along z through the centre) on a Mach 0.5 freestream along x,
+    characteristic farfield BCs, Roe flux, MUSCL (kappa = 1/3) with van Albada
+    limiter, implicit Euler LU-SGS at CFL 5.  n = 64 and 200 iterations is the
+    reference's CPU-runnable benchmark configuration.
+  * ``bump_case``: 3D channel with a sin^2 bump on the lower wall, slip walls,
+    characteristic inflow/outflow, Roe + MUSCL, LU-SGS: one steady request of
+    the serving workload.
+"""
+import math
+import os
+import struct
+
+import numpy as np
+
+AIR_N = 2.5
+AIR_MOLAR = 28.97e-3
+R_UNIVERSAL = 8.3144598
+
+
+def air_gas_constant():
+    return R_UNIVERSAL / AIR_MOLAR
+
+
+def write_plot3d(path, blocks):
+    """blocks: list of (x, y, z) arrays shaped (ni+1, nj+1, nk+1) [i, j, k]."""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", len(blocks)))
+        for x, _, _ in blocks:
+            f.write(struct.pack("<3i", *x.shape))
+        for x, y, z in blocks:
+            for c in (x, y, z):
+                # Plot3D order: i fastest
+                f.write(np.ascontiguousarray(c.transpose(2, 1, 0)).astype("<f8").tobytes())
+
+
+def cell_centres(x, y, z):
+    """Cell centroids in the reference's summation order (src/plot3d.cpp:36-44)."""
+    def cen(a):
+        s = a[:-1, :-1, :-1].copy()
+        s = s + a[1:, :-1, :-1]
+        s = s + a[:-1, 1:, :-1]
+        s = s + a[1:, 1:, :-1]
+        s = s + a[:-1, :-1, 1:]
+        s = s + a[1:, :-1, 1:]
+        s = s + a[:-1, 1:, 1:]
+        s = s + a[1:, 1:, 1:]
+        return 0.125 * s
+    return cen(x), cen(y), cen(z)
+
+
+def write_cloud(path, xc, yc, zc, rho, u, v, w, p, species="air"):
+    """Point-cloud initial condition; values in SI units, %.17g."""
+    n = xc.size
+    with open(path, "w") as f:
+        f.write(f"{n}\n{species}\n")
+        cols = [a.transpose(2, 1, 0).reshape(-1) for a in (xc, yc, zc, rho, u, v, w, p)]
+        zero = np.zeros(n)
+        one = np.ones(n)
+        data = np.stack(cols + [zero, zero, one], axis=1)
+        np.savetxt(f, data, fmt="%.17g")
+
+
+def _surfaces_box(ni, nj, nk, types):
+    """types: dict surface-name -> (bctype, tag) for imin..kmax."""
+    lines = []
+    for name, (a, b, c, d, e, f) in (
+        ("imin", (0, 0, 0, nj, 0, nk)), ("imax", (ni, ni, 0, nj, 0, nk)),
+        ("jmin", (0, ni, 0, 0, 0, nk)), ("jmax", (0, ni, nj, nj, 0, nk)),
+        ("kmin", (0, ni, 0, nj, 0, 0)), ("kmax", (0, ni, 0, nj, nk, nk)),
+    ):
+        bc, tag = types[name]
+        lines.append(f"   {bc:<18} {a:5d} {b:5d} {c:5d} {d:5d} {e:5d} {f:5d} {tag:5d}")
+    return "2 2 2\n#i_surfaces\n" + "\n".join(lines[0:2]) + "\n#j_surfaces\n" + \
+        "\n".join(lines[2:4]) + "\n#k_surfaces\n" + "\n".join(lines[4:6]) + "\n"
+
+
+def vortex_state(xc, yc, beta=5.0, rc=1.0, x0=5.0, y0=5.0, t_inf=288.15, rho_inf=1.225, mach=0.5):
+    """Isentropic vortex on a freestream along x (SI units)."""
+    gamma = (AIR_N + 1.0) / AIR_N
+    R = air_gas_constant()
+    p_inf = rho_inf * R * t_inf
+    a_inf = math.sqrt(gamma * R * t_inf)
+    u_inf = mach * a_inf
+    r2 = ((xc - x0) ** 2 + (yc - y0) ** 2) / (rc * rc)
+    tr = 1.0 - (gamma - 1.0) * beta ** 2 / (8.0 * gamma * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = rho_inf * tr ** (1.0 / (gamma - 1.0))
+    p = p_inf * tr ** (gamma / (gamma - 1.0))
+    scale = beta / (2.0 * math.pi) * np.exp(0.5 * (1.0 - r2)) * math.sqrt(p_inf / rho_inf)
+    u = u_inf - scale * (yc - y0) / rc
+    v = scale * (xc - x0) / rc
+    return rho, u, v, np.zeros_like(u), p, dict(p_inf=p_inf, rho_inf=rho_inf, u_inf=u_inf, t_inf=t_inf)
+
+
+def vortex_case(outdir, n=64, iterations=200, name="vortex", length=10.0, cfl=5.0, limiter="vanAlbada",
+                recon="thirdOrder", solver="lusgs", sweeps=1, flux="roe", ni=None, nj=None, nk=None):
+    """Write the isentropic-vortex case; returns the .inp path."""
+    os.makedirs(outdir, exist_ok=True)
+    ni = ni or n
+    nj = nj or n
+    nk = nk or n
+    h = length / n
+    i, j, k = np.meshgrid(np.arange(ni + 1), np.arange(nj + 1), np.arange(nk + 1), indexing="ij")
+    x, y, z = i * h, j * h, k * h
+    write_plot3d(os.path.join(outdir, name + ".xyz"), [(x, y, z)])
+    xc, yc, zc = cell_centres(x, y, z)
+    c = 0.5 * length
+    rho, u, v, w, p, fs = vortex_state(xc, yc, x0=c, y0=c)
+    write_cloud(os.path.join(outdir, name + ".cloud"), xc, yc, zc, rho, u, v, w, p)
+    bcs = _surfaces_box(ni, nj, nk, {s: ("characteristic", 1) for s in
+                                     ("imin", "imax", "jmin", "jmax", "kmin", "kmax")})
+    inp = f"""# isentropic vortex, {ni}x{nj}x{nk} cells on [0,{length}]^3
+gridName: {name}
+equationSet: euler
+timeIntegration: implicitEuler
+cflStart: {cfl}
+cflMax: {cfl}
+faceReconstruction: {recon}
+limiter: {limiter}
+inviscidFlux: {flux}
+iterations: {iterations}
+outputFrequency: {iterations}
+restartFrequency: 0
+referenceTemperature: {fs['t_inf']!r}
+referenceDensity: {fs['rho_inf']!r}
+referenceLength: 1.0
+initialConditions: <icState(tag=-1; file={name}.cloud)>
+matrixSolver: {solver}
+matrixSweeps: {sweeps}
+matrixRelaxation: 1.0
+boundaryStates: <characteristic(tag=1; pressure={fs['p_inf']!r}; density={fs['rho_inf']!r}; velocity=[{fs['u_inf']!r}, 0.0, 0.0])>
+boundaryConditions: 1
+{bcs}"""
+    path = os.path.join(outdir, name + ".inp")
+    with open(path, "w") as f:
+        f.write(inp)
+    return path
+
+
+def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0):
+    """3D channel with a sin^2 bump on the lower wall; parameters seeded."""
+    rng = np.random.default_rng(seed)
+    n = n or int(rng.integers(24, 97))
+    hl = float(rng.uniform(0.02, 0.10))
+    mach = float(rng.uniform(0.2, 0.8))
+    name = name or f"bump{seed}"
+    os.makedirs(outdir, exist_ok=True)
+    L, H, W = 3.0, 1.0, 1.0
+    ell = 1.0
+    x0 = 1.0
+    i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    x = L * i / n
+    xi = np.clip((x - x0) / ell, 0.0, 1.0)
+    yw = hl * ell * np.sin(np.pi * xi) ** 2
+    y = yw + (H - yw) * j / n
+    z = W * k / n
+    write_plot3d(os.path.join(outdir, name + ".xyz"), [(x, y, z)])
+    gamma = (AIR_N + 1.0) / AIR_N
+    R = air_gas_constant()
+    t_inf, rho_inf = 288.15, 1.225
+    p_inf = rho_inf * R * t_inf
+    u_inf = mach * math.sqrt(gamma * R * t_inf)
+    bcs = _surfaces_box(n, n, n, {"imin": ("characteristic", 1), "imax": ("characteristic", 1),
+                                  "jmin": ("slipWall", 2), "jmax": ("slipWall", 2),
+                                  "kmin": ("slipWall", 2), "kmax": ("slipWall", 2)})
+    inp = f"""# bump channel seed {seed}: n={n}, h/l={hl:.4f}, Mach={mach:.4f}
+gridName: {name}
+equationSet: euler
+timeIntegration: implicitEuler
+cflStart: {cfl}
+cflMax: {cfl}
+faceReconstruction: thirdOrder
+limiter: vanAlbada
+iterations: {iterations}
+referenceTemperature: {t_inf!r}
+referenceDensity: {rho_inf!r}
+initialConditions: <icState(tag=-1; pressure={p_inf!r}; density={rho_inf!r}; velocity=[{u_inf!r}, 0.0, 0.0])>
+matrixSolver: lusgs
+matrixSweeps: 1
+matrixRelaxation: 1.0
+boundaryStates: <characteristic(tag=1; pressure={p_inf!r}; density={rho_inf!r}; velocity=[{u_inf!r}, 0.0, 0.0])>
+boundaryConditions: 1
+{bcs}"""
+    path = os.path.join(outdir, name + ".inp")
+    with open(path, "w") as f:
+        f.write(inp)
+    return path, dict(n=n, h_over_l=hl, mach=mach)

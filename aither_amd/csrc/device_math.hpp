@@ -1,0 +1,278 @@
+// Per-face and per-cell arithmetic of the hot path, as device functions.
+//
+// Each function restates the reference routine it names, keeping the
+// reference's operation order.  The residual path (reconstruction, inviscid
+// flux, spectral radius, boundary states) is FP64; the implicit-operator
+// helpers are templated on the working precision T (double or float).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ath {
+
+constexpr double kEps = 1.0e-30;  // include/macros.hpp.in:21
+
+struct Gas {
+  double R;      // nondimensional gas constant
+  double n;      // Cv / R
+  double gamma;  // Cp / Cv
+  double cps;    // R (n + 1)
+  double cvs;    // R n
+};
+
+template <typename T>
+struct GasT {
+  T R, gamma, cps, cvs;
+};
+
+template <typename T>
+__device__ __forceinline__ GasT<T> Cast(const Gas &g) {
+  return {T(g.R), T(g.gamma), T(g.cps), T(g.cvs)};
+}
+
+__device__ __forceinline__ double Dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double Dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+
+// idealGas::Temperature (src/eos.cpp:103-111), SoS (include/arrayView.hpp:384)
+template <typename T>
+__device__ __forceinline__ T Temperature(const GasT<T> &g, T rho, T p) {
+#pragma clang fp contract(off)
+  return p / (rho * g.R);
+}
+template <typename T>
+__device__ __forceinline__ T SoS(const GasT<T> &g, T rho, T p) {
+  return sqrt(g.gamma * p / rho);
+}
+template <typename T>
+__device__ __forceinline__ T VelMag(T u, T v, T w) {
+#pragma clang fp contract(off)
+  return sqrt((u * u + v * v) + w * w);
+}
+// EnthalpyFunc / InternalEnergy (include/arrayView.hpp:420-444; hf = 0)
+template <typename T>
+__device__ __forceinline__ T Enthalpy(const GasT<T> &g, T rho, T u, T v, T w, T p) {
+#pragma clang fp contract(off)
+  const T vel = VelMag(u, v, w);
+  return g.cps * Temperature(g, rho, p) + T(0.5) * vel * vel;
+}
+template <typename T>
+__device__ __forceinline__ T TotalEnergy(const GasT<T> &g, T rho, T u, T v, T w, T p) {
+#pragma clang fp contract(off)
+  const T vel = VelMag(u, v, w);
+  return g.cvs * Temperature(g, rho, p) + T(0.5) * vel * vel;
+}
+
+// PrimToCons (include/primitive.hpp:151-169)
+template <typename T>
+__device__ __forceinline__ void PrimToCons(const GasT<T> &g, const T W[5], T U[5]) {
+#pragma clang fp contract(off)
+  U[0] = W[0];
+  U[1] = W[0] * W[1];
+  U[2] = W[0] * W[2];
+  U[3] = W[0] * W[3];
+  U[4] = W[0] * TotalEnergy(g, W[0], W[1], W[2], W[3], W[4]);
+}
+// primitive(cons, phys) + PressFromEnergy (include/primitive.hpp:129-148,
+// src/eos.cpp:40-52, src/thermodynamic.cpp:107-113)
+template <typename T>
+__device__ __forceinline__ void ConsToPrim(const GasT<T> &g, const T U[5], T W[5]) {
+#pragma clang fp contract(off)
+  const T rho = U[0];
+  W[0] = U[0];
+  W[1] = U[1] / rho;
+  W[2] = U[2] / rho;
+  W[3] = U[3] / rho;
+  const T en = U[4] / rho;
+  const T vel = VelMag(W[1], W[2], W[3]);
+  const T t = (en - T(0.5) * vel * vel) / g.cvs;
+  W[4] = (W[0] * g.R) * t;
+}
+// UpdatePrimWithCons (include/primitive.hpp:206-231); with a single species
+// the mass-fraction clip leaves the density unchanged
+template <typename T>
+__device__ __forceinline__ void UpdatePrim(const GasT<T> &g, const T W[5], const T du[5], T out[5]) {
+#pragma clang fp contract(off)
+  T U[5];
+  PrimToCons(g, W, U);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) U[e] = U[e] + du[e];
+  ConsToPrim(g, U, out);
+}
+
+// inviscidFlux::ConstructFromPrim (include/inviscidFlux.hpp:128-159)
+template <typename T>
+__device__ __forceinline__ void PhysFlux(const GasT<T> &g, const T W[5], T nx, T ny, T nz, T F[5]) {
+  const T vn = (W[1] * nx + W[2] * ny) + W[3] * nz;
+  const T rvn = W[0] * vn;
+  F[0] = W[0] * vn;
+  F[1] = rvn * W[1] + W[4] * nx;
+  F[2] = rvn * W[2] + W[4] * ny;
+  F[3] = rvn * W[3] + W[4] * nz;
+  F[4] = rvn * Enthalpy(g, W[0], W[1], W[2], W[3], W[4]);
+}
+
+// RoeFlux (include/inviscidFlux.hpp:259-382, src/inviscidFlux.cpp:26-32),
+// RoeAveragedState (include/primitive.hpp:245-280)
+__device__ inline void RoeFlux(const GasT<double> &g, const double L[5], const double R[5],
+                               double nx, double ny, double nz, double F[5]) {
+  const double denRatio = sqrt(R[0] / L[0]);
+  const double rr = L[0] * denRatio;
+  const double inv = 1.0 + denRatio;
+  const double ur = (L[1] + denRatio * R[1]) / inv;
+  const double vr = (L[2] + denRatio * R[2]) / inv;
+  const double wr = (L[3] + denRatio * R[3]) / inv;
+  const double pr = (L[4] + denRatio * R[4]) / inv;
+  const double hR = Enthalpy(g, rr, ur, vr, wr, pr);
+  const double aR = SoS(g, rr, pr);
+  const double velNormR = (ur * nx + vr * ny) + wr * nz;
+  const double mfR = rr / rr;
+  const double d0 = R[0] - L[0], d1 = R[1] - L[1], d2 = R[2] - L[2], d3 = R[3] - L[3],
+               d4 = R[4] - L[4];
+  const double normVelDiff = (d1 * nx + d2 * ny) + d3 * nz;
+  const double aR2 = aR * aR;
+  constexpr double entropyFix = 0.1;
+  double D0, D1, D2, D3, D4;
+
+  // left moving acoustic wave
+  double ws = fabs(velNormR - aR);
+  if (ws < entropyFix) ws = 0.5 * (ws * ws / entropyFix + entropyFix);
+  double wst = (d4 - rr * aR * normVelDiff) / (2.0 * aR * aR);
+  double wss = ws * wst;
+  D0 = wss * mfR;
+  D1 = wss * (ur - aR * nx);
+  D2 = wss * (vr - aR * ny);
+  D3 = wss * (wr - aR * nz);
+  D4 = wss * (hR - aR * velNormR);
+
+  // entropy wave
+  ws = fabs(velNormR);
+  wst = -d4 / aR2;
+  wss = ws * wst;
+  D0 += wss * mfR + ws * d0;
+  wst = d0 - d4 / aR2;
+  wss = ws * wst;
+  D1 += wss * ur;
+  D2 += wss * vr;
+  D3 += wss * wr;
+  D4 += wss * 0.5 * ((ur * ur + vr * vr) + wr * wr);
+
+  // shear wave
+  wss = ws * rr;
+  D1 += wss * (d1 - normVelDiff * nx);
+  D2 += wss * (d2 - normVelDiff * ny);
+  D3 += wss * (d3 - normVelDiff * nz);
+  D4 += wss * (((ur * d1 + vr * d2) + wr * d3) - velNormR * normVelDiff);
+
+  // right moving acoustic wave
+  ws = fabs(velNormR + aR);
+  if (ws < entropyFix) ws = 0.5 * (ws * ws / entropyFix + entropyFix);
+  wst = (d4 + rr * aR * normVelDiff) / (2.0 * aR * aR);
+  wss = ws * wst;
+  D0 += wss * mfR;
+  D1 += wss * (ur + aR * nx);
+  D2 += wss * (vr + aR * ny);
+  D3 += wss * (wr + aR * nz);
+  D4 += wss * (hR + aR * velNormR);
+
+  double FL[5], FR[5];
+  PhysFlux(g, L, nx, ny, nz, FL);
+  PhysFlux(g, R, nx, ny, nz, FR);
+  F[0] = (FL[0] + (FR[0] - D0)) * 0.5;
+  F[1] = (FL[1] + (FR[1] - D1)) * 0.5;
+  F[2] = (FL[2] + (FR[2] - D2)) * 0.5;
+  F[3] = (FL[3] + (FR[3] - D3)) * 0.5;
+  F[4] = (FL[4] + (FR[4] - D4)) * 0.5;
+}
+
+__device__ __forceinline__ double Sign(double x) { return (double)((0.0 < x) - (x < 0.0)); }
+
+// AUSMFlux / AUSMPW+ (include/inviscidFlux.hpp:395-481, member :161-209)
+__device__ inline void AusmFlux(const GasT<double> &g, const double L[5], const double R[5],
+                                double nx, double ny, double nz, double F[5]) {
+  const double velNormL = (L[1] * nx + L[2] * ny) + L[3] * nz;
+  const double velNormR = (R[1] * nx + R[2] * ny) + R[3] * nz;
+  const double sosStar = sqrt(SoS(g, L[0], L[4]) * SoS(g, R[0], R[4]));
+  const double vel = 0.5 * (velNormL + velNormR);
+  double sos = sosStar;
+  if (vel < 0.0) sos = sosStar * sosStar / Dmax(velNormR, sosStar);
+  else if (vel > 0.0) sos = sosStar * sosStar / Dmax(velNormL, sosStar);
+  const double ml = velNormL / sos, mr = velNormR / sos;
+  const double mlp = (ml + 1.0) * (ml + 1.0), mrm = (mr - 1.0) * (mr - 1.0);
+  const double mPlusL = fabs(ml) <= 1.0 ? 0.25 * mlp : 0.5 * (ml + fabs(ml));
+  const double mMinusR = fabs(mr) <= 1.0 ? -0.25 * mrm : 0.5 * (mr - fabs(mr));
+  const double pPlus = fabs(ml) <= 1.0 ? 0.25 * mlp * (2.0 - ml) : 0.5 * (1.0 + Sign(ml));
+  const double pMinus = fabs(mr) <= 1.0 ? 0.25 * mrm * (2.0 + mr) : 0.5 * (1.0 - Sign(mr));
+  const double ps = pPlus * L[4] + pMinus * R[4];
+  const double pm = Dmin(L[4] / R[4], R[4] / L[4]);
+  const double w = 1.0 - pm * pm * pm;
+  const double fl = fabs(ml) < 1.0 ? L[4] / ps - 1.0 : 0.0;
+  const double fr = fabs(mr) < 1.0 ? R[4] / ps - 1.0 : 0.0;
+  const double mavg = mPlusL + mMinusR;
+  const double mPlusLBar = mavg >= 0.0 ? mPlusL + mMinusR * ((1.0 - w) * (1.0 + fr) - fl)
+                                       : mPlusL * w * (1.0 + fl);
+  const double mMinusRBar = mavg >= 0.0 ? mMinusR * w * (1.0 + fr)
+                                        : mMinusR + mPlusL * ((1.0 - w) * (1.0 + fl) - fr);
+  const double vl = mPlusLBar * sos, vr = mMinusRBar * sos;
+  const double rvl = L[0] * vl, rvr = R[0] * vr;
+  F[0] = L[0] * vl + R[0] * vr;
+  F[1] = (rvl * L[1] + pPlus * L[4] * nx) + (rvr * R[1] + pMinus * R[4] * nx);
+  F[2] = (rvl * L[2] + pPlus * L[4] * ny) + (rvr * R[2] + pMinus * R[4] * ny);
+  F[3] = (rvl * L[3] + pPlus * L[4] * nz) + (rvr * R[3] + pMinus * R[4] * nz);
+  F[4] = rvl * Enthalpy(g, L[0], L[1], L[2], L[3], L[4]) +
+         rvr * Enthalpy(g, R[0], R[1], R[2], R[3], R[4]);
+}
+
+// LimiterVanAlbada / LimiterMinmod (src/limiter.cpp:24-53)
+__device__ __forceinline__ double VanAlbada(double r) {
+  const double r2 = r * r;
+  return Dmax(0.0, (r + r2) / (1.0 + r2));
+}
+__device__ __forceinline__ double Minmod(double r) { return Dmax(0.0, Dmin(1.0, r)); }
+
+// FaceReconMUSCL for one variable (include/reconstruction.hpp:109-154)
+template <int LIM>
+__device__ __forceinline__ double Muscl(double u2, double u1, double d1, double kappa,
+                                        double dPlus, double dMinus) {
+  const double r = (kEps + (d1 - u1) * dPlus) / (kEps + (u1 - u2) * dMinus);
+  double lim = 1.0, inv = 1.0;
+  if (LIM == 1) {
+    lim = VanAlbada(r);
+    inv = VanAlbada(1.0 / r);
+  } else if (LIM == 2) {
+    lim = Minmod(r);
+    inv = Minmod(1.0 / r);
+  }
+  return u1 + (0.25 * ((u1 - u2) * dMinus)) * ((1.0 - kappa) * lim + (1.0 + kappa) * r * inv);
+}
+
+// InvCellSpectralRadius (include/spectralRadius.hpp:43-64)
+__device__ __forceinline__ double InvCellSpecRad(const GasT<double> &g, double rho, double u,
+                                                 double v, double w, double p, double4 aL,
+                                                 double4 aR) {
+  const double sx = 0.5 * (aL.x + aR.x), sy = 0.5 * (aL.y + aR.y), sz = 0.5 * (aL.z + aR.z);
+  const double m = sqrt((sx * sx + sy * sy) + sz * sz);
+  const double nx = sx / m, ny = sy / m, nz = sz / m;
+  const double fMag = 0.5 * (aL.w + aR.w);
+  return (fabs((u * nx + v * ny) + w * nz) + SoS(g, rho, p)) * fMag;
+}
+
+// RusanovScalarOffDiagonal for the inviscid equations
+// (src/fluxJacobian.cpp:122-162): 0.5|A| (F(U_nb + du) - F(U_nb)) +/- lambda du
+// with lambda = InvFaceSpectralRadius (include/spectralRadius.hpp:66-80)
+template <typename T>
+__device__ __forceinline__ void ScalarOffDiag(const GasT<T> &g, const T Wnb[5], const T du[5],
+                                              T nx, T ny, T nz, T mag, bool positive, T out[5]) {
+  T Wu[5], Fo[5], Fn[5];
+  UpdatePrim(g, Wnb, du, Wu);
+  PhysFlux(g, Wnb, nx, ny, nz, Fo);
+  PhysFlux(g, Wu, nx, ny, nz, Fn);
+  const T half = T(0.5) * mag;
+  const T lam = T(0.5) * mag * (fabs((Wnb[1] * nx + Wnb[2] * ny) + Wnb[3] * nz) + SoS(g, Wnb[0], Wnb[4]));
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const T fc = half * (Fn[e] - Fo[e]);
+    out[e] = positive ? fc + du[e] * lam : fc - du[e] * lam;
+  }
+}
+
+}  // namespace ath

@@ -1,0 +1,115 @@
+// Host side of the drop-in boundary: the reference's input file, Plot3D grid,
+// boundary-condition tables and FP64 cell geometry, assembled on the CPU and
+// handed to the device solver.
+//
+// Geometry is computed once, in FP64, with the operation order of
+// src/plot3d.cpp:36-359 and src/procBlock.cpp:2160-2268, 6397-6411, so that the
+// device sees the same volumes, face vectors and cell widths as the reference.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+namespace ath {
+
+constexpr int kNeq = 5;  // rho, u, v, w, p (single species, no turbulence)
+
+enum class Recon { Constant, MUSCL };
+enum class Limiter { None = 0, VanAlbada = 1, Minmod = 2 };
+enum class Flux { Roe = 0, AUSM = 1 };
+enum class TimeInt { ExplicitEuler = 0, ImplicitEuler = 1, CrankNicholson = 2, BDF2 = 3 };
+enum class Solver { LUSGS = 0, DPLUR = 1 };
+
+// boundary condition types handled by the inviscid ghost pass
+enum BcType : int {
+  BC_SLIPWALL = 0,  // viscousWall is treated as slipWall in the inviscid pass
+  BC_CHARACTERISTIC = 1,
+  BC_SUPERSONIC_INFLOW = 2,
+  BC_SUPERSONIC_OUTFLOW = 3,
+  BC_STAGNATION_INLET = 4,
+  BC_PRESSURE_OUTLET = 5,
+};
+
+struct BcState {  // boundaryStates entry, nondimensional
+  std::string kind;
+  int tag = -1;
+  double p = 0, rho = 0;
+  std::array<double, 3> vel{0, 0, 0};
+  double p0 = 0, t0 = 0;
+  std::array<double, 3> dir{0, 0, 0};
+};
+
+struct Surface {  // boundarySurface: bc type, i/j/k ranges, tag
+  std::string type;
+  std::array<int, 7> d{};  // imin imax jmin jmax kmin kmax tag
+  int SurfType() const;     // 1..6 = imin, imax, jmin, jmax, kmin, kmax
+};
+
+struct Vec3 {
+  double x = 0, y = 0, z = 0;
+};
+
+struct HostBlock {
+  int ni = 0, nj = 0, nk = 0, g = 0;
+  int NI = 0, NJ = 0, NK = 0;  // padded cell dims
+  // padded arrays (ghost layers included), i fastest
+  std::vector<double> vol;
+  std::vector<Vec3> ctr;
+  std::array<std::vector<std::array<double, 4>>, 3> fa;  // unit normal + mag
+  std::array<std::vector<Vec3>, 3> fc;
+  std::array<std::vector<double>, 3> cw;
+  std::vector<double> W0;  // initial primitive state, physical cells, AoS
+  std::vector<Surface> surf;
+
+  long Cell(int i, int j, int k) const {
+    return (long)(i + g) + (long)(j + g) * NI + (long)(k + g) * NI * NJ;
+  }
+  long Face(int d, int i, int j, int k) const {
+    const int FI = NI + (d == 0), FJ = NJ + (d == 1);
+    return (long)(i + g) + (long)(j + g) * FI + (long)(k + g) * FI * FJ;
+  }
+  long NumFaces(int d) const {
+    return (long)(NI + (d == 0)) * (NJ + (d == 1)) * (NK + (d == 2));
+  }
+  long NumCells() const { return (long)ni * nj * nk; }
+  long NumPadded() const { return (long)NI * NJ * NK; }
+  int N(int d) const { return d == 0 ? ni : (d == 1 ? nj : nk); }
+};
+
+struct HostCase {
+  std::string dir, grid;
+  int iterations = 1, nonlinear = 1, sweeps = 1;
+  double rRef = -1, tRef = -1, lRef = 1, aRef = 0;
+  double dtIn = -1;
+  double cflStart = 1, cflMax = 1, cflStep = 0;
+  double theta = 1, zeta = 0, relax = 1, dualCFL = -1;
+  double kappa = -2;
+  Recon recon = Recon::Constant;
+  Limiter limiter = Limiter::None;
+  Flux flux = Flux::Roe;
+  TimeInt timeint = TimeInt::ExplicitEuler;
+  Solver solver = Solver::LUSGS;
+  double gasR = 0, gasN = 0, gamma = 0;
+  double totalCells = 0;
+  std::vector<BcState> bcs;
+  std::vector<HostBlock> blocks;
+
+  bool Implicit() const { return timeint != TimeInt::ExplicitEuler; }
+  bool MultiLevel() const { return timeint == TimeInt::BDF2; }
+  bool InitMatrix() const { return solver == Solver::DPLUR || sweeps > 1; }
+  int Ghosts() const { return recon == Recon::Constant ? 1 : 2; }
+  double Cfl(int nn) const;
+};
+
+// Parse `inp_path` and its grid; returns an empty string on success or an
+// error message.  Supported subset: see the README of the package.
+std::string LoadCase(const std::string &inp_path, HostCase *out);
+
+// HyperplaneReorder (src/utility.cpp:377-398): cells sorted by i+j+k, then
+// k, j, i ascending.  Returns packed linear physical indices and the plane
+// offsets (size planes+1).
+void HyperplaneOrder(int ni, int nj, int nk, std::vector<int> *cells,
+                     std::vector<int> *offsets);
+
+}  // namespace ath

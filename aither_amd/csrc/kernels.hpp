@@ -1,0 +1,717 @@
+// Device kernels of the hot path.  Launch geometry and data layout:
+//
+//  * state W: SoA per variable, ghost padded (ni+2g)(nj+2g)(nk+2g), i fastest;
+//  * residual, spectral radius, dt, diagonal, du, time-level states: SoA over
+//    physical cells, i fastest;
+//  * face vectors: one double4 (unit normal, magnitude) per physical face.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace ath {
+
+struct DevBlock {
+  int ni, nj, nk, g;
+  int NI, NJ;
+  long npad, ncell;
+  double *W;             // 5 * npad
+  const double *cw;      // 3 * npad
+  const double4 *fa[3];  // physical faces per direction
+  const double *vol;     // ncell
+  double *R;             // 5 * ncell
+  double *sr, *dt, *A;   // ncell
+  double *Ainv;          // ncell (FP64 operator)
+  float *Ainv32;         // ncell (FP32 operator)
+  double *consN, *consNm1;  // 5 * ncell
+  double *x;             // 5 * ncell (FP64 operator)
+  float *x32;            // 5 * ncell (FP32 operator)
+  double *xold;          // 5 * ncell, DP-LUR Jacobi copy
+  float *xold32;
+  long cellBase;         // running cell offset of this block (L-inf index)
+  int blockId;
+};
+
+struct Params {
+  Gas gas;
+  double kappa;
+  double theta, zeta, relax, dualCFL;
+  double dtFixed;  // nondimensional global time step, <= 0 for local dt
+  int implicit, multilevel, initMatrix, explicitEuler;
+};
+
+struct BcStateDev {
+  double p, rho, u, v, w, p0, t0, dx, dy, dz;
+};
+
+struct BcTask {
+  int dir, r1s, n1, r2s, n2, r3;
+  int gC, srcC, stype, bctype, layer, state;
+  long base;
+};
+
+__device__ __forceinline__ long PadIdx(const DevBlock &b, int i, int j, int k) {
+  return (long)(i + b.g) + (long)(j + b.g) * b.NI + (long)(k + b.g) * b.NI * b.NJ;
+}
+__device__ __forceinline__ long FaceIdx(const DevBlock &b, int d, int i, int j, int k) {
+  const int FI = b.ni + (d == 0), FJ = b.nj + (d == 1);
+  return (long)i + (long)j * FI + (long)k * FI * FJ;
+}
+__device__ __forceinline__ void Map3(int dir, int d3, int d1, int d2, int &i, int &j, int &k) {
+  if (dir == 0) { i = d3; j = d1; k = d2; }
+  else if (dir == 1) { i = d2; j = d3; k = d1; }
+  else { i = d1; j = d2; k = d3; }
+}
+
+// ---------------------------------------------------------------------------
+// boundary ghost cells: GetGhostState (src/ghostStates.cpp:62-689, inviscid
+// branches) and ExtrapolateHoldMixture (:691-709), driven by
+// procBlock::AssignInviscidGhostCells (src/procBlock.cpp:2449-2529).  One
+// thread per (surface patch cell, layer); every layer depends only on interior
+// cells, so all layers of all surfaces run in one launch.
+
+__device__ inline void ExtrapHold(const double bnd[5], double factor, const double in[5],
+                                  double out[5]) {
+  const double ghostRho = factor * bnd[0] - in[0];
+  if (ghostRho <= 0.0) {
+    for (int e = 0; e < 5; ++e) out[e] = bnd[e];
+    return;
+  }
+  const double mf = bnd[0] / bnd[0];
+  for (int e = 0; e < 5; ++e) out[e] = factor * bnd[e] - in[e];
+  out[0] = Dmax(ghostRho * mf, 0.0);
+}
+
+__device__ inline void GhostState(const GasT<double> &g, const double in[5], int bctype,
+                                  double ax, double ay, double az, int stype, int layer,
+                                  const BcStateDev *s, double gh[5]) {
+  for (int e = 0; e < 5; ++e) gh[e] = in[e];
+  const bool lower = (stype % 2 == 1);
+  const double nx = lower ? -1.0 * ax : ax, ny = lower ? -1.0 * ay : ay,
+               nz = lower ? -1.0 * az : az;
+  if (bctype == 0) {  // slipWall
+    const double vn = (in[1] * nx + in[2] * ny) + in[3] * nz;
+    gh[1] = in[1] - (nx * 2.0) * vn;
+    gh[2] = in[2] - (ny * 2.0) * vn;
+    gh[3] = in[3] - (nz * 2.0) * vn;
+  } else if (bctype == 1) {  // characteristic
+    const double fs[5] = {s->rho * 1.0, s->u, s->v, s->w, s->p};
+    const double vn = (in[1] * nx + in[2] * ny) + in[3] * nz;
+    const double c = SoS(g, in[0], in[4]);
+    const double mach = fabs(vn) / c;
+    if (mach >= 1.0 && vn < 0.0) {
+      for (int e = 0; e < 5; ++e) gh[e] = fs[e];
+    } else if (mach >= 1.0) {
+      // supersonic outflow: interior values
+    } else if (vn < 0.0) {
+      const double rc = in[0] * c;
+      const double dv = ((fs[1] - in[1]) * nx + (fs[2] - in[2]) * ny) + (fs[3] - in[3]) * nz;
+      gh[4] = 0.5 * (fs[4] + in[4] - rc * dv);
+      const double dp = fs[4] - gh[4];
+      const double rho = fs[0] - dp / (c * c);
+      gh[0] = rho * (fs[0] / fs[0]);
+      gh[1] = fs[1] - nx * dp / rc;
+      gh[2] = fs[2] - ny * dp / rc;
+      gh[3] = fs[3] - nz * dp / rc;
+    } else {
+      const double rc = in[0] * c;
+      const double dp = in[4] - fs[4];
+      const double rho = in[0] - dp / (c * c);
+      gh[0] = rho * (in[0] / in[0]);
+      gh[1] = in[1] + nx * dp / rc;
+      gh[2] = in[2] + ny * dp / rc;
+      gh[3] = in[3] + nz * dp / rc;
+      gh[4] = fs[4];
+    }
+    double t[5];
+    ExtrapHold(gh, 2.0, in, t);
+    if (layer > 1) ExtrapHold(t, (double)layer, in, gh);
+    else for (int e = 0; e < 5; ++e) gh[e] = t[e];
+  } else if (bctype == 2) {  // supersonicInflow
+    gh[0] = s->rho * 1.0;
+    gh[1] = s->u;
+    gh[2] = s->v;
+    gh[3] = s->w;
+    gh[4] = s->p;
+  } else if (bctype == 3) {  // supersonicOutflow
+    if (layer > 1)
+      for (int e = 0; e < 5; ++e) gh[e] = (double)layer * gh[e] - in[e];
+  } else if (bctype == 4) {  // stagnationInlet
+    const double gm1 = g.gamma - 1.0;
+    const double vn = (in[1] * nx + in[2] * ny) + in[3] * nz;
+    const double c = SoS(g, in[0], in[4]);
+    const double rNeg = vn - 2.0 * c / gm1;
+    const double vmag2 = (in[1] * in[1] + in[2] * in[2]) + in[3] * in[3];
+    const double cosT = -1.0 * vn / sqrt(vmag2);
+    const double stag = c * c + 0.5 * gm1 * vmag2;
+    const double sosB = -1.0 * rNeg * gm1 / (gm1 * cosT * cosT + 2.0) *
+                        (1.0 + cosT * sqrt((gm1 * cosT * cosT + 2.0) * stag / (gm1 * rNeg * rNeg) -
+                                           0.5 * gm1));
+    const double ratio = sosB * sosB / stag;
+    const double tb = s->t0 * ratio;
+    const double pb = s->p0 * pow(ratio, g.gamma / gm1);
+    const double vb = sqrt(2.0 / gm1 * (s->t0 - tb));
+    gh[0] = (pb / (g.R * tb)) * 1.0;
+    gh[1] = vb * s->dx;
+    gh[2] = vb * s->dy;
+    gh[3] = vb * s->dz;
+    gh[4] = pb;
+    double t[5];
+    ExtrapHold(gh, 2.0, in, t);
+    if (layer > 1) ExtrapHold(t, (double)layer, in, gh);
+    else for (int e = 0; e < 5; ++e) gh[e] = t[e];
+  } else if (bctype == 5) {  // pressureOutlet (reflecting)
+    const double c = SoS(g, in[0], in[4]);
+    const double rc = in[0] * c;
+    gh[4] = s->p;
+    const double dp = in[4] - gh[4];
+    gh[0] = (in[0] - dp / (c * c)) * (in[0] / in[0]);
+    gh[1] = in[1] + nx * dp / rc;
+    gh[2] = in[2] + ny * dp / rc;
+    gh[3] = in[3] + nz * dp / rc;
+    const double gvn = (gh[1] * nx + gh[2] * ny) + gh[3] * nz;
+    if (gvn / SoS(g, gh[0], gh[4]) >= 1.0)
+      for (int e = 0; e < 5; ++e) gh[e] = in[e];
+    for (int e = 0; e < 5; ++e) gh[e] = 2.0 * gh[e] - in[e];
+    if (layer > 1)
+      for (int e = 0; e < 5; ++e) gh[e] = (double)layer * gh[e] - in[e];
+  }
+}
+
+__global__ void __launch_bounds__(256) KBoundary(DevBlock b, const BcTask *__restrict__ tasks,
+                                                 int ntasks, long total,
+                                                 const BcStateDev *__restrict__ states, Gas gas) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int q = 0;
+  while (q + 1 < ntasks && tasks[q + 1].base <= t) ++q;
+  const BcTask tk = tasks[q];
+  const long loc = t - tk.base;
+  const int q1 = (int)(loc % tk.n1), q2 = (int)(loc / tk.n1);
+  int fi, fj, fk, ci, cj, ck, gi, gj, gk;
+  Map3(tk.dir, tk.r3, tk.r1s + q1, tk.r2s + q2, fi, fj, fk);
+  Map3(tk.dir, tk.srcC, tk.r1s + q1, tk.r2s + q2, ci, cj, ck);
+  Map3(tk.dir, tk.gC, tk.r1s + q1, tk.r2s + q2, gi, gj, gk);
+  const double4 a = b.fa[tk.dir][FaceIdx(b, tk.dir, fi, fj, fk)];
+  const long src = PadIdx(b, ci, cj, ck), dst = PadIdx(b, gi, gj, gk);
+  double in[5], gh[5];
+  for (int e = 0; e < 5; ++e) in[e] = b.W[e * b.npad + src];
+  const GasT<double> g = Cast<double>(gas);
+  GhostState(g, in, tk.bctype, a.x, a.y, a.z, tk.stype, tk.layer,
+             tk.state >= 0 ? &states[tk.state] : nullptr, gh);
+  for (int e = 0; e < 5; ++e) b.W[e * b.npad + dst] = gh[e];
+}
+
+// ---------------------------------------------------------------------------
+// fused residual: procBlock::CalcInvFluxI/J/K (src/procBlock.cpp:384-769),
+// CalcBlockTimeStep (:782-820), linearSolver::AddDiagonalTerms + Invert
+// (src/linearSolver.cpp:146-188).
+//
+// One workgroup per 8x8x4 tile of cells.  The workgroup evaluates every face
+// flux that touches the tile exactly once (896 faces: 288 i, 288 j, 320 k),
+// keeps F*|A| in LDS, then each thread assembles its cell's residual in the
+// reference's accumulation order (-F_i,lo +F_i,hi -F_j,lo +F_j,hi -F_k,lo
+// +F_k,hi), the cell spectral radius (i, j, k order), dt, the scalar diagonal
+// and its inverse.
+constexpr int kTI = 8, kTJ = 8, kTK = 4;
+constexpr int kNFI = (kTI + 1) * kTJ * kTK;
+constexpr int kNFJ = kTI * (kTJ + 1) * kTK;
+constexpr int kNFK = kTI * kTJ * (kTK + 1);
+constexpr int kNF = kNFI + kNFJ + kNFK;
+
+template <int RECON, int LIM, int FLUX>
+__device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &g, double kappa,
+                                         int d, int i, int j, int k, double F[5]) {
+  const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
+  const long c0 = PadIdx(b, i, j, k);  // upper cell of the face
+  double WL[5], WR[5];
+  if (RECON == 0) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      WL[e] = b.W[e * b.npad + c0 - s];
+      WR[e] = b.W[e * b.npad + c0];
+    }
+  } else {
+    const double *cw = b.cw + d * b.npad;
+    const double wm2 = cw[c0 - 2 * s], wm1 = cw[c0 - s], w0 = cw[c0], wp1 = cw[c0 + s];
+    const double dPL = (wm1 + wm1) / (wm1 + w0), dML = (wm1 + wm1) / (wm1 + wm2);
+    const double dPR = (w0 + w0) / (w0 + wm1), dMR = (w0 + w0) / (w0 + wp1);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double *We = b.W + e * b.npad;
+      const double um2 = We[c0 - 2 * s], um1 = We[c0 - s], u0 = We[c0], up1 = We[c0 + s];
+      WL[e] = Muscl<LIM>(um2, um1, u0, kappa, dPL, dML);
+      WR[e] = Muscl<LIM>(up1, u0, um1, kappa, dPR, dMR);
+    }
+  }
+  const double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+  if (FLUX == 0) RoeFlux(g, WL, WR, a.x, a.y, a.z, F);
+  else AusmFlux(g, WL, WR, a.x, a.y, a.z, F);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) F[e] = F[e] * a.w;
+}
+
+template <int RECON, int LIM, int FLUX>
+__global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
+                                                 const double *__restrict__ cflp) {
+  __shared__ double sF[5][kNF];
+  const int i0 = blockIdx.x * kTI, j0 = blockIdx.y * kTJ, k0 = blockIdx.z * kTK;
+  const GasT<double> g = Cast<double>(P.gas);
+  for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
+    int d, li, lj, lk;
+    if (f < kNFI) {
+      d = 0; li = f % (kTI + 1); lj = (f / (kTI + 1)) % kTJ; lk = f / ((kTI + 1) * kTJ);
+    } else if (f < kNFI + kNFJ) {
+      const int q = f - kNFI;
+      d = 1; li = q % kTI; lj = (q / kTI) % (kTJ + 1); lk = q / (kTI * (kTJ + 1));
+    } else {
+      const int q = f - kNFI - kNFJ;
+      d = 2; li = q % kTI; lj = (q / kTI) % kTJ; lk = q / (kTI * kTJ);
+    }
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    const bool ok = (i < b.ni + (d == 0)) && (j < b.nj + (d == 1)) && (k < b.nk + (d == 2));
+    if (ok) {
+      double F[5];
+      FaceFlux<RECON, LIM, FLUX>(b, g, P.kappa, d, i, j, k, F);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sF[e][f] = F[e];
+    }
+  }
+  __syncthreads();
+  const int li = threadIdx.x % kTI, lj = (threadIdx.x / kTI) % kTJ, lk = threadIdx.x / (kTI * kTJ);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  if (i >= b.ni || j >= b.nj || k >= b.nk) return;
+  const int fi = lk * (kTI + 1) * kTJ + lj * (kTI + 1) + li;
+  const int fj = kNFI + lk * kTI * (kTJ + 1) + lj * kTI + li;
+  const int fk = kNFI + kNFJ + lk * kTI * kTJ + lj * kTI + li;
+  const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    double r = -sF[e][fi];
+    r += sF[e][fi + 1];
+    r -= sF[e][fj];
+    r += sF[e][fj + kTI];
+    r -= sF[e][fk];
+    r += sF[e][fk + kTI * kTJ];
+    b.R[e * b.ncell + p] = r;
+  }
+  const long c = PadIdx(b, i, j, k);
+  const double rho = b.W[c], u = b.W[b.npad + c], v = b.W[2 * b.npad + c],
+               w = b.W[3 * b.npad + c], pr = b.W[4 * b.npad + c];
+  const double li_ = InvCellSpecRad(g, rho, u, v, w, pr, b.fa[0][FaceIdx(b, 0, i, j, k)],
+                                    b.fa[0][FaceIdx(b, 0, i + 1, j, k)]);
+  const double lj_ = InvCellSpecRad(g, rho, u, v, w, pr, b.fa[1][FaceIdx(b, 1, i, j, k)],
+                                    b.fa[1][FaceIdx(b, 1, i, j + 1, k)]);
+  const double lk_ = InvCellSpecRad(g, rho, u, v, w, pr, b.fa[2][FaceIdx(b, 2, i, j, k)],
+                                    b.fa[2][FaceIdx(b, 2, i, j, k + 1)]);
+  const double sr = (li_ + lj_) + lk_;
+  b.sr[p] = sr;
+  const double vol = b.vol[p];
+  const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*cflp) * (vol / Dmax(sr, 0.0));
+  b.dt[p] = dt;
+  if (P.implicit) {
+    double dvt = (vol * (1.0 + P.zeta)) / (dt * P.theta);
+    if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
+    double A = sr * P.relax;
+    A += dvt;
+    b.A[p] = A;
+    const double ainv = 1.0 / A;
+    if (b.Ainv) b.Ainv[p] = ainv;
+    if (b.Ainv32) b.Ainv32[p] = (float)ainv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// implicit operator
+
+// b = -R/theta + forcing + SolDeltaNm1 - SolDeltaMmN
+// (src/linearSolver.cpp:360-378, src/procBlock.cpp:1010-1034); forcing is
+// zero on the single grid level
+__device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, long c,
+                                    double out[5]) {
+#pragma clang fp contract(off)
+  const GasT<double> g = Cast<double>(P.gas);
+  const double W[5] = {b.W[c], b.W[b.npad + c], b.W[2 * b.npad + c], b.W[3 * b.npad + c],
+                       b.W[4 * b.npad + c]};
+  double U[5];
+  PrimToCons(g, W, U);
+  const double vol = b.vol[p], dt = b.dt[p];
+  const double coeff = (vol * (1.0 + P.zeta)) / (dt * P.theta);
+  const double thetaInv = 1.0 / P.theta;
+  const double c2 = P.multilevel ? (vol * P.zeta) / (dt * P.theta) : 0.0;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const double nm1 = P.multilevel ? c2 * (b.consN[e * b.ncell + p] - b.consNm1[e * b.ncell + p]) : 0.0;
+    const double mmn = coeff * (U[e] - b.consN[e * b.ncell + p]);
+    double v = -thetaInv * b.R[e * b.ncell + p];
+    out[e] = (v + nm1) - mmn;
+  }
+}
+
+template <typename T>
+struct OpBuf;
+template <>
+struct OpBuf<double> {
+  static __device__ __forceinline__ double *X(const DevBlock &b) { return b.x; }
+  static __device__ __forceinline__ double *Xold(const DevBlock &b) { return b.xold; }
+  static __device__ __forceinline__ const double *Ainv(const DevBlock &b) { return b.Ainv; }
+};
+template <>
+struct OpBuf<float> {
+  static __device__ __forceinline__ float *X(const DevBlock &b) { return b.x32; }
+  static __device__ __forceinline__ float *Xold(const DevBlock &b) { return b.xold32; }
+  static __device__ __forceinline__ const float *Ainv(const DevBlock &b) { return b.Ainv32; }
+};
+
+// sum over the lower (upper == false) or upper neighbours of the Rusanov
+// off-diagonal products: procBlock::ImplicitLower / ImplicitUpper
+// (src/procBlock.cpp:1056-1161), neighbours in i, j, k order
+template <typename T>
+__device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const T *x,
+                                           int i, int j, int k, bool upper, T out[5]) {
+#pragma unroll
+  for (int e = 0; e < 5; ++e) out[e] = T(0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int s = upper ? 1 : -1;
+    const int ni = i + (d == 0 ? s : 0), nj = j + (d == 1 ? s : 0), nk = k + (d == 2 ? s : 0);
+    if (ni < 0 || nj < 0 || nk < 0 || ni >= b.ni || nj >= b.nj || nk >= b.nk) continue;
+    const double4 a = upper ? b.fa[d][FaceIdx(b, d, ni, nj, nk)] : b.fa[d][FaceIdx(b, d, i, j, k)];
+    const long c = PadIdx(b, ni, nj, nk);
+    const long q = (long)ni + (long)nj * b.ni + (long)nk * b.ni * b.nj;
+    T W[5], du[5], o[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      W[e] = T(b.W[e * b.npad + c]);
+      du[e] = x[e * b.ncell + q];
+    }
+    ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) out[e] += o[e];
+  }
+}
+
+// InitializeMatrixUpdate (src/linearSolver.cpp:111-144)
+template <typename T>
+__global__ void __launch_bounds__(256) KInitUpdate(DevBlock b, Params P) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  T *x = OpBuf<T>::X(b);
+  if (!P.initMatrix) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = T(0);
+    return;
+  }
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  double rb[5];
+  Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  const T ainv = OpBuf<T>::Ainv(b)[p];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = T(rb[e]) * ainv;
+}
+
+// lusgs::LUSGS_Forward over one hyperplane (src/linearSolver.cpp:341-383)
+template <typename T>
+__global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
+                                                    const int *__restrict__ cells, int n,
+                                                    int useUpper) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long p = cells[t];
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<T> g = Cast<T>(P.gas);
+  T *x = OpBuf<T>::X(b);
+  T L[5];
+  OffDiagSum<T>(b, g, x, i, j, k, false, L);
+  if (useUpper) {
+    T U[5];
+    OffDiagSum<T>(b, g, x, i, j, k, true, U);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) L[e] -= U[e];
+  }
+  double rb[5];
+  Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  const T ainv = OpBuf<T>::Ainv(b)[p];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = (T(rb[e]) + L[e]) * ainv;
+}
+
+// lusgs::LUSGS_Backward over one hyperplane (src/linearSolver.cpp:385-428)
+template <typename T>
+__global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
+                                                     const int *__restrict__ cells, int n,
+                                                     int full) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long p = cells[t];
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<T> g = Cast<T>(P.gas);
+  T *x = OpBuf<T>::X(b);
+  T U[5];
+  OffDiagSum<T>(b, g, x, i, j, k, true, U);
+  const T ainv = OpBuf<T>::Ainv(b)[p];
+  if (full) {
+    T L[5];
+    OffDiagSum<T>(b, g, x, i, j, k, false, L);
+    double rb[5];
+    Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = ((T(rb[e]) + L[e]) - U[e]) * ainv;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = x[e * b.ncell + p] - U[e] * ainv;
+  }
+}
+
+// dplur::DPLUR Jacobi sweep (src/linearSolver.cpp:473-507): reads the copy
+// xold, writes x
+template <typename T>
+__global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<T> g = Cast<T>(P.gas);
+  const T *xo = OpBuf<T>::Xold(b);
+  T L[5], U[5];
+  OffDiagSum<T>(b, g, xo, i, j, k, false, L);
+  OffDiagSum<T>(b, g, xo, i, j, k, true, U);
+  double rb[5];
+  Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  const T ainv = OpBuf<T>::Ainv(b)[p];
+  T *x = OpBuf<T>::X(b);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = ((T(rb[e]) + T(0)) + (L[e] - U[e])) * ainv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) KCopyUpdate(DevBlock b) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= 5 * b.ncell) return;
+  OpBuf<T>::Xold(b)[p] = OpBuf<T>::X(b)[p];
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reductions: each workgroup reduces its 256 values in a fixed
+// tree and writes one partial; a single-workgroup kernel sums the partials in
+// index order.
+
+constexpr int kRedThreads = 256;
+constexpr int kNormSlots = 7;  // 5 x sum R^2, linf value, matrix-residual sum
+
+struct Partial {
+  double l2[5];
+  double linf;
+  long long linfIdx;  // ((cellBase + p) * 5 + e), -1 if none
+  double mres;
+};
+
+__device__ __forceinline__ bool LinfBetter(double v, long long idx, double bv, long long bidx) {
+  if (idx < 0) return false;
+  if (bidx < 0) return true;
+  return v > bv || (v == bv && idx < bidx);
+}
+
+// linearSolver::Residual / AXmB (src/linearSolver.cpp:58-109), squared and
+// summed as in mgSolution::CycleAtLevel (src/mgSolution.cpp:197-206)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Params P,
+                                                               Partial *part) {
+  __shared__ double s[kRedThreads];
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (p < b.ncell) {
+    const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+    const GasT<T> g = Cast<T>(P.gas);
+    const T *x = OpBuf<T>::X(b);
+    T L[5], U[5];
+    OffDiagSum<T>(b, g, x, i, j, k, false, L);
+    OffDiagSum<T>(b, g, x, i, j, k, true, U);
+    double rb[5];
+    Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+    const double A = b.A[p];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double ax = (double)x[e * b.ncell + p] * A;
+      const double axmb = (ax - (double)(L[e] - U[e])) - rb[e];
+      const double r = 0.0 - axmb;
+      acc += r * r;
+    }
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x].mres = s[0];
+}
+
+// procBlock::UpdateBlock (src/procBlock.cpp:826-871) with
+// ImplicitTimeAdvance (:902-916) or ExplicitEulerTimeAdvance (:880-896)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) KUpdate(DevBlock b, Params P, Partial *part,
+                                                       int *flag) {
+  __shared__ double sl2[5][kRedThreads];
+  __shared__ double sv[kRedThreads];
+  __shared__ long long si[kRedThreads];
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double l2[5] = {0, 0, 0, 0, 0};
+  double bv = 0.0;
+  long long bi = -1;
+  if (p < b.ncell) {
+    const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+    const long c = PadIdx(b, i, j, k);
+    const GasT<double> g = Cast<double>(P.gas);
+    double W[5], Wn[5], R[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      W[e] = b.W[e * b.npad + c];
+      R[e] = b.R[e * b.ncell + p];
+    }
+    if (P.explicitEuler) {
+      double U[5];
+      PrimToCons(g, W, U);
+      const double f = b.dt[p] / b.vol[p];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) U[e] -= f * R[e];
+      ConsToPrim(g, U, Wn);
+    } else {
+      const T *x = OpBuf<T>::X(b);
+      double du[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) du[e] = (double)x[e * b.ncell + p];
+      UpdatePrim(g, W, du, Wn);
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) b.W[e * b.npad + c] = Wn[e];
+    if (!(Wn[0] > 0.0) || !(Wn[4] > 0.0)) atomicOr(flag, 1);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      l2[e] = R[e] * R[e];
+      if (R[e] > bv) {  // strict '>' from 0: first maximum wins
+        bv = R[e];
+        bi = (long long)(b.cellBase + p) * 5 + e;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) sl2[e][threadIdx.x] = l2[e];
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sl2[e][threadIdx.x] += sl2[e][threadIdx.x + w];
+      if (LinfBetter(sv[threadIdx.x + w], si[threadIdx.x + w], sv[threadIdx.x], si[threadIdx.x])) {
+        sv[threadIdx.x] = sv[threadIdx.x + w];
+        si[threadIdx.x] = si[threadIdx.x + w];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    Partial &o = part[blockIdx.x];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o.l2[e] = sl2[e][0];
+    o.linf = sv[0];
+    o.linfIdx = si[0];
+  }
+}
+
+// sums partials [0, n) in index order; writes slot `out` =
+// {l2[5], linf, linfIdx (as bits), mres}
+__global__ void __launch_bounds__(kRedThreads) KReduceNorms(const Partial *part, int n,
+                                                           double *hist, int *counter,
+                                                           int withMres) {
+  __shared__ double sl2[5][kRedThreads];
+  __shared__ double sm[kRedThreads];
+  __shared__ double sv[kRedThreads];
+  __shared__ long long si[kRedThreads];
+  double l2[5] = {0, 0, 0, 0, 0}, m = 0.0, bv = 0.0;
+  long long bi = -1;
+  for (int q = threadIdx.x; q < n; q += kRedThreads) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) l2[e] += part[q].l2[e];
+    if (withMres) m += part[q].mres;
+    if (LinfBetter(part[q].linf, part[q].linfIdx, bv, bi)) {
+      bv = part[q].linf;
+      bi = part[q].linfIdx;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) sl2[e][threadIdx.x] = l2[e];
+  sm[threadIdx.x] = m;
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sl2[e][threadIdx.x] += sl2[e][threadIdx.x + w];
+      sm[threadIdx.x] += sm[threadIdx.x + w];
+      if (LinfBetter(sv[threadIdx.x + w], si[threadIdx.x + w], sv[threadIdx.x], si[threadIdx.x])) {
+        sv[threadIdx.x] = sv[threadIdx.x + w];
+        si[threadIdx.x] = si[threadIdx.x + w];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int slot = *counter;
+    double *o = hist + (long)slot * 8;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o[e] = sl2[e][0];
+    o[5] = sv[0];
+    o[6] = __longlong_as_double(si[0]);
+    o[7] = sm[0];
+    *counter = slot + 1;
+  }
+}
+
+__global__ void KSetScalar(double *dst, double v) { *dst = v; }
+__global__ void KSetInt(int *dst, int v) { *dst = v; }
+
+// StoreOldSolution (src/procBlock.cpp:1037-1053)
+__global__ void __launch_bounds__(256) KBeginStep(DevBlock b, Params P, int copyNm1) {
+#pragma clang fp contract(off)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const long c = PadIdx(b, i, j, k);
+  const GasT<double> g = Cast<double>(P.gas);
+  const double W[5] = {b.W[c], b.W[b.npad + c], b.W[2 * b.npad + c], b.W[3 * b.npad + c],
+                       b.W[4 * b.npad + c]};
+  double U[5];
+  PrimToCons(g, W, U);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    b.consN[e * b.ncell + p] = U[e];
+    if (copyNm1) b.consNm1[e * b.ncell + p] = U[e];
+  }
+}
+
+// AssignSolToTimeNm1 after the last nonlinear iteration (src/gridLevel.cpp:424-428)
+__global__ void __launch_bounds__(256) KCopyNm1(DevBlock b) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= 5 * b.ncell) return;
+  b.consNm1[p] = b.consN[p];
+}
+
+// batch face-flux evaluation used by the parity tests
+__global__ void __launch_bounds__(256) KEvalFlux(int flux, Gas gas, const double *WL,
+                                                 const double *WR, const double *n, int count,
+                                                 double *F) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const GasT<double> g = Cast<double>(gas);
+  double L[5], R[5], out[5];
+  for (int e = 0; e < 5; ++e) {
+    L[e] = WL[5 * t + e];
+    R[e] = WR[5 * t + e];
+  }
+  if (flux == 0) RoeFlux(g, L, R, n[3 * t], n[3 * t + 1], n[3 * t + 2], out);
+  else AusmFlux(g, L, R, n[3 * t], n[3 * t + 1], n[3 * t + 2], out);
+  for (int e = 0; e < 5; ++e) F[5 * t + e] = out[e];
+}
+
+}  // namespace ath

@@ -1,0 +1,854 @@
+// C-ABI implementation: context, device arena, launch orchestration, graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aither_amd.h"
+#include "host_case.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int Fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return Fail(ATH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+struct BlockRes {
+  ath::DevBlock d{};
+  std::vector<int> planeOff;  // host copy of hyperplane offsets
+  int *cells = nullptr;       // device hyperplane cell list
+  int nparts = 0;             // reduction partials of this block
+  int partBase = 0;
+  std::vector<ath::BcTask> tasks;
+  ath::BcTask *dtasks = nullptr;
+  long bcTotal = 0;
+};
+
+}  // namespace
+
+struct ath_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ath::HostCase hc;
+  ath::Params P{};
+  std::vector<BlockRes> blk;
+  std::vector<void *> allocs;
+  ath::BcStateDev *dstates = nullptr;
+  double *dcfl = nullptr;
+  ath::Partial *dpart = nullptr;
+  int nparts = 0;
+  double *dhist = nullptr;  // per-iteration norms ring
+  int histCap = 0;
+  int *dcounter = nullptr;
+  int *dflag = nullptr;
+  int fp32 = 0;
+  int matres = 1;
+  double cfl = 0;
+  double l2first[ATH_MAX_EQ]{};
+  // graph of one nonlinear iteration (after the boundary pass)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  int graphFp32 = -1, graphMatres = -1;
+  // timing
+  int timing = 0;
+  std::vector<hipEvent_t> evs;
+  ath_timing tm{};
+
+  ~ath_ctx() {
+    if (graphExec) hipGraphExecDestroy(graphExec);
+    if (graph) hipGraphDestroy(graph);
+    for (auto e : evs) hipEventDestroy(e);
+    for (void *p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <typename T>
+  int Alloc(T **p, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+      return Fail(ATH_ERR_NOMEM, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return ATH_OK;
+  }
+};
+
+namespace {
+
+constexpr int kBlk = 256;
+
+int Grid(long n, int b = kBlk) { return (int)((n + b - 1) / b); }
+
+int Upload(ath_ctx *c) {
+  using namespace ath;
+  const HostCase &hc = c->hc;
+  // physics / numerics parameters
+  c->P.gas.R = hc.gasR;
+  c->P.gas.n = hc.gasN;
+  c->P.gas.gamma = hc.gamma;
+  c->P.gas.cps = hc.gasR * (hc.gasN + 1.0);
+  c->P.gas.cvs = hc.gasR * hc.gasN;
+  c->P.kappa = hc.kappa;
+  c->P.theta = hc.theta;
+  c->P.zeta = hc.zeta;
+  c->P.relax = hc.relax;
+  c->P.dualCFL = hc.dualCFL;
+  c->P.dtFixed = hc.dtIn > 0.0 ? hc.dtIn * hc.aRef / hc.lRef : -1.0;
+  c->P.implicit = hc.Implicit() ? 1 : 0;
+  c->P.multilevel = hc.MultiLevel() ? 1 : 0;
+  c->P.initMatrix = hc.InitMatrix() ? 1 : 0;
+  c->P.explicitEuler = hc.timeint == TimeInt::ExplicitEuler ? 1 : 0;
+
+  // boundary states by tag
+  std::vector<BcStateDev> st;
+  for (const auto &s : hc.bcs)
+    st.push_back({s.p, s.rho, s.vel[0], s.vel[1], s.vel[2], s.p0, s.t0, s.dir[0], s.dir[1], s.dir[2]});
+  if (int r = c->Alloc(&c->dstates, st.size())) return r;
+  if (!st.empty())
+    HIP_OK(hipMemcpy(c->dstates, st.data(), st.size() * sizeof(BcStateDev), hipMemcpyHostToDevice));
+  if (int r = c->Alloc(&c->dcfl, 1)) return r;
+  if (int r = c->Alloc(&c->dcounter, 1)) return r;
+  if (int r = c->Alloc(&c->dflag, 1)) return r;
+  HIP_OK(hipMemset(c->dcounter, 0, sizeof(int)));
+  HIP_OK(hipMemset(c->dflag, 0, sizeof(int)));
+
+  long cellBase = 0;
+  int partBase = 0;
+  c->blk.resize(hc.blocks.size());
+  for (size_t bq = 0; bq < hc.blocks.size(); ++bq) {
+    const HostBlock &h = hc.blocks[bq];
+    BlockRes &br = c->blk[bq];
+    DevBlock &d = br.d;
+    d.ni = h.ni; d.nj = h.nj; d.nk = h.nk; d.g = h.g;
+    d.NI = h.NI; d.NJ = h.NJ;
+    d.npad = h.NumPadded();
+    d.ncell = h.NumCells();
+    d.cellBase = cellBase;
+    d.blockId = (int)bq;
+    cellBase += d.ncell;
+    const long np = d.npad, nc = d.ncell;
+    // state: SoA padded
+    {
+      std::vector<double> W(5 * (size_t)np, 0.0);
+      for (int k = 0; k < h.nk; ++k)
+        for (int j = 0; j < h.nj; ++j)
+          for (int i = 0; i < h.ni; ++i) {
+            const long p = i + (long)j * h.ni + (long)k * h.ni * h.nj;
+            for (int e = 0; e < 5; ++e) W[e * np + h.Cell(i, j, k)] = h.W0[5 * p + e];
+          }
+      if (int r = c->Alloc(&d.W, 5 * np)) return r;
+      HIP_OK(hipMemcpy(d.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    {
+      std::vector<double> cw(3 * (size_t)np);
+      for (int dd = 0; dd < 3; ++dd) std::copy(h.cw[dd].begin(), h.cw[dd].end(), cw.begin() + dd * np);
+      double *dcw = nullptr;
+      if (int r = c->Alloc(&dcw, 3 * np)) return r;
+      HIP_OK(hipMemcpy(dcw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice));
+      d.cw = dcw;
+    }
+    for (int dd = 0; dd < 3; ++dd) {
+      const int FI = h.ni + (dd == 0), FJ = h.nj + (dd == 1), FK = h.nk + (dd == 2);
+      std::vector<double4> fa((size_t)FI * FJ * FK);
+      for (int k = 0; k < FK; ++k)
+        for (int j = 0; j < FJ; ++j)
+          for (int i = 0; i < FI; ++i) {
+            const auto &a = h.fa[dd][h.Face(dd, i, j, k)];
+            fa[(size_t)i + (size_t)j * FI + (size_t)k * FI * FJ] = make_double4(a[0], a[1], a[2], a[3]);
+          }
+      double4 *dfa = nullptr;
+      if (int r = c->Alloc(&dfa, fa.size())) return r;
+      HIP_OK(hipMemcpy(dfa, fa.data(), fa.size() * sizeof(double4), hipMemcpyHostToDevice));
+      d.fa[dd] = dfa;
+    }
+    {
+      std::vector<double> vol((size_t)nc);
+      for (int k = 0; k < h.nk; ++k)
+        for (int j = 0; j < h.nj; ++j)
+          for (int i = 0; i < h.ni; ++i) vol[i + (size_t)j * h.ni + (size_t)k * h.ni * h.nj] = h.vol[h.Cell(i, j, k)];
+      double *dv = nullptr;
+      if (int r = c->Alloc(&dv, nc)) return r;
+      HIP_OK(hipMemcpy(dv, vol.data(), vol.size() * sizeof(double), hipMemcpyHostToDevice));
+      d.vol = dv;
+    }
+    if (int r = c->Alloc(&d.R, 5 * nc)) return r;
+    if (int r = c->Alloc(&d.sr, nc)) return r;
+    if (int r = c->Alloc(&d.dt, nc)) return r;
+    if (int r = c->Alloc(&d.A, nc)) return r;
+    if (int r = c->Alloc(&d.Ainv, nc)) return r;
+    if (int r = c->Alloc(&d.consN, 5 * nc)) return r;
+    if (int r = c->Alloc(&d.consNm1, 5 * nc)) return r;
+    if (int r = c->Alloc(&d.x, 5 * nc)) return r;
+    if (int r = c->Alloc(&d.xold, 5 * nc)) return r;
+    HIP_OK(hipMemset(d.x, 0, 5 * nc * sizeof(double)));
+    HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
+    d.Ainv32 = nullptr;
+    d.x32 = nullptr;
+    d.xold32 = nullptr;
+    // hyperplane order
+    std::vector<int> cells;
+    HyperplaneOrder(h.ni, h.nj, h.nk, &cells, &br.planeOff);
+    if (int r = c->Alloc(&br.cells, cells.size())) return r;
+    HIP_OK(hipMemcpy(br.cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    // reduction partials
+    br.nparts = Grid(nc, kRedThreads);
+    br.partBase = partBase;
+    partBase += br.nparts;
+    // boundary tasks (procBlock::AssignInviscidGhostCells, src/procBlock.cpp:2449-2529)
+    long total = 0;
+    for (int layer = 1; layer <= h.g; ++layer)
+      for (const auto &sf : h.surf) {
+        const int stp = sf.SurfType();
+        const int dir = (stp - 1) / 2;
+        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+        const int r3 = sf.d[2 * dir];
+        const int n3 = h.N(dir);
+        BcTask t{};
+        t.dir = dir;
+        t.r1s = sf.d[2 * d1];
+        t.n1 = sf.d[2 * d1 + 1] - t.r1s;
+        t.r2s = sf.d[2 * d2];
+        t.n2 = sf.d[2 * d2 + 1] - t.r2s;
+        t.r3 = r3;
+        int iC, aC;
+        if (stp % 2 == 0) {
+          t.gC = r3 + layer - 1;
+          iC = std::max(r3 - layer, 0);
+          aC = r3 - 1;
+        } else {
+          t.gC = r3 - layer;
+          iC = std::min(r3 + layer - 1, n3 - 1);
+          aC = r3;
+        }
+        std::string ty = sf.type == "viscousWall" ? "slipWall" : sf.type;
+        t.bctype = ty == "slipWall" ? BC_SLIPWALL
+                   : ty == "characteristic" ? BC_CHARACTERISTIC
+                   : ty == "supersonicInflow" ? BC_SUPERSONIC_INFLOW
+                   : ty == "supersonicOutflow" ? BC_SUPERSONIC_OUTFLOW
+                   : ty == "stagnationInlet" ? BC_STAGNATION_INLET
+                                             : BC_PRESSURE_OUTLET;
+        t.srcC = t.bctype == BC_SLIPWALL ? iC : aC;
+        t.stype = stp;
+        t.layer = layer;
+        t.state = -1;
+        for (size_t q = 0; q < hc.bcs.size(); ++q)
+          if (hc.bcs[q].tag == sf.d[6]) { t.state = (int)q; break; }
+        t.base = total;
+        total += (long)t.n1 * t.n2;
+        if (t.n1 > 0 && t.n2 > 0) br.tasks.push_back(t);
+      }
+    br.bcTotal = total;
+    if (!br.tasks.empty()) {
+      if (int r = c->Alloc(&br.dtasks, br.tasks.size())) return r;
+      HIP_OK(hipMemcpy(br.dtasks, br.tasks.data(), br.tasks.size() * sizeof(BcTask), hipMemcpyHostToDevice));
+    }
+  }
+  c->nparts = partBase;
+  if (int r = c->Alloc(&c->dpart, c->nparts)) return r;
+  c->histCap = 4096;
+  if (int r = c->Alloc(&c->dhist, 8 * (size_t)c->histCap)) return r;
+  return ATH_OK;
+}
+
+int EnsureFp32(ath_ctx *c) {
+  for (auto &br : c->blk) {
+    auto &d = br.d;
+    if (d.x32) continue;
+    if (int r = c->Alloc(&d.Ainv32, d.ncell)) return r;
+    if (int r = c->Alloc(&d.x32, 5 * d.ncell)) return r;
+    if (int r = c->Alloc(&d.xold32, 5 * d.ncell)) return r;
+    HIP_OK(hipMemset(d.x32, 0, 5 * d.ncell * sizeof(float)));
+  }
+  HIP_OK(hipDeviceSynchronize());
+  return ATH_OK;
+}
+
+// --- launch helpers (all on c->stream) -------------------------------------
+
+void LaunchBoundary(ath_ctx *c) {
+  for (auto &br : c->blk)
+    if (br.bcTotal > 0)
+      hipLaunchKernelGGL(ath::KBoundary, dim3(Grid(br.bcTotal)), dim3(kBlk), 0, c->stream, br.d,
+                         br.dtasks, (int)br.tasks.size(), br.bcTotal, c->dstates, c->P.gas);
+}
+
+template <int RECON, int LIM, int FLUX>
+void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
+  const auto &d = br.d;
+  dim3 grid((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
+            (d.nk + ath::kTK - 1) / ath::kTK);
+  hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+}
+
+void LaunchResidual(ath_ctx *c, const BlockRes &br) {
+  const auto &hc = c->hc;
+  const int recon = hc.recon == ath::Recon::MUSCL ? 1 : 0;
+  const int lim = (int)hc.limiter;
+  const int flux = (int)hc.flux;
+  ath::DevBlock d = br.d;
+  if (!c->fp32) d.Ainv32 = nullptr;
+  else d.Ainv = nullptr;
+  BlockRes tmp = br;
+  tmp.d = d;
+#define ATH_RES(R, L, F) if (recon == R && lim == L && flux == F) return LaunchResidualT<R, L, F>(c, tmp)
+  ATH_RES(0, 0, 0); ATH_RES(0, 0, 1);
+  ATH_RES(1, 0, 0); ATH_RES(1, 1, 0); ATH_RES(1, 2, 0);
+  ATH_RES(1, 0, 1); ATH_RES(1, 1, 1); ATH_RES(1, 2, 1);
+  // constant reconstruction ignores the limiter
+  if (recon == 0) return flux == 0 ? LaunchResidualT<0, 0, 0>(c, tmp) : LaunchResidualT<0, 0, 1>(c, tmp);
+#undef ATH_RES
+}
+
+template <typename T>
+void LaunchRelaxT(ath_ctx *c, bool timed, int evBase) {
+  const auto &hc = c->hc;
+  for (auto &br : c->blk)
+    hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
+  if (timed) hipEventRecord(c->evs[evBase], c->stream);
+  for (int sw = 0; sw < hc.sweeps; ++sw) {
+    const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
+    if (hc.solver == ath::Solver::LUSGS) {
+      for (auto &br : c->blk) {
+        const int np = (int)br.planeOff.size() - 1;
+        for (int p = 0; p < np; ++p) {
+          const int n = br.planeOff[p + 1] - br.planeOff[p];
+          hipLaunchKernelGGL(ath::KLusgsForward<T>, dim3(Grid(n, 64)), dim3(64), 0, c->stream, br.d, c->P,
+                             br.cells + br.planeOff[p], n, full);
+        }
+      }
+      for (auto &br : c->blk) {
+        const int np = (int)br.planeOff.size() - 1;
+        for (int p = np - 1; p >= 0; --p) {
+          const int n = br.planeOff[p + 1] - br.planeOff[p];
+          hipLaunchKernelGGL(ath::KLusgsBackward<T>, dim3(Grid(n, 64)), dim3(64), 0, c->stream, br.d, c->P,
+                             br.cells + br.planeOff[p], n, full);
+        }
+      }
+    } else {
+      for (auto &br : c->blk) {
+        hipLaunchKernelGGL(ath::KCopyUpdate<T>, dim3(Grid(5 * br.d.ncell)), dim3(kBlk), 0, c->stream, br.d);
+        hipLaunchKernelGGL(ath::KDplur<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
+      }
+    }
+  }
+  if (timed) hipEventRecord(c->evs[evBase + 1], c->stream);
+  if (c->matres)
+    for (auto &br : c->blk)
+      hipLaunchKernelGGL(ath::KMatrixResidual<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d,
+                         c->P, c->dpart + br.partBase);
+}
+
+template <typename T>
+void LaunchUpdateT(ath_ctx *c) {
+  for (auto &br : c->blk)
+    hipLaunchKernelGGL(ath::KUpdate<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d, c->P,
+                       c->dpart + br.partBase, c->dflag);
+  hipLaunchKernelGGL(ath::KReduceNorms, dim3(1), dim3(ath::kRedThreads), 0, c->stream, c->dpart, c->nparts,
+                     c->dhist, c->dcounter, (c->P.implicit && c->matres) ? 1 : 0);
+}
+
+// everything after the residual: relax (implicit) + update + reduction
+void LaunchTail(ath_ctx *c, bool timed, int evBase) {
+  if (c->P.implicit) {
+    if (c->fp32) LaunchRelaxT<float>(c, timed, evBase);
+    else LaunchRelaxT<double>(c, timed, evBase);
+  }
+  if (c->fp32) LaunchUpdateT<float>(c);
+  else LaunchUpdateT<double>(c);
+}
+
+void LaunchCopyNm1(ath_ctx *c) {
+  for (auto &br : c->blk)
+    hipLaunchKernelGGL(ath::KCopyNm1, dim3(Grid(5 * br.d.ncell)), dim3(kBlk), 0, c->stream, br.d);
+}
+
+int CheckLaunch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return ATH_OK;
+}
+
+// read norms slot `slot` and fill `out` (+ PrintResiduals normalisation)
+int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
+  double h[8];
+  HIP_OK(hipMemcpy(h, c->dhist + 8 * (size_t)slot, sizeof h, hipMemcpyDeviceToHost));
+  int flag = 0;
+  HIP_OK(hipMemcpy(&flag, c->dflag, sizeof flag, hipMemcpyDeviceToHost));
+  ath_norms n{};
+  n.num_equations = 5;
+  for (int e = 0; e < 5; ++e) {
+    n.l2_sq[e] = h[e];
+    n.l2[e] = std::sqrt(h[e]);
+  }
+  n.linf = h[5];
+  long long idx;
+  std::memcpy(&idx, &h[6], sizeof idx);
+  if (idx >= 0) {
+    const long long cell = idx / 5;
+    n.linf_eqn = (int)(idx % 5) + 1;
+    long long base = 0;
+    for (const auto &br : c->blk) {
+      if (cell < base + br.d.ncell) {
+        const long long p = cell - base;
+        n.linf_block = br.d.blockId;
+        n.linf_i = (int)(p % br.d.ni);
+        n.linf_j = (int)((p / br.d.ni) % br.d.nj);
+        n.linf_k = (int)(p / ((long long)br.d.ni * br.d.nj));
+        break;
+      }
+      base += br.d.ncell;
+    }
+  }
+  // mgSolution::CycleAtLevel divides by the padded size of x
+  // (src/mgSolution.cpp:197-206); main divides by cells * neq
+  // (src/main.cpp:269-270)
+  double padded = 0;
+  for (const auto &br : c->blk) padded += (double)br.d.npad * 5;
+  const double cyc = c->P.implicit ? h[7] / padded : 0.0;
+  n.matrix_resid = std::sqrt(cyc / (c->hc.totalCells * 5));
+  n.cfl = c->cfl;
+  // PrintResiduals (src/output.cpp:1028-1063)
+  constexpr double eps = 1.0e-30;
+  if (nn == 0 && mm == 0) {
+    for (int e = 0; e < 5; ++e) c->l2first[e] = n.l2[e];
+  } else if (nn < 5 && mm == 0) {
+    if (n.l2[0] > c->l2first[0]) c->l2first[0] = n.l2[0];
+    for (int e = 1; e < 5; ++e) c->l2first[e] = std::max(c->l2first[e], n.l2[e]);
+  }
+  for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (c->l2first[e] + eps);
+  if (out) *out = n;
+  if (flag) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
+  return ATH_OK;
+}
+
+int BuildGraph(ath_ctx *c) {
+  if (c->graphExec && c->graphFp32 == c->fp32 && c->graphMatres == c->matres) return ATH_OK;
+  if (c->graphExec) { hipGraphExecDestroy(c->graphExec); c->graphExec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  LaunchTail(c, false, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIP_OK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+  c->graphFp32 = c->fp32;
+  c->graphMatres = c->matres;
+  return ATH_OK;
+}
+
+
+}  // namespace
+
+extern "C" {
+
+const char *ath_last_error(void) { return g_err.c_str(); }
+
+int ath_open(const char *inp_path, int device, ath_ctx **out) {
+  if (!inp_path || !out) return Fail(ATH_ERR_ARG, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return Fail(ATH_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return Fail(ATH_ERR_ARG, "device index out of range");
+  auto c = std::make_unique<ath_ctx>();
+  std::string err = ath::LoadCase(inp_path, &c->hc);
+  if (!err.empty()) return Fail(err.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, err);
+  c->device = device;
+  HIP_OK(hipSetDevice(device));
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (int r = Upload(c.get())) return r;
+  c->cfl = c->hc.Cfl(0);
+  HIP_OK(hipMemcpy(c->dcfl, &c->cfl, sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipDeviceSynchronize());
+  *out = c.release();
+  return ATH_OK;
+}
+
+void ath_close(ath_ctx *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+static void FillInfo(const ath::HostCase &hc, ath_info *o);
+
+int ath_get_info(const ath_ctx *c, ath_info *o) {
+  if (!c || !o) return Fail(ATH_ERR_ARG, "null argument");
+  FillInfo(c->hc, o);
+  return ATH_OK;
+}
+
+int ath_block_dims(const ath_ctx *c, int b, int *ni, int *nj, int *nk) {
+  if (!c || b < 0 || b >= (int)c->blk.size()) return Fail(ATH_ERR_ARG, "bad block index");
+  *ni = c->blk[b].d.ni;
+  *nj = c->blk[b].d.nj;
+  *nk = c->blk[b].d.nk;
+  return ATH_OK;
+}
+
+int ath_set_implicit_precision(ath_ctx *c, int precision) {
+  if (!c || (precision != ATH_IMPLICIT_FP64 && precision != ATH_IMPLICIT_FP32))
+    return Fail(ATH_ERR_ARG, "bad precision");
+  HIP_OK(hipSetDevice(c->device));
+  if (precision == ATH_IMPLICIT_FP32)
+    if (int r = EnsureFp32(c)) return r;
+  c->fp32 = precision;
+  return ATH_OK;
+}
+
+int ath_set_matrix_residual(ath_ctx *c, int on) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  c->matres = on ? 1 : 0;
+  return ATH_OK;
+}
+
+static int CopyState(ath_ctx *c, int b, double *host, bool toHost) {
+  if (!c || !host || b < 0 || b >= (int)c->blk.size()) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const auto &d = c->blk[b].d;
+  std::vector<double> W(5 * (size_t)d.npad);
+  HIP_OK(hipMemcpy(W.data(), d.W, W.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < d.nk; ++k)
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.ni; ++i) {
+        const long p = i + (long)j * d.ni + (long)k * d.ni * d.nj;
+        const long q = (i + d.g) + (long)(j + d.g) * d.NI + (long)(k + d.g) * d.NI * d.NJ;
+        for (int e = 0; e < 5; ++e) {
+          if (toHost) host[5 * p + e] = W[e * d.npad + q];
+          else W[e * d.npad + q] = host[5 * p + e];
+        }
+      }
+  if (!toHost) HIP_OK(hipMemcpy(d.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+  return ATH_OK;
+}
+
+int ath_get_state(ath_ctx *c, int b, double *prim) { return CopyState(c, b, prim, true); }
+int ath_set_state(ath_ctx *c, int b, const double *prim) {
+  return CopyState(c, b, const_cast<double *>(prim), false);
+}
+
+int ath_begin_step(ath_ctx *c, int nn) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  c->cfl = c->hc.Cfl(nn);
+  hipLaunchKernelGGL(ath::KSetScalar, dim3(1), dim3(1), 0, c->stream, c->dcfl, c->cfl);
+  // StoreOldSolution only when the scheme needs U^n
+  // (input::NeedToStoreTimeN, include/input.hpp:174-176)
+  if (c->P.implicit) {
+    const int copyNm1 = (c->P.multilevel && nn == 0) ? 1 : 0;
+    for (auto &br : c->blk)
+      hipLaunchKernelGGL(ath::KBeginStep, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P,
+                         copyNm1);
+  }
+  return CheckLaunch();
+}
+
+int ath_assign_ghost_cells(ath_ctx *c) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  LaunchBoundary(c);
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return ATH_OK;
+}
+
+int ath_calc_residual(ath_ctx *c) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  for (auto &br : c->blk) LaunchResidual(c, br);
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return ATH_OK;
+}
+
+int ath_relax(ath_ctx *c, double *cycle_value) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  if (!c->P.implicit) return Fail(ATH_ERR_ARG, "relax requires an implicit time integration");
+  HIP_OK(hipSetDevice(c->device));
+  if (c->fp32) LaunchRelaxT<float>(c, false, 0);
+  else LaunchRelaxT<double>(c, false, 0);
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (cycle_value) {
+    std::vector<ath::Partial> part(c->nparts);
+    HIP_OK(hipMemcpy(part.data(), c->dpart, part.size() * sizeof(ath::Partial), hipMemcpyDeviceToHost));
+    double s = 0, padded = 0;
+    for (auto &p : part) s += p.mres;
+    for (const auto &br : c->blk) padded += (double)br.d.npad * 5;
+    *cycle_value = c->matres ? s / padded : 0.0;
+  }
+  return ATH_OK;
+}
+
+static int ResetCounterIfFull(ath_ctx *c, int need) {
+  int slot = 0;
+  HIP_OK(hipMemcpyAsync(&slot, c->dcounter, sizeof slot, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (slot + need > c->histCap) {
+    hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+    slot = 0;
+  }
+  return slot;
+}
+
+int ath_update(ath_ctx *c, int mm, ath_norms *norms) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  const int slot = ResetCounterIfFull(c, 1);
+  if (slot < 0) return slot;
+  if (c->fp32) LaunchUpdateT<float>(c);
+  else LaunchUpdateT<double>(c);
+  if (c->P.multilevel && mm == c->hc.nonlinear - 1) LaunchCopyNm1(c);
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  // raw norms; the first-iteration normalisation state is only advanced by
+  // ath_iterate / ath_run, which know the outer step
+  ath_norms n{};
+  const double keep[5] = {c->l2first[0], c->l2first[1], c->l2first[2], c->l2first[3], c->l2first[4]};
+  const int r = ReadNorms(c, slot, 1 << 30, mm, &n);
+  std::memcpy(c->l2first, keep, sizeof keep);
+  if (norms) *norms = n;
+  return r;
+}
+
+int ath_iterate(ath_ctx *c, int nn, int mm, ath_norms *norms) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  HIP_OK(hipSetDevice(c->device));
+  const int slot = ResetCounterIfFull(c, 1);
+  if (slot < 0) return slot;
+  LaunchBoundary(c);
+  for (auto &br : c->blk) LaunchResidual(c, br);
+  LaunchTail(c, false, 0);
+  if (c->P.multilevel && mm == c->hc.nonlinear - 1) LaunchCopyNm1(c);
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  ath_norms n{};
+  const int r = ReadNorms(c, slot, nn, mm, &n);
+  if (norms) *norms = n;
+  return r;
+}
+
+int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
+  if (!c || nsteps < 0) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  const int nl = c->hc.nonlinear;
+  const long iters = (long)nsteps * nl;
+  if (iters == 0) return ATH_OK;
+  if (int r = BuildGraph(c)) return r;
+  const int nflux = (int)c->blk.size();
+  if (c->timing) {
+    const size_t need = (size_t)iters * (2 * nflux + 2) + 2;
+    while (c->evs.size() < need) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      c->evs.push_back(e);
+    }
+  }
+  hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+  if (c->timing) HIP_OK(hipEventRecord(c->evs[0], c->stream));
+  long it = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    const int nn = nn0 + s;
+    if (int r = ath_begin_step(c, nn)) return r;
+    for (int mm = 0; mm < nl; ++mm, ++it) {
+      LaunchBoundary(c);
+      const size_t eb = 2 + (size_t)it * (2 * nflux + 2);
+      for (size_t q = 0; q < c->blk.size(); ++q) {
+        if (c->timing) hipEventRecord(c->evs[eb + 2 * q], c->stream);
+        LaunchResidual(c, c->blk[q]);
+        if (c->timing) hipEventRecord(c->evs[eb + 2 * q + 1], c->stream);
+      }
+      if (c->timing) hipEventRecord(c->evs[eb + 2 * nflux], c->stream);
+      HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
+      if (c->timing) hipEventRecord(c->evs[eb + 2 * nflux + 1], c->stream);
+      if (c->P.multilevel && mm == nl - 1) LaunchCopyNm1(c);
+      if (int r = CheckLaunch()) return r;
+      // the norm history holds histCap iterations; drain it when full
+      if ((it + 1) % c->histCap == 0 && it + 1 < iters) {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (long q = it + 1 - c->histCap; q <= it; ++q) {
+          const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
+          if (qnn < 5 && qmm == 0) ReadNorms(c, (int)(q % c->histCap), qnn, qmm, nullptr);
+        }
+        hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+      }
+    }
+  }
+  if (c->timing) HIP_OK(hipEventRecord(c->evs[1], c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  // advance the PrintResiduals normalisation through the iterations it
+  // depends on, then report the last one
+  const long first = (iters - 1) / c->histCap * c->histCap;
+  int r = ATH_OK;
+  for (long q = first; q < iters; ++q) {
+    const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
+    ath_norms n{};
+    const bool want = (q == iters - 1) || (qnn < 5 && qmm == 0);
+    if (!want) continue;
+    r = ReadNorms(c, (int)(q % c->histCap), qnn, qmm, &n);
+    if (q == iters - 1 && last) *last = n;
+    if (r != ATH_OK) break;
+  }
+  if (c->timing) {
+    ath_timing t{};
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, c->evs[0], c->evs[1]));
+    t.run_ms = ms;
+    for (long q = 0; q < iters; ++q) {
+      const size_t eb = 2 + (size_t)q * (2 * nflux + 2);
+      for (int b = 0; b < nflux; ++b) {
+        HIP_OK(hipEventElapsedTime(&ms, c->evs[eb + 2 * b], c->evs[eb + 2 * b + 1]));
+        t.flux_ms += ms;
+        t.flux_launches++;
+      }
+      HIP_OK(hipEventElapsedTime(&ms, c->evs[eb + 2 * nflux], c->evs[eb + 2 * nflux + 1]));
+      t.sweep_ms += ms;
+      t.sweep_launches++;
+    }
+    c->tm = t;
+  }
+  return r;
+}
+
+int ath_enable_timing(ath_ctx *c, int on) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  c->timing = on ? 1 : 0;
+  return ATH_OK;
+}
+
+int ath_get_timing(const ath_ctx *c, ath_timing *out) {
+  if (!c || !out) return Fail(ATH_ERR_ARG, "null argument");
+  *out = c->tm;
+  return ATH_OK;
+}
+
+static int GetPhys(ath_ctx *c, int b, double *out, int which) {
+  if (!c || !out || b < 0 || b >= (int)c->blk.size()) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const auto &d = c->blk[b].d;
+  const long n = d.ncell;
+  if (which == 0 || which == 4) {  // residual / update: SoA -> AoS
+    std::vector<double> t(5 * (size_t)n);
+    if (which == 0) {
+      HIP_OK(hipMemcpy(t.data(), d.R, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (c->fp32) {
+      std::vector<float> f(5 * (size_t)n);
+      HIP_OK(hipMemcpy(f.data(), d.x32, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < f.size(); ++q) t[q] = f[q];
+    } else {
+      HIP_OK(hipMemcpy(t.data(), d.x, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (long p = 0; p < n; ++p)
+      for (int e = 0; e < 5; ++e) out[5 * p + e] = t[e * n + p];
+    return ATH_OK;
+  }
+  const double *src = which == 1 ? d.dt : which == 2 ? d.sr : d.A;
+  HIP_OK(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+  return ATH_OK;
+}
+
+int ath_get_residual(ath_ctx *c, int b, double *r) { return GetPhys(c, b, r, 0); }
+int ath_get_dt(ath_ctx *c, int b, double *dt) { return GetPhys(c, b, dt, 1); }
+int ath_get_spectral_radius(ath_ctx *c, int b, double *sr) { return GetPhys(c, b, sr, 2); }
+int ath_get_diagonal(ath_ctx *c, int b, double *a) { return GetPhys(c, b, a, 3); }
+int ath_get_update(ath_ctx *c, int b, double *du) { return GetPhys(c, b, du, 4); }
+
+static void FillInfo(const ath::HostCase &hc, ath_info *o) {
+  o->num_blocks = (int)hc.blocks.size();
+  o->num_equations = 5;
+  o->num_ghosts = hc.Ghosts();
+  o->iterations = hc.iterations;
+  o->nonlinear_iterations = hc.nonlinear;
+  o->implicit = hc.Implicit() ? 1 : 0;
+  o->total_cells = hc.totalCells;
+  o->r_ref = hc.rRef;
+  o->t_ref = hc.tRef;
+  o->l_ref = hc.lRef;
+  o->a_ref = hc.aRef;
+  o->gas_constant = hc.gasR;
+  o->gamma = hc.gamma;
+}
+
+int ath_inspect_info(const char *inp_path, ath_info *out) {
+  if (!inp_path || !out) return Fail(ATH_ERR_ARG, "null argument");
+  ath::HostCase hc;
+  const std::string err = ath::LoadCase(inp_path, &hc);
+  if (!err.empty()) return Fail(err.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, err);
+  FillInfo(hc, out);
+  return ATH_OK;
+}
+
+int ath_inspect_field(const char *inp_path, int block, int which, double *out, long count, long *needed) {
+  if (!inp_path || which < 0 || which > 7) return Fail(ATH_ERR_ARG, "bad argument");
+  ath::HostCase hc;
+  const std::string err = ath::LoadCase(inp_path, &hc);
+  if (!err.empty()) return Fail(err.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, err);
+  if (block < 0 || block >= (int)hc.blocks.size()) return Fail(ATH_ERR_ARG, "bad block index");
+  const auto &b = hc.blocks[block];
+  std::vector<double> v;
+  if (which == 0) {
+    for (int k = 0; k < b.nk; ++k)
+      for (int j = 0; j < b.nj; ++j)
+        for (int i = 0; i < b.ni; ++i) v.push_back(b.vol[b.Cell(i, j, k)]);
+  } else if (which <= 3) {
+    v = b.cw[which - 1];
+  } else if (which <= 6) {
+    for (const auto &a : b.fa[which - 4]) v.insert(v.end(), a.begin(), a.end());
+  } else {
+    v = b.W0;
+  }
+  if (needed) *needed = (long)v.size();
+  if (out) {
+    if (count < (long)v.size()) return Fail(ATH_ERR_ARG, "output buffer too small");
+    std::copy(v.begin(), v.end(), out);
+  }
+  return ATH_OK;
+}
+
+int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n, const double *WL,
+                       const double *WR, const double *n, int count, double *F) {
+  if (count <= 0 || !WL || !WR || !n || !F || (flux != 0 && flux != 1)) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(device));
+  ath::Gas g{};
+  g.R = gas_r;
+  g.n = gas_n;
+  const double cp = 0.0 + 1.0 * (gas_r * (gas_n + 1.0)), cv = 0.0 + 1.0 * (gas_r * gas_n);
+  g.gamma = cp / cv;
+  g.cps = gas_r * (gas_n + 1.0);
+  g.cvs = gas_r * gas_n;
+  double *buf = nullptr;
+  const size_t nd = (size_t)count * 18;
+  HIP_OK(hipMalloc(&buf, nd * sizeof(double)));
+  double *dL = buf, *dR = buf + 5 * (size_t)count, *dn = buf + 10 * (size_t)count, *dF = buf + 13 * (size_t)count;
+  hipError_t e = hipMemcpy(dL, WL, 5 * (size_t)count * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dR, WR, 5 * (size_t)count * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dn, n, 3 * (size_t)count * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(ath::KEvalFlux, dim3(Grid(count)), dim3(kBlk), 0, 0, flux, g, dL, dR, dn, count, dF);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(F, dF, 5 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(buf);
+  if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("eval_face_flux: ") + hipGetErrorString(e));
+  return ATH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+/*
+ * aither_amd: MI355X-native implementation of aither's per-nonlinear-iteration
+ * hot path (ghost-cell boundary conditions, face-flux residual, spectral-radius
+ * time step, implicit operator, solution update), behind a plain C ABI.
+ *
+ * All arrays crossing this boundary are host arrays owned by the caller and
+ * copied synchronously; the library keeps no pointer to them after a call
+ * returns.  Device memory is owned by the context.  State arrays are
+ * nondimensional primitive variables [rho, u, v, w, p] per physical cell,
+ * AoS, i fastest then j then k -- the ordering of the reference's
+ * blkMultiArray3d<primitive> (include/multiArray3d.hpp:104-113).
+ *
+ * Every call returns ATH_OK (0) or a negative ATH_ERR_* status; the message of
+ * the last failure on the calling thread is available from ath_last_error().
+ * The library never calls exit().  One context is driven by one host thread
+ * at a time and owns one HIP stream.
+ */
+#ifndef AITHER_AMD_H
+#define AITHER_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATH_OK 0
+#define ATH_ERR_ARG (-1)         /* bad argument or unsupported combination */
+#define ATH_ERR_CONFIG (-2)      /* input file / grid / BC table problem */
+#define ATH_ERR_IO (-3)          /* file could not be opened or read */
+#define ATH_ERR_NOMEM (-4)       /* device or host allocation failed */
+#define ATH_ERR_NONPHYSICAL (-5) /* rho <= 0, p <= 0 or NaN after an update */
+#define ATH_ERR_HIP (-6)         /* HIP runtime error */
+#define ATH_ERR_DEVICE (-7)      /* device-side check failed (e.g. spin limit) */
+
+#define ATH_MAX_EQ 8
+
+typedef struct ath_ctx ath_ctx;
+
+/* case summary (input::ReadInput results plus derived quantities) */
+typedef struct {
+  int num_blocks;
+  int num_equations;     /* input::NumEquations, src/input.cpp:655-671 */
+  int num_ghosts;        /* input::NumberGhostLayers, src/input.cpp:1127-1143 */
+  int iterations;        /* "iterations" */
+  int nonlinear_iterations;
+  int implicit;          /* input::IsImplicit, src/input.cpp:674-682 */
+  double total_cells;
+  double r_ref, t_ref, l_ref, a_ref;
+  double gas_constant;   /* nondimensional R of the single species */
+  double gamma;
+} ath_info;
+
+/* per-iteration norms (what src/main.cpp:252-275 reduces and prints) */
+typedef struct {
+  int num_equations;
+  double l2_sq[ATH_MAX_EQ];    /* sum over cells of R^2 per equation */
+  double l2[ATH_MAX_EQ];       /* sqrt of l2_sq (residual::SquareRoot) */
+  double normalized[ATH_MAX_EQ];/* PrintResiduals values, [0] = mass */
+  double linf;                 /* resid::Linf (signed max, first wins) */
+  int linf_block, linf_i, linf_j, linf_k, linf_eqn;
+  double matrix_resid;         /* sqrt(CycleAtLevel/(cells*neq)) */
+  double cfl;
+} ath_norms;
+
+/* precision of the implicit operator (D^-1, off-diagonals, du) */
+#define ATH_IMPLICIT_FP64 0
+#define ATH_IMPLICIT_FP32 1
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Read a reference-format input file (.inp) and its binary Plot3D grid
+ * (gridName.xyz next to it), build FP64 geometry on the host and upload the
+ * case to device `device`.  Replaces, for the hot path, the sequence
+ * input::ReadInput (src/input.cpp:167-643), ReadP3dGrid
+ * (src/plot3d.cpp:363-445), the procBlock constructor and InitializeStates
+ * (src/procBlock.cpp:59-174, 280-355), AssignGhostCellsGeom
+ * (src/procBlock.cpp:2160-2268) and CalcCellWidths (src/procBlock.cpp:6397)
+ * as called from src/main.cpp:101-164. */
+int ath_open(const char *inp_path, int device, ath_ctx **out);
+void ath_close(ath_ctx *ctx);
+const char *ath_last_error(void);
+
+int ath_get_info(const ath_ctx *ctx, ath_info *out);
+int ath_block_dims(const ath_ctx *ctx, int block, int *ni, int *nj, int *nk);
+
+/* implicit-operator precision (default ATH_IMPLICIT_FP64).  No reference
+ * counterpart: the reference is FP64 throughout. */
+int ath_set_implicit_precision(ath_ctx *ctx, int precision);
+/* compute the matrix residual each iteration (default 1), as
+ * linearSolver::Residual does (src/linearSolver.cpp:95-109) */
+int ath_set_matrix_residual(ath_ctx *ctx, int on);
+
+/* state access: procBlock::States / InsertBlock on physical cells */
+int ath_get_state(ath_ctx *ctx, int block, double *prim);
+int ath_set_state(ath_ctx *ctx, int block, const double *prim);
+
+/* ---- hot path ---------------------------------------------------------- */
+
+/* input::CalcCFL (src/input.cpp:647-649) + mgSolution::StoreOldSolution
+ * (src/mgSolution.cpp:103-114): called once per outer step nn. */
+int ath_begin_step(ath_ctx *ctx, int nn);
+
+/* One nonlinear iteration: mgSolution::Iterate (src/mgSolution.cpp:246-269)
+ * followed by the norm reductions and PrintResiduals normalisation of
+ * src/main.cpp:252-275 / src/output.cpp:1028-1063.  `norms` may be NULL. */
+int ath_iterate(ath_ctx *ctx, int nn, int mm, ath_norms *norms);
+
+/* The main loop of src/main.cpp:231-302 for outer steps [nn0, nn0+nsteps),
+ * each with the input's nonlinear iterations, replayed from a captured HIP
+ * graph.  Norms of the last iteration go to `last` (may be NULL). */
+int ath_run(ath_ctx *ctx, int nn0, int nsteps, ath_norms *last);
+
+/* ---- granular steps (gridLevel / procBlock mirror) ---------------------- */
+
+/* gridLevel::GetBoundaryConditions (src/gridLevel.cpp:287-319) */
+int ath_assign_ghost_cells(ath_ctx *ctx);
+/* gridLevel::CalcResidual + CalcTimeStep + linearSolver::AddDiagonalTerms +
+ * Invert (src/gridLevel.cpp:372-406): one fused launch per block */
+int ath_calc_residual(ath_ctx *ctx);
+/* linearSolver::InitializeMatrixUpdate + Relax (src/linearSolver.cpp:111-144,
+ * 430-470, 509-535); returns the matrix residual (CycleAtLevel value) */
+int ath_relax(ath_ctx *ctx, double *cycle_value);
+/* gridLevel::UpdateBlocks + ResetDiagonal (src/gridLevel.cpp:418-431) */
+int ath_update(ath_ctx *ctx, int mm, ath_norms *norms);
+
+/* field getters (physical cells, i fastest): procBlock::Residual, Dt,
+ * SpectralRadius; linearSolver A (diagonal before inversion) and X (du) */
+int ath_get_residual(ath_ctx *ctx, int block, double *r);     /* neq/cell */
+int ath_get_dt(ath_ctx *ctx, int block, double *dt);
+int ath_get_spectral_radius(ath_ctx *ctx, int block, double *sr);
+int ath_get_diagonal(ath_ctx *ctx, int block, double *a);
+int ath_get_update(ath_ctx *ctx, int block, double *du);      /* neq/cell */
+
+/* ---- timing of the last ath_run (HIP events on the context's stream) ---- */
+typedef struct {
+  double run_ms;            /* whole ath_run */
+  double flux_ms;           /* summed duration of the fused flux kernels */
+  int flux_launches;
+  double sweep_ms;          /* summed LU-SGS / DP-LUR sweep kernels */
+  int sweep_launches;
+} ath_timing;
+int ath_enable_timing(ath_ctx *ctx, int on);
+int ath_get_timing(const ath_ctx *ctx, ath_timing *out);
+
+/* ---- host-only inspection (no device needed) --------------------------- */
+/* Parse the case and build its host geometry exactly as ath_open does, then
+ * return one field of block `block` (physical cells unless noted, i fastest):
+ * which = 0 volume, 1..3 cell width in i/j/k (padded, ghosts included),
+ * 4..6 face vectors in i/j/k (padded face arrays, 4 doubles per face: unit
+ * normal and magnitude), 7 initial primitive state (5 per cell).  `count` is
+ * the capacity of `out` in doubles; the needed size is returned in *needed. */
+int ath_inspect_info(const char *inp_path, ath_info *out);
+int ath_inspect_field(const char *inp_path, int block, int which, double *out,
+                      long count, long *needed);
+
+/* ---- kernel-level entry points used by the parity tests ---------------- */
+/* Evaluate the device inviscid face flux for `count` faces.  flux: 0 Roe
+ * (include/inviscidFlux.hpp:259-382), 1 AUSMPW+ (:395-481).  WL, WR: 5 per
+ * face, n: 3 per face (unit normal), F: 5 per face.  Gas: nondimensional R
+ * and n = Cv/R. */
+int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n,
+                       const double *WL, const double *WR, const double *n,
+                       int count, double *F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
