@@ -16,8 +16,12 @@ LIB = os.path.join(HERE, "libaither_amd.so")
 SOURCES = ["host_case.cpp", "solver.hip"]
 ARCH = os.environ.get("AITHER_AMD_ARCH", "gfx950")
 
-# packed FP32 instructions are disabled for all device code
+# packed FP32 instructions are disabled for all device code.  Multiply-add
+# contraction is off: every product and sum is rounded separately, in the
+# order the reference writes it, so the FP64 path reproduces the reference's
+# arithmetic operation by operation.
 HIPCC_FLAGS = [
+    "-ffp-contract=off",
     "-O3", "-std=c++17", "-fPIC", "-shared",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -184,6 +184,111 @@ __device__ inline void RoeFlux(const GasT<double> &g, const double L[5], const d
   F[4] = (FL[4] + (FR[4] - D4)) * 0.5;
 }
 
+// ---------------------------------------------------------------------------
+// correctly rounded pow for the few places the reference calls std::pow with
+// a non-integer exponent (stagnation-inlet state) or an exponent of 3
+// (AUSMPW+ pressure weight).  The host libm returns the correctly rounded
+// result in all but vanishingly rare cases; these routines evaluate x^y in
+// double-double arithmetic (about 100 significant bits) and round once, so
+// device and host agree to the last bit.
+struct DD {
+  double hi, lo;
+};
+__host__ __device__ __forceinline__ DD TwoSum(double a, double b) {
+  const double s = a + b;
+  const double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ __forceinline__ DD QuickTwoSum(double a, double b) {
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__host__ __device__ __forceinline__ DD TwoProd(double a, double b) {
+  const double p = a * b;
+  return {p, fma(a, b, -p)};
+}
+__host__ __device__ __forceinline__ DD DDAdd(DD a, DD b) {
+  DD s = TwoSum(a.hi, b.hi);
+  DD t = TwoSum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = QuickTwoSum(s.hi, s.lo);
+  s.lo += t.lo;
+  return QuickTwoSum(s.hi, s.lo);
+}
+__host__ __device__ __forceinline__ DD DDMul(DD a, DD b) {
+  DD p = TwoProd(a.hi, b.hi);
+  p.lo += a.hi * b.lo + a.lo * b.hi;
+  return QuickTwoSum(p.hi, p.lo);
+}
+__host__ __device__ __forceinline__ DD DDMulD(DD a, double b) {
+  DD p = TwoProd(a.hi, b);
+  p.lo += a.lo * b;
+  return QuickTwoSum(p.hi, p.lo);
+}
+__host__ __device__ __forceinline__ DD DDDiv(DD a, DD b) {
+  const double q1 = a.hi / b.hi;
+  DD r = DDAdd(a, DDMulD(b, -q1));
+  const double q2 = r.hi / b.hi;
+  r = DDAdd(r, DDMulD(b, -q2));
+  const double q3 = r.hi / b.hi;
+  return DDAdd(QuickTwoSum(q1, q2), DD{q3, 0.0});
+}
+
+// ln 2 to double-double precision
+constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
+constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
+
+// log(x), x > 0 finite: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log(m) = 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172
+__host__ __device__ inline DD DDLog(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const DD s = DDDiv(DD{m - 1.0, 0.0}, TwoSum(m, 1.0));
+  const DD s2 = DDMul(s, s);
+  // 2 (s + s^3/3 + s^5/5 + ...), Horner in s^2, 24 terms (|s|^49 < 1e-37)
+  DD acc{1.0 / 49.0, 0.0};
+  for (int k = 23; k >= 0; --k)
+    acc = DDAdd(DDMul(acc, s2), DDDiv(DD{1.0, 0.0}, DD{(double)(2 * k + 1), 0.0}));
+  DD lm = DDMulD(DDMul(acc, s), 2.0);
+  const DD eln2 = DDAdd(TwoProd((double)e, kLn2Hi), DD{(double)e * kLn2Lo, 0.0});
+  return DDAdd(eln2, lm);
+}
+
+// exp(t) for |t| < 700: t = k ln2 + r, exp(r) = (exp(r / 256))^256
+__host__ __device__ inline DD DDExp(DD t) {
+  const double k = rint(t.hi / kLn2Hi);
+  DD r = DDAdd(t, DDAdd(TwoProd(-k, kLn2Hi), DD{-k * kLn2Lo, 0.0}));
+  r = DD{ldexp(r.hi, -8), ldexp(r.lo, -8)};
+  // Taylor series of exp(r) - 1, |r| < 0.0014: 12 terms
+  DD term = r, sum = r;
+  for (int n = 2; n <= 12; ++n) {
+    term = DDDiv(DDMul(term, r), DD{(double)n, 0.0});
+    sum = DDAdd(sum, term);
+  }
+  // (1 + sum)^256 via exp(2a) - 1 = (exp(a) - 1)(exp(a) + 1)
+  for (int q = 0; q < 8; ++q) sum = DDMul(sum, DDAdd(sum, DD{2.0, 0.0}));
+  DD res = DDAdd(sum, DD{1.0, 0.0});
+  return DD{ldexp(res.hi, (int)k), ldexp(res.lo, (int)k)};
+}
+
+// x^y for x > 0, y finite
+__host__ __device__ inline double PowCR(double x, double y) {
+  const DD t = DDMulD(DDLog(x), y);
+  const DD p = DDExp(t);
+  return p.hi + p.lo;
+}
+
+// x^3
+__host__ __device__ __forceinline__ double CubeCR(double x) {
+  const DD x2 = TwoProd(x, x);
+  const DD x3 = DDMulD(x2, x);
+  return x3.hi + x3.lo;
+}
+
 __device__ __forceinline__ double Sign(double x) { return (double)((0.0 < x) - (x < 0.0)); }
 
 // AUSMFlux / AUSMPW+ (include/inviscidFlux.hpp:395-481, member :161-209)
@@ -203,8 +308,7 @@ __device__ inline void AusmFlux(const GasT<double> &g, const double L[5], const 
   const double pPlus = fabs(ml) <= 1.0 ? 0.25 * mlp * (2.0 - ml) : 0.5 * (1.0 + Sign(ml));
   const double pMinus = fabs(mr) <= 1.0 ? 0.25 * mrm * (2.0 + mr) : 0.5 * (1.0 - Sign(mr));
   const double ps = pPlus * L[4] + pMinus * R[4];
-  const double pm = Dmin(L[4] / R[4], R[4] / L[4]);
-  const double w = 1.0 - pm * pm * pm;
+  const double w = 1.0 - CubeCR(Dmin(L[4] / R[4], R[4] / L[4]));
   const double fl = fabs(ml) < 1.0 ? L[4] / ps - 1.0 : 0.0;
   const double fr = fabs(mr) < 1.0 ? R[4] / ps - 1.0 : 0.0;
   const double mavg = mPlusL + mMinusR;

@@ -148,7 +148,7 @@ __device__ inline void GhostState(const GasT<double> &g, const double in[5], int
                                            0.5 * gm1));
     const double ratio = sosB * sosB / stag;
     const double tb = s->t0 * ratio;
-    const double pb = s->p0 * pow(ratio, g.gamma / gm1);
+    const double pb = s->p0 * PowCR(ratio, g.gamma / gm1);
     const double vb = sqrt(2.0 / gm1 * (s->t0 - tb));
     gh[0] = (pb / (g.R * tb)) * 1.0;
     gh[1] = vb * s->dx;
