@@ -88,6 +88,7 @@ _SIGS = {
                        ctypes.POINTER(ctypes.c_int)],
     "ath_set_implicit_precision": [_P, ctypes.c_int],
     "ath_set_matrix_residual": [_P, ctypes.c_int],
+    "ath_set_sweep_ordering": [_P, ctypes.c_int],
     "ath_get_state": [_P, ctypes.c_int, _D],
     "ath_set_state": [_P, ctypes.c_int, _D],
     "ath_begin_step": [_P, ctypes.c_int],
@@ -212,6 +213,12 @@ class Solver:
     # --- configuration
     def set_implicit_precision(self, p):
         _check(self._lib.ath_set_implicit_precision(self._h, p))
+
+    TILED = 0
+    HYPERPLANE = 1
+
+    def set_sweep_ordering(self, ordering):
+        _check(self._lib.ath_set_sweep_ordering(self._h, ordering))
 
     def set_matrix_residual(self, on):
         _check(self._lib.ath_set_matrix_residual(self._h, 1 if on else 0))

@@ -715,3 +715,265 @@ __global__ void __launch_bounds__(256) KEvalFlux(int flux, Gas gas, const double
 }
 
 }  // namespace ath
+
+namespace ath {
+
+// ---------------------------------------------------------------------------
+// tiled, dataflow-synchronised LU-SGS sweep (lusgs::LUSGS_Forward/Backward,
+// src/linearSolver.cpp:341-428).
+//
+// The hyperplane order of HyperplaneReorder (src/utility.cpp:377-398) is one
+// topological order of the Gauss-Seidel dependencies (a cell needs its lower
+// neighbours in the forward sweep, its upper neighbours in the backward
+// sweep); any other topological order gives the same result bit for bit.
+// This kernel uses a blocked one: the block is cut into kLT = 8x8x4 tiles,
+// tiles are taken in tile-hyperplane order from an atomic ticket by
+// persistent workgroups, and a tile starts once the (at most three) tiles it
+// depends on have published their du.  Inside a tile the workgroup walks the
+// 18 intra-tile hyperplanes with LDS-resident state, du, right-hand side and
+// face vectors.  Cross-tile visibility uses agent-scope release/acquire on a
+// per-tile flag carrying the sweep's epoch; every wait is bounded and reports
+// through `err` instead of hanging.
+constexpr int kLTI = 8, kLTJ = 8, kLTK = 4;
+constexpr int kLT = kLTI * kLTJ * kLTK;  // 256 cells, one per thread
+constexpr int kLTPlanes = kLTI + kLTJ + kLTK - 2;
+constexpr int kHaloI = kLTJ * kLTK, kHaloJ = kLTI * kLTK, kHaloK = kLTI * kLTJ;
+constexpr int kHalo = kHaloI + kHaloJ + kHaloK;
+constexpr long kSpinLimit = 1L << 20;
+
+struct SweepCtl {
+  int *ticket;      // reset to 0 before each sweep
+  int *epoch;       // incremented before each sweep
+  int *flags;       // one per tile, holds the epoch of its last publication
+  int *err;         // bit 1: a dependency wait exceeded the spin limit
+  const int4 *tiles;  // (a, b, c, -) in tile-hyperplane order
+  int ntiles, ta, tb, tc;  // tile count and tile grid dims
+};
+
+__global__ void KSweepPrologue(SweepCtl s) {
+  *s.ticket = 0;
+  *s.epoch = *s.epoch + 1;
+}
+
+__device__ __forceinline__ int TileId(const SweepCtl &s, int a, int b, int c) {
+  return a + s.ta * (b + s.tb * c);
+}
+
+// off-diagonal product of one neighbour: state/du from LDS or global
+template <typename T>
+__device__ __forceinline__ void NbOffDiag(const GasT<T> &g, const double *Wnb, const T *du,
+                                          double4 a, bool positive, T out[5]) {
+  T W[5], d[5];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    W[e] = T(Wnb[e]);
+    d[e] = du[e];
+  }
+  ScalarOffDiag<T>(g, W, d, T(a.x), T(a.y), T(a.z), T(a.w), positive, out);
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl s, int full) {
+  __shared__ double sW[5][kLT];
+  __shared__ T sX[5][kLT];
+  __shared__ double sFa[3][4][kLT];  // face shared with the dependency neighbour
+  __shared__ double sHW[5][kHalo];   // halo (outside-tile dependency) state
+  __shared__ T sHX[5][kHalo];        // halo du
+  __shared__ int sTicket;
+  __shared__ int sEpoch;
+
+  const int tid = threadIdx.x;
+  const int li = tid % kLTI, lj = (tid / kLTI) % kLTJ, lk = tid / (kLTI * kLTJ);
+  const GasT<T> g = Cast<T>(P.gas);
+  T *x = OpBuf<T>::X(b);
+  const T *ainvG = OpBuf<T>::Ainv(b);
+  const int sgn = FWD ? -1 : 1;  // direction of the dependency neighbours
+
+  // Thread 0 owns the ticket and the flag.  Exactly one `tid == 0` block
+  // sits between two barriers at the end of each pass (publish the tile,
+  // fetch the next ticket); with separate blocks on both sides of the loop's
+  // back edge the compiler may merge them into a lane-0-only path around the
+  // loop header, leaving the other lanes to pass the header barrier with a
+  // stale ticket.
+  if (tid == 0) {
+    sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sEpoch = __hip_atomic_load(s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const int epoch = sEpoch;
+  for (;;) {
+    const int t = sTicket;
+    if (t >= s.ntiles) break;
+    const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
+    const int i = tl.x * kLTI + li, j = tl.y * kLTJ + lj, k = tl.z * kLTK + lk;
+    const bool valid = i < b.ni && j < b.nj && k < b.nk;
+    const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
+
+    // --- static data (independent of du) before the dependency wait; the
+    // right-hand side and D^-1 of the thread's own cell stay in registers
+    double rb[5] = {0, 0, 0, 0, 0};
+    T ainv = T(0);
+    if (valid) {
+      const long c = PadIdx(b, i, j, k);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sW[e][tid] = b.W[e * b.npad + c];
+      if (FWD || full) Rhs(b, P, p, c, rb);
+      ainv = ainvG[p];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        // forward: the cell's lower face; backward: its upper face
+        const int fi = i + (FWD ? 0 : (d == 0)), fj = j + (FWD ? 0 : (d == 1)),
+                  fk = k + (FWD ? 0 : (d == 2));
+        const int n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
+        const int f3 = d == 0 ? fi : (d == 1 ? fj : fk);
+        double4 a = make_double4(0, 0, 0, 0);
+        if (f3 > 0 && f3 < n3) a = b.fa[d][FaceIdx(b, d, fi, fj, fk)];
+        sFa[d][0][tid] = a.x;
+        sFa[d][1][tid] = a.y;
+        sFa[d][2][tid] = a.z;
+        sFa[d][3][tid] = a.w;
+      }
+    }
+    // halo state: dependency neighbours just outside the tile
+    {
+      const int di = FWD ? -1 : kLTI, dj = FWD ? -1 : kLTJ, dk = FWD ? -1 : kLTK;
+      if (tid < kHalo) {
+        int hi, hj, hk;
+        if (tid < kHaloI) { hi = di; hj = tid % kLTJ; hk = tid / kLTJ; }
+        else if (tid < kHaloI + kHaloJ) { const int q = tid - kHaloI; hi = q % kLTI; hj = dj; hk = q / kLTI; }
+        else { const int q = tid - kHaloI - kHaloJ; hi = q % kLTI; hj = q / kLTI; hk = dk; }
+        const int gi = tl.x * kLTI + hi, gj = tl.y * kLTJ + hj, gk = tl.z * kLTK + hk;
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+          const long c = PadIdx(b, gi, gj, gk);
+#pragma unroll
+          for (int e = 0; e < 5; ++e) sHW[e][tid] = b.W[e * b.npad + c];
+        }
+      }
+    }
+
+    // --- wait for the tiles this one depends on
+    if (tid == 0) {
+      const int deps[3][3] = {{tl.x + sgn, tl.y, tl.z}, {tl.x, tl.y + sgn, tl.z}, {tl.x, tl.y, tl.z + sgn}};
+      for (int q = 0; q < 3; ++q) {
+        const int a = deps[q][0], bb = deps[q][1], cc = deps[q][2];
+        if (a < 0 || bb < 0 || cc < 0 || a >= s.ta || bb >= s.tb || cc >= s.tc) continue;
+        const int id = TileId(s, a, bb, cc);
+        long spins = 0;
+        while (__hip_atomic_load(&s.flags[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+          // once any wait of the launch has timed out, stop waiting at all
+          if ((++spins & 255) == 0 &&
+              (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2))
+            break;
+          if (spins > kSpinLimit) {
+            __hip_atomic_fetch_or(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // halo du, published by the dependency tiles
+    {
+      const int di = FWD ? -1 : kLTI, dj = FWD ? -1 : kLTJ, dk = FWD ? -1 : kLTK;
+      if (tid < kHalo) {
+        int hi, hj, hk;
+        if (tid < kHaloI) { hi = di; hj = tid % kLTJ; hk = tid / kLTJ; }
+        else if (tid < kHaloI + kHaloJ) { const int q = tid - kHaloI; hi = q % kLTI; hj = dj; hk = q / kLTI; }
+        else { const int q = tid - kHaloI - kHaloJ; hi = q % kLTI; hj = q / kLTI; hk = dk; }
+        const int gi = tl.x * kLTI + hi, gj = tl.y * kLTJ + hj, gk = tl.z * kLTK + hk;
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+          const long q = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
+#pragma unroll
+          for (int e = 0; e < 5; ++e) sHX[e][tid] = x[e * b.ncell + q];
+        }
+      }
+    }
+    if (!FWD && valid) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sX[e][tid] = x[e * b.ncell + p];  // forward result
+    }
+    __syncthreads();
+
+    // --- intra-tile hyperplanes
+    const int plane = li + lj + lk;
+    for (int st = 0; st < kLTPlanes; ++st) {
+      const int pl = FWD ? st : kLTPlanes - 1 - st;
+      if (valid && plane == pl) {
+        T D[5];  // dependency-side sum: L (forward) or U (backward)
+#pragma unroll
+        for (int e = 0; e < 5; ++e) D[e] = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+          if (gi < 0 || gj < 0 || gk < 0 || gi >= b.ni || gj >= b.nj || gk >= b.nk) continue;
+          const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
+          const int L3 = d == 0 ? kLTI : (d == 1 ? kLTJ : kLTK);
+          const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
+          double Wn[5];
+          T dn[5];
+          if (inTile) {
+            const int nt = tid + sgn * (d == 0 ? 1 : (d == 1 ? kLTI : kLTI * kLTJ));
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+              Wn[e] = sW[e][nt];
+              dn[e] = sX[e][nt];
+            }
+          } else {
+            const int h = d == 0 ? lj + kLTJ * lk : (d == 1 ? kHaloI + li + kLTI * lk : kHaloI + kHaloJ + li + kLTI * lj);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+              Wn[e] = sHW[e][h];
+              dn[e] = sHX[e][h];
+            }
+          }
+          const double4 a = make_double4(sFa[d][0][tid], sFa[d][1][tid], sFa[d][2][tid], sFa[d][3][tid]);
+          T o[5];
+          NbOffDiag<T>(g, Wn, dn, a, FWD, o);
+#pragma unroll
+          for (int e = 0; e < 5; ++e) D[e] += o[e];
+        }
+        T xn[5];
+        if (full) {
+          // the other side, from the previous sweep's values in global memory
+          T O[5];
+          OffDiagSum<T>(b, g, x, i, j, k, FWD, O);
+          if (FWD) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) xn[e] = (T(rb[e]) + (D[e] - O[e])) * ainv;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) xn[e] = ((T(rb[e]) + O[e]) - D[e]) * ainv;
+          }
+        } else if (FWD) {
+#pragma unroll
+          for (int e = 0; e < 5; ++e) xn[e] = (T(rb[e]) + D[e]) * ainv;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 5; ++e) xn[e] = sX[e][tid] - D[e] * ainv;
+        }
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+          sX[e][tid] = xn[e];
+          x[e * b.ncell + p] = xn[e];
+        }
+      }
+      __syncthreads();
+    }
+
+    // --- publish: every thread releases its stores, then one flag store;
+    // the same thread takes the next ticket
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_store(&s.flags[TileId(s, tl.x, tl.y, tl.z)], epoch, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace ath

@@ -38,6 +38,8 @@ struct BlockRes {
   std::vector<ath::BcTask> tasks;
   ath::BcTask *dtasks = nullptr;
   long bcTotal = 0;
+  ath::SweepCtl sweep{};  // tiled LU-SGS dataflow state
+  int sweepGrid = 0;
 };
 
 }  // namespace
@@ -59,12 +61,13 @@ struct ath_ctx {
   int *dflag = nullptr;
   int fp32 = 0;
   int matres = 1;
+  int planeSweep = 1;  // 1: one launch per hyperplane (reference ordering)
   double cfl = 0;
   double l2first[ATH_MAX_EQ]{};
   // graph of one nonlinear iteration (after the boundary pass)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
-  int graphFp32 = -1, graphMatres = -1;
+  int graphFp32 = -1, graphMatres = -1, graphPlane = -1;
   // timing
   int timing = 0;
   std::vector<hipEvent_t> evs;
@@ -205,6 +208,34 @@ int Upload(ath_ctx *c) {
     HyperplaneOrder(h.ni, h.nj, h.nk, &cells, &br.planeOff);
     if (int r = c->Alloc(&br.cells, cells.size())) return r;
     HIP_OK(hipMemcpy(br.cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    // tiled sweep: tiles in tile-hyperplane order, flags, ticket, epoch
+    {
+      SweepCtl &sc = br.sweep;
+      sc.ta = (h.ni + kLTI - 1) / kLTI;
+      sc.tb = (h.nj + kLTJ - 1) / kLTJ;
+      sc.tc = (h.nk + kLTK - 1) / kLTK;
+      sc.ntiles = sc.ta * sc.tb * sc.tc;
+      std::vector<int4> tl;
+      tl.reserve(sc.ntiles);
+      for (int pl = 0; pl < sc.ta + sc.tb + sc.tc - 2; ++pl)
+        for (int cc = 0; cc < sc.tc; ++cc)
+          for (int bb = 0; bb < sc.tb; ++bb)
+            for (int aa = 0; aa < sc.ta; ++aa)
+              if (aa + bb + cc == pl) tl.push_back(make_int4(aa, bb, cc, 0));
+      int4 *dtl = nullptr;
+      if (int r = c->Alloc(&dtl, tl.size())) return r;
+      HIP_OK(hipMemcpy(dtl, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice));
+      sc.tiles = dtl;
+      if (int r = c->Alloc(&sc.flags, sc.ntiles)) return r;
+      if (int r = c->Alloc(&sc.ticket, 2)) return r;
+      HIP_OK(hipMemset(sc.flags, 0, sc.ntiles * sizeof(int)));
+      HIP_OK(hipMemset(sc.ticket, 0, 2 * sizeof(int)));
+      sc.epoch = sc.ticket + 1;
+      sc.err = c->dflag;
+      hipDeviceProp_t prop;
+      HIP_OK(hipGetDeviceProperties(&prop, c->device));
+      br.sweepGrid = std::max(1, std::min(sc.ntiles, 2 * prop.multiProcessorCount));
+    }
     // reduction partials
     br.nparts = Grid(nc, kRedThreads);
     br.partBase = partBase;
@@ -322,7 +353,18 @@ void LaunchRelaxT(ath_ctx *c, bool timed, int evBase) {
   if (timed) hipEventRecord(c->evs[evBase], c->stream);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
-    if (hc.solver == ath::Solver::LUSGS) {
+    if (hc.solver == ath::Solver::LUSGS && !c->planeSweep) {
+      for (auto &br : c->blk) {
+        hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
+        hipLaunchKernelGGL((ath::KLusgsTile<T, true>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
+                           c->P, br.sweep, full);
+      }
+      for (auto &br : c->blk) {
+        hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
+        hipLaunchKernelGGL((ath::KLusgsTile<T, false>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
+                           c->P, br.sweep, full);
+      }
+    } else if (hc.solver == ath::Solver::LUSGS) {
       for (auto &br : c->blk) {
         const int np = (int)br.planeOff.size() - 1;
         for (int p = 0; p < np; ++p) {
@@ -432,12 +474,14 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
   }
   for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (c->l2first[e] + eps);
   if (out) *out = n;
-  if (flag) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
+  if (flag & 2) return Fail(ATH_ERR_DEVICE, "a sweep dependency wait exceeded its spin limit");
+  if (flag & 1) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
   return ATH_OK;
 }
 
 int BuildGraph(ath_ctx *c) {
-  if (c->graphExec && c->graphFp32 == c->fp32 && c->graphMatres == c->matres) return ATH_OK;
+  if (c->graphExec && c->graphFp32 == c->fp32 && c->graphMatres == c->matres && c->graphPlane == c->planeSweep)
+    return ATH_OK;
   if (c->graphExec) { hipGraphExecDestroy(c->graphExec); c->graphExec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -449,6 +493,7 @@ int BuildGraph(ath_ctx *c) {
   HIP_OK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
   c->graphFp32 = c->fp32;
   c->graphMatres = c->matres;
+  c->graphPlane = c->planeSweep;
   return ATH_OK;
 }
 
@@ -509,6 +554,13 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
   if (precision == ATH_IMPLICIT_FP32)
     if (int r = EnsureFp32(c)) return r;
   c->fp32 = precision;
+  return ATH_OK;
+}
+
+int ath_set_sweep_ordering(ath_ctx *c, int ordering) {
+  if (!c || (ordering != ATH_SWEEP_TILED && ordering != ATH_SWEEP_HYPERPLANE))
+    return Fail(ATH_ERR_ARG, "bad sweep ordering");
+  c->planeSweep = ordering == ATH_SWEEP_HYPERPLANE ? 1 : 0;
   return ATH_OK;
 }
 
