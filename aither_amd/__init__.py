@@ -216,6 +216,7 @@ class Solver:
 
     TILED = 0
     HYPERPLANE = 1
+    PENCIL = 2
 
     def set_sweep_ordering(self, ordering):
         _check(self._lib.ath_set_sweep_ordering(self._h, ordering))

@@ -326,25 +326,37 @@ __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
 // b = -R/theta + forcing + SolDeltaNm1 - SolDeltaMmN
 // (src/linearSolver.cpp:360-378, src/procBlock.cpp:1010-1034); forcing is
 // zero on the single grid level
-__device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, long c,
-                                    double out[5]) {
+// the same from values already loaded (cNm1 is read only when multilevel)
+__device__ __forceinline__ void RhsFrom(const Params &P, const double W[5], double vol, double dt,
+                                        const double R[5], const double cN[5], const double cNm1[5],
+                                        double out[5]) {
 #pragma clang fp contract(off)
   const GasT<double> g = Cast<double>(P.gas);
-  const double W[5] = {b.W[c], b.W[b.npad + c], b.W[2 * b.npad + c], b.W[3 * b.npad + c],
-                       b.W[4 * b.npad + c]};
   double U[5];
   PrimToCons(g, W, U);
-  const double vol = b.vol[p], dt = b.dt[p];
   const double coeff = (vol * (1.0 + P.zeta)) / (dt * P.theta);
   const double thetaInv = 1.0 / P.theta;
   const double c2 = P.multilevel ? (vol * P.zeta) / (dt * P.theta) : 0.0;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
-    const double nm1 = P.multilevel ? c2 * (b.consN[e * b.ncell + p] - b.consNm1[e * b.ncell + p]) : 0.0;
-    const double mmn = coeff * (U[e] - b.consN[e * b.ncell + p]);
-    double v = -thetaInv * b.R[e * b.ncell + p];
+    const double nm1 = P.multilevel ? c2 * (cN[e] - cNm1[e]) : 0.0;
+    const double mmn = coeff * (U[e] - cN[e]);
+    double v = -thetaInv * R[e];
     out[e] = (v + nm1) - mmn;
   }
+}
+
+__device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, long c,
+                                    double out[5]) {
+  double W[5], R[5], cN[5], cNm1[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    W[e] = b.W[e * b.npad + c];
+    R[e] = b.R[e * b.ncell + p];
+    cN[e] = b.consN[e * b.ncell + p];
+    if (P.multilevel) cNm1[e] = b.consNm1[e * b.ncell + p];
+  }
+  RhsFrom(P, W, b.vol[p], b.dt[p], R, cN, cNm1, out);
 }
 
 template <typename T>
@@ -973,6 +985,373 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
       sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pencil-wavefront LU-SGS sweep.
+//
+// The block is cut into kPW x kPW columns ("pencils") that run the full k
+// extent.  One workgroup of five waves owns a pencil; lane l of every wave
+// stands for column (li, lj) = (l % kPW, l / kPW).  At pencil step s the lanes
+// update the cells of local hyperplane li + lj + k = s (forward; mirrored for
+// the backward sweep), so a pencil walks nk + 2 kPW - 2 steps:
+//
+//  * waves 0, 1, 2 evaluate the dependency-side off-diagonal product of the
+//    i, j and k neighbour (and, when the sweep also needs it, the other side's
+//    product from the previous sweep's du);
+//  * wave 3 sums the products in the order of ImplicitLower/ImplicitUpper,
+//    writes du and publishes the pencil's progress;
+//  * wave 4 loads the right-hand side and D^-1 and stages them in LDS.
+//
+// Everything that does not depend on this sweep's du (neighbour states, face
+// vectors, right-hand side, D^-1, du across a pencil face) is loaded two
+// steps ahead into registers, so the step-to-step chain is LDS, barriers and
+// arithmetic.  In-pencil neighbours come from LDS (the previous step's du).
+// The neighbour across a pencil face comes from global memory once that
+// pencil's progress counter covers it; the i- and j-neighbour pencils run at
+// least kPW steps ahead, so a 64^3 block's dependency chain stays at the
+// 3*64-2 hyperplanes of HyperplaneReorder plus 14 pencil handoffs.
+//
+// Progress: wave 3 stores du with plain stores, then an agent-scope release
+// fence, then the step count with an agent-scope store.  A consumer reads the
+// count with agent-scope loads and issues its loads of that du only after it
+// has seen a count that covers them; those loads are agent-scope (coherent)
+// loads, so no acquire fence (which would drain the prefetch loads in flight)
+// is needed on the consumer side.
+//
+// Pencils are taken from an atomic ticket in pencil-hyperplane order, so a
+// pencil only ever waits on pencils whose workgroups are already running: no
+// co-residency assumption.  Every wait is bounded.  Two divergent regions with
+// the same condition are always separated by a barrier, so the compiler cannot
+// merge them across the loop's back edge.
+constexpr int kPW = 8;
+constexpr int kPenLanes = kPW * kPW;
+constexpr int kPenThreads = 5 * kPenLanes;
+
+struct PencilCtl {
+  int *ticket;        // reset to 0 before each sweep
+  int *prog;          // per pencil: steps completed in the current sweep
+  int *err;           // bit 1: a dependency wait exceeded the spin limit
+  const int2 *order;  // pencils (a, b) in a+b order
+  int npen, pa, pb;   // pencil count and pencil grid
+};
+
+__global__ void __launch_bounds__(256) KPencilPrologue(PencilCtl s) {
+  if (threadIdx.x == 0) *s.ticket = 0;
+  for (int q = threadIdx.x; q < s.npen; q += blockDim.x) s.prog[q] = 0;
+}
+
+// pencil width (cells) in i or j
+__device__ __forceinline__ int PenWidth(int n, int a) { return min(kPW, n - a * kPW); }
+
+template <typename T>
+__device__ __forceinline__ T CoherentLoad(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// spin until pencil `id` has completed `need` steps; uniform across the wave
+__device__ __forceinline__ void PencilSpin(const PencilCtl &s, int id, int need, int &known) {
+  long spins = 0;
+  while (known < need) {
+    known = __builtin_amdgcn_readfirstlane(CoherentLoad(&s.prog[id]));
+    if (known >= need) break;
+    if ((++spins & 255) == 0 && (__builtin_amdgcn_readfirstlane(CoherentLoad(s.err)) & 2)) break;
+    if (spins > kSpinLimit) {
+      __hip_atomic_fetch_or(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// per-lane geometry of a pencil
+struct PenLane {
+  int i, j, li, lj, wi, wj, off, nsteps;
+  bool col;
+  // k of the lane's cell at step s, or -1 when the lane is idle
+  __device__ __forceinline__ int K(int s, int nk, bool fwd) const {
+    const int kk = s - off;
+    if (!col || kk < 0 || kk >= nk) return -1;
+    return fwd ? kk : nk - 1 - kk;
+  }
+};
+
+// What one wave needs for one step, loaded two steps ahead.  Waves 0-2 read
+// it as a neighbour record, wave 4 as a right-hand-side record; one register
+// array holds either, so the kernel carries one set of prefetch registers.
+//   neighbour:  v[0..4] state, v[5..8] face, v[9..13] du across the pencil
+//               face; full mode: v[14..18] other-side state, v[19..22] its
+//               face, v[23..27] its previous-sweep du
+//   right side: v[0..4] W, v[5..9] R, v[10..14] U^n, v[15..19] U^(n-1),
+//               v[20] volume, v[21] dt, v[22] D^-1; backward sweep without
+//               the other side: v[0..4] the cell's forward-sweep du
+template <bool FULL>
+struct StepPre {
+  static constexpr int kN = FULL ? 28 : 23;
+  double v[kN];
+  int flags;  // neighbour: 1 dependency neighbour, 2 across the pencil face, 4 other side; rhs: 1 active
+};
+
+template <typename T, bool FWD, bool FULL>
+__device__ __forceinline__ void NbLoad(const DevBlock &b, const T *x, int d, const PenLane &L, int s,
+                                       StepPre<FULL> &q) {
+  q.flags = 0;
+  const int k = L.K(s, b.nk, FWD);
+  if (k < 0) return;
+  const int sgn = FWD ? -1 : 1;
+  const int i = L.i, j = L.j;
+  const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+  if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+    q.flags = 1;
+    const double4 a = FWD ? b.fa[d][FaceIdx(b, d, i, j, k)] : b.fa[d][FaceIdx(b, d, gi, gj, gk)];
+    q.v[5] = a.x;
+    q.v[6] = a.y;
+    q.v[7] = a.z;
+    q.v[8] = a.w;
+    const long c = PadIdx(b, gi, gj, gk);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) q.v[e] = b.W[e * b.npad + c];
+    const bool cross = (d == 0 && (FWD ? L.li == 0 : L.li == L.wi - 1)) ||
+                       (d == 1 && (FWD ? L.lj == 0 : L.lj == L.wj - 1));
+    if (cross) {
+      q.flags |= 2;
+      const long p = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) q.v[9 + e] = double(CoherentLoad(&x[e * b.ncell + p]));
+    }
+  }
+  if (FULL) {
+    const int hi = i - (d == 0 ? sgn : 0), hj = j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
+    if (hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
+      q.flags |= 4;
+      const double4 a = FWD ? b.fa[d][FaceIdx(b, d, hi, hj, hk)] : b.fa[d][FaceIdx(b, d, i, j, k)];
+      q.v[StepPre<FULL>::kN - 9] = a.x;
+      q.v[StepPre<FULL>::kN - 8] = a.y;
+      q.v[StepPre<FULL>::kN - 7] = a.z;
+      q.v[StepPre<FULL>::kN - 6] = a.w;
+      const long c = PadIdx(b, hi, hj, hk);
+      const long p = (long)hi + (long)hj * b.ni + (long)hk * b.ni * b.nj;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        q.v[14 + e] = b.W[e * b.npad + c];
+        q.v[StepPre<FULL>::kN - 5 + e] = double(x[e * b.ncell + p]);
+      }
+    }
+  }
+}
+
+template <typename T, bool FWD, bool FULL>
+__device__ __forceinline__ void RhsLoad(const DevBlock &b, const Params &P, const T *x, const T *ainvG,
+                                        const PenLane &L, int s, StepPre<FULL> &q) {
+  const int k = L.K(s, b.nk, FWD);
+  q.flags = k >= 0 ? 1 : 0;
+  if (k < 0) return;
+  const long p = (long)L.i + (long)L.j * b.ni + (long)k * b.ni * b.nj;
+  q.v[22] = double(ainvG[p]);
+  if (FWD || FULL) {
+    const long c = PadIdx(b, L.i, L.j, k);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      q.v[e] = b.W[e * b.npad + c];
+      q.v[5 + e] = b.R[e * b.ncell + p];
+      q.v[10 + e] = b.consN[e * b.ncell + p];
+      q.v[15 + e] = P.multilevel ? b.consNm1[e * b.ncell + p] : 0.0;
+    }
+    q.v[20] = b.vol[p];
+    q.v[21] = b.dt[p];
+  } else {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) q.v[e] = double(x[e * b.ncell + p]);
+  }
+}
+
+template <typename T, bool FWD>
+struct PencilShared {
+  T sX[5][kPenLanes];     // du of the previous step
+  T sO[3][5][kPenLanes];  // dependency-side products of this step
+  T sU[3][5][kPenLanes];  // other-side products (full mode)
+  double sRB[5][kPenLanes];
+  T sXo[5][kPenLanes];
+  T sAinv[kPenLanes];
+  int ticket;
+};
+
+// one pencil step for waves 0-4; `qn`/`qr` hold this step's prefetched data
+// and are refilled with step st + 2
+template <typename T, bool FWD, bool FULL>
+__device__ __forceinline__ void PencilStep(const DevBlock &b, const Params &P, const PencilCtl &s,
+                                           PencilShared<T, FWD> &sh, const PenLane &L, int wave, int lane,
+                                           int st, int pid, int nbId, int nbW, int nbTotal, int &known,
+                                           int &pend, StepPre<FULL> &q) {
+  const int sgn = FWD ? -1 : 1;
+  T *x = OpBuf<T>::X(b);
+  constexpr int kN = StepPre<FULL>::kN;
+  if (wave < 3) {
+    const int d = wave;
+    // progress of the neighbour pencil: the count polled last step, and a new poll
+    if (nbId >= 0) {
+      known = max(known, __builtin_amdgcn_readfirstlane(pend));
+      pend = CoherentLoad(&s.prog[nbId]);
+    }
+    const GasT<T> g = Cast<T>(P.gas);
+    T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+    T u[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (q.flags & 1) {
+      T dn[5];
+      if (q.flags & 2) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) dn[e] = T(q.v[9 + e]);
+      } else {
+        const int nl = lane + sgn * (d == 0 ? 1 : (d == 1 ? kPW : 0));
+#pragma unroll
+        for (int e = 0; e < 5; ++e) dn[e] = sh.sX[e][nl];
+      }
+      NbOffDiag<T>(g, q.v, dn, make_double4(q.v[5], q.v[6], q.v[7], q.v[8]), FWD, o);
+    }
+    if (FULL && (q.flags & 4)) {
+      T du[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) du[e] = T(q.v[kN - 5 + e]);
+      NbOffDiag<T>(g, q.v + 14, du, make_double4(q.v[kN - 9], q.v[kN - 8], q.v[kN - 7], q.v[kN - 6]), !FWD, u);
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      sh.sO[d][e][lane] = o[e];
+      if (FULL) sh.sU[d][e][lane] = u[e];
+    }
+    // prefetch step st + 2; du across the pencil face only once it is published
+    if (st + 2 < L.nsteps) {
+      if (nbId >= 0) PencilSpin(s, nbId, min(st + 2 + nbW, nbTotal), known);
+      NbLoad<T, FWD, FULL>(b, x, d, L, st + 2, q);
+    }
+  } else if (wave == 3) {
+    // publish the previous step: its du stores are ordered before the count
+    if (st > 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0) __hip_atomic_store(&s.prog[pid], st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else {
+    if (q.flags & 1) {
+      sh.sAinv[lane] = T(q.v[22]);
+      if (FWD || FULL) {
+        double rb[5];
+        RhsFrom(P, q.v, q.v[20], q.v[21], q.v + 5, q.v + 10, q.v + 15, rb);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) sh.sRB[e][lane] = rb[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) sh.sXo[e][lane] = T(q.v[e]);
+      }
+    }
+    if (st + 2 < L.nsteps) RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, st + 2, q);
+  }
+  __syncthreads();
+  const int k = L.K(st, b.nk, FWD);
+  if (wave == 3 && k >= 0) {
+    const long p = (long)L.i + (long)L.j * b.ni + (long)k * b.ni * b.nj;
+    const T ainv = sh.sAinv[lane];
+    T D[5], O[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      D[e] = T(0);
+      O[e] = T(0);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int gi = L.i + (d == 0 ? sgn : 0), gj = L.j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+      if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) D[e] += sh.sO[d][e][lane];
+      }
+      const int hi = L.i - (d == 0 ? sgn : 0), hj = L.j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
+      if (FULL && hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) O[e] += sh.sU[d][e][lane];
+      }
+    }
+    T xn[5];
+    if (FULL) {
+      if (FWD) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) xn[e] = (T(sh.sRB[e][lane]) + (D[e] - O[e])) * ainv;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) xn[e] = ((T(sh.sRB[e][lane]) + O[e]) - D[e]) * ainv;
+      }
+    } else if (FWD) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) xn[e] = (T(sh.sRB[e][lane]) + D[e]) * ainv;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) xn[e] = sh.sXo[e][lane] - D[e] * ainv;
+    }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      sh.sX[e][lane] = xn[e];
+      x[e * b.ncell + p] = xn[e];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, bool FWD, bool FULL>
+__global__ void __launch_bounds__(kPenThreads) KLusgsPencil(DevBlock b, Params P, PencilCtl s) {
+  __shared__ PencilShared<T, FWD> sh;
+  const int tid = threadIdx.x, wave = tid / kPenLanes, lane = tid % kPenLanes;
+  if (tid == 0) sh.ticket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int t = sh.ticket;
+  if (t >= s.npen) return;
+  const int2 pc = s.order[FWD ? t : s.npen - 1 - t];
+  PenLane L;
+  L.li = lane % kPW;
+  L.lj = lane / kPW;
+  L.wi = PenWidth(b.ni, pc.x);
+  L.wj = PenWidth(b.nj, pc.y);
+  L.nsteps = L.wi + L.wj + b.nk - 2;
+  L.i = pc.x * kPW + L.li;
+  L.j = pc.y * kPW + L.lj;
+  L.col = L.li < L.wi && L.lj < L.wj;
+  L.off = FWD ? L.li + L.lj : (L.wi - 1 - L.li) + (L.wj - 1 - L.lj);
+  const int pid = pc.x + s.pa * pc.y;
+  const int sgn = FWD ? -1 : 1;
+  T *x = OpBuf<T>::X(b);
+
+  // the pencil across the face this wave handles (waves 0 and 1 only)
+  int nbId = -1, nbW = 0, nbTotal = 0, known = 0, pend = 0;
+  if (wave < 2) {
+    const int na = pc.x + (wave == 0 ? sgn : 0), nb = pc.y + (wave == 1 ? sgn : 0);
+    if (na >= 0 && nb >= 0 && na < s.pa && nb < s.pb) {
+      nbId = na + s.pa * nb;
+      const int nwi = PenWidth(b.ni, na), nwj = PenWidth(b.nj, nb);
+      nbW = wave == 0 ? nwi : nwj;
+      nbTotal = nwi + nwj + b.nk - 2;
+    }
+  }
+
+  // prefetch steps 0 and 1
+  StepPre<FULL> q0, q1;
+  q0.flags = q1.flags = 0;
+  if (wave < 3) {
+    if (nbId >= 0) PencilSpin(s, nbId, min(1 + nbW, nbTotal), known);
+    NbLoad<T, FWD, FULL>(b, x, wave, L, 0, q0);
+    if (L.nsteps > 1) NbLoad<T, FWD, FULL>(b, x, wave, L, 1, q1);
+    if (nbId >= 0) pend = CoherentLoad(&s.prog[nbId]);
+  } else if (wave == 4) {
+    RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, 0, q0);
+    if (L.nsteps > 1) RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, 1, q1);
+  }
+  __syncthreads();
+
+  for (int st = 0; st < L.nsteps; st += 2) {
+    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st, pid, nbId, nbW, nbTotal, known, pend, q0);
+    if (st + 1 < L.nsteps)
+      PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st + 1, pid, nbId, nbW, nbTotal, known, pend, q1);
+  }
+  if (wave == 3) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_store(&s.prog[pid], L.nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -40,6 +40,7 @@ struct BlockRes {
   long bcTotal = 0;
   ath::SweepCtl sweep{};  // tiled LU-SGS dataflow state
   int sweepGrid = 0;
+  ath::PencilCtl pencil{};  // pencil-wavefront LU-SGS state
 };
 
 }  // namespace
@@ -61,13 +62,13 @@ struct ath_ctx {
   int *dflag = nullptr;
   int fp32 = 0;
   int matres = 1;
-  int planeSweep = 1;  // 1: one launch per hyperplane (reference ordering)
+  int sweepOrder = ATH_SWEEP_HYPERPLANE;  // ATH_SWEEP_*
   double cfl = 0;
   double l2first[ATH_MAX_EQ]{};
   // graph of one nonlinear iteration (after the boundary pass)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
-  int graphFp32 = -1, graphMatres = -1, graphPlane = -1;
+  int graphFp32 = -1, graphMatres = -1, graphOrder = -1;
   // timing
   int timing = 0;
   std::vector<hipEvent_t> evs;
@@ -236,6 +237,28 @@ int Upload(ath_ctx *c) {
       HIP_OK(hipGetDeviceProperties(&prop, c->device));
       br.sweepGrid = std::max(1, std::min(sc.ntiles, 2 * prop.multiProcessorCount));
     }
+    // pencil sweep: kPW x kPW columns in a+b order, progress counters, ticket
+    {
+      PencilCtl &pc = br.pencil;
+      pc.pa = (h.ni + kPW - 1) / kPW;
+      pc.pb = (h.nj + kPW - 1) / kPW;
+      pc.npen = pc.pa * pc.pb;
+      std::vector<int2> order;
+      order.reserve(pc.npen);
+      for (int pl = 0; pl < pc.pa + pc.pb - 1; ++pl)
+        for (int bb = 0; bb < pc.pb; ++bb)
+          for (int aa = 0; aa < pc.pa; ++aa)
+            if (aa + bb == pl) order.push_back(make_int2(aa, bb));
+      int2 *dord = nullptr;
+      if (int r = c->Alloc(&dord, order.size())) return r;
+      HIP_OK(hipMemcpy(dord, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+      pc.order = dord;
+      if (int r = c->Alloc(&pc.prog, pc.npen)) return r;
+      if (int r = c->Alloc(&pc.ticket, 1)) return r;
+      HIP_OK(hipMemset(pc.prog, 0, pc.npen * sizeof(int)));
+      HIP_OK(hipMemset(pc.ticket, 0, sizeof(int)));
+      pc.err = c->dflag;
+    }
     // reduction partials
     br.nparts = Grid(nc, kRedThreads);
     br.partBase = partBase;
@@ -353,7 +376,26 @@ void LaunchRelaxT(ath_ctx *c, bool timed, int evBase) {
   if (timed) hipEventRecord(c->evs[evBase], c->stream);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
-    if (hc.solver == ath::Solver::LUSGS && !c->planeSweep) {
+    if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_PENCIL) {
+      for (auto &br : c->blk) {
+        hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
+        if (full)
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
+                             0, c->stream, br.d, c->P, br.pencil);
+        else
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, false>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
+                             0, c->stream, br.d, c->P, br.pencil);
+      }
+      for (auto &br : c->blk) {
+        hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
+        if (full)
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, true>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
+                             0, c->stream, br.d, c->P, br.pencil);
+        else
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
+                             0, c->stream, br.d, c->P, br.pencil);
+      }
+    } else if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_TILED) {
       for (auto &br : c->blk) {
         hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
         hipLaunchKernelGGL((ath::KLusgsTile<T, true>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
@@ -480,7 +522,7 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
 }
 
 int BuildGraph(ath_ctx *c) {
-  if (c->graphExec && c->graphFp32 == c->fp32 && c->graphMatres == c->matres && c->graphPlane == c->planeSweep)
+  if (c->graphExec && c->graphFp32 == c->fp32 && c->graphMatres == c->matres && c->graphOrder == c->sweepOrder)
     return ATH_OK;
   if (c->graphExec) { hipGraphExecDestroy(c->graphExec); c->graphExec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -493,7 +535,7 @@ int BuildGraph(ath_ctx *c) {
   HIP_OK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
   c->graphFp32 = c->fp32;
   c->graphMatres = c->matres;
-  c->graphPlane = c->planeSweep;
+  c->graphOrder = c->sweepOrder;
   return ATH_OK;
 }
 
@@ -558,9 +600,10 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
 }
 
 int ath_set_sweep_ordering(ath_ctx *c, int ordering) {
-  if (!c || (ordering != ATH_SWEEP_TILED && ordering != ATH_SWEEP_HYPERPLANE))
+  if (!c || (ordering != ATH_SWEEP_TILED && ordering != ATH_SWEEP_HYPERPLANE &&
+             ordering != ATH_SWEEP_PENCIL))
     return Fail(ATH_ERR_ARG, "bad sweep ordering");
-  c->planeSweep = ordering == ATH_SWEEP_HYPERPLANE ? 1 : 0;
+  c->sweepOrder = ordering;
   return ATH_OK;
 }
 

@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-iters", type=int, default=12)
     p.add_argument("--no-fp32-leg", action="store_true")
+    p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
+                   help="LU-SGS execution order (all give identical results)")
     return p.parse_args()
 
 
@@ -83,6 +85,9 @@ def main():
     s = Solver(inp)
     prec = Solver.FP32 if args.precision == "fp32" else Solver.FP64
     s.set_implicit_precision(prec)
+    order = None if args.sweep == "default" else getattr(Solver, args.sweep.upper())
+    if order is not None:
+        s.set_sweep_ordering(order)
     W0 = s.get_state()
     ncell = s.ncells(0)
 
@@ -132,7 +137,7 @@ def main():
             "algorithmic_bytes_per_launch": FLUX_BYTES_PER_CELL * ncell,
         },
         "sweep": {
-            "kernels": "LU-SGS forward+backward hyperplane launches of one iteration",
+            "kernels": "LU-SGS forward+backward sweeps of one iteration",
             "avg_per_iteration_us": sweep_avg_s * 1e6,
             "algorithmic_GBs": sweep_gbs,
             "frac_hbm": sweep_gbs / HBM_PEAK_GBS,
@@ -147,11 +152,15 @@ def main():
         import numpy as np
         ref = None
         s64 = Solver(inp)
+        if order is not None:
+            s64.set_sweep_ordering(order)
         s64.run(0, args.steps)
         ref = s64.get_state()
         s64.close()
         s32 = Solver(inp)
         s32.set_implicit_precision(Solver.FP32)
+        if order is not None:
+            s32.set_sweep_ordering(order)
         s32.run(0, min(args.warmup, 5))
         s32.set_state(W0)
         t0 = time.perf_counter()

@@ -88,10 +88,12 @@ int ath_set_implicit_precision(ath_ctx *ctx, int precision);
 /* LU-SGS execution order.  ATH_SWEEP_TILED: 8x8x4 tiles in tile-hyperplane
  * order, dataflow-synchronised, one persistent launch per sweep.
  * ATH_SWEEP_HYPERPLANE (default): one launch per i+j+k hyperplane in the order of
- * HyperplaneReorder (src/utility.cpp:377-398).  Both are topological orders of
- * the same Gauss-Seidel dependencies and give identical results. */
+ * HyperplaneReorder (src/utility.cpp:377-398).  ATH_SWEEP_PENCIL: 8x8 columns
+ * walking k on a skewed wavefront, one launch per sweep.  All are topological
+ * orders of the same Gauss-Seidel dependencies and give identical results. */
 #define ATH_SWEEP_TILED 0
 #define ATH_SWEEP_HYPERPLANE 1
+#define ATH_SWEEP_PENCIL 2
 int ath_set_sweep_ordering(ath_ctx *ctx, int ordering);
 /* compute the matrix residual each iteration (default 1), as
  * linearSolver::Residual does (src/linearSolver.cpp:95-109) */

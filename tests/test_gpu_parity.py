@@ -278,20 +278,23 @@ def test_full_size_vortex_first_iterations(tmp_path):
 
 
 @pytest.mark.parametrize("dims,sweeps,precision", [((20, 20, 20), 1, 0), ((13, 11, 7), 1, 0),
-                                                   ((13, 11, 7), 3, 0), ((17, 9, 10), 2, 1)])
-def test_tiled_sweep_equals_hyperplane_order(tmp_path, dims, sweeps, precision):
-    """The tiled dataflow sweep and the one-launch-per-hyperplane sweep are two
-    topological orders of the same Gauss-Seidel dependencies: identical bits."""
+                                                   ((13, 11, 7), 3, 0), ((17, 9, 10), 2, 1),
+                                                   ((9, 17, 3), 1, 1)])
+@pytest.mark.parametrize("ordering", ["PENCIL", "TILED"])
+def test_sweep_orderings_equal_hyperplane_order(tmp_path, dims, sweeps, precision, ordering):
+    """The pencil-wavefront and tiled dataflow sweeps and the one-launch-per-
+    hyperplane sweep are topological orders of the same Gauss-Seidel
+    dependencies: identical bits."""
     ni, nj, nk = dims
     base = cases.vortex_case(str(tmp_path), n=max(dims), iterations=4, ni=ni, nj=nj, nk=nk)
     text = open(base).read().replace("matrixSweeps: 1", f"matrixSweeps: {sweeps}")
     open(base, "w").write(text)
-    say("tiled vs hyperplane", dims, sweeps, precision)
+    say(ordering, "vs hyperplane", dims, sweeps, precision)
     out = []
-    for ordering in (Solver.TILED, Solver.HYPERPLANE):
+    for order in (getattr(Solver, ordering), Solver.HYPERPLANE):
         s = Solver(base)
         s.set_implicit_precision(precision)
-        s.set_sweep_ordering(ordering)
+        s.set_sweep_ordering(order)
         for nn in range(4):
             s.begin_step(nn)
             s.iterate(nn, 0)
