@@ -27,6 +27,7 @@ struct DevBlock {
   float *x32;            // 5 * ncell (FP32 operator)
   double *xold;          // 5 * ncell, DP-LUR Jacobi copy
   float *xold32;
+  double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
   long cellBase;         // running cell offset of this block (L-inf index)
   int blockId;
 };
@@ -1077,153 +1078,180 @@ struct PenLane {
   }
 };
 
-// What one wave needs for one step, loaded two steps ahead.  Waves 0-2 read
-// it as a neighbour record, wave 4 as a right-hand-side record; one register
-// array holds either, so the kernel carries one set of prefetch registers.
-//   neighbour:  v[0..4] state, v[5..8] face, v[9..13] du across the pencil
-//               face; full mode: v[14..18] other-side state, v[19..22] its
-//               face, v[23..27] its previous-sweep du
-//   right side: v[0..4] W, v[5..9] R, v[10..14] U^n, v[15..19] U^(n-1),
-//               v[20] volume, v[21] dt, v[22] D^-1; backward sweep without
-//               the other side: v[0..4] the cell's forward-sweep du
-template <bool FULL>
-struct StepPre {
-  static constexpr int kN = FULL ? 28 : 23;
-  double v[kN];
-  int flags;  // neighbour: 1 dependency neighbour, 2 across the pencil face, 4 other side; rhs: 1 active
-};
-
-template <typename T, bool FWD, bool FULL>
-__device__ __forceinline__ void NbLoad(const DevBlock &b, const T *x, int d, const PenLane &L, int s,
-                                       StepPre<FULL> &q) {
-  q.flags = 0;
-  const int k = L.K(s, b.nk, FWD);
-  if (k < 0) return;
-  const int sgn = FWD ? -1 : 1;
-  const int i = L.i, j = L.j;
-  const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-  if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
-    q.flags = 1;
-    const double4 a = FWD ? b.fa[d][FaceIdx(b, d, i, j, k)] : b.fa[d][FaceIdx(b, d, gi, gj, gk)];
-    q.v[5] = a.x;
-    q.v[6] = a.y;
-    q.v[7] = a.z;
-    q.v[8] = a.w;
-    const long c = PadIdx(b, gi, gj, gk);
+// The right-hand side b of every cell, computed before the pencil sweeps
+// (it depends only on data the sweeps do not modify)
+__global__ void __launch_bounds__(256) KLusgsRhs(DevBlock b, Params P) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  double rb[5];
+  Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
-    for (int e = 0; e < 5; ++e) q.v[e] = b.W[e * b.npad + c];
-    const bool cross = (d == 0 && (FWD ? L.li == 0 : L.li == L.wi - 1)) ||
-                       (d == 1 && (FWD ? L.lj == 0 : L.lj == L.wj - 1));
-    if (cross) {
-      q.flags |= 2;
-      const long p = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
-#pragma unroll
-      for (int e = 0; e < 5; ++e) q.v[9 + e] = double(CoherentLoad(&x[e * b.ncell + p]));
-    }
-  }
-  if (FULL) {
-    const int hi = i - (d == 0 ? sgn : 0), hj = j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
-    if (hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
-      q.flags |= 4;
-      const double4 a = FWD ? b.fa[d][FaceIdx(b, d, hi, hj, hk)] : b.fa[d][FaceIdx(b, d, i, j, k)];
-      q.v[StepPre<FULL>::kN - 9] = a.x;
-      q.v[StepPre<FULL>::kN - 8] = a.y;
-      q.v[StepPre<FULL>::kN - 7] = a.z;
-      q.v[StepPre<FULL>::kN - 6] = a.w;
-      const long c = PadIdx(b, hi, hj, hk);
-      const long p = (long)hi + (long)hj * b.ni + (long)hk * b.ni * b.nj;
-#pragma unroll
-      for (int e = 0; e < 5; ++e) {
-        q.v[14 + e] = b.W[e * b.npad + c];
-        q.v[StepPre<FULL>::kN - 5 + e] = double(x[e * b.ncell + p]);
-      }
-    }
-  }
+  for (int e = 0; e < 5; ++e) b.rhs[e * b.ncell + p] = rb[e];
 }
 
+// ---- operand staging of the pencil sweep.
+//
+// Only wave 4 issues the static loads of the main loop: for the step after
+// next it loads, for every lane, the three dependency-side neighbour states
+// and faces plus the right-hand side (or, in the one-sided backward sweep,
+// the cell's forward du) and D^-1 into one of two register sets, waits for
+// them one step later and writes them into one of two LDS slots, which the
+// other waves read the step after.  Waves 0 and 1 load only the du across
+// their pencil face, two steps ahead, into registers of their own.  Each
+// register set is loaded on one path only, so the compiler's wait counts stay
+// exact and a wave waits only for the records it is about to use.  Idle
+// lanes and absent neighbours read a clamped valid address.
+template <typename T>
+struct Stage {  // one step, one lane, in registers
+  double W[3][5];
+  double F[3][4];
+  double v5[5];  // right-hand side, or the forward du in the one-sided backward sweep
+  T ainv;
+};
+
+template <typename T>
+struct StageSlot {  // one step, all lanes, in LDS
+  double W[3][5][kPenLanes];
+  double F[3][4][kPenLanes];
+  double v5[5][kPenLanes];
+  T ainv[kPenLanes];
+};
+
+__device__ __forceinline__ int Clamp(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
 template <typename T, bool FWD, bool FULL>
-__device__ __forceinline__ void RhsLoad(const DevBlock &b, const Params &P, const T *x, const T *ainvG,
-                                        const PenLane &L, int s, StepPre<FULL> &q) {
+__device__ __forceinline__ void StageLoad(const DevBlock &b, const PenLane &L, int s, Stage<T> &q) {
   const int k = L.K(s, b.nk, FWD);
-  q.flags = k >= 0 ? 1 : 0;
-  if (k < 0) return;
-  const long p = (long)L.i + (long)L.j * b.ni + (long)k * b.ni * b.nj;
-  q.v[22] = double(ainvG[p]);
-  if (FWD || FULL) {
-    const long c = PadIdx(b, L.i, L.j, k);
+  const int i = Clamp(L.i, b.ni), j = Clamp(L.j, b.nj), kc = k < 0 ? 0 : k;
+  const int sgn = FWD ? -1 : 1;
+  const long p = (long)i + (long)j * b.ni + (long)kc * b.ni * b.nj;
 #pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      q.v[e] = b.W[e * b.npad + c];
-      q.v[5 + e] = b.R[e * b.ncell + p];
-      q.v[10 + e] = b.consN[e * b.ncell + p];
-      q.v[15 + e] = P.multilevel ? b.consNm1[e * b.ncell + p] : 0.0;
-    }
-    q.v[20] = b.vol[p];
-    q.v[21] = b.dt[p];
-  } else {
+  for (int d = 0; d < 3; ++d) {
+    const int ci = Clamp(i + (d == 0 ? sgn : 0), b.ni), cj = Clamp(j + (d == 1 ? sgn : 0), b.nj),
+              ck = Clamp(kc + (d == 2 ? sgn : 0), b.nk);
+    const long c = PadIdx(b, ci, cj, ck);
+    const double *fa = reinterpret_cast<const double *>(b.fa[d]) +
+                       4 * (FWD ? FaceIdx(b, d, i, j, kc) : FaceIdx(b, d, ci, cj, ck));
 #pragma unroll
-    for (int e = 0; e < 5; ++e) q.v[e] = double(x[e * b.ncell + p]);
+    for (int e = 0; e < 5; ++e) q.W[d][e] = b.W[e * b.npad + c];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q.F[d][e] = fa[e];
   }
+  if (FWD || FULL) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) q.v5[e] = b.rhs[e * b.ncell + p];
+  } else {
+    const T *x = OpBuf<T>::X(b);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) q.v5[e] = double(x[e * b.ncell + p]);
+  }
+  q.ainv = OpBuf<T>::Ainv(b)[p];
+}
+
+template <typename T>
+__device__ __forceinline__ void StageStore(StageSlot<T> &sl, int lane, const Stage<T> &q) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) sl.W[d][e][lane] = q.W[d][e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) sl.F[d][e][lane] = q.F[d][e];
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) sl.v5[e][lane] = q.v5[e];
+  sl.ainv[lane] = q.ainv;
+}
+
+// du across the pencil face for wave d at step s (clamped address when unused)
+template <typename T, bool FWD>
+__device__ __forceinline__ void HaloLoad(const DevBlock &b, const T *x, int d, const PenLane &L, int s, T h[5]) {
+  const int k = L.K(s, b.nk, FWD);
+  const int sgn = FWD ? -1 : 1;
+  const int ci = Clamp(Clamp(L.i, b.ni) + (d == 0 ? sgn : 0), b.ni),
+            cj = Clamp(Clamp(L.j, b.nj) + (d == 1 ? sgn : 0), b.nj), ck = k < 0 ? 0 : k;
+  const long p = (long)ci + (long)cj * b.ni + (long)ck * b.ni * b.nj;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) h[e] = CoherentLoad(&x[e * b.ncell + p]);
 }
 
 template <typename T, bool FWD>
 struct PencilShared {
+  StageSlot<T> st[2];     // staged operands, slot = step parity
   T sX[5][kPenLanes];     // du of the previous step
   T sO[3][5][kPenLanes];  // dependency-side products of this step
-  T sU[3][5][kPenLanes];  // other-side products (full mode)
-  double sRB[5][kPenLanes];
-  T sXo[5][kPenLanes];
-  T sAinv[kPenLanes];
+  T sU[3][5][kPenLanes];  // other-side products (two-sided sweep)
   int ticket;
 };
 
-// one pencil step for waves 0-4; `qn`/`qr` hold this step's prefetched data
-// and are refilled with step st + 2
+// one pencil step.  `cur`/`nxt` (wave 4): the register sets holding step
+// st+1 (stored into LDS now) and receiving step st+2.  `h` (waves 0, 1): the
+// du across the pencil face for step st, refilled with step st+2.
 template <typename T, bool FWD, bool FULL>
 __device__ __forceinline__ void PencilStep(const DevBlock &b, const Params &P, const PencilCtl &s,
                                            PencilShared<T, FWD> &sh, const PenLane &L, int wave, int lane,
                                            int st, int pid, int nbId, int nbW, int nbTotal, int &known,
-                                           int &pend, StepPre<FULL> &q) {
+                                           int &pend, Stage<T> &cur, Stage<T> &nxt, T h[5]) {
   const int sgn = FWD ? -1 : 1;
   T *x = OpBuf<T>::X(b);
-  constexpr int kN = StepPre<FULL>::kN;
+  const StageSlot<T> &sl = sh.st[st & 1];
+  const int k = L.K(st, b.nk, FWD);
   if (wave < 3) {
     const int d = wave;
-    // progress of the neighbour pencil: the count polled last step, and a new poll
-    if (nbId >= 0) {
+    // progress of the neighbour pencil: the count polled last step, and a new
+    // poll (a pencil without that neighbour polls its own count, unused)
+    if (d < 2) {
       known = max(known, __builtin_amdgcn_readfirstlane(pend));
-      pend = CoherentLoad(&s.prog[nbId]);
+      pend = CoherentLoad(&s.prog[nbId >= 0 ? nbId : pid]);
     }
     const GasT<T> g = Cast<T>(P.gas);
     T o[5] = {T(0), T(0), T(0), T(0), T(0)};
     T u[5] = {T(0), T(0), T(0), T(0), T(0)};
-    if (q.flags & 1) {
-      T dn[5];
-      if (q.flags & 2) {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) dn[e] = T(q.v[9 + e]);
-      } else {
+    if (k >= 0) {
+      const int gi = L.i + (d == 0 ? sgn : 0), gj = L.j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+      if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+        const bool cross = (d == 0 && (FWD ? L.li == 0 : L.li == L.wi - 1)) ||
+                           (d == 1 && (FWD ? L.lj == 0 : L.lj == L.wj - 1));
+        double Wn[5];
+        T dn[5];
         const int nl = lane + sgn * (d == 0 ? 1 : (d == 1 ? kPW : 0));
 #pragma unroll
-        for (int e = 0; e < 5; ++e) dn[e] = sh.sX[e][nl];
+        for (int e = 0; e < 5; ++e) {
+          Wn[e] = sl.W[d][e][lane];
+          dn[e] = cross ? h[e] : sh.sX[e][nl];
+        }
+        NbOffDiag<T>(g, Wn, dn, make_double4(sl.F[d][0][lane], sl.F[d][1][lane], sl.F[d][2][lane],
+                                             sl.F[d][3][lane]), FWD, o);
       }
-      NbOffDiag<T>(g, q.v, dn, make_double4(q.v[5], q.v[6], q.v[7], q.v[8]), FWD, o);
-    }
-    if (FULL && (q.flags & 4)) {
-      T du[5];
+      if (FULL) {
+        // other side from the previous sweep's du, as in OffDiagSum (not staged)
+        const int hi = L.i - (d == 0 ? sgn : 0), hj = L.j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
+        if (hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
+          const double4 a = FWD ? b.fa[d][FaceIdx(b, d, hi, hj, hk)] : b.fa[d][FaceIdx(b, d, L.i, L.j, k)];
+          const long c = PadIdx(b, hi, hj, hk);
+          const long q = (long)hi + (long)hj * b.ni + (long)hk * b.ni * b.nj;
+          double Wn[5];
+          T dn[5];
 #pragma unroll
-      for (int e = 0; e < 5; ++e) du[e] = T(q.v[kN - 5 + e]);
-      NbOffDiag<T>(g, q.v + 14, du, make_double4(q.v[kN - 9], q.v[kN - 8], q.v[kN - 7], q.v[kN - 6]), !FWD, u);
+          for (int e = 0; e < 5; ++e) {
+            Wn[e] = b.W[e * b.npad + c];
+            dn[e] = x[e * b.ncell + q];
+          }
+          NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
+        }
+      }
     }
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
       sh.sO[d][e][lane] = o[e];
       if (FULL) sh.sU[d][e][lane] = u[e];
     }
-    // prefetch step st + 2; du across the pencil face only once it is published
-    if (st + 2 < L.nsteps) {
-      if (nbId >= 0) PencilSpin(s, nbId, min(st + 2 + nbW, nbTotal), known);
-      NbLoad<T, FWD, FULL>(b, x, d, L, st + 2, q);
+    // du across the pencil face for step st+2, once it is published (loaded
+    // also without a neighbour pencil, from a clamped address, unused)
+    if (d < 2) {
+      const int need = min(st + 2 + nbW, nbTotal);
+      if (nbId >= 0 && known < need) PencilSpin(s, nbId, need, known);
+      HaloLoad<T, FWD>(b, x, d, L, st + 2, h);
     }
   } else if (wave == 3) {
     // publish the previous step: its du stores are ordered before the count
@@ -1232,25 +1260,13 @@ __device__ __forceinline__ void PencilStep(const DevBlock &b, const Params &P, c
       if (lane == 0) __hip_atomic_store(&s.prog[pid], st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   } else {
-    if (q.flags & 1) {
-      sh.sAinv[lane] = T(q.v[22]);
-      if (FWD || FULL) {
-        double rb[5];
-        RhsFrom(P, q.v, q.v[20], q.v[21], q.v + 5, q.v + 10, q.v + 15, rb);
-#pragma unroll
-        for (int e = 0; e < 5; ++e) sh.sRB[e][lane] = rb[e];
-      } else {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) sh.sXo[e][lane] = T(q.v[e]);
-      }
-    }
-    if (st + 2 < L.nsteps) RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, st + 2, q);
+    StageStore<T>(sh.st[(st + 1) & 1], lane, cur);
+    StageLoad<T, FWD, FULL>(b, L, st + 2, nxt);
   }
   __syncthreads();
-  const int k = L.K(st, b.nk, FWD);
   if (wave == 3 && k >= 0) {
     const long p = (long)L.i + (long)L.j * b.ni + (long)k * b.ni * b.nj;
-    const T ainv = sh.sAinv[lane];
+    const T ainv = sl.ainv[lane];
     T D[5], O[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
@@ -1274,17 +1290,17 @@ __device__ __forceinline__ void PencilStep(const DevBlock &b, const Params &P, c
     if (FULL) {
       if (FWD) {
 #pragma unroll
-        for (int e = 0; e < 5; ++e) xn[e] = (T(sh.sRB[e][lane]) + (D[e] - O[e])) * ainv;
+        for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + (D[e] - O[e])) * ainv;
       } else {
 #pragma unroll
-        for (int e = 0; e < 5; ++e) xn[e] = ((T(sh.sRB[e][lane]) + O[e]) - D[e]) * ainv;
+        for (int e = 0; e < 5; ++e) xn[e] = ((T(sl.v5[e][lane]) + O[e]) - D[e]) * ainv;
       }
     } else if (FWD) {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) xn[e] = (T(sh.sRB[e][lane]) + D[e]) * ainv;
+      for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + D[e]) * ainv;
     } else {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) xn[e] = sh.sXo[e][lane] - D[e] * ainv;
+      for (int e = 0; e < 5; ++e) xn[e] = T(sl.v5[e][lane]) - D[e] * ainv;
     }
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
@@ -1330,24 +1346,27 @@ __global__ void __launch_bounds__(kPenThreads) KLusgsPencil(DevBlock b, Params P
     }
   }
 
-  // prefetch steps 0 and 1
-  StepPre<FULL> q0, q1;
-  q0.flags = q1.flags = 0;
-  if (wave < 3) {
+  // staging of steps 0 and 1; du across the pencil faces of steps 0 and 1
+  Stage<T> sa, sb;
+  T h0[5], h1[5];
+  if (wave == 4) {
+    StageLoad<T, FWD, FULL>(b, L, 0, sa);
+    StageLoad<T, FWD, FULL>(b, L, 1, sb);
+    StageStore<T>(sh.st[0], lane, sa);
+  } else if (wave < 2) {
     if (nbId >= 0) PencilSpin(s, nbId, min(1 + nbW, nbTotal), known);
-    NbLoad<T, FWD, FULL>(b, x, wave, L, 0, q0);
-    if (L.nsteps > 1) NbLoad<T, FWD, FULL>(b, x, wave, L, 1, q1);
-    if (nbId >= 0) pend = CoherentLoad(&s.prog[nbId]);
-  } else if (wave == 4) {
-    RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, 0, q0);
-    if (L.nsteps > 1) RhsLoad<T, FWD, FULL>(b, P, x, OpBuf<T>::Ainv(b), L, 1, q1);
+    HaloLoad<T, FWD>(b, x, wave, L, 0, h0);
+    HaloLoad<T, FWD>(b, x, wave, L, 1, h1);
+    pend = CoherentLoad(&s.prog[nbId >= 0 ? nbId : pid]);
   }
   __syncthreads();
 
-  for (int st = 0; st < L.nsteps; st += 2) {
-    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st, pid, nbId, nbW, nbTotal, known, pend, q0);
-    if (st + 1 < L.nsteps)
-      PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st + 1, pid, nbId, nbW, nbTotal, known, pend, q1);
+  // an even number of steps (a trailing step has no active lane)
+  const int nsteps2 = L.nsteps + (L.nsteps & 1);
+  for (int st = 0; st < nsteps2; st += 2) {
+    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st, pid, nbId, nbW, nbTotal, known, pend, sb, sa, h0);
+    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st + 1, pid, nbId, nbW, nbTotal, known, pend, sa, sb,
+                             h1);
   }
   if (wave == 3) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");

@@ -199,6 +199,7 @@ int Upload(ath_ctx *c) {
     if (int r = c->Alloc(&d.consNm1, 5 * nc)) return r;
     if (int r = c->Alloc(&d.x, 5 * nc)) return r;
     if (int r = c->Alloc(&d.xold, 5 * nc)) return r;
+    if (int r = c->Alloc(&d.rhs, 5 * nc)) return r;
     HIP_OK(hipMemset(d.x, 0, 5 * nc * sizeof(double)));
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
     d.Ainv32 = nullptr;
@@ -377,6 +378,9 @@ void LaunchRelaxT(ath_ctx *c, bool timed, int evBase) {
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_PENCIL) {
+      if (sw == 0)
+        for (auto &br : c->blk)
+          hipLaunchKernelGGL(ath::KLusgsRhs, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
       for (auto &br : c->blk) {
         hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
         if (full)
