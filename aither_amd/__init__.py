@@ -72,7 +72,7 @@ class Timing(ctypes.Structure):
     _fields_ = [
         ("run_ms", ctypes.c_double), ("flux_ms", ctypes.c_double),
         ("flux_launches", ctypes.c_int), ("sweep_ms", ctypes.c_double),
-        ("sweep_launches", ctypes.c_int),
+        ("sweep_launches", ctypes.c_int), ("sweep_passes", ctypes.c_int),
     ]
 
 

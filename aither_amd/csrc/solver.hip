@@ -62,7 +62,7 @@ struct ath_ctx {
   int *dflag = nullptr;
   int fp32 = 0;
   int matres = 1;
-  int sweepOrder = ATH_SWEEP_HYPERPLANE;  // ATH_SWEEP_*
+  int sweepOrder = ATH_SWEEP_PENCIL;  // ATH_SWEEP_*
   double cfl = 0;
   double l2first[ATH_MAX_EQ]{};
   // graph of one nonlinear iteration (after the boundary pass)
@@ -369,76 +369,93 @@ void LaunchResidual(ath_ctx *c, const BlockRes &br) {
 #undef ATH_RES
 }
 
+// One relax.  When `timed`, an event pair brackets every sweep pass (a
+// forward or backward LU-SGS pass, or one DP-LUR sweep, over all blocks)
+// from c->evs[evBase]; returns the number of sweep kernel launches.
 template <typename T>
-void LaunchRelaxT(ath_ctx *c, bool timed, int evBase) {
+int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
   const auto &hc = c->hc;
+  int launches = 0, pass = 0;
+  auto mark = [&](bool begin) {
+    if (timed) hipEventRecord(c->evs[evBase + 2 * pass + (begin ? 0 : 1)], c->stream);
+    if (!begin) ++pass;
+  };
   for (auto &br : c->blk)
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
-  if (timed) hipEventRecord(c->evs[evBase], c->stream);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_PENCIL) {
       if (sw == 0)
         for (auto &br : c->blk)
           hipLaunchKernelGGL(ath::KLusgsRhs, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
-      for (auto &br : c->blk) {
-        hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
-        if (full)
-          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
-                             0, c->stream, br.d, c->P, br.pencil);
-        else
-          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, false>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
-                             0, c->stream, br.d, c->P, br.pencil);
-      }
-      for (auto &br : c->blk) {
-        hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
-        if (full)
-          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, true>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
-                             0, c->stream, br.d, c->P, br.pencil);
-        else
-          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), dim3(br.pencil.npen), dim3(ath::kPenThreads),
-                             0, c->stream, br.d, c->P, br.pencil);
+      for (int dir = 0; dir < 2; ++dir) {
+        mark(true);
+        for (auto &br : c->blk) {
+          hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
+          const dim3 grid(br.pencil.npen), blk(ath::kPenThreads);
+          if (dir == 0 && full)
+            hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
+          else if (dir == 0)
+            hipLaunchKernelGGL((ath::KLusgsPencil<T, true, false>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
+          else if (full)
+            hipLaunchKernelGGL((ath::KLusgsPencil<T, false, true>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
+          else
+            hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
+          ++launches;
+        }
+        mark(false);
       }
     } else if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_TILED) {
-      for (auto &br : c->blk) {
-        hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
-        hipLaunchKernelGGL((ath::KLusgsTile<T, true>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
-                           c->P, br.sweep, full);
-      }
-      for (auto &br : c->blk) {
-        hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
-        hipLaunchKernelGGL((ath::KLusgsTile<T, false>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
-                           c->P, br.sweep, full);
+      for (int dir = 0; dir < 2; ++dir) {
+        mark(true);
+        for (auto &br : c->blk) {
+          hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
+          if (dir == 0)
+            hipLaunchKernelGGL((ath::KLusgsTile<T, true>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
+                               c->P, br.sweep, full);
+          else
+            hipLaunchKernelGGL((ath::KLusgsTile<T, false>), dim3(br.sweepGrid), dim3(ath::kLT), 0, c->stream, br.d,
+                               c->P, br.sweep, full);
+          ++launches;
+        }
+        mark(false);
       }
     } else if (hc.solver == ath::Solver::LUSGS) {
+      mark(true);
       for (auto &br : c->blk) {
         const int np = (int)br.planeOff.size() - 1;
-        for (int p = 0; p < np; ++p) {
+        for (int p = 0; p < np; ++p, ++launches) {
           const int n = br.planeOff[p + 1] - br.planeOff[p];
           hipLaunchKernelGGL(ath::KLusgsForward<T>, dim3(Grid(n, 64)), dim3(64), 0, c->stream, br.d, c->P,
                              br.cells + br.planeOff[p], n, full);
         }
       }
+      mark(false);
+      mark(true);
       for (auto &br : c->blk) {
         const int np = (int)br.planeOff.size() - 1;
-        for (int p = np - 1; p >= 0; --p) {
+        for (int p = np - 1; p >= 0; --p, ++launches) {
           const int n = br.planeOff[p + 1] - br.planeOff[p];
           hipLaunchKernelGGL(ath::KLusgsBackward<T>, dim3(Grid(n, 64)), dim3(64), 0, c->stream, br.d, c->P,
                              br.cells + br.planeOff[p], n, full);
         }
       }
+      mark(false);
     } else {
+      mark(true);
       for (auto &br : c->blk) {
         hipLaunchKernelGGL(ath::KCopyUpdate<T>, dim3(Grid(5 * br.d.ncell)), dim3(kBlk), 0, c->stream, br.d);
         hipLaunchKernelGGL(ath::KDplur<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
+        ++launches;
       }
+      mark(false);
     }
   }
-  if (timed) hipEventRecord(c->evs[evBase + 1], c->stream);
   if (c->matres)
     for (auto &br : c->blk)
       hipLaunchKernelGGL(ath::KMatrixResidual<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d,
                          c->P, c->dpart + br.partBase);
+  return launches;
 }
 
 template <typename T>
@@ -451,13 +468,22 @@ void LaunchUpdateT(ath_ctx *c) {
 }
 
 // everything after the residual: relax (implicit) + update + reduction
-void LaunchTail(ath_ctx *c, bool timed, int evBase) {
+// returns the number of sweep kernel launches
+int LaunchTail(ath_ctx *c, bool timed, size_t evBase) {
+  int launches = 0;
   if (c->P.implicit) {
-    if (c->fp32) LaunchRelaxT<float>(c, timed, evBase);
-    else LaunchRelaxT<double>(c, timed, evBase);
+    if (c->fp32) launches = LaunchRelaxT<float>(c, timed, evBase);
+    else launches = LaunchRelaxT<double>(c, timed, evBase);
   }
   if (c->fp32) LaunchUpdateT<float>(c);
   else LaunchUpdateT<double>(c);
+  return launches;
+}
+
+// sweep passes of one relax (event pairs needed to time it)
+int SweepPasses(const ath_ctx *c) {
+  if (!c->P.implicit) return 0;
+  return c->hc.solver == ath::Solver::LUSGS ? 2 * c->hc.sweeps : c->hc.sweeps;
 }
 
 void LaunchCopyNm1(ath_ctx *c) {
@@ -752,8 +778,13 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   if (iters == 0) return ATH_OK;
   if (int r = BuildGraph(c)) return r;
   const int nflux = (int)c->blk.size();
+  // timed runs launch the tail eagerly with an event pair around every sweep
+  // pass; untimed runs replay the captured graph
+  const int npass = SweepPasses(c);
+  const size_t perIt = 2 * (size_t)nflux + 2 * (size_t)npass;
+  int sweepLaunches = 0;
   if (c->timing) {
-    const size_t need = (size_t)iters * (2 * nflux + 2) + 2;
+    const size_t need = (size_t)iters * perIt + 2;
     while (c->evs.size() < need) {
       hipEvent_t e;
       HIP_OK(hipEventCreate(&e));
@@ -768,15 +799,14 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
     if (int r = ath_begin_step(c, nn)) return r;
     for (int mm = 0; mm < nl; ++mm, ++it) {
       LaunchBoundary(c);
-      const size_t eb = 2 + (size_t)it * (2 * nflux + 2);
+      const size_t eb = 2 + (size_t)it * perIt;
       for (size_t q = 0; q < c->blk.size(); ++q) {
         if (c->timing) hipEventRecord(c->evs[eb + 2 * q], c->stream);
         LaunchResidual(c, c->blk[q]);
         if (c->timing) hipEventRecord(c->evs[eb + 2 * q + 1], c->stream);
       }
-      if (c->timing) hipEventRecord(c->evs[eb + 2 * nflux], c->stream);
-      HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
-      if (c->timing) hipEventRecord(c->evs[eb + 2 * nflux + 1], c->stream);
+      if (c->timing) sweepLaunches += LaunchTail(c, true, eb + 2 * nflux);
+      else HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
       if (c->P.multilevel && mm == nl - 1) LaunchCopyNm1(c);
       if (int r = CheckLaunch()) return r;
       // the norm history holds histCap iterations; drain it when full
@@ -811,16 +841,19 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
     HIP_OK(hipEventElapsedTime(&ms, c->evs[0], c->evs[1]));
     t.run_ms = ms;
     for (long q = 0; q < iters; ++q) {
-      const size_t eb = 2 + (size_t)q * (2 * nflux + 2);
+      const size_t eb = 2 + (size_t)q * perIt;
       for (int b = 0; b < nflux; ++b) {
         HIP_OK(hipEventElapsedTime(&ms, c->evs[eb + 2 * b], c->evs[eb + 2 * b + 1]));
         t.flux_ms += ms;
         t.flux_launches++;
       }
-      HIP_OK(hipEventElapsedTime(&ms, c->evs[eb + 2 * nflux], c->evs[eb + 2 * nflux + 1]));
-      t.sweep_ms += ms;
-      t.sweep_launches++;
+      for (int ps = 0; ps < npass; ++ps) {
+        HIP_OK(hipEventElapsedTime(&ms, c->evs[eb + 2 * nflux + 2 * ps], c->evs[eb + 2 * nflux + 2 * ps + 1]));
+        t.sweep_ms += ms;
+      }
     }
+    t.sweep_launches = sweepLaunches;
+    t.sweep_passes = (int)(iters * npass);
     c->tm = t;
   }
   return r;

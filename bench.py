@@ -2,7 +2,7 @@
 
 Workload (a "step" is one nonlinear iteration of the whole hot path: ghost
 cells, fused face-flux residual + spectral radius + dt + diagonal, LU-SGS
-hyperplane sweeps, matrix residual, update and norm reductions):
+forward and backward sweeps, matrix residual, update and norm reductions):
 inviscid Euler, calorically perfect air, one Cartesian block 64^3 on
 [0,10]^3, isentropic vortex (strength 5, core radius 1) on a Mach 0.5
 freestream, characteristic farfield BCs, Roe flux, MUSCL (kappa = 1/3) with
@@ -23,8 +23,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLUX_BYTES_PER_CELL = 224      # W 40 + geometry 128 + R 40 + lambda 8 + diag 8 (FP64)
-SWEEP_BYTES_PER_CELL = 504     # LU-SGS forward + backward, FP64 operator
+# LU-SGS, FP64 operator: forward pass W 40 + R 40 + U^n 40 + geometry 96 +
+# V 8 + dt 8 + D^-1 8 + x 40 = 280; backward pass W 40 + geometry 96 + D^-1 8
+# + x read 40 + x write 40 = 224; a pass is one forward or backward launch
+SWEEP_BYTES_PER_CELL = {"fwd": 280, "bwd": 224}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch on the 64^3 workload (FP64), from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by the same counters on a 256 MiB device
+# copy (FETCH_SIZE reads half the copied bytes: factor 2.0; WRITE_SIZE 1.0);
+# see profiles/euler-vortex-64_hbm_counters.md
+FLUX_TRAFFIC_BYTES = 172.2e6
+SWEEP_TRAFFIC_BYTES = 0.5 * (150.7e6 + 132.2e6)   # pencil order, forward and backward
 
 
 def parse():
@@ -40,19 +49,9 @@ def parse():
     p.add_argument("--no-fp32-leg", action="store_true")
     p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
                    help="LU-SGS execution order (all give identical results)")
+    p.add_argument("--hbm-calibration", action="store_true",
+                   help="only run a device-to-device copy of known size (to scale HBM counters)")
     return p.parse_args()
-
-
-def traffic_from_profiles():
-    """HBM bytes per flux launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "flux_hbm_traffic.json")
-    if not os.path.exists(path):
-        return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    except Exception:
-        return None
 
 
 def cpu_baseline(inp, iters):
@@ -72,8 +71,27 @@ def cpu_baseline(inp, iters):
                       f"1 thread ({dt:.1f} s)"}
 
 
+def hbm_calibration():
+    """Copy 256 MiB device to device 5 times with torch's elementwise copy
+    kernel: FETCH_SIZE and WRITE_SIZE of that kernel should each read 256 MiB
+    per dispatch; their ratio to it scales the counters of the solver's
+    kernels.  Runs in its own process (the solver library is not loaded)."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_dispatch": n,
+                      "bytes_written_per_dispatch": n, "dispatches": 5}), flush=True)
+
+
 def main():
     args = parse()
+    if args.hbm_calibration:
+        hbm_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -104,9 +122,13 @@ def main():
     value = ncell * args.steps / wall
 
     flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
-    sweep_avg_s = tm.sweep_ms / max(tm.sweep_launches, 1) * 1e-3
+    pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
     flux_gbs = FLUX_BYTES_PER_CELL * ncell / flux_avg_s / 1e9
-    sweep_gbs = SWEEP_BYTES_PER_CELL * ncell / sweep_avg_s / 1e9
+    pass_bytes = 0.5 * (SWEEP_BYTES_PER_CELL["fwd"] + SWEEP_BYTES_PER_CELL["bwd"]) * ncell
+    if prec == Solver.FP32:
+        pass_bytes -= 0.5 * (4 + 20 + 4 + 20 + 20) * ncell  # D^-1 and x words in FP32
+    sweep_gbs = pass_bytes / pass_avg_s / 1e9
+    order_name = args.sweep if args.sweep != "default" else "pencil"
     out = {
         "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
         "value": value,
@@ -125,22 +147,32 @@ def main():
                         "van Albada, implicit Euler LU-SGS CFL 5, characteristic farfield",
             "cells": ncell, "iterations": args.steps, "parallelism": "single GPU",
         },
+        # the dominant kernel: one LU-SGS sweep pass (forward or backward)
         "roofline": {
+            "kernel": f"LU-SGS sweep pass ({order_name} order), average of forward and backward",
+            "bound": "hbm",
+            "achieved": sweep_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": sweep_gbs / HBM_PEAK_GBS,
+            "traffic": SWEEP_TRAFFIC_BYTES if (order_name == "pencil" and args.n == 64 and
+                                               prec == Solver.FP64) else None,
+            "avg_launch_us": pass_avg_s * 1e6,
+            "algorithmic_bytes_per_launch": pass_bytes,
+            "launches_per_pass": tm.sweep_launches / max(tm.sweep_passes, 1),
+            "share_of_step": tm.sweep_ms / max(tm.run_ms, 1e-30),
+        },
+        "roofline_flux": {
             "kernel": "KResidual (fused face flux + spectral radius + dt + diagonal)",
             "bound": "hbm",
             "achieved": flux_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": flux_gbs / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
+            "traffic": FLUX_TRAFFIC_BYTES if args.n == 64 else None,
             "avg_launch_us": flux_avg_s * 1e6,
             "algorithmic_bytes_per_launch": FLUX_BYTES_PER_CELL * ncell,
-        },
-        "sweep": {
-            "kernels": "LU-SGS forward+backward sweeps of one iteration",
-            "avg_per_iteration_us": sweep_avg_s * 1e6,
-            "algorithmic_GBs": sweep_gbs,
-            "frac_hbm": sweep_gbs / HBM_PEAK_GBS,
+            "share_of_step": tm.flux_ms / max(tm.run_ms, 1e-30),
         },
         "final_normalized_residual": list(last.normalized[:5]),
     }

@@ -85,12 +85,12 @@ int ath_block_dims(const ath_ctx *ctx, int block, int *ni, int *nj, int *nk);
 /* implicit-operator precision (default ATH_IMPLICIT_FP64).  No reference
  * counterpart: the reference is FP64 throughout. */
 int ath_set_implicit_precision(ath_ctx *ctx, int precision);
-/* LU-SGS execution order.  ATH_SWEEP_TILED: 8x8x4 tiles in tile-hyperplane
- * order, dataflow-synchronised, one persistent launch per sweep.
- * ATH_SWEEP_HYPERPLANE (default): one launch per i+j+k hyperplane in the order of
- * HyperplaneReorder (src/utility.cpp:377-398).  ATH_SWEEP_PENCIL: 8x8 columns
- * walking k on a skewed wavefront, one launch per sweep.  All are topological
- * orders of the same Gauss-Seidel dependencies and give identical results. */
+/* LU-SGS execution order.  ATH_SWEEP_PENCIL (default): 8x8 columns walking k
+ * on a skewed wavefront, one launch per sweep pass.  ATH_SWEEP_TILED: 8x8x4
+ * tiles in tile-hyperplane order, dataflow-synchronised, one launch per pass.
+ * ATH_SWEEP_HYPERPLANE: one launch per i+j+k hyperplane in the order of
+ * HyperplaneReorder (src/utility.cpp:377-398).  All are topological orders of
+ * the same Gauss-Seidel dependencies and give identical results. */
 #define ATH_SWEEP_TILED 0
 #define ATH_SWEEP_HYPERPLANE 1
 #define ATH_SWEEP_PENCIL 2
@@ -145,9 +145,14 @@ typedef struct {
   double run_ms;            /* whole ath_run */
   double flux_ms;           /* summed duration of the fused flux kernels */
   int flux_launches;
-  double sweep_ms;          /* summed LU-SGS / DP-LUR sweep kernels */
-  int sweep_launches;
+  double sweep_ms;          /* summed sweep passes: each forward or backward
+                               LU-SGS pass (or DP-LUR sweep) over all blocks,
+                               including its one-thread prologue */
+  int sweep_launches;       /* sweep kernel launches in those passes */
+  int sweep_passes;
 } ath_timing;
+/* Timed runs launch the work after the residual eagerly (untimed runs replay
+ * a captured HIP graph) so that events can bracket every sweep pass. */
 int ath_enable_timing(ath_ctx *ctx, int on);
 int ath_get_timing(const ath_ctx *ctx, ath_timing *out);
 
