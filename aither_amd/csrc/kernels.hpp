@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
 // merge them across the loop's back edge.
 constexpr int kPW = 8;
 constexpr int kPenLanes = kPW * kPW;
-constexpr int kPenThreads = 5 * kPenLanes;
+
 
 struct PencilCtl {
   int *ticket;        // reset to 0 before each sweep
@@ -1036,6 +1036,7 @@ struct PencilCtl {
   int *err;           // bit 1: a dependency wait exceeded the spin limit
   const int2 *order;  // pencils (a, b) in a+b order
   int npen, pa, pb;   // pencil count and pencil grid
+  void *sinkv;        // per pencil 5 x 64 operator words: stores of idle lanes
 };
 
 __global__ void __launch_bounds__(256) KPencilPrologue(PencilCtl s) {
@@ -1090,232 +1091,188 @@ __global__ void __launch_bounds__(256) KLusgsRhs(DevBlock b, Params P) {
   for (int e = 0; e < 5; ++e) b.rhs[e * b.ncell + p] = rb[e];
 }
 
-// ---- operand staging of the pencil sweep.
+// ---- decoupled waves of the pencil sweep.
 //
-// Only wave 4 issues the static loads of the main loop: for the step after
-// next it loads, for every lane, the three dependency-side neighbour states
-// and faces plus the right-hand side (or, in the one-sided backward sweep,
-// the cell's forward du) and D^-1 into one of two register sets, waits for
-// them one step later and writes them into one of two LDS slots, which the
-// other waves read the step after.  Waves 0 and 1 load only the du across
-// their pencil face, two steps ahead, into registers of their own.  Each
-// register set is loaded on one path only, so the compiler's wait counts stay
-// exact and a wave waits only for the records it is about to use.  Idle
-// lanes and absent neighbours read a clamped valid address.
-template <typename T>
-struct Stage {  // one step, one lane, in registers
-  double W[3][5];
-  double F[3][4];
-  double v5[5];  // right-hand side, or the forward du in the one-sided backward sweep
-  T ainv;
+// A pencil's workgroup has eight waves that run their own loops and
+// synchronise through step counters in LDS (no workgroup barrier in the
+// loop), so a wave that waits on global memory holds up only the waves that
+// need its result:
+//
+//  * waves 4-7 (stagers): stager j prepares every step s = j mod 4 into LDS
+//    slot j.  It loads, for each lane, the three dependency-side neighbours
+//    and computes everything of their off-diagonal products that does not
+//    depend on this sweep's du (conservative state U, flux F(W), the face
+//    normal, half the face area, the spectral radius term), the du across the
+//    i and j pencil faces (after the neighbour pencil's progress covers it),
+//    the right-hand side (or the cell's forward du) and D^-1, and in the
+//    two-sided sweep the whole other-side sum.  It waits only for the slot to
+//    be released (the owner finished step s-4) and for the neighbour pencils.
+//  * waves 0-2 (neighbours): wave d finishes the dependency-side product of
+//    direction d: U + du, primitive state, flux, and the Rusanov combination
+//    (the operations of RusanovScalarOffDiagonal, in the same order).
+//  * wave 3 (owner): sums the products in the order of ImplicitLower /
+//    ImplicitUpper, forms du, passes it to the neighbour waves through LDS,
+//    stores it, and every kPubEvery steps releases the stores and publishes
+//    the pencil's progress.
+//
+// Counters are monotone; each has one writer and is written with a
+// workgroup-scope release after the data it covers, read with an acquire.
+// Every wait is bounded (an exceeded bound sets err bit 1 and stops waiting).
+constexpr int kStagers = 4;
+constexpr int kPenWaves = 4 + kStagers;
+constexpr int kPubEvery = 2;
+constexpr long kLdsSpinLimit = 1L << 24;
+
+template <typename T, bool FULL>
+struct PenSlot {
+  T U[3][5][kPenLanes];   // neighbour conservative state
+  T Fo[3][5][kPenLanes];  // neighbour flux through the face
+  T n[3][3][kPenLanes];   // face unit normal
+  T half[3][kPenLanes];   // 0.5 |A|
+  T lam[3][kPenLanes];    // 0.5 |A| (|v.n| + c) of the neighbour
+  T halo[2][5][kPenLanes];  // du across the i and j pencil faces
+  double v5[5][kPenLanes];  // right-hand side, or the cell's forward du
+  T ainv[kPenLanes];
+  T O[FULL ? 5 : 1][kPenLanes];  // other-side sum (two-sided sweep)
 };
 
-template <typename T>
-struct StageSlot {  // one step, all lanes, in LDS
-  double W[3][5][kPenLanes];
-  double F[3][4][kPenLanes];
-  double v5[5][kPenLanes];
-  T ainv[kPenLanes];
+template <typename T, bool FULL>
+struct PenShared {
+  PenSlot<T, FULL> slot[kStagers];
+  T sX[5][kPenLanes];     // du of the owner's last step
+  T sO[3][5][kPenLanes];  // dependency-side products of the current step
+  int stageReady[kStagers];  // slot j holds step stageReady[j] - 1
+  int oReady[3];             // wave d has written sO for oReady[d] steps
+  int xReady;                // the owner has finished xReady steps
+  int ticket;
 };
+
+__device__ __forceinline__ int LdsLoad(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void LdsStore(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// wait until an LDS counter reaches `need`; uniform across the wave
+__device__ __forceinline__ void LdsWait(const int *p, int need, int *err) {
+  long spins = 0;
+  while (__builtin_amdgcn_readfirstlane(LdsLoad(p)) < need) {
+    if (++spins > kLdsSpinLimit) {
+      __hip_atomic_fetch_or(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
 
 __device__ __forceinline__ int Clamp(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
 
+__device__ __forceinline__ bool InBlock(const DevBlock &b, int i, int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk;
+}
+
+// Lane divergence and the step counters.  A counter is stored after data
+// written by every lane of the wave; a lane-divergent branch between the
+// data and the store would let the compiler duplicate the store into both
+// sides of the branch and run the idle lanes' copy first, publishing the
+// counter before the active lanes have written.  So everything from a
+// wave's loads to its counter store is straight-line code: idle lanes and
+// absent neighbours compute on clamped, valid addresses, their results are
+// selected away, and their stores go to a per-pencil sink.
+
+// stager: prepare step st into `sl`
 template <typename T, bool FWD, bool FULL>
-__device__ __forceinline__ void StageLoad(const DevBlock &b, const PenLane &L, int s, Stage<T> &q) {
-  const int k = L.K(s, b.nk, FWD);
-  const int i = Clamp(L.i, b.ni), j = Clamp(L.j, b.nj), kc = k < 0 ? 0 : k;
+__device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, const PencilCtl &s,
+                                         PenSlot<T, FULL> &sl, const PenLane &L, int lane, int st,
+                                         const int nbId[2], const int nbW[2], const int nbTotal[2], int known[2]) {
   const int sgn = FWD ? -1 : 1;
+  const GasT<T> g = Cast<T>(P.gas);
+  const T *x = OpBuf<T>::X(b);
+  // the du across the i and j pencil faces must be published first (uniform)
+  bool fresh = false;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int need = min(st + nbW[h], nbTotal[h]);
+    if (nbId[h] >= 0 && known[h] < need) {
+      PencilSpin(s, nbId[h], need, known[h]);
+      fresh = true;
+    }
+  }
+  if (fresh) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const int k = L.K(st, b.nk, FWD);
+  const int i = Clamp(L.i, b.ni), j = Clamp(L.j, b.nj), kc = k < 0 ? 0 : k;
   const long p = (long)i + (long)j * b.ni + (long)kc * b.ni * b.nj;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int ci = Clamp(i + (d == 0 ? sgn : 0), b.ni), cj = Clamp(j + (d == 1 ? sgn : 0), b.nj),
               ck = Clamp(kc + (d == 2 ? sgn : 0), b.nk);
+    const double4 a = FWD ? b.fa[d][FaceIdx(b, d, i, j, kc)] : b.fa[d][FaceIdx(b, d, ci, cj, ck)];
     const long c = PadIdx(b, ci, cj, ck);
-    const double *fa = reinterpret_cast<const double *>(b.fa[d]) +
-                       4 * (FWD ? FaceIdx(b, d, i, j, kc) : FaceIdx(b, d, ci, cj, ck));
+    T W[5], U[5], Fo[5];
 #pragma unroll
-    for (int e = 0; e < 5; ++e) q.W[d][e] = b.W[e * b.npad + c];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) q.F[d][e] = fa[e];
-  }
-  if (FWD || FULL) {
-#pragma unroll
-    for (int e = 0; e < 5; ++e) q.v5[e] = b.rhs[e * b.ncell + p];
-  } else {
-    const T *x = OpBuf<T>::X(b);
-#pragma unroll
-    for (int e = 0; e < 5; ++e) q.v5[e] = double(x[e * b.ncell + p]);
-  }
-  q.ainv = OpBuf<T>::Ainv(b)[p];
-}
-
-template <typename T>
-__device__ __forceinline__ void StageStore(StageSlot<T> &sl, int lane, const Stage<T> &q) {
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int e = 0; e < 5; ++e) sl.W[d][e][lane] = q.W[d][e];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) sl.F[d][e][lane] = q.F[d][e];
-  }
-#pragma unroll
-  for (int e = 0; e < 5; ++e) sl.v5[e][lane] = q.v5[e];
-  sl.ainv[lane] = q.ainv;
-}
-
-// du across the pencil face for wave d at step s (clamped address when unused)
-template <typename T, bool FWD>
-__device__ __forceinline__ void HaloLoad(const DevBlock &b, const T *x, int d, const PenLane &L, int s, T h[5]) {
-  const int k = L.K(s, b.nk, FWD);
-  const int sgn = FWD ? -1 : 1;
-  const int ci = Clamp(Clamp(L.i, b.ni) + (d == 0 ? sgn : 0), b.ni),
-            cj = Clamp(Clamp(L.j, b.nj) + (d == 1 ? sgn : 0), b.nj), ck = k < 0 ? 0 : k;
-  const long p = (long)ci + (long)cj * b.ni + (long)ck * b.ni * b.nj;
-#pragma unroll
-  for (int e = 0; e < 5; ++e) h[e] = CoherentLoad(&x[e * b.ncell + p]);
-}
-
-template <typename T, bool FWD>
-struct PencilShared {
-  StageSlot<T> st[2];     // staged operands, slot = step parity
-  T sX[5][kPenLanes];     // du of the previous step
-  T sO[3][5][kPenLanes];  // dependency-side products of this step
-  T sU[3][5][kPenLanes];  // other-side products (two-sided sweep)
-  int ticket;
-};
-
-// one pencil step.  `cur`/`nxt` (wave 4): the register sets holding step
-// st+1 (stored into LDS now) and receiving step st+2.  `h` (waves 0, 1): the
-// du across the pencil face for step st, refilled with step st+2.
-template <typename T, bool FWD, bool FULL>
-__device__ __forceinline__ void PencilStep(const DevBlock &b, const Params &P, const PencilCtl &s,
-                                           PencilShared<T, FWD> &sh, const PenLane &L, int wave, int lane,
-                                           int st, int pid, int nbId, int nbW, int nbTotal, int &known,
-                                           int &pend, Stage<T> &cur, Stage<T> &nxt, T h[5]) {
-  const int sgn = FWD ? -1 : 1;
-  T *x = OpBuf<T>::X(b);
-  const StageSlot<T> &sl = sh.st[st & 1];
-  const int k = L.K(st, b.nk, FWD);
-  if (wave < 3) {
-    const int d = wave;
-    // progress of the neighbour pencil: the count polled last step, and a new
-    // poll (a pencil without that neighbour polls its own count, unused)
-    if (d < 2) {
-      known = max(known, __builtin_amdgcn_readfirstlane(pend));
-      pend = CoherentLoad(&s.prog[nbId >= 0 ? nbId : pid]);
-    }
-    const GasT<T> g = Cast<T>(P.gas);
-    T o[5] = {T(0), T(0), T(0), T(0), T(0)};
-    T u[5] = {T(0), T(0), T(0), T(0), T(0)};
-    if (k >= 0) {
-      const int gi = L.i + (d == 0 ? sgn : 0), gj = L.j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-      if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
-        const bool cross = (d == 0 && (FWD ? L.li == 0 : L.li == L.wi - 1)) ||
-                           (d == 1 && (FWD ? L.lj == 0 : L.lj == L.wj - 1));
-        double Wn[5];
-        T dn[5];
-        const int nl = lane + sgn * (d == 0 ? 1 : (d == 1 ? kPW : 0));
-#pragma unroll
-        for (int e = 0; e < 5; ++e) {
-          Wn[e] = sl.W[d][e][lane];
-          dn[e] = cross ? h[e] : sh.sX[e][nl];
-        }
-        NbOffDiag<T>(g, Wn, dn, make_double4(sl.F[d][0][lane], sl.F[d][1][lane], sl.F[d][2][lane],
-                                             sl.F[d][3][lane]), FWD, o);
-      }
-      if (FULL) {
-        // other side from the previous sweep's du, as in OffDiagSum (not staged)
-        const int hi = L.i - (d == 0 ? sgn : 0), hj = L.j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
-        if (hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
-          const double4 a = FWD ? b.fa[d][FaceIdx(b, d, hi, hj, hk)] : b.fa[d][FaceIdx(b, d, L.i, L.j, k)];
-          const long c = PadIdx(b, hi, hj, hk);
-          const long q = (long)hi + (long)hj * b.ni + (long)hk * b.ni * b.nj;
-          double Wn[5];
-          T dn[5];
-#pragma unroll
-          for (int e = 0; e < 5; ++e) {
-            Wn[e] = b.W[e * b.npad + c];
-            dn[e] = x[e * b.ncell + q];
-          }
-          NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
-        }
-      }
-    }
+    for (int e = 0; e < 5; ++e) W[e] = T(b.W[e * b.npad + c]);
+    const T nx = T(a.x), ny = T(a.y), nz = T(a.z), mag = T(a.w);
+    PrimToCons(g, W, U);
+    PhysFlux(g, W, nx, ny, nz, Fo);
+    const T half = T(0.5) * mag;
+    const T lam = T(0.5) * mag * (fabs((W[1] * nx + W[2] * ny) + W[3] * nz) + SoS(g, W[0], W[4]));
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
-      sh.sO[d][e][lane] = o[e];
-      if (FULL) sh.sU[d][e][lane] = u[e];
+      sl.U[d][e][lane] = U[e];
+      sl.Fo[d][e][lane] = Fo[e];
     }
-    // du across the pencil face for step st+2, once it is published (loaded
-    // also without a neighbour pencil, from a clamped address, unused)
+    sl.n[d][0][lane] = nx;
+    sl.n[d][1][lane] = ny;
+    sl.n[d][2][lane] = nz;
+    sl.half[d][lane] = half;
+    sl.lam[d][lane] = lam;
     if (d < 2) {
-      const int need = min(st + 2 + nbW, nbTotal);
-      if (nbId >= 0 && known < need) PencilSpin(s, nbId, need, known);
-      HaloLoad<T, FWD>(b, x, d, L, st + 2, h);
-    }
-  } else if (wave == 3) {
-    // publish the previous step: its du stores are ordered before the count
-    if (st > 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (lane == 0) __hip_atomic_store(&s.prog[pid], st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  } else {
-    StageStore<T>(sh.st[(st + 1) & 1], lane, cur);
-    StageLoad<T, FWD, FULL>(b, L, st + 2, nxt);
-  }
-  __syncthreads();
-  if (wave == 3 && k >= 0) {
-    const long p = (long)L.i + (long)L.j * b.ni + (long)k * b.ni * b.nj;
-    const T ainv = sl.ainv[lane];
-    T D[5], O[5];
+      // du of the neighbour; used only across the pencil face
+      const long q = (long)ci + (long)cj * b.ni + (long)ck * b.ni * b.nj;
 #pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      D[e] = T(0);
-      O[e] = T(0);
+      for (int e = 0; e < 5; ++e) sl.halo[d][e][lane] = x[e * b.ncell + q];
     }
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) sl.v5[e][lane] = (FWD || FULL) ? b.rhs[e * b.ncell + p] : double(x[e * b.ncell + p]);
+  sl.ainv[lane] = OpBuf<T>::Ainv(b)[p];
+  if (FULL) {
+    // other side from the previous sweep's du, as in OffDiagSum (a missing
+    // neighbour adds +0, which leaves the sum unchanged: it starts at +0 and
+    // a sum of finite terms reached from +0 is never -0)
+    T O[5] = {T(0), T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int gi = L.i + (d == 0 ? sgn : 0), gj = L.j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-      if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+      const int hi = i - (d == 0 ? sgn : 0), hj = j - (d == 1 ? sgn : 0), hk = kc - (d == 2 ? sgn : 0);
+      const bool ok = k >= 0 && InBlock(b, hi, hj, hk);
+      const int ui = Clamp(hi, b.ni), uj = Clamp(hj, b.nj), uk = Clamp(hk, b.nk);
+      const double4 a = FWD ? b.fa[d][FaceIdx(b, d, ui, uj, uk)] : b.fa[d][FaceIdx(b, d, i, j, kc)];
+      const long c = PadIdx(b, ui, uj, uk);
+      const long q = (long)ui + (long)uj * b.ni + (long)uk * b.ni * b.nj;
+      double Wn[5];
+      T dn[5], u[5];
 #pragma unroll
-        for (int e = 0; e < 5; ++e) D[e] += sh.sO[d][e][lane];
+      for (int e = 0; e < 5; ++e) {
+        Wn[e] = b.W[e * b.npad + c];
+        dn[e] = x[e * b.ncell + q];
       }
-      const int hi = L.i - (d == 0 ? sgn : 0), hj = L.j - (d == 1 ? sgn : 0), hk = k - (d == 2 ? sgn : 0);
-      if (FULL && hi >= 0 && hj >= 0 && hk >= 0 && hi < b.ni && hj < b.nj && hk < b.nk) {
+      NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
 #pragma unroll
-        for (int e = 0; e < 5; ++e) O[e] += sh.sU[d][e][lane];
-      }
-    }
-    T xn[5];
-    if (FULL) {
-      if (FWD) {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + (D[e] - O[e])) * ainv;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) xn[e] = ((T(sl.v5[e][lane]) + O[e]) - D[e]) * ainv;
-      }
-    } else if (FWD) {
-#pragma unroll
-      for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + D[e]) * ainv;
-    } else {
-#pragma unroll
-      for (int e = 0; e < 5; ++e) xn[e] = T(sl.v5[e][lane]) - D[e] * ainv;
+      for (int e = 0; e < 5; ++e) O[e] += ok ? u[e] : T(0);
     }
 #pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      sh.sX[e][lane] = xn[e];
-      x[e * b.ncell + p] = xn[e];
-    }
+    for (int e = 0; e < 5; ++e) sl.O[FULL ? e : 0][lane] = O[e];
   }
-  __syncthreads();
 }
 
 template <typename T, bool FWD, bool FULL>
-__global__ void __launch_bounds__(kPenThreads) KLusgsPencil(DevBlock b, Params P, PencilCtl s) {
-  __shared__ PencilShared<T, FWD> sh;
+__global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(DevBlock b, Params P, PencilCtl s) {
+  __shared__ PenShared<T, FULL> sh;
   const int tid = threadIdx.x, wave = tid / kPenLanes, lane = tid % kPenLanes;
   if (tid == 0) sh.ticket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid < kStagers) sh.stageReady[tid] = 0;
+  if (tid < 3) sh.oReady[tid] = 0;
+  if (tid == 0) sh.xReady = 0;
   __syncthreads();
   const int t = sh.ticket;
   if (t >= s.npen) return;
@@ -1332,45 +1289,107 @@ __global__ void __launch_bounds__(kPenThreads) KLusgsPencil(DevBlock b, Params P
   L.off = FWD ? L.li + L.lj : (L.wi - 1 - L.li) + (L.wj - 1 - L.lj);
   const int pid = pc.x + s.pa * pc.y;
   const int sgn = FWD ? -1 : 1;
-  T *x = OpBuf<T>::X(b);
+  const int ns = L.nsteps;
 
-  // the pencil across the face this wave handles (waves 0 and 1 only)
-  int nbId = -1, nbW = 0, nbTotal = 0, known = 0, pend = 0;
-  if (wave < 2) {
-    const int na = pc.x + (wave == 0 ? sgn : 0), nb = pc.y + (wave == 1 ? sgn : 0);
-    if (na >= 0 && nb >= 0 && na < s.pa && nb < s.pb) {
-      nbId = na + s.pa * nb;
-      const int nwi = PenWidth(b.ni, na), nwj = PenWidth(b.nj, nb);
-      nbW = wave == 0 ? nwi : nwj;
-      nbTotal = nwi + nwj + b.nk - 2;
+  if (wave >= 4) {
+    // ---- stager j: steps j, j+4, ...
+    const int jst = wave - 4;
+    int nbId[2], nbW[2], nbTotal[2], known[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int na = pc.x + (h == 0 ? sgn : 0), nb = pc.y + (h == 1 ? sgn : 0);
+      nbId[h] = -1;
+      nbW[h] = nbTotal[h] = 0;
+      if (na >= 0 && nb >= 0 && na < s.pa && nb < s.pb) {
+        nbId[h] = na + s.pa * nb;
+        const int nwi = PenWidth(b.ni, na), nwj = PenWidth(b.nj, nb);
+        nbW[h] = h == 0 ? nwi : nwj;
+        nbTotal[h] = nwi + nwj + b.nk - 2;
+      }
     }
-  }
-
-  // staging of steps 0 and 1; du across the pencil faces of steps 0 and 1
-  Stage<T> sa, sb;
-  T h0[5], h1[5];
-  if (wave == 4) {
-    StageLoad<T, FWD, FULL>(b, L, 0, sa);
-    StageLoad<T, FWD, FULL>(b, L, 1, sb);
-    StageStore<T>(sh.st[0], lane, sa);
-  } else if (wave < 2) {
-    if (nbId >= 0) PencilSpin(s, nbId, min(1 + nbW, nbTotal), known);
-    HaloLoad<T, FWD>(b, x, wave, L, 0, h0);
-    HaloLoad<T, FWD>(b, x, wave, L, 1, h1);
-    pend = CoherentLoad(&s.prog[nbId >= 0 ? nbId : pid]);
-  }
-  __syncthreads();
-
-  // an even number of steps (a trailing step has no active lane)
-  const int nsteps2 = L.nsteps + (L.nsteps & 1);
-  for (int st = 0; st < nsteps2; st += 2) {
-    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st, pid, nbId, nbW, nbTotal, known, pend, sb, sa, h0);
-    PencilStep<T, FWD, FULL>(b, P, s, sh, L, wave, lane, st + 1, pid, nbId, nbW, nbTotal, known, pend, sa, sb,
-                             h1);
-  }
-  if (wave == 3) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(&s.prog[pid], L.nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int st = jst; st < ns; st += kStagers) {
+      // the slot is free once the owner has finished step st - kStagers
+      LdsWait(&sh.xReady, st - kStagers + 1, s.err);
+      PenStage<T, FWD, FULL>(b, P, s, sh.slot[jst], L, lane, st, nbId, nbW, nbTotal, known);
+      LdsStore(&sh.stageReady[jst], st + 1);
+    }
+  } else if (wave < 3) {
+    // ---- neighbour wave d
+    const int d = wave;
+    const GasT<T> g = Cast<T>(P.gas);
+    // in-pencil neighbour lane (clamped: lanes across the pencil face read a
+    // valid slot and use the staged du instead)
+    const int nl = min(max(lane + sgn * (d == 0 ? 1 : (d == 1 ? kPW : 0)), 0), kPenLanes - 1);
+    const bool cross = (d == 0 && (FWD ? L.li == 0 : L.li == L.wi - 1)) ||
+                       (d == 1 && (FWD ? L.lj == 0 : L.lj == L.wj - 1));
+    for (int st = 0; st < ns; ++st) {
+      const PenSlot<T, FULL> &sl = sh.slot[st % kStagers];
+      LdsWait(&sh.stageReady[st % kStagers], st + 1, s.err);
+      LdsWait(&sh.xReady, st, s.err);
+      const int k = L.K(st, b.nk, FWD);
+      const bool dep = k >= 0 && InBlock(b, L.i + (d == 0 ? sgn : 0), L.j + (d == 1 ? sgn : 0), k + (d == 2 ? sgn : 0));
+      T du[5], U[5], Wu[5], Fn[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        const T hx = sl.halo[d < 2 ? d : 0][e][lane], px = sh.sX[e][nl];
+        du[e] = cross ? hx : px;
+        U[e] = sl.U[d][e][lane] + du[e];
+      }
+      // UpdatePrim + PhysFlux of the updated state + the Rusanov terms
+      ConsToPrim(g, U, Wu);
+      PhysFlux(g, Wu, sl.n[d][0][lane], sl.n[d][1][lane], sl.n[d][2][lane], Fn);
+      const T half = sl.half[d][lane], lam = sl.lam[d][lane];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        const T fc = half * (Fn[e] - sl.Fo[d][e][lane]);
+        const T o = FWD ? fc + du[e] * lam : fc - du[e] * lam;
+        sh.sO[d][e][lane] = dep ? o : T(0);
+      }
+      LdsStore(&sh.oReady[d], st + 1);
+    }
+  } else {
+    // ---- owner
+    T *x = OpBuf<T>::X(b);
+    T *sink = static_cast<T *>(s.sinkv) + (size_t)pid * 5 * kPenLanes;
+    for (int st = 0; st < ns; ++st) {
+      const PenSlot<T, FULL> &sl = sh.slot[st % kStagers];
+      LdsWait(&sh.oReady[0], st + 1, s.err);
+      LdsWait(&sh.oReady[1], st + 1, s.err);
+      LdsWait(&sh.oReady[2], st + 1, s.err);
+      const int k = L.K(st, b.nk, FWD);
+      const T ainv = sl.ainv[lane];
+      // products of absent neighbours are +0 and leave D unchanged (D starts
+      // at +0 and is never -0)
+      T D[5], xn[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) D[e] = ((T(0) + sh.sO[0][e][lane]) + sh.sO[1][e][lane]) + sh.sO[2][e][lane];
+      if (FULL) {
+        if (FWD) {
+#pragma unroll
+          for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + (D[e] - sl.O[FULL ? e : 0][lane])) * ainv;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 5; ++e) xn[e] = ((T(sl.v5[e][lane]) + sl.O[FULL ? e : 0][lane]) - D[e]) * ainv;
+        }
+      } else if (FWD) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) xn[e] = (T(sl.v5[e][lane]) + D[e]) * ainv;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) xn[e] = T(sl.v5[e][lane]) - D[e] * ainv;
+      }
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sh.sX[e][lane] = xn[e];
+      LdsStore(&sh.xReady, st + 1);
+      const long p = (long)L.i + (long)L.j * b.ni + (long)(k < 0 ? 0 : k) * b.ni * b.nj;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) *(k >= 0 ? &x[e * b.ncell + p] : &sink[e * kPenLanes + lane]) = xn[e];
+      if ((st + 1) % kPubEvery == 0 || st + 1 == ns) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0)
+          __hip_atomic_store(&s.prog[pid], st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 

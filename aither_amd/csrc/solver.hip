@@ -256,6 +256,9 @@ int Upload(ath_ctx *c) {
       pc.order = dord;
       if (int r = c->Alloc(&pc.prog, pc.npen)) return r;
       if (int r = c->Alloc(&pc.ticket, 1)) return r;
+      double *sink = nullptr;
+      if (int r = c->Alloc(&sink, (size_t)pc.npen * 5 * kPenLanes)) return r;
+      pc.sinkv = sink;
       HIP_OK(hipMemset(pc.prog, 0, pc.npen * sizeof(int)));
       HIP_OK(hipMemset(pc.ticket, 0, sizeof(int)));
       pc.err = c->dflag;
@@ -392,7 +395,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         mark(true);
         for (auto &br : c->blk) {
           hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
-          const dim3 grid(br.pencil.npen), blk(ath::kPenThreads);
+          const dim3 grid(br.pencil.npen), blk(ath::kPenWaves * ath::kPenLanes);
           if (dir == 0 && full)
             hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
           else if (dir == 0)
