@@ -147,7 +147,7 @@ boundaryConditions: 1
     return path
 
 
-def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0):
+def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0, recon="thirdOrder"):
     """3D channel with a sin^2 bump on the lower wall; parameters seeded."""
     rng = np.random.default_rng(seed)
     n = n or int(rng.integers(24, 97))
@@ -179,7 +179,7 @@ equationSet: euler
 timeIntegration: implicitEuler
 cflStart: {cfl}
 cflMax: {cfl}
-faceReconstruction: thirdOrder
+faceReconstruction: {recon}
 limiter: vanAlbada
 iterations: {iterations}
 referenceTemperature: {t_inf!r}

@@ -349,6 +349,139 @@ __device__ __forceinline__ double Muscl(double u2, double u1, double d1, double 
   return u1 + (0.25 * ((u1 - u2) * dMinus)) * ((1.0 - kappa) * lim + (1.0 + kappa) * r * inv);
 }
 
+// ---------------------------------------------------------------------------
+// WENO5 / WENO-Z on non-uniform widths (include/reconstruction.hpp:155-310).
+// A stencil is five cells, three upwind (w[0..2], w[2] next to the face) and
+// two downwind; all loop bounds below are compile-time constants, so each
+// coefficient unrolls to straight-line arithmetic in the reference's order.
+
+// StencilWidth (include/utility.hpp:103-114)
+__device__ __forceinline__ double StencilW(const double w[5], int start, int end) {
+  double s = 0.0;
+  if (end > start) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q >= start && q < end) s += w[q];
+  } else if (start > end) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q >= end && q < start) t += w[q];
+    s = -1.0 * t;
+  }
+  return s;
+}
+
+// coefficient jj of LagrangeCoeff(w, DEG, RR, 2) (src/utility.cpp:449-483)
+template <int DEG, int RR>
+__device__ __forceinline__ double LagrangeCoeffAt(const double w[5], int jj) {
+  constexpr int ii = 2;
+  double c = 0.0;
+#pragma unroll
+  for (int mm = 1; mm <= DEG + 1; ++mm) {
+    if (mm < jj + 1) continue;
+    double numer = 0.0, denom = 1.0;
+#pragma unroll
+    for (int ll = 0; ll <= DEG + 1; ++ll) {
+      if (ll == mm) continue;
+      double prod = 1.0;
+#pragma unroll
+      for (int qq = 0; qq <= DEG + 1; ++qq)
+        if (qq != mm && qq != ll) prod *= StencilW(w, ii - RR + qq, ii + 1);
+      numer += prod;
+      denom *= StencilW(w, ii - RR + ll, ii - RR + mm);
+    }
+    c += numer / denom;
+  }
+  return c * w[ii - RR + jj];
+}
+
+// the width-only part of one WENO reconstruction
+struct WenoGeom {
+  double c0[3], c1[3], c2[3];
+  double lw0, lw1, lw2;
+};
+
+__device__ __forceinline__ WenoGeom MakeWenoGeom(const double w[5]) {
+  WenoGeom G;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    G.c0[q] = LagrangeCoeffAt<2, 2>(w, q);
+    G.c1[q] = LagrangeCoeffAt<2, 1>(w, q);
+    G.c2[q] = LagrangeCoeffAt<2, 0>(w, q);
+  }
+  // linear weights as the reference forms them (:273-276): lw1 is the
+  // weight of the downwind stencil and is applied to the central one
+  G.lw0 = LagrangeCoeffAt<4, 2>(w, 0) / G.c0[0];
+  G.lw1 = LagrangeCoeffAt<4, 2>(w, 4) / G.c2[2];
+  G.lw2 = 1.0 - G.lw0 - G.lw1;
+  return G;
+}
+
+// Derivative2nd (include/utility.hpp:116-122)
+__device__ __forceinline__ double Deriv2nd(double x0, double x1, double x2, double y0, double y1,
+                                           double y2) {
+  const double fwd = (y2 - y1) / (0.5 * (x2 + x1));
+  const double bck = (y1 - y0) / (0.5 * (x1 + x0));
+  return (fwd - bck) / (0.25 * (x2 + x0) + 0.5 * x1);
+}
+
+// BetaIntegral (include/reconstruction.hpp:157-181); x^3 correctly rounded
+__device__ __forceinline__ double BetaIntAt(double d1, double d2, double dx, double x) {
+  return (d1 * d1 * x + d1 * d2 * x * x + d2 * d2 * CubeCR(x) / 3.0) * dx +
+         d2 * d2 * x * CubeCR(dx);
+}
+
+// Beta0 / Beta1 / Beta2 (include/reconstruction.hpp:183-240)
+template <int WHICH>
+__device__ __forceinline__ double WenoBeta(double x0, double x1, double x2, double y0, double y1,
+                                           double y2) {
+  const double d2 = Deriv2nd(x0, x1, x2, y0, y1, y2);
+  double d1, h;
+  if (WHICH == 0) {
+    d1 = (y2 - y1) / (0.5 * (x2 + x1)) + 0.5 * x2 * d2;
+    h = x2;
+  } else if (WHICH == 1) {
+    d1 = (y2 - y1) / (0.5 * (x2 + x1)) - 0.5 * x1 * d2;
+    h = x1;
+  } else {
+    d1 = (y1 - y0) / (0.5 * (x1 + x0)) - 0.5 * x0 * d2;
+    h = x0;
+  }
+  return BetaIntAt(d1, d2, h, 0.5 * h) - BetaIntAt(d1, d2, h, -0.5 * h);
+}
+
+// FaceReconWENO for one variable (include/reconstruction.hpp:243-310)
+template <bool WENOZ>
+__device__ __forceinline__ double Weno(const WenoGeom &G, const double w[5], double u3, double u2,
+                                       double u1, double d1, double d2) {
+  const double s0 = G.c0[0] * u3 + G.c0[1] * u2 + G.c0[2] * u1;
+  const double s1 = G.c1[0] * u2 + G.c1[1] * u1 + G.c1[2] * d1;
+  const double s2 = G.c2[0] * u1 + G.c2[1] * d1 + G.c2[2] * d2;
+  const double b0 = WenoBeta<0>(w[0], w[1], w[2], u3, u2, u1);
+  const double b1 = WenoBeta<1>(w[1], w[2], w[3], u2, u1, d1);
+  const double b2 = WenoBeta<2>(w[2], w[3], w[4], u1, d1, d2);
+  double n0, n1, n2;
+  if (WENOZ) {
+    const double tau5 = fabs(b0 - b2);
+    const double t0 = tau5 / (1.0e-40 + b0), t1 = tau5 / (1.0e-40 + b1),
+                 t2 = tau5 / (1.0e-40 + b2);
+    n0 = G.lw0 * (1.0 + t0 * t0);
+    n1 = G.lw1 * (1.0 + t1 * t1);
+    n2 = G.lw2 * (1.0 + t2 * t2);
+  } else {
+    const double e0 = 1.0e-6 + b0, e1 = 1.0e-6 + b1, e2 = 1.0e-6 + b2;
+    n0 = G.lw0 / (e0 * e0);
+    n1 = G.lw1 / (e1 * e1);
+    n2 = G.lw2 / (e2 * e2);
+  }
+  const double sum = n0 + n1 + n2;
+  n0 /= sum;
+  n1 /= sum;
+  n2 /= sum;
+  return n0 * s0 + n1 * s1 + n2 * s2;
+}
+
 // InvCellSpectralRadius (include/spectralRadius.hpp:43-64)
 __device__ __forceinline__ double InvCellSpecRad(const GasT<double> &g, double rho, double u,
                                                  double v, double w, double p, double4 aL,

@@ -472,6 +472,8 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
         else if (val == "central") c.kappa = 1.0;
         else if (val == "thirdOrder") c.kappa = 1.0 / 3.0;
         else if (val == "constant") c.recon = Recon::Constant;
+        else if (val == "weno") c.recon = Recon::WENO;
+        else if (val == "wenoZ") c.recon = Recon::WENOZ;
         else return "unsupported faceReconstruction " + val;
       } else if (key == "limiter") {
         if (val == "none") c.limiter = Limiter::None;

@@ -15,7 +15,7 @@ namespace ath {
 
 constexpr int kNeq = 5;  // rho, u, v, w, p (single species, no turbulence)
 
-enum class Recon { Constant, MUSCL };
+enum class Recon { Constant, MUSCL, WENO, WENOZ };
 enum class Limiter { None = 0, VanAlbada = 1, Minmod = 2 };
 enum class Flux { Roe = 0, AUSM = 1 };
 enum class TimeInt { ExplicitEuler = 0, ImplicitEuler = 1, CrankNicholson = 2, BDF2 = 3 };
@@ -98,7 +98,10 @@ struct HostCase {
   bool Implicit() const { return timeint != TimeInt::ExplicitEuler; }
   bool MultiLevel() const { return timeint == TimeInt::BDF2; }
   bool InitMatrix() const { return solver == Solver::DPLUR || sweeps > 1; }
-  int Ghosts() const { return recon == Recon::Constant ? 1 : 2; }
+  // src/input.cpp:1127-1143
+  int Ghosts() const {
+    return recon == Recon::Constant ? 1 : (recon == Recon::MUSCL ? 2 : 3);
+  }
   double Cfl(int nn) const;
 };
 

@@ -231,6 +231,25 @@ __device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &
       WL[e] = b.W[e * b.npad + c0 - s];
       WR[e] = b.W[e * b.npad + c0];
     }
+  } else if (RECON >= 2) {
+    // src/procBlock.cpp:416-430: the lower state from cells -3..+1 around
+    // the face, the upper state from +2..-2
+    const double *cw = b.cw + d * b.npad;
+    double wc[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) wc[q] = cw[c0 + (q - 3) * s];
+    const double wl[5] = {wc[0], wc[1], wc[2], wc[3], wc[4]};
+    const double wr[5] = {wc[5], wc[4], wc[3], wc[2], wc[1]};
+    const WenoGeom GL = MakeWenoGeom(wl), GR = MakeWenoGeom(wr);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double *We = b.W + e * b.npad + c0;
+      double u[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) u[q] = We[(q - 3) * s];
+      WL[e] = Weno<RECON == 3>(GL, wl, u[0], u[1], u[2], u[3], u[4]);
+      WR[e] = Weno<RECON == 3>(GR, wr, u[5], u[4], u[3], u[2], u[1]);
+    }
   } else {
     const double *cw = b.cw + d * b.npad;
     const double wm2 = cw[c0 - 2 * s], wm1 = cw[c0 - s], w0 = cw[c0], wp1 = cw[c0 + s];

@@ -355,7 +355,7 @@ void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
 
 void LaunchResidual(ath_ctx *c, const BlockRes &br) {
   const auto &hc = c->hc;
-  const int recon = hc.recon == ath::Recon::MUSCL ? 1 : 0;
+  const int recon = (int)hc.recon;
   const int lim = (int)hc.limiter;
   const int flux = (int)hc.flux;
   ath::DevBlock d = br.d;
@@ -367,6 +367,9 @@ void LaunchResidual(ath_ctx *c, const BlockRes &br) {
   ATH_RES(0, 0, 0); ATH_RES(0, 0, 1);
   ATH_RES(1, 0, 0); ATH_RES(1, 1, 0); ATH_RES(1, 2, 0);
   ATH_RES(1, 0, 1); ATH_RES(1, 1, 1); ATH_RES(1, 2, 1);
+  // WENO ignores the limiter
+  if (recon == 2) return flux == 0 ? LaunchResidualT<2, 0, 0>(c, tmp) : LaunchResidualT<2, 0, 1>(c, tmp);
+  if (recon == 3) return flux == 0 ? LaunchResidualT<3, 0, 0>(c, tmp) : LaunchResidualT<3, 0, 1>(c, tmp);
   // constant reconstruction ignores the limiter
   if (recon == 0) return flux == 0 ? LaunchResidualT<0, 0, 0>(c, tmp) : LaunchResidualT<0, 0, 1>(c, tmp);
 #undef ATH_RES

@@ -125,7 +125,7 @@ struct orc_case {
   double cflStart, cflMax, cflStep, cfl;
   double theta, zeta, relax, dualCFL;
   double kappa;
-  int recon;   /* 0 constant, 1 MUSCL */
+  int recon;   /* 0 constant, 1 MUSCL, 2 WENO5, 3 WENO-Z */
   int limiter; /* 0 none, 1 vanAlbada, 2 minmod */
   int flux;    /* 0 roe, 1 ausm */
   int timeint; /* 0 explicitEuler 1 implicitEuler 2 crankNicholson 3 bdf2 */
@@ -443,6 +443,115 @@ void orc_muscl(const double *u2, const double *u1, const double *d1,
     double a = 0.25 * ((u1[e] - u2[e]) * dMinus);
     double b = (1.0 - kappa) * lim + (1.0 + kappa) * r * inv;
     out[e] = u1[e] + a * b;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5 / WENO-Z on non-uniform widths                                     */
+
+/* StencilWidth: include/utility.hpp:103-114 (signed sum of widths in
+ * [start, end), accumulated from the lower index) */
+static double stencil_width(const double *cw, int start, int end) {
+  double w = 0.0;
+  if (end > start) {
+    for (int q = start; q < end; ++q) w += cw[q];
+  } else if (start > end) {
+    double s = 0.0;
+    for (int q = end; q < start; ++q) s += cw[q];
+    w = -1.0 * s;
+  }
+  return w;
+}
+
+/* LagrangeCoeff: src/utility.cpp:449-483 (Shu, ICASE 97-65, eq. 2.20) */
+void orc_lagrange_coeff(const double *cw, int degree, int rr, int ii, double *coeffs) {
+  for (int jj = 0; jj <= degree; ++jj) {
+    coeffs[jj] = 0.0;
+    for (int mm = jj + 1; mm <= degree + 1; ++mm) {
+      double numer = 0.0, denom = 1.0;
+      for (int ll = 0; ll <= degree + 1; ++ll) {
+        if (ll == mm) continue;
+        double prod = 1.0;
+        for (int qq = 0; qq <= degree + 1; ++qq)
+          if (qq != mm && qq != ll) prod *= stencil_width(cw, ii - rr + qq, ii + 1);
+        numer += prod;
+        denom *= stencil_width(cw, ii - rr + ll, ii - rr + mm);
+      }
+      coeffs[jj] += numer / denom;
+    }
+    coeffs[jj] *= cw[ii - rr + jj];
+  }
+}
+
+/* Derivative2nd: include/utility.hpp:116-122 */
+static double deriv2nd(double x0, double x1, double x2, double y0, double y1, double y2) {
+  double fwd = (y2 - y1) / (0.5 * (x2 + x1));
+  double bck = (y1 - y0) / (0.5 * (x1 + x0));
+  return (fwd - bck) / (0.25 * (x2 + x0) + 0.5 * x1);
+}
+
+/* BetaIntegral: include/reconstruction.hpp:157-181 */
+static double beta_int_at(double d1, double d2, double dx, double x) {
+  return (d1 * d1 * x + d1 * d2 * x * x + d2 * d2 * pow(x, 3.0) / 3.0) * dx +
+         d2 * d2 * x * pow(dx, 3.0);
+}
+static double beta_int(double d1, double d2, double dx, double xl, double xh) {
+  return beta_int_at(d1, d2, dx, xh) - beta_int_at(d1, d2, dx, xl);
+}
+
+/* Beta0/Beta1/Beta2: include/reconstruction.hpp:183-240; which = 0, 1, 2 */
+double orc_weno_beta(int which, double x0, double x1, double x2, double y0, double y1,
+                     double y2) {
+  double d2 = deriv2nd(x0, x1, x2, y0, y1, y2);
+  if (which == 0) {
+    double d1 = (y2 - y1) / (0.5 * (x2 + x1)) + 0.5 * x2 * d2;
+    return beta_int(d1, d2, x2, -0.5 * x2, 0.5 * x2);
+  } else if (which == 1) {
+    double d1 = (y2 - y1) / (0.5 * (x2 + x1)) - 0.5 * x1 * d2;
+    return beta_int(d1, d2, x1, -0.5 * x1, 0.5 * x1);
+  }
+  double d1 = (y1 - y0) / (0.5 * (x1 + x0)) - 0.5 * x0 * d2;
+  return beta_int(d1, d2, x0, -0.5 * x0, 0.5 * x0);
+}
+
+/* FaceReconWENO: include/reconstruction.hpp:243-310.  u3, u2, u1 are the
+ * upwind cells (u1 adjacent to the face), d1, d2 the downwind cells; w[5]
+ * their widths in the same order. */
+void orc_weno(const double *u3, const double *u2, const double *u1, const double *d1,
+              const double *d2, const double *w, int wenoZ, double *out) {
+  double c0[3], c1[3], c2[3], full[5];
+  orc_lagrange_coeff(w, 2, 2, 2, c0);
+  orc_lagrange_coeff(w, 2, 1, 2, c1);
+  orc_lagrange_coeff(w, 2, 0, 2, c2);
+  orc_lagrange_coeff(w, 4, 2, 2, full);
+  double lw0 = full[0] / c0[0];
+  double lw1 = full[4] / c2[2];
+  double lw2 = 1.0 - lw0 - lw1;
+  for (int e = 0; e < NEQ; ++e) {
+    double s0 = c0[0] * u3[e] + c0[1] * u2[e] + c0[2] * u1[e];
+    double s1 = c1[0] * u2[e] + c1[1] * u1[e] + c1[2] * d1[e];
+    double s2 = c2[0] * u1[e] + c2[1] * d1[e] + c2[2] * d2[e];
+    double b0 = orc_weno_beta(0, w[0], w[1], w[2], u3[e], u2[e], u1[e]);
+    double b1 = orc_weno_beta(1, w[1], w[2], w[3], u2[e], u1[e], d1[e]);
+    double b2 = orc_weno_beta(2, w[2], w[3], w[4], u1[e], d1[e], d2[e]);
+    double n0, n1, n2;
+    if (wenoZ) {
+      double tau5 = fabs(b0 - b2);
+      double t0 = tau5 / (1.0e-40 + b0), t1 = tau5 / (1.0e-40 + b1), t2 = tau5 / (1.0e-40 + b2);
+      n0 = lw0 * (1.0 + t0 * t0);
+      n1 = lw1 * (1.0 + t1 * t1);
+      n2 = lw2 * (1.0 + t2 * t2);
+    } else {
+      double e0 = 1.0e-6 + b0, e1 = 1.0e-6 + b1, e2 = 1.0e-6 + b2;
+      n0 = lw0 / (e0 * e0);
+      n1 = lw1 / (e1 * e1);
+      n2 = lw2 / (e2 * e2);
+    }
+    double sum = n0 + n1 + n2;
+    n0 /= sum;
+    n1 /= sum;
+    n2 /= sum;
+    out[e] = n0 * s0 + n1 * s1 + n2 * s2;
   }
 }
 
@@ -1114,6 +1223,8 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
       else if (!strcmp(val, "central")) c->kappa = 1.0;
       else if (!strcmp(val, "thirdOrder")) c->kappa = 1.0 / 3.0;
       else if (!strcmp(val, "constant")) c->recon = 0;
+      else if (!strcmp(val, "weno")) c->recon = 2;
+      else if (!strcmp(val, "wenoZ")) c->recon = 3;
       else { seterr(err, errlen, "unsupported faceReconstruction %s", val); goto fail; }
     } else if (!strcmp(key, "limiter")) {
       if (!strcmp(val, "none")) c->limiter = 0;
@@ -1259,7 +1370,7 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
     }
     c->nb = nbg;
     c->blk = (blk_t *)calloc((size_t)nbg, sizeof(blk_t));
-    int g = (c->recon == 0) ? 1 : 2; /* src/input.cpp:1127-1143 */
+    int g = (c->recon == 0) ? 1 : (c->recon == 1 ? 2 : 3); /* src/input.cpp:1127-1143 */
     c->totalCells = 0;
     for (int bq = 0; bq < nbg; ++bq) {
       blk_t *b = &c->blk[bq];
@@ -1597,6 +1708,20 @@ static void inv_flux_dir(orc_case *c, blk_t *b, int d) {
         if (c->recon == 0) {
           memcpy(WL, Wm1, sizeof WL);
           memcpy(WR, W0, sizeof WR);
+        } else if (c->recon >= 2) {
+          /* src/procBlock.cpp:416-430: lower side from cells -3..+1 around
+           * the face, upper side from +2..-2 */
+          const double *Wc[6];
+          double wc[6];
+          for (int q = 0; q < 6; ++q) {
+            long cc = cidx(b, i + (q - 3) * ei, j + (q - 3) * ej, k + (q - 3) * ek);
+            Wc[q] = &b->W[cc * NEQ];
+            wc[q] = b->cw[d][cc];
+          }
+          double wl[5] = {wc[0], wc[1], wc[2], wc[3], wc[4]};
+          double wr[5] = {wc[5], wc[4], wc[3], wc[2], wc[1]};
+          orc_weno(Wc[0], Wc[1], Wc[2], Wc[3], Wc[4], wl, c->recon == 3, WL);
+          orc_weno(Wc[5], Wc[4], Wc[3], Wc[2], Wc[1], wr, c->recon == 3, WR);
         } else {
           const double *Wm2 = &b->W[cidx(b, i - 2 * ei, j - 2 * ej, k - 2 * ek) * NEQ];
           const double *Wp1 = &b->W[cidx(b, i + ei, j + ej, k + ek) * NEQ];

@@ -108,6 +108,11 @@ void orc_ausm_flux(const orc_gas *g, const double *WL, const double *WR,
 void orc_phys_flux(const orc_gas *g, const double *W, const double *n,
                    double *F);
 /* limiter: 0 none, 1 vanAlbada, 2 minmod */
+void orc_lagrange_coeff(const double *cw, int degree, int rr, int ii, double *coeffs);
+double orc_weno_beta(int which, double x0, double x1, double x2, double y0, double y1,
+                     double y2);
+void orc_weno(const double *u3, const double *u2, const double *u1, const double *d1,
+              const double *d2, const double *w, int wenoZ, double *out);
 void orc_muscl(const double *u2, const double *u1, const double *d1,
                double kappa, int limiter, double uw2, double uw, double dw,
                double *out);

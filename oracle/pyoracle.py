@@ -69,6 +69,10 @@ def lib():
         L.orc_phys_flux.argtypes = [G, _D, _D, _D]
         L.orc_muscl.argtypes = [_D, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, _D]
+        L.orc_lagrange_coeff.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_weno_beta.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6
+        L.orc_weno_beta.restype = ctypes.c_double
+        L.orc_weno.argtypes = [_D] * 6 + [ctypes.c_int, _D]
         L.orc_pyramid_volume.argtypes = [_D] * 5
         L.orc_pyramid_volume.restype = ctypes.c_double
         L.orc_hyperplane_order.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)]

@@ -234,6 +234,28 @@ def test_explicit_and_constant_reconstruction(tmp_path):
     s.close()
 
 
+@pytest.mark.parametrize("recon", ["weno", "wenoZ"])
+@pytest.mark.parametrize("grid", ["vortex", "bump"])
+def test_weno_reconstruction(tmp_path, recon, grid):
+    """WENO5 / WENO-Z with three ghost layers, on the uniform vortex grid and
+    the bump grid (non-uniform widths), Roe and AUSMPW+."""
+    if grid == "vortex":
+        path = cases.vortex_case(str(tmp_path), n=10, iterations=6, recon=recon)
+    else:
+        path = cases.bump_case(str(tmp_path), seed=4, n=10, iterations=6, recon=recon)[0]
+    for flux in ("roe", "ausm"):
+        text = open(path).read()
+        if flux == "ausm":
+            text = text.replace("boundaryConditions:", "inviscidFlux: ausm\nboundaryConditions:", 1)
+        p = os.path.join(str(tmp_path), f"{flux}.inp")
+        open(p, "w").write(text)
+        say("WENO parity", recon, grid, flux)
+        s, o, hs, ho = run_both(p, 6)
+        assert s.info.num_ghosts == 3
+        assert rel(s.get_state(), o.state()) <= 1e-10
+        s.close()
+
+
 def test_ausm_minmod(tmp_path):
     path = cases.vortex_case(str(tmp_path), n=10, iterations=5, flux="ausm", limiter="minmod")
     say("AUSMPW+ with minmod")

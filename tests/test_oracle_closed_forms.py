@@ -232,7 +232,7 @@ def test_hyperplane_order():
         assert len(set(map(tuple, cells))) == ni * nj * nk
 
 
-def _distorted_case(tmp, n=6, seed=0, bc="characteristic"):
+def _distorted_case(tmp, n=6, seed=0, bc="characteristic", recon="thirdOrder"):
     rng = np.random.default_rng(seed)
     i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
     h = 1.0 / n
@@ -247,7 +247,7 @@ equationSet: euler
 timeIntegration: implicitEuler
 cflStart: 5
 cflMax: 5
-faceReconstruction: thirdOrder
+faceReconstruction: {recon}
 limiter: vanAlbada
 iterations: 3
 referenceTemperature: 288.15
@@ -260,15 +260,25 @@ boundaryConditions: 1
     return os.path.join(tmp, "dist.inp")
 
 
-def test_freestream_preservation(tmp_path):
+@pytest.mark.parametrize("recon", ["thirdOrder", "weno", "wenoZ"])
+def test_freestream_preservation(tmp_path, recon):
     """Uniform state with farfield BCs at the same state: R = 0 to rounding on
-    a distorted grid (ghost geometry, MUSCL, Roe, face areas, accumulation)."""
-    inp = _distorted_case(str(tmp_path))
+    a distorted grid (ghost geometry, MUSCL or WENO with three ghost layers,
+    Roe, face areas, accumulation).
+
+    With three ghost layers the extrapolating BCs set the third layer to
+    layer * ghost - interior = 2 W for a uniform W
+    (src/ghostStates.cpp:380-382), so the boundary faces' WENO stencils are
+    not uniform; there the check covers the cells whose faces are all
+    interior."""
+    inp = _distorted_case(str(tmp_path), recon=recon)
     c = pyoracle.OracleCase(inp)
     c.bc()
     c.calc_residual()
-    R = c.residual()
+    R = c.residual().reshape(6, 6, 6, 5)  # k, j, i
     flux_scale = (1.0 / 1.4) * (1.0 / 6) ** 2  # p |A|, nondimensional
+    if recon != "thirdOrder":
+        R = R[1:-1, 1:-1, 1:-1]
     assert np.abs(R).max() < 1e-13 * flux_scale
 
 
@@ -312,3 +322,139 @@ def test_dplur_and_lusgs_solve_the_same_system(tmp_path):
         assert mr < 1e-20
     scale = np.abs(xs["lusgs"]).max()
     assert np.abs(xs["lusgs"] - xs["dplur"]).max() < 1e-10 * scale
+
+
+# ---------------------------------------------------------------------------
+# WENO5 / WENO-Z (include/reconstruction.hpp:155-310, src/utility.cpp:449-483)
+
+def lagrange(widths, degree, rr):
+    w = arr(widths)
+    out = np.zeros(degree + 1)
+    lib().orc_lagrange_coeff(P(w), degree, rr, 2, P(out))
+    return out
+
+
+def weno(cells, widths, z=False):
+    """cells: five 5-vectors (three upwind, u1 next to the face, two downwind)."""
+    cs = [arr(c) for c in cells]
+    w = arr(widths)
+    out = np.zeros(5)
+    lib().orc_weno(*[P(c) for c in cs], P(w), int(z), P(out))
+    return out
+
+
+def test_lagrange_coefficients_uniform_are_shu_values():
+    # Shu (ICASE 97-65) Table 2.1, k = 3, and the k = 5 row for r = 2
+    for h in (1.0, 0.37):
+        w = [h] * 5
+        assert np.allclose(lagrange(w, 2, 2), [1 / 3, -7 / 6, 11 / 6], rtol=1e-14, atol=1e-14)
+        assert np.allclose(lagrange(w, 2, 1), [-1 / 6, 5 / 6, 1 / 3], rtol=1e-14, atol=1e-14)
+        assert np.allclose(lagrange(w, 2, 0), [1 / 3, 5 / 6, -1 / 6], rtol=1e-14, atol=1e-14)
+        assert np.allclose(lagrange(w, 4, 2), np.array([2, -13, 47, 27, -3]) / 60.0,
+                           rtol=1e-14, atol=1e-14)
+
+
+def test_lagrange_coefficients_exact_for_polynomial_cell_averages():
+    # on arbitrary widths, the stencil applied to exact cell averages of a
+    # polynomial of degree <= stencil size - 1 returns its value at the face
+    # between cell 2 and cell 3
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        w = rng.uniform(0.3, 2.0, 5)
+        edges = np.concatenate([[0.0], np.cumsum(w)])
+        xf = edges[3]
+        for degree, rr, cells in ((2, 2, [0, 1, 2]), (2, 1, [1, 2, 3]), (2, 0, [2, 3, 4]),
+                                  (4, 2, [0, 1, 2, 3, 4])):
+            c = rng.normal(size=degree + 1)
+            prim = np.polynomial.Polynomial(c).integ()
+            avg = np.array([(prim(edges[q + 1]) - prim(edges[q])) / w[q] for q in cells])
+            got = lagrange(w, degree, rr) @ avg
+            want = np.polynomial.Polynomial(c)(xf)
+            assert abs(got - want) <= 1e-11 * max(1.0, abs(want)), (degree, rr)
+
+
+def test_weno_smoothness_indicators_are_jiang_shu_on_uniform_widths():
+    rng = np.random.default_rng(3)
+    L = lib()
+    for _ in range(50):
+        h = rng.uniform(0.1, 3.0)
+        y0, y1, y2 = rng.normal(size=3)
+        # Jiang & Shu (1996) eq. 3.2-3.4: the reference's integrals reduce to
+        # these for any uniform width (the indicator is scale free)
+        js0 = 13 / 12 * (y0 - 2 * y1 + y2) ** 2 + 0.25 * (y0 - 4 * y1 + 3 * y2) ** 2
+        js1 = 13 / 12 * (y0 - 2 * y1 + y2) ** 2 + 0.25 * (y0 - y2) ** 2
+        js2 = 13 / 12 * (y0 - 2 * y1 + y2) ** 2 + 0.25 * (3 * y0 - 4 * y1 + y2) ** 2
+        for which, want in ((0, js0), (1, js1), (2, js2)):
+            got = L.orc_weno_beta(which, h, h, h, y0, y1, y2)
+            assert abs(got - want) <= 1e-12 * max(1.0, want), (which, got, want)
+
+
+def test_weno_reproduces_quadratics_and_constants():
+    rng = np.random.default_rng(11)
+    for z in (False, True):
+        for _ in range(10):
+            w = rng.uniform(0.5, 1.5, 5)
+            edges = np.concatenate([[0.0], np.cumsum(w)])
+            coef = rng.normal(size=(5, 3))
+            cells = []
+            for q in range(5):
+                cells.append([np.polynomial.Polynomial(coef[e]).integ()(edges[q + 1]) / w[q] -
+                              np.polynomial.Polynomial(coef[e]).integ()(edges[q]) / w[q]
+                              for e in range(5)])
+            want = [np.polynomial.Polynomial(coef[e])(edges[3]) for e in range(5)]
+            assert np.allclose(weno(cells, w, z), want, rtol=1e-10, atol=1e-10)
+        const = [[1.3, -0.2, 0.5, 0.1, 2.5]] * 5
+        assert np.allclose(weno(const, rng.uniform(0.5, 1.5, 5), z), const[0], rtol=1e-15)
+
+
+def test_weno_linear_weights_as_the_reference_assigns_them():
+    # Mirror-symmetric data about the face-adjacent cell (u3 = d2, u2 = d1)
+    # on uniform widths gives beta0 = beta2, so WENO-Z's weights are exactly
+    # its linear weights.  The reference forms lw0 = 0.1 for the upwind
+    # stencil, lw1 = full[4] / c2[2] = 0.3 and lw2 = 0.6, and applies lw1 to
+    # the central stencil (include/reconstruction.hpp:273-306); Shu's
+    # textbook assignment (0.1, 0.6, 0.3) gives a different value here.
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        u3, u2, u1 = rng.normal(size=(3, 5))
+        cells = [u3, u2, u1, u2, u3]
+        s0 = u3 / 3 - 7 / 6 * u2 + 11 / 6 * u1
+        s1 = -u2 / 6 + 5 / 6 * u1 + u2 / 3
+        s2 = u1 / 3 + 5 / 6 * u2 - u3 / 6
+        ref = 0.1 * s0 + 0.3 * s1 + 0.6 * s2
+        shu = 0.1 * s0 + 0.6 * s1 + 0.3 * s2
+        got = weno(cells, [0.8] * 5, z=True)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+        assert np.max(np.abs(ref - shu)) > 1e-3
+
+
+def test_weno_weights_follow_smoothness():
+    # a jump inside the upwind stencil: the classic weights put almost all of
+    # the weight on the smooth downwind stencil (u1, d1, d2)
+    cells = [[10.0] * 5, [10.0] * 5, [1.0] * 5, [1.0] * 5, [1.0] * 5]
+    for z in (False, True):
+        got = weno(cells, [1.0] * 5, z)
+        assert np.allclose(got, 1.0, atol=1e-6)
+
+
+def test_weno_smoothness_indicators_on_nonuniform_widths():
+    # linear point data y = s*x at the cell centres of arbitrary widths: the
+    # second difference vanishes and each indicator is (s * h)^2 with h the
+    # width of the cell it is integrated over (x2, x1, x0 for beta 0, 1, 2);
+    # every indicator is scale free in the widths and quadratic in the data
+    rng = np.random.default_rng(9)
+    L = lib()
+    for _ in range(50):
+        x = rng.uniform(0.2, 2.0, 3)
+        ctr = np.array([0.5 * x[0], x[0] + 0.5 * x[1], x[0] + x[1] + 0.5 * x[2]])
+        s, a = rng.normal(size=2)
+        y = a + s * ctr
+        for which, h in ((0, x[2]), (1, x[1]), (2, x[0])):
+            got = L.orc_weno_beta(which, *x, *y)
+            assert abs(got - (s * h) ** 2) <= 1e-12 * max(1.0, (s * h) ** 2), which
+        yq = rng.normal(size=3)
+        lam, alpha = rng.uniform(0.1, 10.0), rng.uniform(0.1, 10.0)
+        for which in range(3):
+            b = L.orc_weno_beta(which, *x, *yq)
+            assert abs(L.orc_weno_beta(which, *(lam * x), *yq) - b) <= 1e-12 * b
+            assert abs(L.orc_weno_beta(which, *x, *(alpha * yq)) - alpha ** 2 * b) <= 1e-12 * alpha ** 2 * b
