@@ -114,7 +114,20 @@ typedef struct {
   int nsurf;
   surf_t *surf;
   int ictag;
+  int nconn;
+  struct conn_s *conn;
+  unsigned char *isconn[6]; /* per boundary face of each surface: 1 on a connection */
 } blk_t;
+
+/* one side of an interblock connection, as seen from the block that owns
+ * the ghost cells; patch-local cell (p1, p2) along this block's d1, d2 maps
+ * to the partner's patch-local cell (q1, q2):
+ *   c = swap ? (p2, p1) : (p1, p2);  q1 = rev1 ? pn1-1-c1 : c1;  q2 likewise */
+typedef struct conn_s {
+  int s, dir, r3, r1s, n1, r2s, n2;
+  int pb, ps, pdir, pr3, pr1s, pn1, pr2s, pn2;
+  int swap, rev1, rev2;
+} conn_t;
 
 struct orc_case {
   char dir[1024];
@@ -820,6 +833,260 @@ static void cell_widths(blk_t *b) {
       }
 }
 
+/* ------------------------------------------------------------------------ */
+/* interblock connections                                                   */
+
+static void seterr(char *err, int errlen, const char *fmt, ...);
+
+/* partner block and surface from the tag: src/boundaryConditions.cpp:2455-2495 */
+static int tag_partner_surface(int tag) {
+  if (tag < 2000) return 1;
+  if (tag < 3000) return 2;
+  if (tag < 4000) return 3;
+  if (tag < 5000) return 4;
+  if (tag < 6000) return 5;
+  if (tag < 7000) return 6;
+  return -1;
+}
+
+static v3 node_at(const v3 *nodes, const blk_t *b, int dir, int r3, int a1, int a2) {
+  int i, j, k;
+  map3(dir, r3, a1, a2, &i, &j, &k);
+  return nodes[(long)i + (long)(b->ni + 1) * (j + (long)(b->nj + 1) * k)];
+}
+
+static int same_point(v3 a, v3 b) {
+  /* vector3d::CompareWithTol, include/vector3d.hpp:319-327 */
+  return fabs(a.x - b.x) < 1.0e-10 && fabs(a.y - b.y) < 1.0e-10 && fabs(a.z - b.z) < 1.0e-10;
+}
+
+static void conn_map(const conn_t *cn, int p1, int p2, int cells, int *q1, int *q2) {
+  int c1 = cn->swap ? p2 : p1, c2 = cn->swap ? p1 : p2;
+  int m1 = cells ? cn->pn1 - 1 : cn->pn1, m2 = cells ? cn->pn2 - 1 : cn->pn2;
+  *q1 = cn->rev1 ? m1 - c1 : c1;
+  *q2 = cn->rev2 ? m2 - c2 : c2;
+}
+
+/* patch match by three corners (TestPatchMatch, src/boundaryConditions.cpp:
+ * 729-831): find how the partner's patch axes map onto this patch's */
+static int orient_patch(conn_t *cn, const orc_case *c, const v3 *const *nodes, int a) {
+  const blk_t *A = &c->blk[a], *B = &c->blk[cn->pb];
+  int corners[3][2] = {{0, 0}, {cn->n1, 0}, {0, cn->n2}};
+  for (int swap = 0; swap < 2; ++swap)
+    for (int rev1 = 0; rev1 < 2; ++rev1)
+      for (int rev2 = 0; rev2 < 2; ++rev2) {
+        cn->swap = swap;
+        cn->rev1 = rev1;
+        cn->rev2 = rev2;
+        if (swap ? (cn->n1 != cn->pn2 || cn->n2 != cn->pn1) : (cn->n1 != cn->pn1 || cn->n2 != cn->pn2))
+          continue;
+        int ok = 1;
+        for (int q = 0; q < 3 && ok; ++q) {
+          int b1, b2;
+          conn_map(cn, corners[q][0], corners[q][1], 0, &b1, &b2);
+          v3 pa = node_at(nodes[a], A, cn->dir, cn->r3, cn->r1s + corners[q][0], cn->r2s + corners[q][1]);
+          v3 pbn = node_at(nodes[cn->pb], B, cn->pdir, cn->pr3, cn->pr1s + b1, cn->pr2s + b2);
+          ok = same_point(pa, pbn);
+        }
+        if (ok) return 0;
+      }
+  return -1;
+}
+
+/* this side's ghost cell at layer l (1..g) and the partner's interior cell at
+ * layer l, patch-local (p1, p2): the ghost cells continue the partner block
+ * as if the grid were not cut (src/boundaryConditions.cpp:3006-3185) */
+static long conn_ghost(const conn_t *cn, const blk_t *A, int l, int p1, int p2) {
+  int n3 = (cn->s % 2 == 1) ? -l : ndir(A, cn->dir) + l - 1;
+  int i, j, k;
+  map3(cn->dir, n3, cn->r1s + p1, cn->r2s + p2, &i, &j, &k);
+  return cidx(A, i, j, k);
+}
+static long conn_source(const conn_t *cn, const blk_t *B, int l, int p1, int p2) {
+  int q1, q2;
+  conn_map(cn, p1, p2, 1, &q1, &q2);
+  int n3 = (cn->ps % 2 == 1) ? l - 1 : ndir(B, cn->pdir) - l;
+  int i, j, k;
+  map3(cn->pdir, n3, cn->pr1s + q1, cn->pr2s + q2, &i, &j, &k);
+  return cidx(B, i, j, k);
+}
+
+/* SwapGeomSlice / procBlock::PutGeomSlice (src/utility.cpp:213-256,
+ * src/procBlock.cpp:3167-4011), restricted to the patch: each ghost cell
+ * takes the partner cell's volume and centroid; its faces normal to the
+ * patch (the boundary face included) take the partner's faces at the same
+ * distance from the cut, with the area direction reversed when both patches
+ * are lower or both upper surfaces; its faces along the patch take the
+ * partner's, swapped and reversed where the partner's axis runs backwards.
+ * All swaps read the geometry as it was before any of them. */
+typedef struct { double *vol; v3 *ctr; uv3 *fa[3]; v3 *fc[3]; } geom_copy;
+
+static void swap_geometry(orc_case *c) {
+  geom_copy *gc = (geom_copy *)calloc((size_t)c->nb, sizeof(geom_copy));
+  for (int bq = 0; bq < c->nb; ++bq) {
+    blk_t *b = &c->blk[bq];
+    long nc = ncells_padded(b);
+    gc[bq].vol = (double *)malloc(sizeof(double) * (size_t)nc);
+    gc[bq].ctr = (v3 *)malloc(sizeof(v3) * (size_t)nc);
+    memcpy(gc[bq].vol, b->vol, sizeof(double) * (size_t)nc);
+    memcpy(gc[bq].ctr, b->ctr, sizeof(v3) * (size_t)nc);
+    for (int d = 0; d < 3; ++d) {
+      long nf = nfaces_padded(b, d);
+      gc[bq].fa[d] = (uv3 *)malloc(sizeof(uv3) * (size_t)nf);
+      gc[bq].fc[d] = (v3 *)malloc(sizeof(v3) * (size_t)nf);
+      memcpy(gc[bq].fa[d], b->fa[d], sizeof(uv3) * (size_t)nf);
+      memcpy(gc[bq].fc[d], b->fc[d], sizeof(v3) * (size_t)nf);
+    }
+  }
+  for (int a = 0; a < c->nb; ++a) {
+    blk_t *A = &c->blk[a];
+    for (int q = 0; q < A->nconn; ++q) {
+      const conn_t *cn = &A->conn[q];
+      const blk_t *B = &c->blk[cn->pb];
+      const geom_copy *G = &gc[cn->pb];
+      double f3 = ((cn->s + cn->ps) % 2 == 0) ? -1.0 : 1.0;
+      int lowA = cn->s % 2 == 1, lowB = cn->ps % 2 == 1;
+      int nA = ndir(A, cn->dir), nB = ndir(B, cn->pdir);
+      int dA1 = (cn->dir + 1) % 3, dA2 = (cn->dir + 2) % 3;
+      int dB1 = (cn->pdir + 1) % 3, dB2 = (cn->pdir + 2) % 3;
+      for (int p2 = 0; p2 < cn->n2; ++p2)
+        for (int p1 = 0; p1 < cn->n1; ++p1) {
+          int q1, q2;
+          conn_map(cn, p1, p2, 1, &q1, &q2);
+          /* faces normal to the patch at distance t = 0..g from the cut */
+          for (int t = 0; t <= A->g; ++t) {
+            int fa3 = lowA ? -t : nA + t, fb3 = lowB ? t : nB - t;
+            int i, j, k, bi, bj, bk;
+            map3(cn->dir, fa3, cn->r1s + p1, cn->r2s + p2, &i, &j, &k);
+            map3(cn->pdir, fb3, cn->pr1s + q1, cn->pr2s + q2, &bi, &bj, &bk);
+            uv3 v = G->fa[cn->pdir][fidx(B, cn->pdir, bi, bj, bk)];
+            if (f3 < 0.0) v.u = vscl(-1.0, v.u);
+            A->fa[cn->dir][fidx(A, cn->dir, i, j, k)] = v;
+            A->fc[cn->dir][fidx(A, cn->dir, i, j, k)] = G->fc[cn->pdir][fidx(B, cn->pdir, bi, bj, bk)];
+          }
+          for (int l = 1; l <= A->g; ++l) {
+            long ga = conn_ghost(cn, A, l, p1, p2), sb = conn_source(cn, B, l, p1, p2);
+            A->vol[ga] = G->vol[sb];
+            A->ctr[ga] = G->ctr[sb];
+            int na3 = lowA ? -l : nA + l - 1, nb3 = lowB ? l - 1 : nB - l;
+            int gi, gj, gk, si, sj, sk;
+            map3(cn->dir, na3, cn->r1s + p1, cn->r2s + p2, &gi, &gj, &gk);
+            map3(cn->pdir, nb3, cn->pr1s + q1, cn->pr2s + q2, &si, &sj, &sk);
+            /* faces along the patch: axis dA1 <-> (swap ? dB2 : dB1), dA2 <-> the other */
+            for (int w = 0; w < 2; ++w) {
+              int da = w == 0 ? dA1 : dA2;
+              int useB1 = (w == 0) != (cn->swap != 0);
+              int db = useB1 ? dB1 : dB2;
+              int rev = useB1 ? cn->rev1 : cn->rev2;
+              for (int up = 0; up < 2; ++up) {
+                int ai = gi + (da == 0 && up), aj = gj + (da == 1 && up), ak = gk + (da == 2 && up);
+                int bup = rev ? !up : up;
+                int xi = si + (db == 0 && bup), xj = sj + (db == 1 && bup), xk = sk + (db == 2 && bup);
+                uv3 v = G->fa[db][fidx(B, db, xi, xj, xk)];
+                if (rev) v.u = vscl(-1.0, v.u);
+                A->fa[da][fidx(A, da, ai, aj, ak)] = v;
+                A->fc[da][fidx(A, da, ai, aj, ak)] = G->fc[db][fidx(B, db, xi, xj, xk)];
+              }
+            }
+          }
+        }
+    }
+  }
+  for (int bq = 0; bq < c->nb; ++bq) {
+    free(gc[bq].vol);
+    free(gc[bq].ctr);
+    for (int d = 0; d < 3; ++d) { free(gc[bq].fa[d]); free(gc[bq].fc[d]); }
+  }
+  free(gc);
+}
+
+/* pair every interblock surface with its partner patch
+ * (GetBlockInterConnBCs, src/boundaryConditions.cpp:603-642) */
+static int connect_blocks(orc_case *c, const v3 *const *nodes, char *err, int errlen) {
+  for (int a = 0; a < c->nb; ++a) {
+    blk_t *A = &c->blk[a];
+    for (int q = 0; q < 6; ++q) {
+      int d = (q / 2), d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+      int n = (d1 == 0 ? A->ni : d1 == 1 ? A->nj : A->nk) * (d2 == 0 ? A->ni : d2 == 1 ? A->nj : A->nk);
+      A->isconn[q] = (unsigned char *)calloc((size_t)n, 1);
+    }
+    for (int s = 0; s < A->nsurf; ++s) {
+      const surf_t *sf = &A->surf[s];
+      if (strcmp(sf->type, "interblock")) continue;
+      conn_t cn;
+      memset(&cn, 0, sizeof cn);
+      cn.s = surf_type(sf);
+      cn.dir = (cn.s - 1) / 2;
+      int d1 = (cn.dir + 1) % 3, d2 = (cn.dir + 2) % 3;
+      cn.r3 = sf->d[2 * cn.dir];
+      cn.r1s = sf->d[2 * d1];
+      cn.n1 = sf->d[2 * d1 + 1] - cn.r1s;
+      cn.r2s = sf->d[2 * d2];
+      cn.n2 = sf->d[2 * d2 + 1] - cn.r2s;
+      cn.ps = tag_partner_surface(sf->d[6]);
+      cn.pb = sf->d[6] - 1000 * cn.ps;
+      if (cn.ps < 0 || cn.pb < 0 || cn.pb >= c->nb) {
+        seterr(err, errlen, "block %d: bad interblock tag %d", a, sf->d[6]);
+        return -1;
+      }
+      cn.pdir = (cn.ps - 1) / 2;
+      const blk_t *B = &c->blk[cn.pb];
+      int found = 0;
+      for (int t = 0; t < B->nsurf && !found; ++t) {
+        const surf_t *pf = &B->surf[t];
+        if (strcmp(pf->type, "interblock")) continue;
+        int pps = tag_partner_surface(pf->d[6]);
+        if (surf_type(pf) != cn.ps || pps != cn.s || pf->d[6] - 1000 * pps != a) continue;
+        if (cn.pb == a && pf == sf) continue;
+        int e1 = (cn.pdir + 1) % 3, e2 = (cn.pdir + 2) % 3;
+        cn.pr3 = pf->d[2 * cn.pdir];
+        cn.pr1s = pf->d[2 * e1];
+        cn.pn1 = pf->d[2 * e1 + 1] - cn.pr1s;
+        cn.pr2s = pf->d[2 * e2];
+        cn.pn2 = pf->d[2 * e2 + 1] - cn.pr2s;
+        if (orient_patch(&cn, c, nodes, a) == 0) found = 1;
+      }
+      if (!found) {
+        seterr(err, errlen, "block %d: no matching partner patch for interblock tag %d", a, sf->d[6]);
+        return -1;
+      }
+      if (ndir(B, cn.pdir) < A->g) {
+        seterr(err, errlen, "block %d: partner block %d is thinner than the ghost layers", a, cn.pb);
+        return -1;
+      }
+      A->conn = (conn_t *)realloc(A->conn, sizeof(conn_t) * (size_t)(A->nconn + 1));
+      A->conn[A->nconn++] = cn;
+      int n1full = d1 == 0 ? A->ni : d1 == 1 ? A->nj : A->nk;
+      for (int p2 = 0; p2 < cn.n2; ++p2)
+        for (int p1 = 0; p1 < cn.n1; ++p1)
+          A->isconn[cn.s - 1][(cn.r1s + p1) + (long)n1full * (cn.r2s + p2)] = 1;
+    }
+  }
+  swap_geometry(c);
+  return 0;
+}
+
+/* SwapStateSlice / SwapImplicitUpdate (src/procBlock.cpp:3050-3055,
+ * src/utility.cpp:400-423): ghost cells of every connection take the
+ * partner's interior values; the sources are interior cells, so the order
+ * of the swaps does not matter */
+static void swap_halo(orc_case *c, int which) {
+  for (int a = 0; a < c->nb; ++a) {
+    blk_t *A = &c->blk[a];
+    for (int q = 0; q < A->nconn; ++q) {
+      const conn_t *cn = &A->conn[q];
+      const blk_t *B = &c->blk[cn->pb];
+      double *dst = which ? A->x : A->W;
+      const double *src = which ? B->x : B->W;
+      for (int l = 1; l <= A->g; ++l)
+        for (int p2 = 0; p2 < cn->n2; ++p2)
+          for (int p1 = 0; p1 < cn->n1; ++p1)
+            memcpy(&dst[conn_ghost(cn, A, l, p1, p2) * NEQ], &src[conn_source(cn, B, l, p1, p2) * NEQ],
+                   sizeof(double) * NEQ);
+    }
+  }
+}
+
 /* HyperplaneReorder: src/utility.cpp:377-398 */
 void orc_hyperplane_order(int ni, int nj, int nk, int *out) {
   int np = ni + nj + nk - 2;
@@ -1387,16 +1654,21 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
       b->surf = surfsPer[bq];
       surfsPer[bq] = NULL;
       for (int s = 0; s < b->nsurf; ++s) {
-        if (!strcmp(b->surf[s].type, "interblock") || !strcmp(b->surf[s].type, "periodic")) {
-          seterr(err, errlen, "interblock/periodic connections are not supported");
+        if (!strcmp(b->surf[s].type, "periodic")) {
+          seterr(err, errlen, "periodic connections are not supported");
           goto fail2;
         }
       }
       qsort(b->surf, (size_t)b->nsurf, sizeof(surf_t), surf_cmp);
       if (build_geometry(b, coords[bq])) { seterr(err, errlen, "negative volume in block %d", bq); goto fail2; }
       ghost_geometry(b);
-      cell_widths(b);
       orc_hyperplane_order(b->ni, b->nj, b->nk, b->order);
+    }
+    /* src/gridLevel.cpp:67-74 (geometry swap), then src/main.cpp:169
+     * (cell widths, ghost cells included) */
+    if (connect_blocks(c, (const v3 *const *)coords, err, errlen)) goto fail2;
+    for (int bq = 0; bq < nbg; ++bq) {
+      cell_widths(&c->blk[bq]);
       free(coords[bq]);
     }
     free(coords);
@@ -1491,7 +1763,8 @@ static void free_case(orc_case *c) {
     for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
     free(b->W); free(b->x); free(b->R); free(b->sr); free(b->dt);
     free(b->A); free(b->Ainv); free(b->consN); free(b->consNm1);
-    free(b->order); free(b->surf);
+    free(b->order); free(b->surf); free(b->conn);
+    for (int q = 0; q < 6; ++q) free(b->isconn[q]);
   }
   free(c->blk);
   free(c);
@@ -1667,6 +1940,7 @@ static void assign_ghosts(orc_case *c, blk_t *b) {
         aC = r3;
         if (iC >= n3) iC = n3 - 1;
       }
+      if (!strcmp(sf->type, "interblock")) continue; /* filled by swap_halo */
       char bcn[32];
       snprintf(bcn, sizeof bcn, "%s", sf->type);
       if (!strcmp(bcn, "viscousWall")) strcpy(bcn, "slipWall");
@@ -1687,8 +1961,11 @@ static void assign_ghosts(orc_case *c, blk_t *b) {
   }
 }
 
+/* gridLevel::GetBoundaryConditions: src/gridLevel.cpp:287-319 (edge ghost
+ * cells, which only the viscous terms read, are not formed) */
 void orc_bc(orc_case *c) {
   for (int bq = 0; bq < c->nb; ++bq) assign_ghosts(c, &c->blk[bq]);
+  swap_halo(c, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1831,7 +2108,14 @@ static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
     int ei = (d == 0), ej = (d == 1), ek = (d == 2);
     int s = upper ? 1 : -1;
     int ni = i + s * ei, nj = j + s * ej, nk = k + s * ek;
-    if (ni < 0 || nj < 0 || nk < 0 || ni >= b->ni || nj >= b->nj || nk >= b->nk) continue;
+    if (ni < 0 || nj < 0 || nk < 0 || ni >= b->ni || nj >= b->nj || nk >= b->nk) {
+      /* a ghost neighbour counts only across a connection (BCIsConnection,
+       * src/procBlock.cpp:1064,1078,1092) */
+      int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+      int c1 = d1 == 0 ? i : d1 == 1 ? j : k, c2 = d2 == 0 ? i : d2 == 1 ? j : k;
+      int n1 = d1 == 0 ? b->ni : d1 == 1 ? b->nj : b->nk;
+      if (!b->isconn[2 * d + (upper ? 1 : 0)][c1 + (long)n1 * c2]) continue;
+    }
     /* face between the cell and its neighbour */
     int fi = upper ? ni : i, fj = upper ? nj : j, fk = upper ? nk : k;
     uv3 a = b->fa[d][fidx(b, d, fi, fj, fk)];
@@ -1945,15 +2229,22 @@ static double matrix_residual(orc_case *c) {
   return l2 / (double)total;
 }
 
+/* lusgs::Relax / dplur::Relax (src/linearSolver.cpp:432-468, 509-535):
+ * the connections' ghost updates are swapped before every pass, so the
+ * blocks see each other's update lagged by one pass */
 double orc_relax(orc_case *c) {
   for (int sw = 0; sw < c->sweeps; ++sw) {
+    swap_halo(c, 1);
     for (int bq = 0; bq < c->nb; ++bq) {
       if (c->solver == 0) lusgs_forward(c, &c->blk[bq], sw);
       else dplur_sweep(c, &c->blk[bq]);
     }
-    if (c->solver == 0)
+    if (c->solver == 0) {
+      swap_halo(c, 1);
       for (int bq = 0; bq < c->nb; ++bq) lusgs_backward(c, &c->blk[bq], sw);
+    }
   }
+  swap_halo(c, 1);
   return c->matres_on ? matrix_residual(c) : 0.0;
 }
 

@@ -6,7 +6,10 @@ relative (testCases/regressionTests.py:93-115).  These two cases cover the
 Euler path the oracle restates: implicit Euler + LU-SGS with stagnation
 inlet / pressure outlet / slip walls and unlimited third-order MUSCL
 (subsonicCylinder), explicit Euler with van Albada MUSCL and supersonic
-inflow/outflow (supersonicWedge).
+inflow/outflow (supersonicWedge), two blocks joined by interblock
+connections with AUSMPW+ and a CFL ramp (multiblockCylinder), and a two-block
+Sod shock tube with WENO5, BDF2 dual time stepping (five nonlinear iterations
+per step, dual-time CFL) and per-block initial states (shockTube).
 """
 import json
 import os
