@@ -368,7 +368,180 @@ class CloudSearch {
   std::vector<int> head_, next_;
 };
 
+// ---------------------------------------------------------------------------
+// interblock connections
+
+// partner surface from the tag (src/boundaryConditions.cpp:2468-2495); the
+// partner block is tag - 1000 * surface
+int TagPartnerSurface(int tag) {
+  for (int s = 1; s <= 6; ++s)
+    if (tag < 1000 * (s + 1)) return s;
+  return -1;
+}
+
+// vector3d::CompareWithTol (include/vector3d.hpp:319-327)
+bool SamePoint(const Vec3 &a, const Vec3 &b) {
+  return std::fabs(a.x - b.x) < 1.0e-10 && std::fabs(a.y - b.y) < 1.0e-10 && std::fabs(a.z - b.z) < 1.0e-10;
+}
+
+Vec3 NodeAt(const std::vector<Vec3> &X, const HostBlock &b, int dir, int r3, int a1, int a2) {
+  int i, j, k;
+  Map3(dir, r3, a1, a2, &i, &j, &k);
+  return X[(size_t)i + (size_t)(b.ni + 1) * ((size_t)j + (size_t)(b.nj + 1) * k)];
+}
+
+// the patch axes' correspondence, from three corners (TestPatchMatch,
+// src/boundaryConditions.cpp:729-831)
+bool OrientPatch(Connection &cn, const HostBlock &A, const std::vector<Vec3> &XA, const HostBlock &B,
+                 const std::vector<Vec3> &XB) {
+  const int corners[3][2] = {{0, 0}, {cn.n1, 0}, {0, cn.n2}};
+  for (int sw = 0; sw < 2; ++sw)
+    for (int r1 = 0; r1 < 2; ++r1)
+      for (int r2 = 0; r2 < 2; ++r2) {
+        cn.swap = sw;
+        cn.rev1 = r1;
+        cn.rev2 = r2;
+        if (sw ? (cn.n1 != cn.pn2 || cn.n2 != cn.pn1) : (cn.n1 != cn.pn1 || cn.n2 != cn.pn2)) continue;
+        bool ok = true;
+        for (int q = 0; q < 3 && ok; ++q) {
+          int b1, b2;
+          cn.Map(corners[q][0], corners[q][1], false, &b1, &b2);
+          ok = SamePoint(NodeAt(XA, A, cn.dir, cn.r3, cn.r1s + corners[q][0], cn.r2s + corners[q][1]),
+                         NodeAt(XB, B, cn.pdir, cn.pr3, cn.pr1s + b1, cn.pr2s + b2));
+        }
+        if (ok) return true;
+      }
+  return false;
+}
+
+// SwapGeomSlice / PutGeomSlice (src/utility.cpp:213-256,
+// src/procBlock.cpp:3167-4011) over the patch: a ghost cell takes its
+// partner cell's volume and centroid; its faces normal to the patch (the
+// boundary face included) take the partner's faces at the same distance from
+// the cut, the area direction reversed when both patches are lower or both
+// upper surfaces; its faces along the patch take the partner's, lower and
+// upper exchanged and the area reversed where the partner's axis runs
+// backwards.  Every swap reads the geometry as it was before any of them.
+void SwapGeometry(HostCase &c) {
+  const std::vector<HostBlock> before = c.blocks;
+  for (const auto &cn : c.conns) {
+    HostBlock &A = c.blocks[cn.block];
+    const HostBlock &B = before[cn.pb];
+    const bool flip3 = (cn.s + cn.ps) % 2 == 0;
+    const bool lowA = cn.s % 2 == 1, lowB = cn.ps % 2 == 1;
+    const int nA = A.N(cn.dir), nB = B.N(cn.pdir);
+    const int dA[2] = {(cn.dir + 1) % 3, (cn.dir + 2) % 3};
+    const int dB[2] = {(cn.pdir + 1) % 3, (cn.pdir + 2) % 3};
+    auto neg = [](std::array<double, 4> v) {
+      v[0] = -1.0 * v[0];
+      v[1] = -1.0 * v[1];
+      v[2] = -1.0 * v[2];
+      return v;
+    };
+    for (int p2 = 0; p2 < cn.n2; ++p2)
+      for (int p1 = 0; p1 < cn.n1; ++p1) {
+        int q1, q2;
+        cn.Map(p1, p2, true, &q1, &q2);
+        for (int t = 0; t <= A.g; ++t) {
+          int i, j, k, bi, bj, bk;
+          Map3(cn.dir, lowA ? -t : nA + t, cn.r1s + p1, cn.r2s + p2, &i, &j, &k);
+          Map3(cn.pdir, lowB ? t : nB - t, cn.pr1s + q1, cn.pr2s + q2, &bi, &bj, &bk);
+          const auto &v = B.fa[cn.pdir][B.Face(cn.pdir, bi, bj, bk)];
+          A.fa[cn.dir][A.Face(cn.dir, i, j, k)] = flip3 ? neg(v) : v;
+          A.fc[cn.dir][A.Face(cn.dir, i, j, k)] = B.fc[cn.pdir][B.Face(cn.pdir, bi, bj, bk)];
+        }
+        for (int l = 1; l <= A.g; ++l) {
+          int gi, gj, gk, si, sj, sk;
+          cn.Ghost(A, l, p1, p2, &gi, &gj, &gk);
+          cn.Source(B, l, p1, p2, &si, &sj, &sk);
+          A.vol[A.Cell(gi, gj, gk)] = B.vol[B.Cell(si, sj, sk)];
+          A.ctr[A.Cell(gi, gj, gk)] = B.ctr[B.Cell(si, sj, sk)];
+          for (int w = 0; w < 2; ++w) {
+            // this block's axis dA[w] runs along the partner's axis dB[w]
+            // (or dB[1-w] when the patch axes are swapped)
+            const bool useB1 = (w == 0) != cn.swap;
+            const int da = dA[w], db = dB[useB1 ? 0 : 1];
+            const bool rev = useB1 ? cn.rev1 : cn.rev2;
+            for (int up = 0; up < 2; ++up) {
+              const int bup = rev ? 1 - up : up;
+              const long fa = A.Face(da, gi + (da == 0 ? up : 0), gj + (da == 1 ? up : 0), gk + (da == 2 ? up : 0));
+              const long fb = B.Face(db, si + (db == 0 ? bup : 0), sj + (db == 1 ? bup : 0), sk + (db == 2 ? bup : 0));
+              A.fa[da][fa] = rev ? neg(B.fa[db][fb]) : B.fa[db][fb];
+              A.fc[da][fa] = B.fc[db][fb];
+            }
+          }
+        }
+      }
+  }
+}
+
+// pair every interblock surface with its partner patch
+// (GetBlockInterConnBCs, src/boundaryConditions.cpp:603-642)
+std::string ConnectBlocks(HostCase &c, const std::vector<std::vector<Vec3>> &nodes) {
+  for (int a = 0; a < (int)c.blocks.size(); ++a) {
+    const HostBlock &A = c.blocks[a];
+    for (const auto &sf : A.surf) {
+      if (sf.type != "interblock") continue;
+      Connection cn;
+      cn.block = a;
+      cn.s = sf.SurfType();
+      cn.dir = (cn.s - 1) / 2;
+      const int d1 = (cn.dir + 1) % 3, d2 = (cn.dir + 2) % 3;
+      cn.r3 = sf.d[2 * cn.dir];
+      cn.r1s = sf.d[2 * d1];
+      cn.n1 = sf.d[2 * d1 + 1] - cn.r1s;
+      cn.r2s = sf.d[2 * d2];
+      cn.n2 = sf.d[2 * d2 + 1] - cn.r2s;
+      cn.ps = TagPartnerSurface(sf.d[6]);
+      cn.pb = sf.d[6] - 1000 * cn.ps;
+      const std::string where = "block " + std::to_string(a) + ": interblock tag " + std::to_string(sf.d[6]);
+      if (cn.ps < 0 || cn.pb < 0 || cn.pb >= (int)c.blocks.size()) return where + " names no block";
+      cn.pdir = (cn.ps - 1) / 2;
+      const HostBlock &B = c.blocks[cn.pb];
+      bool found = false;
+      for (const auto &pf : B.surf) {
+        if (pf.type != "interblock" || pf.SurfType() != cn.ps) continue;
+        const int pps = TagPartnerSurface(pf.d[6]);
+        if (pps != cn.s || pf.d[6] - 1000 * pps != a) continue;
+        if (cn.pb == a && &pf == &sf) continue;
+        const int e1 = (cn.pdir + 1) % 3, e2 = (cn.pdir + 2) % 3;
+        cn.pr3 = pf.d[2 * cn.pdir];
+        cn.pr1s = pf.d[2 * e1];
+        cn.pn1 = pf.d[2 * e1 + 1] - cn.pr1s;
+        cn.pr2s = pf.d[2 * e2];
+        cn.pn2 = pf.d[2 * e2 + 1] - cn.pr2s;
+        if (OrientPatch(cn, A, nodes[a], B, nodes[cn.pb])) {
+          found = true;
+          break;
+        }
+      }
+      if (!found) return where + " has no matching interblock patch in the partner block";
+      if (B.N(cn.pdir) < A.g) return where + ": partner block is thinner than the ghost layers";
+      c.conns.push_back(cn);
+    }
+  }
+  SwapGeometry(c);
+  return "";
+}
+
 }  // namespace
+
+void Connection::Map(int p1, int p2, bool cells, int *q1, int *q2) const {
+  const int c1 = swap ? p2 : p1, c2 = swap ? p1 : p2;
+  const int m1 = cells ? pn1 - 1 : pn1, m2 = cells ? pn2 - 1 : pn2;
+  *q1 = rev1 ? m1 - c1 : c1;
+  *q2 = rev2 ? m2 - c2 : c2;
+}
+
+void Connection::Ghost(const HostBlock &a, int l, int p1, int p2, int *i, int *j, int *k) const {
+  Map3(dir, s % 2 == 1 ? -l : a.N(dir) + l - 1, r1s + p1, r2s + p2, i, j, k);
+}
+
+void Connection::Source(const HostBlock &b, int l, int p1, int p2, int *i, int *j, int *k) const {
+  int q1, q2;
+  Map(p1, p2, true, &q1, &q2);
+  Map3(pdir, ps % 2 == 1 ? l - 1 : b.N(pdir) - l, pr1s + q1, pr2s + q2, i, j, k);
+}
 
 int Surface::SurfType() const {  // src/boundaryConditions.cpp:2424-2450
   if (d[0] == d[1]) return d[1] == 0 ? 1 : 2;
@@ -601,6 +774,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (!g) return "truncated Plot3D header";
   const int ng = c.Ghosts();
   c.blocks.resize(nb);
+  std::vector<std::vector<Vec3>> nodes(nb);
   c.totalCells = 0;
   for (int bq = 0; bq < nb; ++bq) {
     HostBlock &b = c.blocks[bq];
@@ -633,7 +807,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     }
     b.surf = surfs[bq];
     for (const auto &s : b.surf) {
-      if (s.type == "interblock" || s.type == "periodic") return "interblock/periodic connections are not supported";
+      if (s.type == "periodic") return "periodic connections are not supported";
       const int st = s.SurfType();
       if (st == 0) return "boundary surface is neither an i, j nor k surface";
       // the surface must lie on the block boundary and its patch inside it
@@ -654,8 +828,15 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     auto err = BuildGeometry(b, X);
     if (!err.empty()) return err;
     GhostGeometry(b);
-    CellWidths(b);
+    nodes[bq] = std::move(X);
   }
+  // geometry swap across connections (src/gridLevel.cpp:67-74), then the
+  // cell widths, ghost cells included (src/main.cpp:169)
+  {
+    auto err = ConnectBlocks(c, nodes);
+    if (!err.empty()) return err;
+  }
+  for (auto &b : c.blocks) CellWidths(b);
 
   // initial state (procBlock::InitializeStates, src/procBlock.cpp:280-355)
   for (int bq = 0; bq < nb; ++bq) {
@@ -713,7 +894,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     for (const auto &s : b.surf) {
       const auto &t = s.type;
       if (t != "slipWall" && t != "viscousWall" && t != "characteristic" && t != "supersonicInflow" &&
-          t != "supersonicOutflow" && t != "stagnationInlet" && t != "pressureOutlet")
+          t != "supersonicOutflow" && t != "stagnationInlet" && t != "pressureOutlet" && t != "interblock")
         return "unsupported boundary condition " + t;
       if (t == "characteristic" || t == "supersonicInflow" || t == "stagnationInlet" || t == "pressureOutlet") {
         bool found = false;

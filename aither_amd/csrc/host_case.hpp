@@ -77,6 +77,27 @@ struct HostBlock {
   int N(int d) const { return d == 0 ? ni : (d == 1 ? nj : nk); }
 };
 
+// One side of an interblock connection, seen from the block that owns the
+// ghost cells (boundarySurface pairs of src/boundaryConditions.cpp:603-642).
+// Patch-local cell (p1, p2) along this block's in-surface axes d1 = dir+1,
+// d2 = dir+2 (mod 3) continues in the partner's patch-local cell (q1, q2):
+//   c = swap ? (p2, p1) : (p1, p2);  q1 = rev1 ? pn1-1-c1 : c1;  q2 likewise.
+// Ghost layer l (1..g) continues in the partner's interior layer l.
+struct Connection {
+  int block = 0;                   // owner of the ghost cells
+  int s = 0, dir = 0, r3 = 0;      // surface type 1..6, normal axis, face index
+  int r1s = 0, n1 = 0, r2s = 0, n2 = 0;
+  int pb = 0, ps = 0, pdir = 0, pr3 = 0;  // partner block, surface, axis, face index
+  int pr1s = 0, pn1 = 0, pr2s = 0, pn2 = 0;
+  bool swap = false, rev1 = false, rev2 = false;
+
+  void Map(int p1, int p2, bool cells, int *q1, int *q2) const;
+  // (i, j, k) of this block's ghost cell at layer l and of the partner's
+  // interior source cell
+  void Ghost(const struct HostBlock &a, int l, int p1, int p2, int *i, int *j, int *k) const;
+  void Source(const struct HostBlock &b, int l, int p1, int p2, int *i, int *j, int *k) const;
+};
+
 struct HostCase {
   std::string dir, grid;
   int iterations = 1, nonlinear = 1, sweeps = 1;
@@ -94,6 +115,7 @@ struct HostCase {
   double totalCells = 0;
   std::vector<BcState> bcs;
   std::vector<HostBlock> blocks;
+  std::vector<Connection> conns;
 
   bool Implicit() const { return timeint != TimeInt::ExplicitEuler; }
   bool MultiLevel() const { return timeint == TimeInt::BDF2; }

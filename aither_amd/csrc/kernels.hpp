@@ -28,6 +28,12 @@ struct DevBlock {
   double *xold;          // 5 * ncell, DP-LUR Jacobi copy
   float *xold32;
   double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
+  // off-diagonal product of the ghost neighbour across each boundary face
+  // that lies on an interblock connection (zero on other faces), per sweep
+  // pass; null when the block has no connection.  5 * nbf, SoA.
+  double *bt;            // FP64 operator
+  float *bt32;           // FP32 operator
+  long nbf;              // boundary faces: 2 (nj nk + nk ni + ni nj)
   long cellBase;         // running cell offset of this block (L-inf index)
   int blockId;
 };
@@ -56,6 +62,15 @@ __device__ __forceinline__ long PadIdx(const DevBlock &b, int i, int j, int k) {
 __device__ __forceinline__ long FaceIdx(const DevBlock &b, int d, int i, int j, int k) {
   const int FI = b.ni + (d == 0), FJ = b.nj + (d == 1);
   return (long)i + (long)j * FI + (long)k * FI * FJ;
+}
+// slot of a boundary face: surfaces in the order i-lower, i-upper, j-lower,
+// j-upper, k-lower, k-upper; within a surface of normal axis d the face
+// (c1, c2) along axes d+1, d+2 (mod 3) is at c1 + N(d+1) c2
+__device__ __forceinline__ long BtSlot(const DevBlock &b, int d, int upper, int i, int j, int k) {
+  const long sI = (long)b.nj * b.nk, sJ = (long)b.nk * b.ni, sK = (long)b.ni * b.nj;
+  if (d == 0) return upper * sI + j + (long)b.nj * k;
+  if (d == 1) return 2 * sI + upper * sJ + k + (long)b.nk * i;
+  return 2 * sI + 2 * sJ + upper * sK + i + (long)b.ni * j;
 }
 __device__ __forceinline__ void Map3(int dir, int d3, int d1, int d2, int &i, int &j, int &k) {
   if (dir == 0) { i = d3; j = d1; k = d2; }
@@ -383,12 +398,14 @@ template <typename T>
 struct OpBuf;
 template <>
 struct OpBuf<double> {
+  static __device__ __forceinline__ double *Bt(const DevBlock &b) { return b.bt; }
   static __device__ __forceinline__ double *X(const DevBlock &b) { return b.x; }
   static __device__ __forceinline__ double *Xold(const DevBlock &b) { return b.xold; }
   static __device__ __forceinline__ const double *Ainv(const DevBlock &b) { return b.Ainv; }
 };
 template <>
 struct OpBuf<float> {
+  static __device__ __forceinline__ float *Bt(const DevBlock &b) { return b.bt32; }
   static __device__ __forceinline__ float *X(const DevBlock &b) { return b.x32; }
   static __device__ __forceinline__ float *Xold(const DevBlock &b) { return b.xold32; }
   static __device__ __forceinline__ const float *Ainv(const DevBlock &b) { return b.Ainv32; }
@@ -396,17 +413,28 @@ struct OpBuf<float> {
 
 // sum over the lower (upper == false) or upper neighbours of the Rusanov
 // off-diagonal products: procBlock::ImplicitLower / ImplicitUpper
-// (src/procBlock.cpp:1056-1161), neighbours in i, j, k order
+// (src/procBlock.cpp:1056-1161), neighbours in i, j, k order.  A ghost
+// neighbour counts only across a connection (src/procBlock.cpp:1064, 1078,
+// 1092); its product is the precomputed boundary term, zero elsewhere, and
+// adding +0 leaves the sum unchanged (it starts at +0 and never becomes -0).
 template <typename T>
 __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const T *x,
                                            int i, int j, int k, bool upper, T out[5]) {
 #pragma unroll
   for (int e = 0; e < 5; ++e) out[e] = T(0);
+  const T *bt = OpBuf<T>::Bt(b);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int s = upper ? 1 : -1;
     const int ni = i + (d == 0 ? s : 0), nj = j + (d == 1 ? s : 0), nk = k + (d == 2 ? s : 0);
-    if (ni < 0 || nj < 0 || nk < 0 || ni >= b.ni || nj >= b.nj || nk >= b.nk) continue;
+    if (ni < 0 || nj < 0 || nk < 0 || ni >= b.ni || nj >= b.nj || nk >= b.nk) {
+      if (bt) {
+        const long q = BtSlot(b, d, upper ? 1 : 0, i, j, k);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) out[e] += bt[e * b.nbf + q];
+      }
+      continue;
+    }
     const double4 a = upper ? b.fa[d][FaceIdx(b, d, ni, nj, nk)] : b.fa[d][FaceIdx(b, d, i, j, k)];
     const long c = PadIdx(b, ni, nj, nk);
     const long q = (long)ni + (long)nj * b.ni + (long)nk * b.ni * b.nj;
@@ -804,6 +832,56 @@ __device__ __forceinline__ void NbOffDiag(const GasT<T> &g, const double *Wnb, c
   ScalarOffDiag<T>(g, W, d, T(a.x), T(a.y), T(a.z), T(a.w), positive, out);
 }
 
+// ---------------------------------------------------------------------------
+// interblock connections
+
+// ghost cell <- partner interior cell, all layers (SwapStateSlice,
+// src/procBlock.cpp:3050-3055; sources are interior cells, so the swaps are
+// independent of each other)
+struct HaloTask {
+  int dst, src;
+  long dpad, spad;
+};
+
+__global__ void __launch_bounds__(256) KHaloState(const DevBlock *__restrict__ blocks,
+                                                  const HaloTask *__restrict__ tasks, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const HaloTask h = tasks[t];
+  const DevBlock &A = blocks[h.dst], &B = blocks[h.src];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) A.W[e * A.npad + h.dpad] = B.W[e * B.npad + h.spad];
+}
+
+// the off-diagonal product of the ghost neighbour across one connection
+// face, from the partner's current du (SwapImplicitUpdate before each pass,
+// src/linearSolver.cpp:444-463, then OffDiagonal with the ghost state)
+struct ConnTask {
+  int blk, src, d, upper;
+  long gpad, scell, face, slot;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ blocks,
+                                                  const ConnTask *__restrict__ tasks, int n, Gas gas) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const ConnTask c = tasks[t];
+  const DevBlock &A = blocks[c.blk], &B = blocks[c.src];
+  const T *xb = OpBuf<T>::X(B);
+  double Wn[5];
+  T dn[5], o[5];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    Wn[e] = A.W[e * A.npad + c.gpad];
+    dn[e] = xb[e * B.ncell + c.scell];
+  }
+  NbOffDiag<T>(Cast<T>(gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o);
+  T *bt = OpBuf<T>::Bt(A);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) bt[e * A.nbf + c.slot] = o[e];
+}
+
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl s, int full) {
   __shared__ double sW[5][kLT];
@@ -940,7 +1018,16 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-          if (gi < 0 || gj < 0 || gk < 0 || gi >= b.ni || gj >= b.nj || gk >= b.nk) continue;
+          if (gi < 0 || gj < 0 || gk < 0 || gi >= b.ni || gj >= b.nj || gk >= b.nk) {
+            // ghost neighbour: the connection term (zero off connections)
+            const T *bt = OpBuf<T>::Bt(b);
+            if (bt) {
+              const long q = BtSlot(b, d, FWD ? 0 : 1, i, j, k);
+#pragma unroll
+              for (int e = 0; e < 5; ++e) D[e] += bt[e * b.nbf + q];
+            }
+            continue;
+          }
           const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
           const int L3 = d == 0 ? kLTI : (d == 1 ? kLTJ : kLTK);
           const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
@@ -1235,10 +1322,20 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
     PhysFlux(g, W, nx, ny, nz, Fo);
     const T half = T(0.5) * mag;
     const T lam = T(0.5) * mag * (fabs((W[1] * nx + W[2] * ny) + W[3] * nz) + SoS(g, W[0], W[4]));
+    // a neighbour outside the block: the slot of Fo carries the connection
+    // term instead (zero off connections), selected by the neighbour wave
+    const bool outside = !InBlock(b, i + (d == 0 ? sgn : 0), j + (d == 1 ? sgn : 0), kc + (d == 2 ? sgn : 0));
+    T bv[5] = {T(0), T(0), T(0), T(0), T(0)};
+    const T *bt = OpBuf<T>::Bt(b);
+    if (bt) {
+      const long q = BtSlot(b, d, FWD ? 0 : 1, i, j, kc);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) bv[e] = bt[e * b.nbf + q];
+    }
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
       sl.U[d][e][lane] = U[e];
-      sl.Fo[d][e][lane] = Fo[e];
+      sl.Fo[d][e][lane] = outside ? bv[e] : Fo[e];
     }
     sl.n[d][0][lane] = nx;
     sl.n[d][1][lane] = ny;
@@ -1276,8 +1373,15 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
         dn[e] = x[e * b.ncell + q];
       }
       NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
+      T bv[5] = {T(0), T(0), T(0), T(0), T(0)};
+      const T *bt = OpBuf<T>::Bt(b);
+      if (bt) {
+        const long qb = BtSlot(b, d, FWD ? 1 : 0, i, j, kc);
 #pragma unroll
-      for (int e = 0; e < 5; ++e) O[e] += ok ? u[e] : T(0);
+        for (int e = 0; e < 5; ++e) bv[e] = bt[e * b.nbf + qb];
+      }
+#pragma unroll
+      for (int e = 0; e < 5; ++e) O[e] += ok ? u[e] : bv[e];
     }
 #pragma unroll
     for (int e = 0; e < 5; ++e) sl.O[FULL ? e : 0][lane] = O[e];
@@ -1358,11 +1462,14 @@ __global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(DevBlock b
       ConsToPrim(g, U, Wu);
       PhysFlux(g, Wu, sl.n[d][0][lane], sl.n[d][1][lane], sl.n[d][2][lane], Fn);
       const T half = sl.half[d][lane], lam = sl.lam[d][lane];
+      // a neighbour outside the block contributes the staged connection
+      // term (zero off connections); idle steps contribute zero
+      const bool outside = k >= 0 && !dep;
 #pragma unroll
       for (int e = 0; e < 5; ++e) {
         const T fc = half * (Fn[e] - sl.Fo[d][e][lane]);
         const T o = FWD ? fc + du[e] * lam : fc - du[e] * lam;
-        sh.sO[d][e][lane] = dep ? o : T(0);
+        sh.sO[d][e][lane] = dep ? o : (outside ? sl.Fo[d][e][lane] : T(0));
       }
       LdsStore(&sh.oReady[d], st + 1);
     }

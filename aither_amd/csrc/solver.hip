@@ -69,6 +69,13 @@ struct ath_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphFp32 = -1, graphMatres = -1, graphOrder = -1;
+  // interblock connections: a device copy of every block's DevBlock, the
+  // ghost-state gather list and the per-face connection terms
+  ath::DevBlock *dblocks = nullptr;
+  ath::HaloTask *dhalo = nullptr;
+  int nhalo = 0;
+  ath::ConnTask *dconn = nullptr;
+  int nconn = 0;
   // timing
   int timing = 0;
   std::vector<hipEvent_t> evs;
@@ -97,6 +104,77 @@ namespace {
 constexpr int kBlk = 256;
 
 int Grid(long n, int b = kBlk) { return (int)((n + b - 1) / b); }
+
+// boundary-face slot, as BtSlot in the kernels
+long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) {
+  const long sI = (long)h.nj * h.nk, sJ = (long)h.nk * h.ni, sK = (long)h.ni * h.nj;
+  if (d == 0) return upper * sI + j + (long)h.nj * k;
+  if (d == 1) return 2 * sI + upper * sJ + k + (long)h.nk * i;
+  return 2 * sI + 2 * sJ + upper * sK + i + (long)h.ni * j;
+}
+
+// the device copy of the blocks' DevBlocks (pointers change when the FP32
+// operator buffers are created)
+int SyncBlocks(ath_ctx *c) {
+  if (c->hc.conns.empty()) return ATH_OK;
+  std::vector<ath::DevBlock> v;
+  for (const auto &br : c->blk) v.push_back(br.d);
+  if (!c->dblocks)
+    if (int r = c->Alloc(&c->dblocks, v.size())) return r;
+  HIP_OK(hipMemcpy(c->dblocks, v.data(), v.size() * sizeof(ath::DevBlock), hipMemcpyHostToDevice));
+  return ATH_OK;
+}
+
+// gather lists of the interblock connections
+int UploadConnections(ath_ctx *c) {
+  using namespace ath;
+  const HostCase &hc = c->hc;
+  if (hc.conns.empty()) return ATH_OK;
+  std::vector<HaloTask> halo;
+  std::vector<ConnTask> conn;
+  for (const auto &cn : hc.conns) {
+    const HostBlock &A = hc.blocks[cn.block], &B = hc.blocks[cn.pb];
+    const bool upper = cn.s % 2 == 0;
+    for (int p2 = 0; p2 < cn.n2; ++p2)
+      for (int p1 = 0; p1 < cn.n1; ++p1) {
+        for (int l = 1; l <= A.g; ++l) {
+          int gi, gj, gk, si, sj, sk;
+          cn.Ghost(A, l, p1, p2, &gi, &gj, &gk);
+          cn.Source(B, l, p1, p2, &si, &sj, &sk);
+          halo.push_back({cn.block, cn.pb, A.Cell(gi, gj, gk), B.Cell(si, sj, sk)});
+          if (l > 1) continue;
+          ConnTask t{};
+          t.blk = cn.block;
+          t.src = cn.pb;
+          t.d = cn.dir;
+          t.upper = upper ? 1 : 0;
+          t.gpad = A.Cell(gi, gj, gk);
+          t.scell = si + (long)sj * B.ni + (long)sk * B.ni * B.nj;
+          // the block's own boundary cell and face
+          int bi, bj, bk;
+          {
+            const int n3 = upper ? A.N(cn.dir) - 1 : 0, c1 = cn.r1s + p1, c2 = cn.r2s + p2;
+            bi = cn.dir == 0 ? n3 : (cn.dir == 1 ? c2 : c1);
+            bj = cn.dir == 0 ? c1 : (cn.dir == 1 ? n3 : c2);
+            bk = cn.dir == 0 ? c2 : (cn.dir == 1 ? c1 : n3);
+          }
+          const int fi = bi + (cn.dir == 0 && upper), fj = bj + (cn.dir == 1 && upper),
+                    fk = bk + (cn.dir == 2 && upper);
+          const int FI = A.ni + (cn.dir == 0), FJ = A.nj + (cn.dir == 1);
+          t.face = fi + (long)fj * FI + (long)fk * FI * FJ;
+          t.slot = HostBtSlot(A, cn.dir, t.upper, bi, bj, bk);
+          conn.push_back(t);
+        }
+      }
+  }
+  c->nhalo = (int)halo.size();
+  c->nconn = (int)conn.size();
+  if (int r = c->Alloc(&c->dhalo, halo.size())) return r;
+  if (int r = c->Alloc(&c->dconn, conn.size())) return r;
+  HIP_OK(hipMemcpy(c->dhalo, halo.data(), halo.size() * sizeof(HaloTask), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->dconn, conn.data(), conn.size() * sizeof(ConnTask), hipMemcpyHostToDevice));
+  return SyncBlocks(c);
+}
 
 int Upload(ath_ctx *c) {
   using namespace ath;
@@ -205,6 +283,15 @@ int Upload(ath_ctx *c) {
     d.Ainv32 = nullptr;
     d.x32 = nullptr;
     d.xold32 = nullptr;
+    d.nbf = 2 * ((long)h.nj * h.nk + (long)h.nk * h.ni + (long)h.ni * h.nj);
+    d.bt = nullptr;
+    d.bt32 = nullptr;
+    bool hasConn = false;
+    for (const auto &cn : hc.conns) hasConn = hasConn || cn.block == (int)bq;
+    if (hasConn) {
+      if (int r = c->Alloc(&d.bt, 5 * d.nbf)) return r;
+      HIP_OK(hipMemset(d.bt, 0, 5 * d.nbf * sizeof(double)));
+    }
     // hyperplane order
     std::vector<int> cells;
     HyperplaneOrder(h.ni, h.nj, h.nk, &cells, &br.planeOff);
@@ -271,6 +358,7 @@ int Upload(ath_ctx *c) {
     long total = 0;
     for (int layer = 1; layer <= h.g; ++layer)
       for (const auto &sf : h.surf) {
+        if (sf.type == "interblock") continue;  // filled by the halo gather
         const int stp = sf.SurfType();
         const int dir = (stp - 1) / 2;
         const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
@@ -320,6 +408,7 @@ int Upload(ath_ctx *c) {
   if (int r = c->Alloc(&c->dpart, c->nparts)) return r;
   c->histCap = 4096;
   if (int r = c->Alloc(&c->dhist, 8 * (size_t)c->histCap)) return r;
+  if (int r = UploadConnections(c)) return r;
   return ATH_OK;
 }
 
@@ -331,18 +420,37 @@ int EnsureFp32(ath_ctx *c) {
     if (int r = c->Alloc(&d.x32, 5 * d.ncell)) return r;
     if (int r = c->Alloc(&d.xold32, 5 * d.ncell)) return r;
     HIP_OK(hipMemset(d.x32, 0, 5 * d.ncell * sizeof(float)));
+    if (d.bt) {
+      if (int r = c->Alloc(&d.bt32, 5 * d.nbf)) return r;
+      HIP_OK(hipMemset(d.bt32, 0, 5 * d.nbf * sizeof(float)));
+    }
   }
+  if (int r = SyncBlocks(c)) return r;
   HIP_OK(hipDeviceSynchronize());
   return ATH_OK;
 }
 
 // --- launch helpers (all on c->stream) -------------------------------------
 
+// gridLevel::GetBoundaryConditions (src/gridLevel.cpp:287-319): ghost
+// states of the boundary patches, then the connections' ghost cells (edge
+// ghost cells, which only viscous terms read, are not formed)
 void LaunchBoundary(ath_ctx *c) {
   for (auto &br : c->blk)
     if (br.bcTotal > 0)
       hipLaunchKernelGGL(ath::KBoundary, dim3(Grid(br.bcTotal)), dim3(kBlk), 0, c->stream, br.d,
                          br.dtasks, (int)br.tasks.size(), br.bcTotal, c->dstates, c->P.gas);
+  if (c->nhalo > 0)
+    hipLaunchKernelGGL(ath::KHaloState, dim3(Grid(c->nhalo)), dim3(kBlk), 0, c->stream, c->dblocks, c->dhalo,
+                       c->nhalo);
+}
+
+// connection terms from the partners' current du, before each pass
+template <typename T>
+void LaunchConnTerms(ath_ctx *c) {
+  if (c->nconn > 0)
+    hipLaunchKernelGGL(ath::KConnTerms<T>, dim3(Grid(c->nconn)), dim3(kBlk), 0, c->stream, c->dblocks, c->dconn,
+                       c->nconn, c->P.gas);
 }
 
 template <int RECON, int LIM, int FLUX>
@@ -395,6 +503,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         for (auto &br : c->blk)
           hipLaunchKernelGGL(ath::KLusgsRhs, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
       for (int dir = 0; dir < 2; ++dir) {
+        LaunchConnTerms<T>(c);
         mark(true);
         for (auto &br : c->blk) {
           hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
@@ -413,6 +522,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
       }
     } else if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_TILED) {
       for (int dir = 0; dir < 2; ++dir) {
+        LaunchConnTerms<T>(c);
         mark(true);
         for (auto &br : c->blk) {
           hipLaunchKernelGGL(ath::KSweepPrologue, dim3(1), dim3(1), 0, c->stream, br.sweep);
@@ -427,6 +537,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         mark(false);
       }
     } else if (hc.solver == ath::Solver::LUSGS) {
+      LaunchConnTerms<T>(c);
       mark(true);
       for (auto &br : c->blk) {
         const int np = (int)br.planeOff.size() - 1;
@@ -437,6 +548,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         }
       }
       mark(false);
+      LaunchConnTerms<T>(c);
       mark(true);
       for (auto &br : c->blk) {
         const int np = (int)br.planeOff.size() - 1;
@@ -448,6 +560,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
       }
       mark(false);
     } else {
+      LaunchConnTerms<T>(c);
       mark(true);
       for (auto &br : c->blk) {
         hipLaunchKernelGGL(ath::KCopyUpdate<T>, dim3(Grid(5 * br.d.ncell)), dim3(kBlk), 0, c->stream, br.d);
@@ -457,10 +570,12 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
       mark(false);
     }
   }
-  if (c->matres)
+  if (c->matres) {
+    LaunchConnTerms<T>(c);
     for (auto &br : c->blk)
       hipLaunchKernelGGL(ath::KMatrixResidual<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d,
                          c->P, c->dpart + br.partBase);
+  }
   return launches;
 }
 

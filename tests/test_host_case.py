@@ -32,8 +32,15 @@ def test_library_exports_every_declared_symbol():
 def _cases(tmp):
     out = [os.path.join(ROOT, "tests", "golden", "subsonicCylinder", "subsonicCylinder.inp"),
            os.path.join(ROOT, "tests", "golden", "supersonicWedge", "supersonicWedge.inp"),
+           os.path.join(ROOT, "tests", "golden", "shockTube", "shockTube.inp"),
+           os.path.join(ROOT, "tests", "golden", "multiblockCylinder", "multiblockCylinder.inp"),
            cases.vortex_case(os.path.join(tmp, "v"), n=7, iterations=1),
            cases.bump_case(os.path.join(tmp, "b"), seed=3, n=9)[0]]
+    # two-block cases with interblock connections in several orientations
+    import conn_cases as cc
+    pairs = cc.orientation_set()
+    for q in (0, 97, 211, 350, 463, 575):
+        out.append(cc.write_pair(os.path.join(tmp, f"c{q}"), "cut", *pairs[q], recon="weno")[0])
     return out
 
 
