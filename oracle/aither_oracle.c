@@ -72,6 +72,8 @@ typedef struct {
   v3 dir;
   int nonreflecting;
   int has_mf;
+  int isothermal; /* viscousWall: wall temperature given */
+  double twall;
 } bcstate_t;
 
 /* src/boundaryConditions.cpp:2424-2450 */
@@ -110,6 +112,7 @@ typedef struct {
   double *A, *Ainv;
   double *consN, *consNm1;
   double *x;      /* padded, NEQ per cell */
+  double *T, *mu; /* padded: temperature, viscosity (viscous cases) */
   int *order;     /* 3 per physical cell */
   int nsurf;
   surf_t *surf;
@@ -161,6 +164,12 @@ struct orc_case {
   double totalCells;
   int matres_on;
   double l2first[NEQ];
+  /* viscous (laminar Navier-Stokes) */
+  int viscous;
+  int viscRecon4; /* viscousFaceReconstruction: 0 central, 1 centralFourth */
+  double suth[4]; /* Sutherland C1, S for viscosity, then conductivity */
+  double muRef, kNonDim, vscale; /* reference viscosity, conductivity scale, mu/(rho a L) */
+  double viscCoeff;
 };
 
 static long cidx(const blk_t *b, int i, int j, int k) {
@@ -592,8 +601,8 @@ static double inv_face_specrad(const orc_gas *g, const double *W, uv3 a) {
 /* RusanovScalarOffDiagonal (inviscid, no turbulence):
  * src/fluxJacobian.cpp:122-162; ConvectiveFluxUpdate
  * include/inviscidFlux.hpp:543-562 */
-static void scalar_offdiag(const orc_gas *g, const double *Wnb,
-                           const double *du, uv3 a, int positive, double *out) {
+static void scalar_offdiag_v(const orc_gas *g, const double *Wnb, const double *du, uv3 a,
+                             int positive, int viscous, double lamVisc, double *out) {
   double Wu[NEQ];
   update_prim(g, Wnb, du, Wu);
   double n[3] = {a.u.x, a.u.y, a.u.z};
@@ -602,11 +611,16 @@ static void scalar_offdiag(const orc_gas *g, const double *Wnb,
   orc_phys_flux(g, Wu, n, Fnew);
   double half = 0.5 * a.m;
   double lam = inv_face_specrad(g, Wnb, a);
+  if (viscous) lam += lamVisc;
   for (int e = 0; e < NEQ; ++e) {
     double fc = half * (Fnew[e] - Fold[e]);
     double sd = du[e] * lam;
     out[e] = positive ? fc + sd : fc - sd;
   }
+}
+static void scalar_offdiag(const orc_gas *g, const double *Wnb,
+                           const double *du, uv3 a, int positive, double *out) {
+  scalar_offdiag_v(g, Wnb, du, a, positive, 0, 0.0, out);
 }
 void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
                         const double *area, int positive, double *out) {
@@ -837,6 +851,7 @@ static void cell_widths(blk_t *b) {
 /* interblock connections                                                   */
 
 static void seterr(char *err, int errlen, const char *fmt, ...);
+static void edge_ghosts(orc_case *c, blk_t *b, int viscous);
 
 /* partner block and surface from the tag: src/boundaryConditions.cpp:2455-2495 */
 static int tag_partner_surface(int tag) {
@@ -1202,13 +1217,17 @@ static void entry_name(const char *e, char *name, int n) {
 /* fluid properties: read <name>.dat from the case directory (as
  * src/fluid.cpp:105-118 does), else the built-in values of
  * fluidDatabase/air.dat:14-15 and fluidDatabase/N2.dat */
-static int fluid_props(const char *dir, const char *name, double *n, double *M) {
+/* fluid database entry (fluidDatabase/<name>.dat, src/fluid.cpp): n, molar
+ * mass and the Sutherland coefficients of viscosity and conductivity */
+static int fluid_props(const char *dir, const char *name, double *n, double *M, double *suth) {
   char path[1200];
   snprintf(path, sizeof path, "%s/%s.dat", dir, name);
   FILE *f = fopen(path, "r");
   if (f) {
     char line[512];
     int got = 0;
+    static const char *sk[4] = {"sutherlandViscosityC1", "sutherlandViscosityS",
+                                "sutherlandConductivityC1", "sutherlandConductivityS"};
     while (fgets(line, sizeof line, f)) {
       trim(line);
       char *c = strchr(line, ':');
@@ -1219,12 +1238,22 @@ static int fluid_props(const char *dir, const char *name, double *n, double *M) 
       trim(key);
       if (strcmp(key, "n") == 0) { *n = strtod(c + 1, NULL); got |= 1; }
       if (strcmp(key, "molarMass") == 0) { *M = strtod(c + 1, NULL) / 1000.; got |= 2; }
+      for (int q = 0; q < 4; ++q)
+        if (strcmp(key, sk[q]) == 0) { suth[q] = strtod(c + 1, NULL); got |= 4 << q; }
     }
     fclose(f);
-    if (got == 3) return 0;
+    if (got == 63) return 0;
   }
-  if (strcmp(name, "air") == 0) { *n = 2.5; *M = 28.97 / 1000.; return 0; }
-  if (strcmp(name, "N2") == 0) { *n = 2.5; *M = 28.0134 / 1000.; return 0; }
+  if (strcmp(name, "air") == 0) {
+    *n = 2.5; *M = 28.97 / 1000.;
+    suth[0] = 1.458e-6; suth[1] = 110.4; suth[2] = 2.495e-3; suth[3] = 194.0;
+    return 0;
+  }
+  if (strcmp(name, "N2") == 0) {
+    *n = 2.5; *M = 28.0134 / 1000.;
+    suth[0] = 1.4742e-06; suth[1] = 1.2846e+02; suth[2] = 2.6834e-03; suth[3] = 2.5615e+02;
+    return 0;
+  }
   return -1;
 }
 
@@ -1360,6 +1389,8 @@ static int alloc_block(blk_t *b) {
   }
   b->W = (double *)calloc((size_t)nc * NEQ, sizeof(double));
   b->x = (double *)calloc((size_t)nc * NEQ, sizeof(double));
+  b->T = (double *)calloc((size_t)nc, sizeof(double));
+  b->mu = (double *)calloc((size_t)nc, sizeof(double));
   b->R = (double *)calloc((size_t)np * NEQ, sizeof(double));
   b->sr = (double *)calloc((size_t)np, sizeof(double));
   b->dt = (double *)calloc((size_t)np, sizeof(double));
@@ -1499,7 +1530,15 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
       else if (!strcmp(val, "minmod")) c->limiter = 2;
       else { seterr(err, errlen, "unsupported limiter %s", val); goto fail; }
     } else if (!strcmp(key, "equationSet")) {
-      if (strcmp(val, "euler")) { seterr(err, errlen, "unsupported equationSet %s", val); goto fail; }
+      if (!strcmp(val, "euler")) c->viscous = 0;
+      else if (!strcmp(val, "navierStokes")) c->viscous = 1;
+      else { seterr(err, errlen, "unsupported equationSet %s", val); goto fail; }
+    } else if (!strcmp(key, "viscousFaceReconstruction")) {
+      if (!strcmp(val, "central")) c->viscRecon4 = 0;
+      else if (!strcmp(val, "centralFourth")) c->viscRecon4 = 1;
+      else { seterr(err, errlen, "unsupported viscousFaceReconstruction %s", val); goto fail; }
+    } else if (!strcmp(key, "turbulenceModel")) {
+      if (strcmp(val, "none")) { seterr(err, errlen, "unsupported turbulenceModel %s", val); goto fail; }
     } else if (!strcmp(key, "matrixSolver")) {
       if (!strcmp(val, "lusgs")) c->solver = 0;
       else if (!strcmp(val, "dplur")) c->solver = 1;
@@ -1556,6 +1595,15 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
         if (get_param(ents[q], "t0", v, sizeof v)) s->t0 = strtod(v, NULL);
         if (get_param(ents[q], "direction", v, sizeof v)) s->dir = parse_vec(v);
         if (get_param(ents[q], "nonreflecting", v, sizeof v)) s->nonreflecting = !strcmp(v, "true");
+        if (get_param(ents[q], "temperature", v, sizeof v)) { s->isothermal = 1; s->twall = strtod(v, NULL); }
+        if (get_param(ents[q], "heatFlux", v, sizeof v) && strtod(v, NULL) != 0.0) {
+          seterr(err, errlen, "viscousWall heat flux is not supported");
+          goto fail;
+        }
+        if (get_param(ents[q], "wallTreatment", v, sizeof v) && strcmp(v, "lowRe")) {
+          seterr(err, errlen, "wall laws are not supported");
+          goto fail;
+        }
         if (s->nonreflecting) { seterr(err, errlen, "nonreflecting BCs are not supported"); goto fail; }
       }
     } else if (!strcmp(key, "boundaryConditions")) {
@@ -1573,7 +1621,7 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
   f = NULL;
 
   if (c->rRef <= 0 || c->tRef <= 0) { seterr(err, errlen, "reference density/temperature missing"); goto fail2; }
-  if (fluid_props(c->dir, fluidName, &c->fluidN, &c->fluidM)) {
+  if (fluid_props(c->dir, fluidName, &c->fluidN, &c->fluidM, c->suth)) {
     seterr(err, errlen, "unknown fluid %s", fluidName);
     goto fail2;
   }
@@ -1607,8 +1655,22 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
     c->ic[q].rho /= c->rRef;
     c->ic[q].p /= c->rRef * c->aRef * c->aRef;
   }
+  /* Sutherland transport and its nondimensional scaling:
+   * src/transport.cpp:29-66, include/transport.hpp:40-44 */
+  {
+    c->muRef = c->suth[0] * pow(c->tRef, 1.5) / (c->tRef + c->suth[1]);
+    c->kNonDim = (c->aRef * c->aRef * c->muRef) / c->tRef;
+    c->vscale = c->muRef / (c->rRef * c->aRef * c->lRef);
+    /* input::ViscousCFLCoefficient, src/input.cpp:1110-1118 */
+    c->viscCoeff = c->kappa == 1.0 ? 4.0 : (c->kappa == -2.0 ? 2.0 : 1.0);
+  }
   for (int q = 0; q < c->nbc; ++q) {
     bcstate_t *s = &c->bcs[q];
+    if (!strcmp(s->kind, "viscousWall")) { /* src/inputStates.cpp:775-783 */
+      s->vel = vdiv(s->vel, c->aRef);
+      s->twall /= c->tRef;
+      continue;
+    }
     if (!strcmp(s->kind, "stagnationInlet")) {
       s->p0 /= c->rRef * c->aRef * c->aRef;
       s->t0 /= c->tRef;
@@ -1761,7 +1823,7 @@ static void free_case(orc_case *c) {
     blk_t *b = &c->blk[bq];
     free(b->vol); free(b->ctr);
     for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
-    free(b->W); free(b->x); free(b->R); free(b->sr); free(b->dt);
+    free(b->W); free(b->x); free(b->T); free(b->mu); free(b->R); free(b->sr); free(b->dt);
     free(b->A); free(b->Ainv); free(b->consN); free(b->consNm1);
     free(b->order); free(b->surf); free(b->conn);
     for (int q = 0; q < 6; ++q) free(b->isconn[q]);
@@ -1961,11 +2023,374 @@ static void assign_ghosts(orc_case *c, blk_t *b) {
   }
 }
 
-/* gridLevel::GetBoundaryConditions: src/gridLevel.cpp:287-319 (edge ghost
- * cells, which only the viscous terms read, are not formed) */
+/* gridLevel::GetBoundaryConditions: src/gridLevel.cpp:287-319 */
 void orc_bc(orc_case *c) {
   for (int bq = 0; bq < c->nb; ++bq) assign_ghosts(c, &c->blk[bq]);
   swap_halo(c, 0);
+  for (int bq = 0; bq < c->nb; ++bq) edge_ghosts(c, &c->blk[bq], 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* viscous terms (laminar Navier-Stokes)                                     */
+
+/* sutherland::SpeciesViscosity / SpeciesConductivity (src/transport.cpp:
+ * 97-117), nondimensional */
+static double visc_of_t(const orc_case *c, double t) {
+  double temp = t * c->tRef;
+  double mu = (c->suth[0] * pow(temp, 1.5)) / (temp + c->suth[1]);
+  return mu / c->muRef;
+}
+static double cond_of_t(const orc_case *c, double t) {
+  double temp = t * c->tRef;
+  double k = (c->suth[2] * pow(temp, 1.5)) / (temp + c->suth[3]);
+  return k / c->kNonDim;
+}
+/* thermodynamic::Prandtl (include/thermodynamic.hpp:61-64) */
+static double prandtl(const orc_gas *g) {
+  double gamma = orc_gamma(g);
+  return (4.0 * gamma) / (9.0 * gamma - 5.0);
+}
+static double std_max(double a, double b) { return (a < b) ? b : a; }
+
+/* the part of Visc{Cell,Face}SpectralRadius (include/spectralRadius.hpp:
+ * 93-151) that depends on the state: max(4/(3 rho), gamma/rho) and
+ * NondimScaling (mu / Pr + mut / Prt), mut = 0 */
+static double visc_state_factor(const orc_case *c, const double *W, double mu, double *viscTerm) {
+  double rho = 0.0 + W[0];
+  double maxTerm = std_max(4.0 / (3.0 * rho), orc_gamma(&c->gas) / rho);
+  *viscTerm = c->vscale * (mu / prandtl(&c->gas) + 0.0 / 0.9);
+  return maxTerm;
+}
+static double visc_cell_specrad(const orc_case *c, const double *W, uv3 aL, uv3 aR, double vol,
+                                double mu) {
+  double fMag = 0.5 * (aL.m + aR.m);
+  double vt;
+  double mt = visc_state_factor(c, W, mu, &vt);
+  return mt * vt * fMag * fMag / vol;
+}
+static double visc_face_specrad(const orc_case *c, const double *W, uv3 a, double dist, double mu) {
+  double vt;
+  double mt = visc_state_factor(c, W, mu, &vt);
+  return a.m / dist * mt * vt;
+}
+
+static int at_corner(const blk_t *b, int i, int j, int k) {
+  return (i < 0 || i >= b->ni) && (j < 0 || j >= b->nj) && (k < 0 || k >= b->nk);
+}
+
+/* procBlock::UpdateAuxillaryVariables (src/procBlock.cpp:6171-6189): every
+ * cell but the block corners, ghost cells included */
+static void update_aux(const orc_case *c, blk_t *b) {
+  for (int k = -b->g; k < b->nk + b->g; ++k)
+    for (int j = -b->g; j < b->nj + b->g; ++j)
+      for (int i = -b->g; i < b->ni + b->g; ++i) {
+        if (at_corner(b, i, j, k)) continue;
+        long q = cidx(b, i, j, k);
+        b->T[q] = temperature(&c->gas, &b->W[q * NEQ]);
+        if (c->viscous) b->mu[q] = visc_of_t(c, b->T[q]);
+      }
+}
+
+/* boundaryConditions::GetBCSurface (src/boundaryConditions.cpp:109-190): the
+ * surface of the same normal direction whose patch holds face (i, j, k) */
+static const surf_t *bc_surface_at(const blk_t *b, int i, int j, int k, int st) {
+  int dir = (st - 1) / 2;
+  int x[3] = {i, j, k};
+  for (int s = 0; s < b->nsurf; ++s) {
+    const surf_t *sf = &b->surf[s];
+    if ((surf_type(sf) - 1) / 2 != dir) continue;
+    int ok = 1;
+    for (int q = 0; q < 3 && ok; ++q) {
+      if (q == dir) ok = x[q] >= sf->d[2 * q] && x[q] <= sf->d[2 * q + 1];
+      else ok = x[q] >= sf->d[2 * q] && x[q] < sf->d[2 * q + 1];
+    }
+    if (ok) return sf;
+  }
+  return NULL;
+}
+
+/* procBlock::AssignInviscidGhostCellsEdge (src/procBlock.cpp:2565-2714)
+ * (viscous == 0) and AssignViscousGhostCellsEdge (:2874-3030): the ghost
+ * cells along the 12 block edges, line by line, layer by layer */
+static void edge_ghosts(orc_case *c, blk_t *b, int viscous) {
+  for (int dd = 0; dd < 3; ++dd) {
+    int d2 = (dd + 1) % 3, d3 = (dd + 2) % 3;
+    int max1 = ndir(b, dd), max2 = ndir(b, d2), max3 = ndir(b, d3);
+    int surfStart2 = 2 * d2 + 1, surfStart3 = 2 * d3 + 1;
+    for (int layer3 = 1; layer3 <= b->g; ++layer3)
+      for (int layer2 = 1; layer2 <= b->g; ++layer2)
+        for (int cc = 0; cc < 4; ++cc) {
+          int upper2 = cc > 1, upper3 = cc % 2 == 1;
+          int pC2 = upper2 ? max2 + layer2 - 2 : 1 - layer2;
+          int gC2 = upper2 ? pC2 + 1 : pC2 - 1;
+          int pC3 = upper3 ? max3 + layer3 - 2 : 1 - layer3;
+          int gC3 = upper3 ? pC3 + 1 : pC3 - 1;
+          int surf2 = upper2 ? surfStart2 + 1 : surfStart2;
+          int surf3 = upper3 ? surfStart3 + 1 : surfStart3;
+          int cF2_2 = upper2 ? max2 : 0, cF2_3 = upper3 ? max3 - 1 : 0;
+          int cF3_2 = upper2 ? max2 - 1 : 0, cF3_3 = upper3 ? max3 : 0;
+          for (int d1 = 0; d1 < max1; ++d1) {
+            int i, j, k;
+            map3(dd, d1, cF2_2, cF2_3, &i, &j, &k);
+            const surf_t *s2 = bc_surface_at(b, i, j, k, surf2);
+            map3(dd, d1, cF3_2, cF3_3, &i, &j, &k);
+            const surf_t *s3 = bc_surface_at(b, i, j, k, surf3);
+            map3(dd, d1, cF2_2, gC3, &i, &j, &k);
+            v3 fa2 = b->fa[d2][fidx(b, d2, i, j, k)].u;
+            map3(dd, d1, gC2, cF3_3, &i, &j, &k);
+            v3 fa3 = b->fa[d3][fidx(b, d3, i, j, k)].u;
+            char bc2[32], bc3[32];
+            snprintf(bc2, sizeof bc2, "%s", s2 ? s2->type : "");
+            snprintf(bc3, sizeof bc3, "%s", s3 ? s3->type : "");
+            if (!viscous) {
+              if (!strcmp(bc2, "viscousWall")) strcpy(bc2, "slipWall");
+              if (!strcmp(bc3, "viscousWall")) strcpy(bc3, "slipWall");
+            }
+            int slip2 = !strcmp(bc2, "slipWall"), slip3 = !strcmp(bc3, "slipWall");
+            long gq, q2, q3;
+            map3(dd, d1, gC2, gC3, &i, &j, &k);
+            gq = cidx(b, i, j, k);
+            map3(dd, d1, pC2, gC3, &i, &j, &k);
+            q2 = cidx(b, i, j, k);
+            map3(dd, d1, gC2, pC3, &i, &j, &k);
+            q3 = cidx(b, i, j, k);
+            double *W = b->W;
+            if (slip2 && !slip3) {
+              ghost_state(c, &W[q2 * NEQ], bc2, fa2, surf2, s2->d[6], layer2, &W[gq * NEQ]);
+            } else if (!slip2 && slip3) {
+              ghost_state(c, &W[q3 * NEQ], bc3, fa3, surf3, s3->d[6], layer3, &W[gq * NEQ]);
+            } else if (!viscous || (!strcmp(bc2, "viscousWall") && !strcmp(bc3, "viscousWall"))) {
+              if (layer2 == layer3) {
+                for (int e = 0; e < NEQ; ++e) W[gq * NEQ + e] = 0.5 * (W[q2 * NEQ + e] + W[q3 * NEQ + e]);
+              } else if (layer2 > layer3) {
+                memcpy(&W[gq * NEQ], &W[q3 * NEQ], sizeof(double) * NEQ);
+              } else {
+                memcpy(&W[gq * NEQ], &W[q2 * NEQ], sizeof(double) * NEQ);
+              }
+            }
+          }
+        }
+  }
+}
+
+/* GetGhostState, viscousWall with the low-Re treatment
+ * (src/ghostStates.cpp:126-145, 185-190, adiabatic default): the velocity
+ * mirrors about the wall velocity; an isothermal wall sets the density from
+ * the mirrored temperature at the interior pressure */
+static void viscous_wall_ghost(const orc_case *c, const double *in, const bcstate_t *s, double *ghost) {
+  memcpy(ghost, in, sizeof(double) * NEQ);
+  v3 vw = s ? s->vel : V(0, 0, 0);
+  ghost[1] = 2.0 * vw.x - in[1];
+  ghost[2] = 2.0 * vw.y - in[2];
+  ghost[3] = 2.0 * vw.z - in[3];
+  if (s && s->isothermal) {
+    double tGhost = 2.0 * s->twall - temperature(&c->gas, in);
+    double R = 0.0 + 1.0 * c->gas.R;
+    double rho = ghost[4] / (R * tGhost);
+    ghost[0] = rho * (in[0] / (0.0 + in[0]));
+  }
+}
+
+/* procBlock::AssignViscousGhostCells (src/procBlock.cpp:2760-2836) */
+static void viscous_ghosts(orc_case *c, blk_t *b) {
+  for (int layer = 1; layer <= b->g; ++layer)
+    for (int s = 0; s < b->nsurf; ++s) {
+      const surf_t *sf = &b->surf[s];
+      if (strcmp(sf->type, "viscousWall")) continue;
+      int st = surf_type(sf), dir = (st - 1) / 2;
+      int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      int r3 = sf->d[2 * dir], n3 = ndir(b, dir);
+      int gC, iC;
+      if (st % 2 == 0) {
+        gC = r3 + layer - 1;
+        iC = r3 - layer;
+        if (iC < 0) iC = 0;
+      } else {
+        gC = r3 - layer;
+        iC = r3 + layer - 1;
+        if (iC >= n3) iC = n3 - 1;
+      }
+      const bcstate_t *bs = find_bc(c, sf->d[6]);
+      for (int q2 = sf->d[2 * d2]; q2 < sf->d[2 * d2 + 1]; ++q2)
+        for (int q1 = sf->d[2 * d1]; q1 < sf->d[2 * d1 + 1]; ++q1) {
+          int ii, ij, ik, gi, gj, gk;
+          map3(dir, iC, q1, q2, &ii, &ij, &ik);
+          map3(dir, gC, q1, q2, &gi, &gj, &gk);
+          viscous_wall_ghost(c, &b->W[cidx(b, ii, ij, ik) * NEQ], bs, &b->W[cidx(b, gi, gj, gk) * NEQ]);
+        }
+    }
+  edge_ghosts(c, b, 1);
+}
+
+/* CalcGradsI/J/K at a face (src/procBlock.cpp:5173-5788), VectorGradGG /
+ * ScalarGradGG (src/utility.cpp:59-188): Green-Gauss over the control
+ * volume spanning the face's lower cell L and upper cell U.  The face values
+ * in direction d are L's and U's; in the other directions the average of
+ * L, U and their two neighbours on that side. */
+static void face_grads(const orc_case *c, const blk_t *b, int d, int i, int j, int k,
+                       double vg[3][3], double tg[3]) {
+  (void)c;
+  int ed[3] = {d == 0, d == 1, d == 2};
+  int L[3] = {i - ed[0], j - ed[1], k - ed[2]}, U[3] = {i, j, k};
+  double au[3][3], al[3][3];
+  for (int q = 0; q < 3; ++q) {
+    int eq[3] = {q == 0, q == 1, q == 2};
+    uv3 f1, f2, f3, f4;
+    if (q == d) {
+      f1 = b->fa[q][fidx(b, q, U[0], U[1], U[2])];
+      f2 = b->fa[q][fidx(b, q, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2])];
+      f3 = f1;
+      f4 = b->fa[q][fidx(b, q, L[0], L[1], L[2])];
+    } else {
+      f1 = b->fa[q][fidx(b, q, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2])];
+      f2 = b->fa[q][fidx(b, q, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2])];
+      f3 = b->fa[q][fidx(b, q, U[0], U[1], U[2])];
+      f4 = b->fa[q][fidx(b, q, L[0], L[1], L[2])];
+    }
+    au[q][0] = 0.5 * (f1.u.x * f1.m + f2.u.x * f2.m);
+    au[q][1] = 0.5 * (f1.u.y * f1.m + f2.u.y * f2.m);
+    au[q][2] = 0.5 * (f1.u.z * f1.m + f2.u.z * f2.m);
+    al[q][0] = 0.5 * (f3.u.x * f3.m + f4.u.x * f4.m);
+    al[q][1] = 0.5 * (f3.u.y * f3.m + f4.u.y * f4.m);
+    al[q][2] = 0.5 * (f3.u.z * f3.m + f4.u.z * f4.m);
+  }
+  long cL = cidx(b, L[0], L[1], L[2]), cU = cidx(b, U[0], U[1], U[2]);
+  double vol = 0.5 * (b->vol[cL] + b->vol[cU]);
+  /* values on the six faces: velocity (3) and temperature */
+  double vu[3][4], vl[3][4];
+  for (int q = 0; q < 3; ++q) {
+    int eq[3] = {q == 0, q == 1, q == 2};
+    for (int v = 0; v < 4; ++v) {
+      double xl = v < 3 ? b->W[cL * NEQ + 1 + v] : b->T[cL];
+      double xu = v < 3 ? b->W[cU * NEQ + 1 + v] : b->T[cU];
+      if (q == d) {
+        vu[q][v] = xu;
+        vl[q][v] = xl;
+      } else {
+        long uP = cidx(b, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2]);
+        long lP = cidx(b, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2]);
+        long uM = cidx(b, U[0] - eq[0], U[1] - eq[1], U[2] - eq[2]);
+        long lM = cidx(b, L[0] - eq[0], L[1] - eq[1], L[2] - eq[2]);
+        double up = v < 3 ? b->W[uP * NEQ + 1 + v] : b->T[uP];
+        double lp = v < 3 ? b->W[lP * NEQ + 1 + v] : b->T[lP];
+        double um = v < 3 ? b->W[uM * NEQ + 1 + v] : b->T[uM];
+        double lm = v < 3 ? b->W[lM * NEQ + 1 + v] : b->T[lM];
+        vu[q][v] = 0.25 * (xl + xu + up + lp);
+        vl[q][v] = 0.25 * (xl + xu + um + lm);
+      }
+    }
+  }
+  double invVol = 1.0 / vol;
+  for (int r = 0; r < 3; ++r) {
+    for (int cc = 0; cc < 3; ++cc)
+      vg[r][cc] = (vu[0][cc] * au[0][r] - vl[0][cc] * al[0][r] + vu[1][cc] * au[1][r] -
+                   vl[1][cc] * al[1][r] + vu[2][cc] * au[2][r] - vl[2][cc] * al[2][r]) * invVol;
+    tg[r] = (vu[0][3] * au[0][r] - vl[0][3] * al[0][r] + vu[1][3] * au[1][r] -
+             vl[1][3] * al[1][r] + vu[2][3] * au[2][r] - vl[2][3] * al[2][r]) * invVol;
+  }
+}
+
+/* viscousFlux::CalcFlux / CalcWallFlux (src/viscousFlux.cpp:58-188) with
+ * TauNormal (src/utility.cpp:426-436), laminar single species.  The wall
+ * form omits the species-enthalpy term (+0). */
+static void viscous_flux(const orc_case *c, const double vg[3][3], const double tg[3], v3 n,
+                         const double *Wf, double muFace, int wall, double *F) {
+  double mu = c->vscale * muFace, mut = c->vscale * 0.0;
+  double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
+  double trace = vg[0][0] + vg[1][1] + vg[2][2];
+  double nn[3] = {n.x, n.y, n.z};
+  double tau[3];
+  for (int r = 0; r < 3; ++r) {
+    double s0 = vg[r][0] + vg[0][r], s1 = vg[r][1] + vg[1][r], s2 = vg[r][2] + vg[2][r];
+    double sa = s0 * nn[0] + s1 * nn[1] + s2 * nn[2];
+    tau[r] = lambda * trace * nn[r] + (mu + mut) * sa;
+  }
+  double t = temperature(&c->gas, Wf);
+  double kk = cond_of_t(c, t) * c->vscale;
+  double kt = mut * cps(&c->gas) / 0.9;
+  double tv = tau[0] * Wf[1] + tau[1] * Wf[2] + tau[2] * Wf[3];
+  double q = (kk + kt) * (tg[0] * nn[0] + tg[1] * nn[1] + tg[2] * nn[2]);
+  F[0] = 0.0;
+  F[1] = tau[0];
+  F[2] = tau[1];
+  F[3] = tau[2];
+  F[4] = wall ? tv + q : tv + q + 0.0;
+}
+
+/* procBlock::CalcViscFluxI/J/K (src/procBlock.cpp:1233-2135), laminar */
+static void visc_flux_dir(orc_case *c, blk_t *b, int d) {
+  int ei = (d == 0), ej = (d == 1), ek = (d == 2);
+  int n3 = ndir(b, d);
+  for (int k = 0; k < b->nk + ek; ++k)
+    for (int j = 0; j < b->nj + ej; ++j)
+      for (int i = 0; i < b->ni + ei; ++i) {
+        int f3 = d == 0 ? i : (d == 1 ? j : k);
+        double vg[3][3], tg[3];
+        face_grads(c, b, d, i, j, k, vg, tg);
+        int wall = 0;
+        if (f3 == 0 || f3 == n3) {
+          const surf_t *sf = bc_surface_at(b, i, j, k, 2 * d + (f3 == 0 ? 1 : 2));
+          wall = sf && !strcmp(sf->type, "viscousWall");
+        }
+        long cL = cidx(b, i - ei, j - ej, k - ek), cU = cidx(b, i, j, k);
+        double Wf[NEQ], muF;
+        if (!c->viscRecon4) {
+          /* FaceReconCentral (include/reconstruction.hpp:314-327) */
+          double w[2] = {b->cw[d][cL], b->cw[d][cU]}, co[2];
+          orc_lagrange_coeff(w, 1, 0, 0, co);
+          for (int e = 0; e < NEQ; ++e) Wf[e] = co[0] * b->W[cU * NEQ + e] + co[1] * b->W[cL * NEQ + e];
+          muF = co[0] * b->mu[cU] + co[1] * b->mu[cL];
+        } else {
+          /* FaceReconCentral4th (:331-349) */
+          long cLL = cidx(b, i - 2 * ei, j - 2 * ej, k - 2 * ek);
+          long cUU = cidx(b, i + ei, j + ej, k + ek);
+          double w[4] = {b->cw[d][cLL], b->cw[d][cL], b->cw[d][cU], b->cw[d][cUU]}, co[4];
+          orc_lagrange_coeff(w, 3, 1, 1, co);
+          for (int e = 0; e < NEQ; ++e)
+            Wf[e] = co[0] * b->W[cLL * NEQ + e] + co[1] * b->W[cL * NEQ + e] +
+                    co[2] * b->W[cU * NEQ + e] + co[3] * b->W[cUU * NEQ + e];
+          muF = co[0] * b->mu[cLL] + co[1] * b->mu[cL] + co[2] * b->mu[cU] + co[3] * b->mu[cUU];
+        }
+        uv3 a = b->fa[d][fidx(b, d, i, j, k)];
+        double F[NEQ];
+        viscous_flux(c, vg, tg, a.u, Wf, muF, wall, F);
+        if (f3 > 0) {
+          double *R = &b->R[pidx(b, i - ei, j - ej, k - ek) * NEQ];
+          for (int e = 0; e < NEQ; ++e) R[e] -= F[e] * a.m;
+        }
+        if (f3 < n3) {
+          long p = pidx(b, i, j, k);
+          double *R = &b->R[p * NEQ];
+          for (int e = 0; e < NEQ; ++e) R[e] += F[e] * a.m;
+          uv3 aU = b->fa[d][fidx(b, d, i + ei, j + ej, k + ek)];
+          double vsr = visc_cell_specrad(c, &b->W[cU * NEQ], a, aU, b->vol[cU], b->mu[cU]);
+          b->sr[p] += vsr * c->viscCoeff;
+          if (c->implicit) b->A[p] += 2.0 * vsr;
+        }
+      }
+}
+
+/* test hooks: temperature/viscosity of every non-corner cell, and the face
+ * gradients (velocity tensor row-major, then temperature) of one face */
+void orc_update_aux(orc_case *c) {
+  for (int bq = 0; bq < c->nb; ++bq) update_aux(c, &c->blk[bq]);
+}
+void orc_face_grads(const orc_case *c, int bq, int d, int i, int j, int k, double *out) {
+  double vg[3][3], tg[3];
+  face_grads(c, &c->blk[bq], d, i, j, k, vg, tg);
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) out[3 * r + q] = vg[r][q];
+  for (int r = 0; r < 3; ++r) out[9 + r] = tg[r];
+}
+double orc_viscosity(const orc_case *c, double t) { return visc_of_t(c, t); }
+double orc_conductivity(const orc_case *c, double t) { return cond_of_t(c, t); }
+
+/* ProjC2CDist (src/procBlock.cpp:6316-6341): centre-to-centre distance
+ * across face (i, j, k) of direction d, projected on its unit normal */
+static double proj_c2c(const blk_t *b, int d, int i, int j, int k) {
+  v3 cu = b->ctr[cidx(b, i, j, k)];
+  v3 cl = b->ctr[cidx(b, i - (d == 0), j - (d == 1), k - (d == 2))];
+  return vdot(vsub(cu, cl), b->fa[d][fidx(b, d, i, j, k)].u);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2037,6 +2462,13 @@ void orc_residual(orc_case *c) {
     inv_flux_dir(c, b, 0);
     inv_flux_dir(c, b, 1);
     inv_flux_dir(c, b, 2);
+    if (c->viscous) {
+      viscous_ghosts(c, b);
+      update_aux(c, b);
+      visc_flux_dir(c, b, 0);
+      visc_flux_dir(c, b, 1);
+      visc_flux_dir(c, b, 2);
+    }
   }
 }
 
@@ -2120,8 +2552,12 @@ static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
     int fi = upper ? ni : i, fj = upper ? nj : j, fk = upper ? nk : k;
     uv3 a = b->fa[d][fidx(b, d, fi, fj, fk)];
     double o[NEQ];
-    scalar_offdiag(&c->gas, &b->W[cidx(b, ni, nj, nk) * NEQ],
-                   &x[cidx(b, ni, nj, nk) * NEQ], a, !upper, o);
+    long cn = cidx(b, ni, nj, nk);
+    /* FaceSpectralRadius with the viscous part (include/spectralRadius.hpp:
+     * 192-203): the neighbour's state and viscosity, ProjC2CDist */
+    double lv = c->viscous ? visc_face_specrad(c, &b->W[cn * NEQ], a, proj_c2c(b, d, fi, fj, fk), b->mu[cn])
+                           : 0.0;
+    scalar_offdiag_v(&c->gas, &b->W[cn * NEQ], &x[cn * NEQ], a, !upper, c->viscous, lv, o);
     for (int e = 0; e < NEQ; ++e) out[e] += o[e];
   }
 }
@@ -2373,6 +2809,10 @@ void orc_set_state(orc_case *c, int bq, const double *in) {
     for (int j = 0; j < b->nj; ++j)
       for (int i = 0; i < b->ni; ++i)
         memcpy(&b->W[cidx(b, i, j, k) * NEQ], &in[pidx(b, i, j, k) * NEQ], sizeof(double) * NEQ);
+}
+void orc_set_state_padded(orc_case *c, int bq, const double *in) {
+  blk_t *b = &c->blk[bq];
+  memcpy(b->W, in, sizeof(double) * NEQ * (size_t)ncells_padded(b));
 }
 void orc_get_state_padded(const orc_case *c, int bq, double *out) {
   const blk_t *b = &c->blk[bq];

@@ -58,6 +58,7 @@ void orc_set_matrix_residual(orc_case *c, int on);
 void orc_get_state(const orc_case *c, int b, double *out);
 void orc_set_state(orc_case *c, int b, const double *in);
 /* padded state (ghosts included), AoS, i fastest over (ni+2g)(nj+2g)(nk+2g) */
+void orc_set_state_padded(orc_case *c, int bq, const double *in);
 void orc_get_state_padded(const orc_case *c, int b, double *out);
 
 /* one outer step: CalcCFL + StoreOldSolution (src/main.cpp:235-238) */
@@ -113,6 +114,10 @@ double orc_weno_beta(int which, double x0, double x1, double x2, double y0, doub
                      double y2);
 void orc_weno(const double *u3, const double *u2, const double *u1, const double *d1,
               const double *d2, const double *w, int wenoZ, double *out);
+void orc_update_aux(orc_case *c);
+void orc_face_grads(const orc_case *c, int bq, int d, int i, int j, int k, double *out);
+double orc_viscosity(const orc_case *c, double t);
+double orc_conductivity(const orc_case *c, double t);
 void orc_muscl(const double *u2, const double *u1, const double *d1,
                double kappa, int limiter, double uw2, double uw, double dw,
                double *out);

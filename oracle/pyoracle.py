@@ -73,6 +73,13 @@ def lib():
         L.orc_weno_beta.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6
         L.orc_weno_beta.restype = ctypes.c_double
         L.orc_weno.argtypes = [_D] * 6 + [ctypes.c_int, _D]
+        L.orc_update_aux.argtypes = [_V]
+        L.orc_set_state_padded.argtypes = [_V, ctypes.c_int, _D]
+        L.orc_face_grads.argtypes = [_V] + [ctypes.c_int] * 5 + [_D]
+        L.orc_viscosity.argtypes = [_V, ctypes.c_double]
+        L.orc_viscosity.restype = ctypes.c_double
+        L.orc_conductivity.argtypes = [_V, ctypes.c_double]
+        L.orc_conductivity.restype = ctypes.c_double
         L.orc_pyramid_volume.argtypes = [_D] * 5
         L.orc_pyramid_volume.restype = ctypes.c_double
         L.orc_hyperplane_order.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)]
@@ -148,6 +155,11 @@ class OracleCase:
 
     def state(self, b=0):
         return self._get(self.L.orc_get_state, b, 5)
+
+    def set_state_padded(self, W, b=0):
+        """all cells, ghost layers included: (nk+2g, nj+2g, ni+2g, 5) flattened"""
+        W = arr(W)
+        self.L.orc_set_state_padded(self.h, b, P(W))
 
     def set_state(self, W, b=0):
         self.L.orc_set_state(self.h, b, P(arr(W).reshape(-1)))
