@@ -147,7 +147,8 @@ boundaryConditions: 1
     return path
 
 
-def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0, recon="thirdOrder"):
+def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0, recon="thirdOrder", viscous=False,
+              visc_recon="central"):
     """3D channel with a sin^2 bump on the lower wall; parameters seeded."""
     rng = np.random.default_rng(seed)
     n = n or int(rng.integers(24, 97))
@@ -164,18 +165,28 @@ def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0, recon="t
     yw = hl * ell * np.sin(np.pi * xi) ** 2
     y = yw + (H - yw) * j / n
     z = W * k / n
-    write_plot3d(os.path.join(outdir, name + ".xyz"), [(x, y, z)])
+    scale = 2e-4 if viscous else 1.0  # viscous: a 0.6 mm channel, Reynolds number of order 1e3
+    write_plot3d(os.path.join(outdir, name + ".xyz"), [(scale * x, scale * y, scale * z)])
     gamma = (AIR_N + 1.0) / AIR_N
     R = air_gas_constant()
     t_inf, rho_inf = 288.15, 1.225
     p_inf = rho_inf * R * t_inf
     u_inf = mach * math.sqrt(gamma * R * t_inf)
+    walls = ("viscousWall", 3), ("viscousWall", 4)
     bcs = _surfaces_box(n, n, n, {"imin": ("characteristic", 1), "imax": ("characteristic", 1),
-                                  "jmin": ("slipWall", 2), "jmax": ("slipWall", 2),
+                                  "jmin": walls[0] if viscous else ("slipWall", 2),
+                                  "jmax": walls[1] if viscous else ("slipWall", 2),
                                   "kmin": ("slipWall", 2), "kmax": ("slipWall", 2)})
+    # viscous: an adiabatic wall on the bump, an isothermal wall moving along
+    # x on the top; laminar at a low Reynolds number (small channel)
+    eq = "navierStokes" if viscous else "euler"
+    extra_states = (f""", viscousWall(tag=3), viscousWall(tag=4; temperature=300.0; velocity=[{0.3 * u_inf!r}, 0.0, 0.0])"""
+                    if viscous else "")
+
     inp = f"""# bump channel seed {seed}: n={n}, h/l={hl:.4f}, Mach={mach:.4f}
 gridName: {name}
-equationSet: euler
+equationSet: {eq}
+viscousFaceReconstruction: {visc_recon}
 timeIntegration: implicitEuler
 cflStart: {cfl}
 cflMax: {cfl}
@@ -188,7 +199,7 @@ initialConditions: <icState(tag=-1; pressure={p_inf!r}; density={rho_inf!r}; vel
 matrixSolver: lusgs
 matrixSweeps: 1
 matrixRelaxation: 1.0
-boundaryStates: <characteristic(tag=1; pressure={p_inf!r}; density={rho_inf!r}; velocity=[{u_inf!r}, 0.0, 0.0])>
+boundaryStates: <characteristic(tag=1; pressure={p_inf!r}; density={rho_inf!r}; velocity=[{u_inf!r}, 0.0, 0.0]){extra_states}>
 boundaryConditions: 1
 {bcs}"""
     path = os.path.join(outdir, name + ".inp")

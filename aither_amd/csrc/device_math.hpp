@@ -355,27 +355,27 @@ __device__ __forceinline__ double Muscl(double u2, double u1, double d1, double 
 // two downwind; all loop bounds below are compile-time constants, so each
 // coefficient unrolls to straight-line arithmetic in the reference's order.
 
-// StencilWidth (include/utility.hpp:103-114)
-__device__ __forceinline__ double StencilW(const double w[5], int start, int end) {
+// StencilWidth (include/utility.hpp:103-114) over N widths
+template <int N>
+__device__ __forceinline__ double StencilWN(const double *w, int start, int end) {
   double s = 0.0;
   if (end > start) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
+    for (int q = 0; q < N; ++q)
       if (q >= start && q < end) s += w[q];
   } else if (start > end) {
     double t = 0.0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
+    for (int q = 0; q < N; ++q)
       if (q >= end && q < start) t += w[q];
     s = -1.0 * t;
   }
   return s;
 }
 
-// coefficient jj of LagrangeCoeff(w, DEG, RR, 2) (src/utility.cpp:449-483)
-template <int DEG, int RR>
-__device__ __forceinline__ double LagrangeCoeffAt(const double w[5], int jj) {
-  constexpr int ii = 2;
+// coefficient jj of LagrangeCoeff(w, DEG, RR, II) (src/utility.cpp:449-483)
+template <int N, int DEG, int RR, int II>
+__device__ __forceinline__ double LagrangeCoeffN(const double *w, int jj) {
   double c = 0.0;
 #pragma unroll
   for (int mm = 1; mm <= DEG + 1; ++mm) {
@@ -387,13 +387,19 @@ __device__ __forceinline__ double LagrangeCoeffAt(const double w[5], int jj) {
       double prod = 1.0;
 #pragma unroll
       for (int qq = 0; qq <= DEG + 1; ++qq)
-        if (qq != mm && qq != ll) prod *= StencilW(w, ii - RR + qq, ii + 1);
+        if (qq != mm && qq != ll) prod *= StencilWN<N>(w, II - RR + qq, II + 1);
       numer += prod;
-      denom *= StencilW(w, ii - RR + ll, ii - RR + mm);
+      denom *= StencilWN<N>(w, II - RR + ll, II - RR + mm);
     }
     c += numer / denom;
   }
-  return c * w[ii - RR + jj];
+  return c * w[II - RR + jj];
+}
+
+// the WENO stencils: five widths, first upwind cell at index 2
+template <int DEG, int RR>
+__device__ __forceinline__ double LagrangeCoeffAt(const double w[5], int jj) {
+  return LagrangeCoeffN<5, DEG, RR, 2>(w, jj);
 }
 
 // the width-only part of one WENO reconstruction
@@ -496,15 +502,19 @@ __device__ __forceinline__ double InvCellSpecRad(const GasT<double> &g, double r
 // RusanovScalarOffDiagonal for the inviscid equations
 // (src/fluxJacobian.cpp:122-162): 0.5|A| (F(U_nb + du) - F(U_nb)) +/- lambda du
 // with lambda = InvFaceSpectralRadius (include/spectralRadius.hpp:66-80)
+// With the viscous terms FaceSpectralRadius adds the viscous face spectral
+// radius `lamv` (include/spectralRadius.hpp:192-203).
 template <typename T>
 __device__ __forceinline__ void ScalarOffDiag(const GasT<T> &g, const T Wnb[5], const T du[5],
-                                              T nx, T ny, T nz, T mag, bool positive, T out[5]) {
+                                              T nx, T ny, T nz, T mag, bool positive, T out[5],
+                                              bool viscous = false, T lamv = T(0)) {
   T Wu[5], Fo[5], Fn[5];
   UpdatePrim(g, Wnb, du, Wu);
   PhysFlux(g, Wnb, nx, ny, nz, Fo);
   PhysFlux(g, Wu, nx, ny, nz, Fn);
   const T half = T(0.5) * mag;
-  const T lam = T(0.5) * mag * (fabs((Wnb[1] * nx + Wnb[2] * ny) + Wnb[3] * nz) + SoS(g, Wnb[0], Wnb[4]));
+  T lam = T(0.5) * mag * (fabs((Wnb[1] * nx + Wnb[2] * ny) + Wnb[3] * nz) + SoS(g, Wnb[0], Wnb[4]));
+  if (viscous) lam += lamv;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     const T fc = half * (Fn[e] - Fo[e]);

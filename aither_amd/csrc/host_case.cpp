@@ -99,8 +99,12 @@ std::array<double, 3> ParseVec(const std::string &s) {
 
 // fluid properties: <name>.dat in the case directory first
 // (src/fluid.cpp:105-118), then the values of fluidDatabase/air.dat and N2.dat
+// fluid database entry (fluidDatabase/<name>.dat): n, molar mass and the
+// Sutherland coefficients of viscosity and conductivity
 bool FluidProps(const std::string &dir, const std::string &name, double *n,
-                double *molar) {
+                double *molar, std::array<double, 4> *suth) {
+  static const char *sk[4] = {"sutherlandViscosityC1", "sutherlandViscosityS",
+                              "sutherlandConductivityC1", "sutherlandConductivityS"};
   std::ifstream f(dir + "/" + name + ".dat");
   if (f) {
     std::string line;
@@ -112,11 +116,23 @@ bool FluidProps(const std::string &dir, const std::string &name, double *n,
       const auto key = Trim(line.substr(0, c));
       if (key == "n") { *n = std::stod(line.substr(c + 1)); got |= 1; }
       if (key == "molarMass") { *molar = std::stod(line.substr(c + 1)) / 1000.; got |= 2; }
+      for (int q = 0; q < 4; ++q)
+        if (key == sk[q]) { (*suth)[q] = std::stod(line.substr(c + 1)); got |= 4 << q; }
     }
-    if (got == 3) return true;
+    if (got == 63) return true;
   }
-  if (name == "air") { *n = 2.5; *molar = 28.97 / 1000.; return true; }
-  if (name == "N2") { *n = 2.5; *molar = 28.0134 / 1000.; return true; }
+  if (name == "air") {
+    *n = 2.5;
+    *molar = 28.97 / 1000.;
+    *suth = {1.458e-6, 110.4, 2.495e-3, 194.0};
+    return true;
+  }
+  if (name == "N2") {
+    *n = 2.5;
+    *molar = 28.0134 / 1000.;
+    *suth = {1.4742e-06, 1.2846e+02, 2.6834e-03, 2.5615e+02};
+    return true;
+  }
   return false;
 }
 
@@ -654,7 +670,15 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
         else if (val == "minmod") c.limiter = Limiter::Minmod;
         else return "unsupported limiter " + val;
       } else if (key == "equationSet") {
-        if (val != "euler") return "unsupported equationSet " + val;
+        if (val == "euler") c.viscous = false;
+        else if (val == "navierStokes") c.viscous = true;
+        else return "unsupported equationSet " + val;
+      } else if (key == "viscousFaceReconstruction") {
+        if (val == "central") c.viscRecon4 = false;
+        else if (val == "centralFourth") c.viscRecon4 = true;
+        else return "unsupported viscousFaceReconstruction " + val;
+      } else if (key == "turbulenceModel") {
+        if (val != "none") return "unsupported turbulenceModel " + val;
       } else if (key == "matrixSolver") {
         if (val == "lusgs") c.solver = Solver::LUSGS;
         else if (val == "dplur") c.solver = Solver::DPLUR;
@@ -704,6 +728,14 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
           if (p.count("direction")) s.dir = ParseVec(p.at("direction"));
           if (p.count("nonreflecting") && p.at("nonreflecting") == "true")
             return "nonreflecting boundary states are not supported";
+          if (p.count("temperature")) {
+            s.isothermal = true;
+            s.twall = std::stod(p.at("temperature"));
+          }
+          if (p.count("heatFlux") && std::stod(p.at("heatFlux")) != 0.0)
+            return "viscousWall heat flux is not supported";
+          if (p.count("wallTreatment") && p.at("wallTreatment") != "lowRe")
+            return "wall laws are not supported";
           c.bcs.push_back(s);
         }
       } else if (key == "boundaryConditions") {
@@ -719,7 +751,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (c.rRef <= 0 || c.tRef <= 0) return "referenceDensity and referenceTemperature are required";
   if (c.timeint == TimeInt::ExplicitEuler) c.nonlinear = 1;  // src/input.cpp:928-934
   double fn = 0, fm = 0;
-  if (!FluidProps(c.dir, fluid, &fn, &fm)) return "unknown fluid " + fluid;
+  if (!FluidProps(c.dir, fluid, &fn, &fm, &c.suth)) return "unknown fluid " + fluid;
 
   // reference speed of sound (src/input.cpp:616-621) and nondimensional gas
   // constant (src/fluid.cpp:89-103)
@@ -738,6 +770,15 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     const double cv = 0.0 + 1.0 * (c.gasR * fn);
     c.gamma = cp / cv;
   }
+  // Sutherland transport and its nondimensional scaling
+  // (src/transport.cpp:29-66, include/transport.hpp:40-44); Prandtl number
+  // (include/thermodynamic.hpp:61-64); viscous CFL coefficient
+  // (src/input.cpp:1110-1118)
+  c.muRef = c.suth[0] * std::pow(c.tRef, 1.5) / (c.tRef + c.suth[1]);
+  c.kNonDim = (c.aRef * c.aRef * c.muRef) / c.tRef;
+  c.vscale = c.muRef / (c.rRef * c.aRef * c.lRef);
+  c.prandtl = (4.0 * c.gamma) / (9.0 * c.gamma - 5.0);
+  c.viscCoeff = c.kappa == 1.0 ? 4.0 : (c.kappa == -2.0 ? 2.0 : 1.0);
   // state nondimensionalisation (src/inputStates.cpp:464-473, 590-599,
   // 674-681); the stagnation direction stays as given (src/inputStates.cpp:594
   // discards the normalised copy)
@@ -749,6 +790,11 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     ic.p /= pScale;
   }
   for (auto &s : c.bcs) {
+    if (s.kind == "viscousWall") {  // src/inputStates.cpp:775-783
+      for (auto &v : s.vel) v /= c.aRef;
+      s.twall /= c.tRef;
+      continue;
+    }
     if (s.kind == "stagnationInlet") {
       s.p0 /= pScale;
       s.t0 /= c.tRef;

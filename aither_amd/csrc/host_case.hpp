@@ -38,6 +38,8 @@ struct BcState {  // boundaryStates entry, nondimensional
   std::array<double, 3> vel{0, 0, 0};
   double p0 = 0, t0 = 0;
   std::array<double, 3> dir{0, 0, 0};
+  bool isothermal = false;  // viscousWall with a given wall temperature
+  double twall = 0;
 };
 
 struct Surface {  // boundarySurface: bc type, i/j/k ranges, tag
@@ -112,6 +114,11 @@ struct HostCase {
   TimeInt timeint = TimeInt::ExplicitEuler;
   Solver solver = Solver::LUSGS;
   double gasR = 0, gasN = 0, gamma = 0;
+  // laminar viscous terms (equationSet navierStokes)
+  bool viscous = false;
+  bool viscRecon4 = false;                // viscousFaceReconstruction centralFourth
+  std::array<double, 4> suth{};           // Sutherland C1, S (viscosity, conductivity)
+  double muRef = 0, kNonDim = 0, vscale = 0, prandtl = 0, viscCoeff = 1;
   double totalCells = 0;
   std::vector<BcState> bcs;
   std::vector<HostBlock> blocks;
@@ -122,7 +129,8 @@ struct HostCase {
   bool InitMatrix() const { return solver == Solver::DPLUR || sweeps > 1; }
   // src/input.cpp:1127-1143
   int Ghosts() const {
-    return recon == Recon::Constant ? 1 : (recon == Recon::MUSCL ? 2 : 3);
+    const int g = recon == Recon::Constant ? 1 : (recon == Recon::MUSCL ? 2 : 3);
+    return viscRecon4 && g < 2 ? 2 : g;
   }
   double Cfl(int nn) const;
 };

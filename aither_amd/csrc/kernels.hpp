@@ -34,20 +34,79 @@ struct DevBlock {
   double *bt;            // FP64 operator
   float *bt32;           // FP32 operator
   long nbf;              // boundary faces: 2 (nj nk + nk ni + ni nj)
+  // viscous cases only
+  double *T, *mu;               // padded: temperature, viscosity
+  const double4 *faP[3];        // padded faces (ghost faces included)
+  const double *volP;           // padded volumes
+  const double *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
+  const unsigned char *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
   long cellBase;         // running cell offset of this block (L-inf index)
   int blockId;
 };
 
+// laminar viscous terms: Sutherland transport (src/transport.cpp:29-176),
+// its nondimensional scaling mu_ref / (rho_ref a_ref L_ref), Prandtl number
+// 4 gamma / (9 gamma - 5), and the viscous CFL coefficient
+struct ViscParams {
+  int on, recon4;
+  double vscale, prandtl, coeff, tRef, muRef, kNonDim;
+  double c1, s1, c2, s2;  // Sutherland C1, S of viscosity, then of conductivity
+};
+
 struct Params {
   Gas gas;
+  ViscParams visc;
   double kappa;
   double theta, zeta, relax, dualCFL;
   double dtFixed;  // nondimensional global time step, <= 0 for local dt
   int implicit, multilevel, initMatrix, explicitEuler;
 };
 
+// ---------------------------------------------------------------------------
+// laminar viscous helpers
+
+// idealGas::Temperature (src/eos.cpp:54-60)
+__device__ __forceinline__ double TempOf(const GasT<double> &g, double rho, double p) {
+  const double rhoR = 0.0 + rho * g.R;
+  return p / rhoR;
+}
+// sutherland::SpeciesViscosity / SpeciesConductivity (src/transport.cpp:
+// 97-117), nondimensional; t^1.5 correctly rounded
+__device__ __forceinline__ double SutherlandMu(const ViscParams &v, double t) {
+  const double temp = t * v.tRef;
+  const double mu = (v.c1 * PowCR(temp, 1.5)) / (temp + v.s1);
+  return mu / v.muRef;
+}
+__device__ __forceinline__ double SutherlandK(const ViscParams &v, double t) {
+  const double temp = t * v.tRef;
+  const double k = (v.c2 * PowCR(temp, 1.5)) / (temp + v.s2);
+  return k / v.kNonDim;
+}
+// max(4/(3 rho), gamma/rho) and NondimScaling (mu / Pr + mut / Prt), mut = 0
+// (include/spectralRadius.hpp:93-151)
+template <typename T>
+__device__ __forceinline__ T ViscFaceSR(const GasT<T> &g, const T W[5], T mag, T dist, T mu,
+                                        const ViscParams &v) {
+  const T rho = T(0) + W[0];
+  const T a = T(4) / (T(3) * rho), bq = g.gamma / rho;
+  const T maxTerm = a < bq ? bq : a;
+  const T viscTerm = T(v.vscale) * (mu / T(v.prandtl) + T(0) / T(0.9));
+  return mag / dist * maxTerm * viscTerm;
+}
+__device__ __forceinline__ double ViscCellSR(const GasT<double> &g, const double W[5], double magL,
+                                             double magR, double vol, double mu, const ViscParams &v) {
+  const double fMag = 0.5 * (magL + magR);
+  const double rho = 0.0 + W[0];
+  const double a = 4.0 / (3.0 * rho), bq = g.gamma / rho;
+  const double maxTerm = a < bq ? bq : a;
+  const double viscTerm = v.vscale * (mu / v.prandtl + 0.0 / 0.9);
+  return maxTerm * viscTerm * fMag * fMag / vol;
+}
+
 struct BcStateDev {
   double p, rho, u, v, w, p0, t0, dx, dy, dz;
+  double tw;  // viscousWall: wall temperature when iso
+  int iso;
 };
 
 struct BcTask {
@@ -285,6 +344,26 @@ __device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &
   for (int e = 0; e < 5; ++e) F[e] = F[e] * a.w;
 }
 
+// CalcCellDt (src/procBlock.cpp:782-820) and AddDiagonalTerms + Invert
+// (src/linearSolver.cpp:146-188) for one cell; `sr` is the cell spectral
+// radius, `Ad` the accumulated main diagonal
+__device__ __forceinline__ void CellTail(const DevBlock &b, const Params &P, long p, double vol, double sr,
+                                         double Ad, const double *cflp) {
+  b.sr[p] = sr;
+  const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*cflp) * (vol / Dmax(sr, 0.0));
+  b.dt[p] = dt;
+  if (P.implicit) {
+    double dvt = (vol * (1.0 + P.zeta)) / (dt * P.theta);
+    if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
+    double A = Ad * P.relax;
+    A += dvt;
+    b.A[p] = A;
+    const double ainv = 1.0 / A;
+    if (b.Ainv) b.Ainv[p] = ainv;
+    if (b.Ainv32) b.Ainv32[p] = (float)ainv;
+  }
+}
+
 template <int RECON, int LIM, int FLUX>
 __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
                                                  const double *__restrict__ cflp) {
@@ -339,22 +418,297 @@ __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
   const double lk_ = InvCellSpecRad(g, rho, u, v, w, pr, b.fa[2][FaceIdx(b, 2, i, j, k)],
                                     b.fa[2][FaceIdx(b, 2, i, j, k + 1)]);
   const double sr = (li_ + lj_) + lk_;
-  b.sr[p] = sr;
-  const double vol = b.vol[p];
-  const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*cflp) * (vol / Dmax(sr, 0.0));
-  b.dt[p] = dt;
-  if (P.implicit) {
-    double dvt = (vol * (1.0 + P.zeta)) / (dt * P.theta);
-    if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
-    double A = sr * P.relax;
-    A += dvt;
-    b.A[p] = A;
-    const double ainv = 1.0 / A;
-    if (b.Ainv) b.Ainv[p] = ainv;
-    if (b.Ainv32) b.Ainv32[p] = (float)ainv;
+  if (P.visc.on) {
+    // the viscous pass adds its spectral radii and forms dt and the diagonal
+    b.sr[p] = sr;
+    return;
+  }
+  CellTail(b, P, p, b.vol[p], sr, sr, cflp);
+}
+
+
+// ---------------------------------------------------------------------------
+// laminar viscous residual: procBlock::CalcResidualNoSource's viscous branch
+// (src/procBlock.cpp:6126-6138)
+
+__device__ __forceinline__ long FacePadIdx(const DevBlock &b, int d, int i, int j, int k) {
+  const int FI = b.NI + (d == 0), FJ = b.NJ + (d == 1);
+  return (long)(i + b.g) + (long)(j + b.g) * FI + (long)(k + b.g) * FI * FJ;
+}
+
+// one ghost cell on a block edge (AssignInviscidGhostCellsEdge,
+// src/procBlock.cpp:2565-2714, and AssignViscousGhostCellsEdge, :2874-3030):
+// action 0 / 1 reflects q2 / q3 across the slip wall of unit normal n,
+// 2 averages q2 and q3, 3 copies q3, 4 copies q2, 5 leaves the cell
+struct EdgeTask {
+  long gq, q2, q3;
+  double nx, ny, nz;
+  int action, stype, layer, pad;
+};
+
+// one thread per edge line; its tasks run in the reference's layer order
+__global__ void __launch_bounds__(64) KEdgeGhosts(DevBlock b, const EdgeTask *__restrict__ tasks, int nlines,
+                                                  int perLine, Gas gas) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const GasT<double> g = Cast<double>(gas);
+  for (int q = 0; q < perLine; ++q) {
+    const EdgeTask t = tasks[(long)line * perLine + q];
+    if (t.action == 5) continue;
+    double a[5], c[5], gh[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      a[e] = b.W[e * b.npad + t.q2];
+      c[e] = b.W[e * b.npad + t.q3];
+    }
+    if (t.action == 0) GhostState(g, a, 0, t.nx, t.ny, t.nz, t.stype, t.layer, nullptr, gh);
+    else if (t.action == 1) GhostState(g, c, 0, t.nx, t.ny, t.nz, t.stype, t.layer, nullptr, gh);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double v = t.action <= 1 ? gh[e] : (t.action == 2 ? 0.5 * (a[e] + c[e]) : (t.action == 3 ? c[e] : a[e]));
+      b.W[e * b.npad + t.gq] = v;
+    }
   }
 }
 
+// GetGhostState, viscousWall, low-Re treatment (src/ghostStates.cpp:126-145,
+// 185-190): velocity mirrored about the wall velocity; an isothermal wall sets
+// the density from the mirrored temperature at the interior pressure
+struct WallTask {
+  long src, dst;
+  int state, pad;
+};
+
+__global__ void __launch_bounds__(256) KViscWall(DevBlock b, const WallTask *__restrict__ tasks, int n,
+                                                 const BcStateDev *__restrict__ states, Gas gas) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const WallTask w = tasks[t];
+  const BcStateDev *s = w.state >= 0 ? &states[w.state] : nullptr;
+  double in[5], gh[5];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) gh[e] = in[e] = b.W[e * b.npad + w.src];
+  const double vx = s ? s->u : 0.0, vy = s ? s->v : 0.0, vz = s ? s->w : 0.0;
+  gh[1] = 2.0 * vx - in[1];
+  gh[2] = 2.0 * vy - in[2];
+  gh[3] = 2.0 * vz - in[3];
+  if (s && s->iso) {
+    const GasT<double> g = Cast<double>(gas);
+    const double tGhost = 2.0 * s->tw - TempOf(g, in[0], in[4]);
+    const double R = 0.0 + 1.0 * g.R;
+    gh[0] = (gh[4] / (R * tGhost)) * (in[0] / (0.0 + in[0]));
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) b.W[e * b.npad + w.dst] = gh[e];
+}
+
+// UpdateAuxillaryVariables (src/procBlock.cpp:6171-6189): temperature and
+// viscosity of every cell but the block corners
+__global__ void __launch_bounds__(256) KAux(DevBlock b, Params P) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.npad) return;
+  const int i = (int)(q % b.NI) - b.g, j = (int)((q / b.NI) % b.NJ) - b.g, k = (int)(q / ((long)b.NI * b.NJ)) - b.g;
+  const bool corner = (i < 0 || i >= b.ni) && (j < 0 || j >= b.nj) && (k < 0 || k >= b.nk);
+  if (corner) return;
+  const double t = TempOf(Cast<double>(P.gas), b.W[q], b.W[4 * b.npad + q]);
+  b.T[q] = t;
+  b.mu[q] = SutherlandMu(P.visc, t);
+}
+
+// CalcGradsI/J/K at a face (src/procBlock.cpp:5173-5788) with VectorGradGG /
+// ScalarGradGG (src/utility.cpp:59-188): Green-Gauss over the control volume
+// spanning the face's lower cell L and upper cell U; velocity gradient
+// vg[r][c] = d v_c / d x_r, temperature gradient tg
+__device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j, int k, double vg[3][3],
+                                          double tg[3]) {
+  const int L[3] = {i - (d == 0), j - (d == 1), k - (d == 2)}, U[3] = {i, j, k};
+  double au[3][3], al[3][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int eq[3] = {q == 0, q == 1, q == 2};
+    const double4 *F = b.faP[q];
+    double4 f1, f2, f3, f4;
+    if (q == d) {
+      f1 = F[FacePadIdx(b, q, U[0], U[1], U[2])];
+      f2 = F[FacePadIdx(b, q, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2])];
+      f3 = f1;
+      f4 = F[FacePadIdx(b, q, L[0], L[1], L[2])];
+    } else {
+      f1 = F[FacePadIdx(b, q, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2])];
+      f2 = F[FacePadIdx(b, q, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2])];
+      f3 = F[FacePadIdx(b, q, U[0], U[1], U[2])];
+      f4 = F[FacePadIdx(b, q, L[0], L[1], L[2])];
+    }
+    au[q][0] = 0.5 * (f1.x * f1.w + f2.x * f2.w);
+    au[q][1] = 0.5 * (f1.y * f1.w + f2.y * f2.w);
+    au[q][2] = 0.5 * (f1.z * f1.w + f2.z * f2.w);
+    al[q][0] = 0.5 * (f3.x * f3.w + f4.x * f4.w);
+    al[q][1] = 0.5 * (f3.y * f3.w + f4.y * f4.w);
+    al[q][2] = 0.5 * (f3.z * f3.w + f4.z * f4.w);
+  }
+  const long cL = PadIdx(b, L[0], L[1], L[2]), cU = PadIdx(b, U[0], U[1], U[2]);
+  const double vol = 0.5 * (b.volP[cL] + b.volP[cU]);
+  // face values: three velocity components, then temperature
+  double vu[3][4], vl[3][4];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int eq[3] = {q == 0, q == 1, q == 2};
+    const long uP = PadIdx(b, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2]);
+    const long lP = PadIdx(b, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2]);
+    const long uM = PadIdx(b, U[0] - eq[0], U[1] - eq[1], U[2] - eq[2]);
+    const long lM = PadIdx(b, L[0] - eq[0], L[1] - eq[1], L[2] - eq[2]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double *A = v < 3 ? b.W + (v + 1) * b.npad : b.T;
+      const double xl = A[cL], xu = A[cU];
+      if (q == d) {
+        vu[q][v] = xu;
+        vl[q][v] = xl;
+      } else {
+        vu[q][v] = 0.25 * (xl + xu + A[uP] + A[lP]);
+        vl[q][v] = 0.25 * (xl + xu + A[uM] + A[lM]);
+      }
+    }
+  }
+  const double invVol = 1.0 / vol;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      vg[r][c] = (vu[0][c] * au[0][r] - vl[0][c] * al[0][r] + vu[1][c] * au[1][r] - vl[1][c] * al[1][r] +
+                  vu[2][c] * au[2][r] - vl[2][c] * al[2][r]) * invVol;
+    tg[r] = (vu[0][3] * au[0][r] - vl[0][3] * al[0][r] + vu[1][3] * au[1][r] - vl[1][3] * al[1][r] +
+             vu[2][3] * au[2][r] - vl[2][3] * al[2][r]) * invVol;
+  }
+}
+
+// viscous flux through face (d, i, j, k) times its area: central or
+// fourth-order face state and viscosity (include/reconstruction.hpp:314-349),
+// viscousFlux::CalcFlux / CalcWallFlux (src/viscousFlux.cpp:58-188),
+// TauNormal (src/utility.cpp:426-436); the wall form has no +0 species term
+template <bool R4>
+__device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P, int d, int i, int j, int k,
+                                             double F[5]) {
+#pragma clang fp contract(off)
+  const GasT<double> g = Cast<double>(P.gas);
+  double vg[3][3], tg[3];
+  FaceGrads(b, d, i, j, k, vg, tg);
+  const int n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
+  const int f3 = d == 0 ? i : (d == 1 ? j : k);
+  const bool wall = (f3 == 0 || f3 == n3) && b.wallf[BtSlot(b, d, f3 == 0 ? 0 : 1, i, j, k)];
+  const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
+  const long cU = PadIdx(b, i, j, k), cL = cU - s;
+  const double *cw = b.cw + d * b.npad;
+  double Wf[5], muF;
+  if (!R4) {
+    const double w[2] = {cw[cL], cw[cU]};
+    const double c0 = LagrangeCoeffN<2, 1, 0, 0>(w, 0), c1 = LagrangeCoeffN<2, 1, 0, 0>(w, 1);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) Wf[e] = c0 * b.W[e * b.npad + cU] + c1 * b.W[e * b.npad + cL];
+    muF = c0 * b.mu[cU] + c1 * b.mu[cL];
+  } else {
+    const long cLL = cL - s, cUU = cU + s;
+    const double w[4] = {cw[cLL], cw[cL], cw[cU], cw[cUU]};
+    double c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = LagrangeCoeffN<4, 3, 1, 1>(w, q);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double *We = b.W + e * b.npad;
+      Wf[e] = c[0] * We[cLL] + c[1] * We[cL] + c[2] * We[cU] + c[3] * We[cUU];
+    }
+    muF = c[0] * b.mu[cLL] + c[1] * b.mu[cL] + c[2] * b.mu[cU] + c[3] * b.mu[cUU];
+  }
+  const double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+  const double n[3] = {a.x, a.y, a.z};
+  const double mu = P.visc.vscale * muF, mut = P.visc.vscale * 0.0;
+  const double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
+  const double trace = vg[0][0] + vg[1][1] + vg[2][2];
+  double tau[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double s0 = vg[r][0] + vg[0][r], s1 = vg[r][1] + vg[1][r], s2 = vg[r][2] + vg[2][r];
+    const double sa = s0 * n[0] + s1 * n[1] + s2 * n[2];
+    tau[r] = lambda * trace * n[r] + (mu + mut) * sa;
+  }
+  const double t = TempOf(g, Wf[0], Wf[4]);
+  const double kk = SutherlandK(P.visc, t) * P.visc.vscale;
+  const double kt = mut * g.cps / 0.9;
+  const double tv = tau[0] * Wf[1] + tau[1] * Wf[2] + tau[2] * Wf[3];
+  const double q = (kk + kt) * (tg[0] * n[0] + tg[1] * n[1] + tg[2] * n[2]);
+  F[0] = 0.0 * a.w;
+  F[1] = tau[0] * a.w;
+  F[2] = tau[1] * a.w;
+  F[3] = tau[2] * a.w;
+  F[4] = (wall ? tv + q : tv + q + 0.0) * a.w;
+}
+
+// procBlock::CalcViscFluxI/J/K (src/procBlock.cpp:1233-2135), laminar.  The
+// same 8x8x4 tiling as KResidual: every viscous face flux of the tile once
+// into LDS, then each cell adds them to its residual in the reference's order
+// (+Fv_i,lo -Fv_i,hi +Fv_j,lo -Fv_j,hi +Fv_k,lo -Fv_k,hi), adds the viscous
+// spectral radii (i, j, k) to its spectral radius (times the viscous CFL
+// coefficient) and twice them to its diagonal, then forms dt and A.
+template <bool R4>
+__global__ void __launch_bounds__(256) KViscFlux(DevBlock b, Params P, const double *__restrict__ cflp) {
+  __shared__ double sF[5][kNF];
+  const int i0 = blockIdx.x * kTI, j0 = blockIdx.y * kTJ, k0 = blockIdx.z * kTK;
+  for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
+    int d, li, lj, lk;
+    if (f < kNFI) {
+      d = 0; li = f % (kTI + 1); lj = (f / (kTI + 1)) % kTJ; lk = f / ((kTI + 1) * kTJ);
+    } else if (f < kNFI + kNFJ) {
+      const int q = f - kNFI;
+      d = 1; li = q % kTI; lj = (q / kTI) % (kTJ + 1); lk = q / (kTI * (kTJ + 1));
+    } else {
+      const int q = f - kNFI - kNFJ;
+      d = 2; li = q % kTI; lj = (q / kTI) % kTJ; lk = q / (kTI * kTJ);
+    }
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    const bool ok = (i < b.ni + (d == 0)) && (j < b.nj + (d == 1)) && (k < b.nk + (d == 2));
+    if (ok) {
+      double F[5];
+      ViscFaceFlux<R4>(b, P, d, i, j, k, F);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sF[e][f] = F[e];
+    }
+  }
+  __syncthreads();
+  const int li = threadIdx.x % kTI, lj = (threadIdx.x / kTI) % kTJ, lk = threadIdx.x / (kTI * kTJ);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  if (i >= b.ni || j >= b.nj || k >= b.nk) return;
+  const int fi = lk * (kTI + 1) * kTJ + lj * (kTI + 1) + li;
+  const int fj = kNFI + lk * kTI * (kTJ + 1) + lj * kTI + li;
+  const int fk = kNFI + kNFJ + lk * kTI * kTJ + lj * kTI + li;
+  const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    double r = b.R[e * b.ncell + p];
+    r += sF[e][fi];
+    r -= sF[e][fi + 1];
+    r += sF[e][fj];
+    r -= sF[e][fj + kTI];
+    r += sF[e][fk];
+    r -= sF[e][fk + kTI * kTJ];
+    b.R[e * b.ncell + p] = r;
+  }
+  const GasT<double> g = Cast<double>(P.gas);
+  const long c = PadIdx(b, i, j, k);
+  double W[5];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) W[e] = b.W[e * b.npad + c];
+  const double vol = b.vol[p], mu = b.mu[c];
+  double sr = b.sr[p], Ad = sr;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int ei = d == 0, ej = d == 1, ek = d == 2;
+    const double v = ViscCellSR(g, W, b.fa[d][FaceIdx(b, d, i, j, k)].w,
+                                b.fa[d][FaceIdx(b, d, i + ei, j + ej, k + ek)].w, vol, mu, P.visc);
+    sr += v * P.visc.coeff;
+    if (P.implicit) Ad += 2.0 * v;
+  }
+  CellTail(b, P, p, vol, sr, Ad, cflp);
+}
 // ---------------------------------------------------------------------------
 // implicit operator
 
@@ -418,8 +772,8 @@ struct OpBuf<float> {
 // 1092); its product is the precomputed boundary term, zero elsewhere, and
 // adding +0 leaves the sum unchanged (it starts at +0 and never becomes -0).
 template <typename T>
-__device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const T *x,
-                                           int i, int j, int k, bool upper, T out[5]) {
+__device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const ViscParams &vp,
+                                           const T *x, int i, int j, int k, bool upper, T out[5]) {
 #pragma unroll
   for (int e = 0; e < 5; ++e) out[e] = T(0);
   const T *bt = OpBuf<T>::Bt(b);
@@ -435,7 +789,8 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       }
       continue;
     }
-    const double4 a = upper ? b.fa[d][FaceIdx(b, d, ni, nj, nk)] : b.fa[d][FaceIdx(b, d, i, j, k)];
+    const long fq = upper ? FaceIdx(b, d, ni, nj, nk) : FaceIdx(b, d, i, j, k);
+    const double4 a = b.fa[d][fq];
     const long c = PadIdx(b, ni, nj, nk);
     const long q = (long)ni + (long)nj * b.ni + (long)nk * b.ni * b.nj;
     T W[5], du[5], o[5];
@@ -444,7 +799,8 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       W[e] = T(b.W[e * b.npad + c]);
       du[e] = x[e * b.ncell + q];
     }
-    ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o);
+    const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
+    ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
 #pragma unroll
     for (int e = 0; e < 5; ++e) out[e] += o[e];
   }
@@ -481,10 +837,10 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T L[5];
-  OffDiagSum<T>(b, g, x, i, j, k, false, L);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
   if (useUpper) {
     T U[5];
-    OffDiagSum<T>(b, g, x, i, j, k, true, U);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
 #pragma unroll
     for (int e = 0; e < 5; ++e) L[e] -= U[e];
   }
@@ -507,11 +863,11 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T U[5];
-  OffDiagSum<T>(b, g, x, i, j, k, true, U);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
   const T ainv = OpBuf<T>::Ainv(b)[p];
   if (full) {
     T L[5];
-    OffDiagSum<T>(b, g, x, i, j, k, false, L);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
@@ -532,8 +888,8 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
   const GasT<T> g = Cast<T>(P.gas);
   const T *xo = OpBuf<T>::Xold(b);
   T L[5], U[5];
-  OffDiagSum<T>(b, g, xo, i, j, k, false, L);
-  OffDiagSum<T>(b, g, xo, i, j, k, true, U);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U);
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
   const T ainv = OpBuf<T>::Ainv(b)[p];
@@ -583,8 +939,8 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Param
     const GasT<T> g = Cast<T>(P.gas);
     const T *x = OpBuf<T>::X(b);
     T L[5], U[5];
-    OffDiagSum<T>(b, g, x, i, j, k, false, L);
-    OffDiagSum<T>(b, g, x, i, j, k, true, U);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
     const double A = b.A[p];
@@ -822,14 +1178,18 @@ __device__ __forceinline__ int TileId(const SweepCtl &s, int a, int b, int c) {
 // off-diagonal product of one neighbour: state/du from LDS or global
 template <typename T>
 __device__ __forceinline__ void NbOffDiag(const GasT<T> &g, const double *Wnb, const T *du,
-                                          double4 a, bool positive, T out[5]) {
+                                          double4 a, bool positive, T out[5],
+                                          const ViscParams *vp = nullptr, double dist = 1.0,
+                                          double mu = 0.0) {
   T W[5], d[5];
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     W[e] = T(Wnb[e]);
     d[e] = du[e];
   }
-  ScalarOffDiag<T>(g, W, d, T(a.x), T(a.y), T(a.z), T(a.w), positive, out);
+  const bool visc = vp && vp->on;
+  const T lv = visc ? ViscFaceSR<T>(g, W, T(a.w), T(dist), T(mu), *vp) : T(0);
+  ScalarOffDiag<T>(g, W, d, T(a.x), T(a.y), T(a.z), T(a.w), positive, out, visc, lv);
 }
 
 // ---------------------------------------------------------------------------
@@ -863,7 +1223,7 @@ struct ConnTask {
 
 template <typename T>
 __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ blocks,
-                                                  const ConnTask *__restrict__ tasks, int n, Gas gas) {
+                                                  const ConnTask *__restrict__ tasks, int n, Params P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const ConnTask c = tasks[t];
@@ -876,7 +1236,11 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
     Wn[e] = A.W[e * A.npad + c.gpad];
     dn[e] = xb[e * B.ncell + c.scell];
   }
-  NbOffDiag<T>(Cast<T>(gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o);
+  if (P.visc.on)
+    NbOffDiag<T>(Cast<T>(P.gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.c2c[c.d][c.face],
+                 A.mu[c.gpad]);
+  else
+    NbOffDiag<T>(Cast<T>(P.gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o);
   T *bt = OpBuf<T>::Bt(A);
 #pragma unroll
   for (int e = 0; e < 5; ++e) bt[e * A.nbf + c.slot] = o[e];
@@ -1050,7 +1414,12 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
           }
           const double4 a = make_double4(sFa[d][0][tid], sFa[d][1][tid], sFa[d][2][tid], sFa[d][3][tid]);
           T o[5];
-          NbOffDiag<T>(g, Wn, dn, a, FWD, o);
+          if (P.visc.on) {
+            const long fq = FWD ? FaceIdx(b, d, i, j, k) : FaceIdx(b, d, gi, gj, gk);
+            NbOffDiag<T>(g, Wn, dn, a, FWD, o, &P.visc, b.c2c[d][fq], b.mu[PadIdx(b, gi, gj, gk)]);
+          } else {
+            NbOffDiag<T>(g, Wn, dn, a, FWD, o);
+          }
 #pragma unroll
           for (int e = 0; e < 5; ++e) D[e] += o[e];
         }
@@ -1058,7 +1427,7 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
         if (full) {
           // the other side, from the previous sweep's values in global memory
           T O[5];
-          OffDiagSum<T>(b, g, x, i, j, k, FWD, O);
+          OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, O);
           if (FWD) {
 #pragma unroll
             for (int e = 0; e < 5; ++e) xn[e] = (T(rb[e]) + (D[e] - O[e])) * ainv;
@@ -1312,7 +1681,8 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
   for (int d = 0; d < 3; ++d) {
     const int ci = Clamp(i + (d == 0 ? sgn : 0), b.ni), cj = Clamp(j + (d == 1 ? sgn : 0), b.nj),
               ck = Clamp(kc + (d == 2 ? sgn : 0), b.nk);
-    const double4 a = FWD ? b.fa[d][FaceIdx(b, d, i, j, kc)] : b.fa[d][FaceIdx(b, d, ci, cj, ck)];
+    const long fq = FWD ? FaceIdx(b, d, i, j, kc) : FaceIdx(b, d, ci, cj, ck);
+    const double4 a = b.fa[d][fq];
     const long c = PadIdx(b, ci, cj, ck);
     T W[5], U[5], Fo[5];
 #pragma unroll
@@ -1321,7 +1691,8 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
     PrimToCons(g, W, U);
     PhysFlux(g, W, nx, ny, nz, Fo);
     const T half = T(0.5) * mag;
-    const T lam = T(0.5) * mag * (fabs((W[1] * nx + W[2] * ny) + W[3] * nz) + SoS(g, W[0], W[4]));
+    T lam = T(0.5) * mag * (fabs((W[1] * nx + W[2] * ny) + W[3] * nz) + SoS(g, W[0], W[4]));
+    if (P.visc.on) lam += ViscFaceSR<T>(g, W, mag, T(b.c2c[d][fq]), T(b.mu[c]), P.visc);
     // a neighbour outside the block: the slot of Fo carries the connection
     // term instead (zero off connections), selected by the neighbour wave
     const bool outside = !InBlock(b, i + (d == 0 ? sgn : 0), j + (d == 1 ? sgn : 0), kc + (d == 2 ? sgn : 0));
@@ -1362,7 +1733,8 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
       const int hi = i - (d == 0 ? sgn : 0), hj = j - (d == 1 ? sgn : 0), hk = kc - (d == 2 ? sgn : 0);
       const bool ok = k >= 0 && InBlock(b, hi, hj, hk);
       const int ui = Clamp(hi, b.ni), uj = Clamp(hj, b.nj), uk = Clamp(hk, b.nk);
-      const double4 a = FWD ? b.fa[d][FaceIdx(b, d, ui, uj, uk)] : b.fa[d][FaceIdx(b, d, i, j, kc)];
+      const long fq = FWD ? FaceIdx(b, d, ui, uj, uk) : FaceIdx(b, d, i, j, kc);
+      const double4 a = b.fa[d][fq];
       const long c = PadIdx(b, ui, uj, uk);
       const long q = (long)ui + (long)uj * b.ni + (long)uk * b.ni * b.nj;
       double Wn[5];
@@ -1372,7 +1744,8 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
         Wn[e] = b.W[e * b.npad + c];
         dn[e] = x[e * b.ncell + q];
       }
-      NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
+      if (P.visc.on) NbOffDiag<T>(g, Wn, dn, a, !FWD, u, &P.visc, b.c2c[d][fq], b.mu[c]);
+      else NbOffDiag<T>(g, Wn, dn, a, !FWD, u);
       T bv[5] = {T(0), T(0), T(0), T(0), T(0)};
       const T *bt = OpBuf<T>::Bt(b);
       if (bt) {

@@ -41,6 +41,11 @@ struct BlockRes {
   ath::SweepCtl sweep{};  // tiled LU-SGS dataflow state
   int sweepGrid = 0;
   ath::PencilCtl pencil{};  // pencil-wavefront LU-SGS state
+  // viscous cases: edge ghost tasks (inviscid and viscous pass) and wall tasks
+  ath::EdgeTask *edgeInv = nullptr, *edgeVisc = nullptr;
+  int edgeLines = 0, edgePerLine = 0;
+  ath::WallTask *wall = nullptr;
+  int nwall = 0;
 };
 
 }  // namespace
@@ -125,6 +130,191 @@ int SyncBlocks(ath_ctx *c) {
   return ATH_OK;
 }
 
+// boundaryConditions::GetBCSurface (src/boundaryConditions.cpp:109-190): the
+// surface of the same normal direction whose patch holds face (i, j, k)
+const ath::Surface *SurfaceAt(const ath::HostBlock &h, int i, int j, int k, int st) {
+  const int dir = (st - 1) / 2, x[3] = {i, j, k};
+  for (const auto &sf : h.surf) {
+    if ((sf.SurfType() - 1) / 2 != dir) continue;
+    bool ok = true;
+    for (int q = 0; q < 3 && ok; ++q)
+      ok = q == dir ? (x[q] >= sf.d[2 * q] && x[q] <= sf.d[2 * q + 1]) : (x[q] >= sf.d[2 * q] && x[q] < sf.d[2 * q + 1]);
+    if (ok) return &sf;
+  }
+  return nullptr;
+}
+
+void HostMap3(int dir, int d3, int d1, int d2, int *i, int *j, int *k) {
+  if (dir == 0) { *i = d3; *j = d1; *k = d2; }
+  else if (dir == 1) { *i = d2; *j = d3; *k = d1; }
+  else { *i = d1; *j = d2; *k = d3; }
+}
+
+// padded geometry, projected centre distances, wall flags and the ghost-cell
+// task lists of a viscous case
+int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
+  using namespace ath;
+  DevBlock &d = br.d;
+  const long np = d.npad;
+  if (int r = c->Alloc(&d.T, np)) return r;
+  if (int r = c->Alloc(&d.mu, np)) return r;
+  HIP_OK(hipMemset(d.T, 0, np * sizeof(double)));
+  HIP_OK(hipMemset(d.mu, 0, np * sizeof(double)));
+  {
+    double *v = nullptr;
+    if (int r = c->Alloc(&v, np)) return r;
+    HIP_OK(hipMemcpy(v, h.vol.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    d.volP = v;
+  }
+  for (int dd = 0; dd < 3; ++dd) {
+    std::vector<double4> fa(h.NumFaces(dd));
+    for (size_t q = 0; q < fa.size(); ++q) {
+      const auto &a = h.fa[dd][q];
+      fa[q] = make_double4(a[0], a[1], a[2], a[3]);
+    }
+    double4 *dfa = nullptr;
+    if (int r = c->Alloc(&dfa, fa.size())) return r;
+    HIP_OK(hipMemcpy(dfa, fa.data(), fa.size() * sizeof(double4), hipMemcpyHostToDevice));
+    d.faP[dd] = dfa;
+    // ProjC2CDist (src/procBlock.cpp:6316-6341) at every physical face
+    const int FI = h.ni + (dd == 0), FJ = h.nj + (dd == 1), FK = h.nk + (dd == 2);
+    std::vector<double> cc((size_t)FI * FJ * FK);
+    for (int k = 0; k < FK; ++k)
+      for (int j = 0; j < FJ; ++j)
+        for (int i = 0; i < FI; ++i) {
+          const Vec3 &u = h.ctr[h.Cell(i, j, k)];
+          const Vec3 &l = h.ctr[h.Cell(i - (dd == 0), j - (dd == 1), k - (dd == 2))];
+          const auto &a = h.fa[dd][h.Face(dd, i, j, k)];
+          const double dx = u.x - l.x, dy = u.y - l.y, dz = u.z - l.z;
+          cc[(size_t)i + (size_t)j * FI + (size_t)k * FI * FJ] = (dx * a[0] + dy * a[1]) + dz * a[2];
+        }
+    double *dc = nullptr;
+    if (int r = c->Alloc(&dc, cc.size())) return r;
+    HIP_OK(hipMemcpy(dc, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
+    d.c2c[dd] = dc;
+  }
+  // viscousWall flag of every boundary face (BtSlot order)
+  {
+    std::vector<unsigned char> wf((size_t)d.nbf, 0);
+    for (int dd = 0; dd < 3; ++dd)
+      for (int up = 0; up < 2; ++up) {
+        const int d1 = (dd + 1) % 3, d2 = (dd + 2) % 3;
+        for (int c2 = 0; c2 < h.N(d2); ++c2)
+          for (int c1 = 0; c1 < h.N(d1); ++c1) {
+            int i, j, k;
+            HostMap3(dd, up ? h.N(dd) : 0, c1, c2, &i, &j, &k);
+            const Surface *sf = SurfaceAt(h, i, j, k, 2 * dd + 1 + up);
+            int ci = i, cj = j, ck = k;
+            if (up) { ci -= dd == 0; cj -= dd == 1; ck -= dd == 2; }
+            wf[HostBtSlot(h, dd, up, ci, cj, ck)] = sf && sf->type == "viscousWall";
+          }
+      }
+    unsigned char *dw = nullptr;
+    if (int r = c->Alloc(&dw, wf.size())) return r;
+    HIP_OK(hipMemcpy(dw, wf.data(), wf.size(), hipMemcpyHostToDevice));
+    d.wallf = dw;
+  }
+  // edge ghost cells, per line in the reference's layer order
+  // (src/procBlock.cpp:2565-2714, 2874-3030)
+  {
+    const int g = h.g;
+    int lines = 0;
+    for (int dd = 0; dd < 3; ++dd) lines += 4 * h.N(dd);
+    br.edgeLines = lines;
+    br.edgePerLine = g * g;
+    std::vector<EdgeTask> inv((size_t)lines * g * g), vis((size_t)lines * g * g);
+    int line = 0;
+    for (int dd = 0; dd < 3; ++dd) {
+      const int d2 = (dd + 1) % 3, d3 = (dd + 2) % 3;
+      const int max1 = h.N(dd), max2 = h.N(d2), max3 = h.N(d3);
+      for (int cc = 0; cc < 4; ++cc)
+        for (int d1 = 0; d1 < max1; ++d1, ++line) {
+          int q = 0;
+          for (int layer3 = 1; layer3 <= g; ++layer3)
+            for (int layer2 = 1; layer2 <= g; ++layer2, ++q) {
+              const bool upper2 = cc > 1, upper3 = cc % 2 == 1;
+              const int pC2 = upper2 ? max2 + layer2 - 2 : 1 - layer2, gC2 = upper2 ? pC2 + 1 : pC2 - 1;
+              const int pC3 = upper3 ? max3 + layer3 - 2 : 1 - layer3, gC3 = upper3 ? pC3 + 1 : pC3 - 1;
+              const int surf2 = 2 * d2 + 1 + (upper2 ? 1 : 0), surf3 = 2 * d3 + 1 + (upper3 ? 1 : 0);
+              const int cF2_2 = upper2 ? max2 : 0, cF2_3 = upper3 ? max3 - 1 : 0;
+              const int cF3_2 = upper2 ? max2 - 1 : 0, cF3_3 = upper3 ? max3 : 0;
+              int i, j, k;
+              HostMap3(dd, d1, cF2_2, cF2_3, &i, &j, &k);
+              const Surface *s2 = SurfaceAt(h, i, j, k, surf2);
+              HostMap3(dd, d1, cF3_2, cF3_3, &i, &j, &k);
+              const Surface *s3 = SurfaceAt(h, i, j, k, surf3);
+              HostMap3(dd, d1, cF2_2, gC3, &i, &j, &k);
+              const auto fa2 = h.fa[d2][h.Face(d2, i, j, k)];
+              HostMap3(dd, d1, gC2, cF3_3, &i, &j, &k);
+              const auto fa3 = h.fa[d3][h.Face(d3, i, j, k)];
+              EdgeTask t{};
+              HostMap3(dd, d1, gC2, gC3, &i, &j, &k);
+              t.gq = h.Cell(i, j, k);
+              HostMap3(dd, d1, pC2, gC3, &i, &j, &k);
+              t.q2 = h.Cell(i, j, k);
+              HostMap3(dd, d1, gC2, pC3, &i, &j, &k);
+              t.q3 = h.Cell(i, j, k);
+              const std::string b2 = s2 ? s2->type : "", b3 = s3 ? s3->type : "";
+              const int avg = layer2 == layer3 ? 2 : (layer2 > layer3 ? 3 : 4);
+              auto fill = [&](EdgeTask &e, bool slip2, bool slip3, bool keep) {
+                e = t;
+                if (slip2 && !slip3) {
+                  e.action = 0;
+                  e.nx = fa2[0]; e.ny = fa2[1]; e.nz = fa2[2];
+                  e.stype = surf2;
+                  e.layer = layer2;
+                } else if (!slip2 && slip3) {
+                  e.action = 1;
+                  e.nx = fa3[0]; e.ny = fa3[1]; e.nz = fa3[2];
+                  e.stype = surf3;
+                  e.layer = layer3;
+                } else {
+                  e.action = keep ? avg : 5;
+                }
+              };
+              const bool w2 = b2 == "slipWall" || b2 == "viscousWall", w3 = b3 == "slipWall" || b3 == "viscousWall";
+              fill(inv[(size_t)line * g * g + q], w2, w3, true);
+              fill(vis[(size_t)line * g * g + q], b2 == "slipWall", b3 == "slipWall",
+                   b2 == "viscousWall" && b3 == "viscousWall");
+            }
+        }
+    }
+    if (int r = c->Alloc(&br.edgeInv, inv.size())) return r;
+    if (int r = c->Alloc(&br.edgeVisc, vis.size())) return r;
+    HIP_OK(hipMemcpy(br.edgeInv, inv.data(), inv.size() * sizeof(EdgeTask), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(br.edgeVisc, vis.data(), vis.size() * sizeof(EdgeTask), hipMemcpyHostToDevice));
+  }
+  // viscousWall ghost cells, all layers (src/procBlock.cpp:2760-2836)
+  {
+    std::vector<WallTask> wt;
+    for (int layer = 1; layer <= h.g; ++layer)
+      for (const auto &sf : h.surf) {
+        if (sf.type != "viscousWall") continue;
+        const int st = sf.SurfType(), dir = (st - 1) / 2;
+        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+        const int r3 = sf.d[2 * dir], n3 = h.N(dir);
+        const int gC = st % 2 == 0 ? r3 + layer - 1 : r3 - layer;
+        const int iC = st % 2 == 0 ? std::max(r3 - layer, 0) : std::min(r3 + layer - 1, n3 - 1);
+        int state = -1;
+        for (size_t q = 0; q < c->hc.bcs.size(); ++q)
+          if (c->hc.bcs[q].tag == sf.d[6]) { state = (int)q; break; }
+        for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
+          for (int q1 = sf.d[2 * d1]; q1 < sf.d[2 * d1 + 1]; ++q1) {
+            int ii, ij, ik, gi, gj, gk;
+            HostMap3(dir, iC, q1, q2, &ii, &ij, &ik);
+            HostMap3(dir, gC, q1, q2, &gi, &gj, &gk);
+            wt.push_back({h.Cell(ii, ij, ik), h.Cell(gi, gj, gk), state, 0});
+          }
+      }
+    br.nwall = (int)wt.size();
+    if (!wt.empty()) {
+      if (int r = c->Alloc(&br.wall, wt.size())) return r;
+      HIP_OK(hipMemcpy(br.wall, wt.data(), wt.size() * sizeof(WallTask), hipMemcpyHostToDevice));
+    }
+  }
+  return ATH_OK;
+}
+
 // gather lists of the interblock connections
 int UploadConnections(ath_ctx *c) {
   using namespace ath;
@@ -195,11 +385,24 @@ int Upload(ath_ctx *c) {
   c->P.multilevel = hc.MultiLevel() ? 1 : 0;
   c->P.initMatrix = hc.InitMatrix() ? 1 : 0;
   c->P.explicitEuler = hc.timeint == TimeInt::ExplicitEuler ? 1 : 0;
+  c->P.visc.on = hc.viscous ? 1 : 0;
+  c->P.visc.recon4 = hc.viscRecon4 ? 1 : 0;
+  c->P.visc.vscale = hc.vscale;
+  c->P.visc.prandtl = hc.prandtl;
+  c->P.visc.coeff = hc.viscCoeff;
+  c->P.visc.tRef = hc.tRef;
+  c->P.visc.muRef = hc.muRef;
+  c->P.visc.kNonDim = hc.kNonDim;
+  c->P.visc.c1 = hc.suth[0];
+  c->P.visc.s1 = hc.suth[1];
+  c->P.visc.c2 = hc.suth[2];
+  c->P.visc.s2 = hc.suth[3];
 
   // boundary states by tag
   std::vector<BcStateDev> st;
   for (const auto &s : hc.bcs)
-    st.push_back({s.p, s.rho, s.vel[0], s.vel[1], s.vel[2], s.p0, s.t0, s.dir[0], s.dir[1], s.dir[2]});
+    st.push_back({s.p, s.rho, s.vel[0], s.vel[1], s.vel[2], s.p0, s.t0, s.dir[0], s.dir[1], s.dir[2], s.twall,
+                  s.isothermal ? 1 : 0});
   if (int r = c->Alloc(&c->dstates, st.size())) return r;
   if (!st.empty())
     HIP_OK(hipMemcpy(c->dstates, st.data(), st.size() * sizeof(BcStateDev), hipMemcpyHostToDevice));
@@ -292,6 +495,15 @@ int Upload(ath_ctx *c) {
       if (int r = c->Alloc(&d.bt, 5 * d.nbf)) return r;
       HIP_OK(hipMemset(d.bt, 0, 5 * d.nbf * sizeof(double)));
     }
+    d.T = d.mu = nullptr;
+    d.volP = nullptr;
+    d.wallf = nullptr;
+    for (int dd = 0; dd < 3; ++dd) {
+      d.faP[dd] = nullptr;
+      d.c2c[dd] = nullptr;
+    }
+    if (hc.viscous)
+      if (int r = UploadViscous(c, h, br)) return r;
     // hyperplane order
     std::vector<int> cells;
     HyperplaneOrder(h.ni, h.nj, h.nk, &cells, &br.planeOff);
@@ -443,6 +655,11 @@ void LaunchBoundary(ath_ctx *c) {
   if (c->nhalo > 0)
     hipLaunchKernelGGL(ath::KHaloState, dim3(Grid(c->nhalo)), dim3(kBlk), 0, c->stream, c->dblocks, c->dhalo,
                        c->nhalo);
+  // edge ghost cells: read only by the viscous gradients
+  if (c->P.visc.on)
+    for (auto &br : c->blk)
+      hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, br.d, br.edgeInv,
+                         br.edgeLines, br.edgePerLine, c->P.gas);
 }
 
 // connection terms from the partners' current du, before each pass
@@ -450,7 +667,7 @@ template <typename T>
 void LaunchConnTerms(ath_ctx *c) {
   if (c->nconn > 0)
     hipLaunchKernelGGL(ath::KConnTerms<T>, dim3(Grid(c->nconn)), dim3(kBlk), 0, c->stream, c->dblocks, c->dconn,
-                       c->nconn, c->P.gas);
+                       c->nconn, c->P);
 }
 
 template <int RECON, int LIM, int FLUX>
@@ -459,6 +676,19 @@ void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
   dim3 grid((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
             (d.nk + ath::kTK - 1) / ath::kTK);
   hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+  if (!c->P.visc.on) return;
+  // viscous ghost cells, temperature and viscosity, viscous fluxes
+  // (src/procBlock.cpp:6126-6138)
+  if (br.nwall > 0)
+    hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(br.nwall)), dim3(kBlk), 0, c->stream, d, br.wall, br.nwall,
+                       c->dstates, c->P.gas);
+  hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, d, br.edgeVisc,
+                     br.edgeLines, br.edgePerLine, c->P.gas);
+  hipLaunchKernelGGL(ath::KAux, dim3(Grid(d.npad)), dim3(kBlk), 0, c->stream, d, c->P);
+  if (c->P.visc.recon4)
+    hipLaunchKernelGGL((ath::KViscFlux<true>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+  else
+    hipLaunchKernelGGL((ath::KViscFlux<false>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
 }
 
 void LaunchResidual(ath_ctx *c, const BlockRes &br) {

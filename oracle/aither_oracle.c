@@ -1700,6 +1700,7 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
     c->nb = nbg;
     c->blk = (blk_t *)calloc((size_t)nbg, sizeof(blk_t));
     int g = (c->recon == 0) ? 1 : (c->recon == 1 ? 2 : 3); /* src/input.cpp:1127-1143 */
+    if (c->viscRecon4 && g < 2) g = 2;
     c->totalCells = 0;
     for (int bq = 0; bq < nbg; ++bq) {
       blk_t *b = &c->blk[bq];

@@ -84,10 +84,10 @@ def state(xc, yc, zc):
     return rho, u, v, w, p
 
 
-def _inp(name, nblocks_bcs, recon, timeint, solver="lusgs", sweeps=1, extra=""):
+def _inp(name, nblocks_bcs, recon, timeint, solver="lusgs", sweeps=1, extra="", eq="euler"):
     fs = FREESTREAM
     return f"""gridName: {name}
-equationSet: euler
+equationSet: {eq}
 timeIntegration: {timeint}
 cflStart: 2.0
 cflMax: 2.0
@@ -106,12 +106,12 @@ boundaryConditions: {len(nblocks_bcs)}
 
 
 def write_pair(outdir, name, rot0=None, rot1=None, cut=4, recon="thirdOrder", timeint="explicitEuler",
-               solver="lusgs", sweeps=1, n=(6, 5, 8)):
+               solver="lusgs", sweeps=1, n=(6, 5, 8), eq="euler", scale=1.0):
     """Write the uncut case (rot0 is None) or the two-block case; returns
     (inp path, cell map).  The cell map takes an uncut cell (i, j, k) to
     (block, (i', j', k')) of the written case."""
     os.makedirs(outdir, exist_ok=True)
-    x, y, z = base_grid(n)
+    x, y, z = (scale * a for a in base_grid(n))
     xc, yc, zc = cases.cell_centres(x, y, z)
     rho, u, v, w, p = state(xc, yc, zc)
     cases.write_cloud(os.path.join(outdir, name + ".cloud"), xc, yc, zc, rho, u, v, w, p)
@@ -146,7 +146,7 @@ def write_pair(outdir, name, rot0=None, rot1=None, cut=4, recon="thirdOrder", ti
             return 1, rotate_cell((i, j, k - cut), dims[1], rot1)
     path = os.path.join(outdir, name + ".inp")
     with open(path, "w") as f:
-        f.write(_inp(name, bcs, recon, timeint, solver, sweeps))
+        f.write(_inp(name, bcs, recon, timeint, solver, sweeps, eq=eq))
     return path, cmap
 
 

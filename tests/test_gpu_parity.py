@@ -10,7 +10,8 @@ Both sides evaluate the reference's operation sequence without fused
 multiply-add, and the GPU state is bit-identical to the oracle's on every
 case that does not call std::pow.  std::pow appears in the stagnation-inlet
 state (non-integer exponent), the AUSMPW+ pressure weight (cube) and the WENO
-smoothness integrals (cubes): the device evaluates it correctly rounded,
+smoothness integrals (cubes) and Sutherland's law (T^1.5): the device
+evaluates it correctly rounded,
 while glibc misrounds about 0.09 % of near-tie inputs, so those cases agree
 to the last bit only until the first such input.  In subsonicCylinder
 (stagnation inlet, CFL 1e5, residual not decaying) a one-ulp difference in a
@@ -119,7 +120,7 @@ def test_residual_dt_diagonal_parity(case_paths, name):
 
 # cases whose free trajectory is not bit-identical (std::pow); their per-step
 # residual is compared from a common state
-RESYNC = {"subsonicCylinder", "multiblockCylinder", "shockTube"}
+RESYNC = {"subsonicCylinder", "multiblockCylinder", "shockTube", "viscousFlatPlate"}
 
 
 def run_both(inp, iterations, precision=0, check_fields=True, step_tol=1e-12, resync=False):
@@ -388,3 +389,81 @@ def test_connections_fp32_operator(tmp_path):
     Wo = np.concatenate([o.state(b) for b in range(o.nb)])
     assert rel(Ws, Wo) <= 1e-6
     s.close()
+
+
+def _parity_from_common_state(s, o, iterations, tol=1e-12):
+    """per-step residual and matrix residual from a common state (the oracle
+    restarts each step from the GPU state)"""
+    for nn in range(iterations):
+        for b in range(o.nb):
+            o.set_state(s.get_state(b), b)
+        s.begin_step(nn)
+        o.begin_step(nn)
+        for mm in range(o.nonlinear):
+            ns = s.iterate(nn, mm)
+            no = o.iterate(nn, mm)
+            Rs = np.concatenate([s.residual(b) for b in range(o.nb)])
+            Ro = np.concatenate([o.residual(b) for b in range(o.nb)])
+            assert rel(Rs, Ro) <= tol, (nn, mm, rel(Rs, Ro))
+            if no["matrix_resid"] > 0:
+                assert abs(ns.matrix_resid - no["matrix_resid"]) <= 1e-10 * no["matrix_resid"]
+
+
+@pytest.mark.parametrize("visc_recon", ["central", "centralFourth"])
+def test_viscous_channel_every_sweep_ordering(tmp_path, visc_recon):
+    """Laminar channel: adiabatic wall on the bump, isothermal wall moving
+    along x on top, characteristic inflow/outflow, slip side walls; the
+    viscous residual (edge and wall ghost cells, face gradients, Sutherland
+    transport, viscous spectral radii) and the viscous implicit terms in the
+    pencil, tiled and hyperplane LU-SGS orders."""
+    path = cases.bump_case(str(tmp_path), seed=4, n=10, iterations=6, viscous=True, visc_recon=visc_recon)[0]
+    for order in ("PENCIL", "TILED", "HYPERPLANE"):
+        say("viscous channel", visc_recon, order)
+        s = Solver(path)
+        s.set_sweep_ordering(getattr(Solver, order))
+        o = pyoracle.OracleCase(path)
+        _parity_from_common_state(s, o, 6)
+        s.close()
+    # the free trajectory
+    s, o, hs, ho = run_both(path, 6, check_fields=False)
+    assert rel(s.get_state(), o.state()) <= 1e-10
+    s.close()
+
+
+def test_viscous_explicit_dplur_fp32(tmp_path):
+    base = cases.bump_case(str(tmp_path), seed=5, n=8, iterations=4, viscous=True)[0]
+    text = open(base).read()
+    variants = {
+        "explicit": text.replace("timeIntegration: implicitEuler", "timeIntegration: explicitEuler")
+                        .replace("cflStart: 5.0", "cflStart: 0.5").replace("cflMax: 5.0", "cflMax: 0.5"),
+        "dplur": text.replace("matrixSolver: lusgs", "matrixSolver: dplur").replace("matrixSweeps: 1", "matrixSweeps: 3"),
+        "lusgs3": text.replace("matrixSweeps: 1", "matrixSweeps: 3"),
+    }
+    for name, t in variants.items():
+        p = os.path.join(str(tmp_path), name + ".inp")
+        open(p, "w").write(t)
+        say("viscous", name)
+        s = Solver(p)
+        o = pyoracle.OracleCase(p)
+        _parity_from_common_state(s, o, 4)
+        s.close()
+    say("viscous fp32 operator")
+    s, o, hs, ho = run_both(base, 4, precision=1, check_fields=False)
+    assert rel(s.get_state(), o.state()) <= 1e-6
+    s.close()
+
+
+def test_viscous_connections(tmp_path):
+    """Two blocks joined in a rotated orientation with the viscous terms on:
+    the connection's ghost viscosity and centre distance enter the implicit
+    terms across the cut."""
+    import conn_cases as cc
+    r0, r1 = cc.orientation_set()[350]
+    path, _ = cc.write_pair(str(tmp_path), "cut", r0, r1, timeint="implicitEuler", eq="navierStokes", scale=1e-3)
+    for order in ("PENCIL", "HYPERPLANE"):
+        say("viscous connections", order)
+        s = Solver(path)
+        s.set_sweep_ordering(getattr(Solver, order))
+        o = pyoracle.OracleCase(path)
+        _parity_from_common_state(s, o, 4)
+        s.close()
