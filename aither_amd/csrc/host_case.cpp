@@ -409,7 +409,7 @@ Vec3 NodeAt(const std::vector<Vec3> &X, const HostBlock &b, int dir, int r3, int
 // the patch axes' correspondence, from three corners (TestPatchMatch,
 // src/boundaryConditions.cpp:729-831)
 bool OrientPatch(Connection &cn, const HostBlock &A, const std::vector<Vec3> &XA, const HostBlock &B,
-                 const std::vector<Vec3> &XB) {
+                 const std::vector<Vec3> &XB, const std::array<double, 3> &shift = {0, 0, 0}) {
   const int corners[3][2] = {{0, 0}, {cn.n1, 0}, {0, cn.n2}};
   for (int sw = 0; sw < 2; ++sw)
     for (int r1 = 0; r1 < 2; ++r1)
@@ -422,8 +422,11 @@ bool OrientPatch(Connection &cn, const HostBlock &A, const std::vector<Vec3> &XA
         for (int q = 0; q < 3 && ok; ++q) {
           int b1, b2;
           cn.Map(corners[q][0], corners[q][1], false, &b1, &b2);
-          ok = SamePoint(NodeAt(XA, A, cn.dir, cn.r3, cn.r1s + corners[q][0], cn.r2s + corners[q][1]),
-                         NodeAt(XB, B, cn.pdir, cn.pr3, cn.pr1s + b1, cn.pr2s + b2));
+          Vec3 pa = NodeAt(XA, A, cn.dir, cn.r3, cn.r1s + corners[q][0], cn.r2s + corners[q][1]);
+          pa.x += shift[0];
+          pa.y += shift[1];
+          pa.z += shift[2];
+          ok = SamePoint(pa, NodeAt(XB, B, cn.pdir, cn.pr3, cn.pr1s + b1, cn.pr2s + b2));
         }
         if (ok) return true;
       }
@@ -441,6 +444,7 @@ bool OrientPatch(Connection &cn, const HostBlock &A, const std::vector<Vec3> &XA
 void SwapGeometry(HostCase &c) {
   const std::vector<HostBlock> before = c.blocks;
   for (const auto &cn : c.conns) {
+    if (cn.periodic) continue;  // periodic ghost cells keep the mirrored geometry
     HostBlock &A = c.blocks[cn.block];
     const HostBlock &B = before[cn.pb];
     const bool flip3 = (cn.s + cn.ps) % 2 == 0;
@@ -497,7 +501,7 @@ std::string ConnectBlocks(HostCase &c, const std::vector<std::vector<Vec3>> &nod
   for (int a = 0; a < (int)c.blocks.size(); ++a) {
     const HostBlock &A = c.blocks[a];
     for (const auto &sf : A.surf) {
-      if (sf.type != "interblock") continue;
+      if (sf.type != "interblock" && sf.type != "periodic") continue;
       Connection cn;
       cn.block = a;
       cn.s = sf.SurfType();
@@ -508,6 +512,48 @@ std::string ConnectBlocks(HostCase &c, const std::vector<std::vector<Vec3>> &nod
       cn.n1 = sf.d[2 * d1 + 1] - cn.r1s;
       cn.r2s = sf.d[2 * d2];
       cn.n2 = sf.d[2 * d2 + 1] - cn.r2s;
+      if (sf.type == "periodic") {
+        // periodic pairs match by geometry once the start surface is moved by
+        // the translation (GetConnectionBCs, src/boundaryConditions.cpp:543-593;
+        // patch::Transform, :2223-2248)
+        const BcState *ps = nullptr;
+        for (const auto &st : c.bcs)
+          if (st.kind == "periodic" && (st.startTag == sf.d[6] || st.endTag == sf.d[6])) {
+            ps = &st;
+            break;
+          }
+        const std::string where = "block " + std::to_string(a) + ": periodic tag " + std::to_string(sf.d[6]);
+        if (!ps) return where + " has no periodic boundary state";
+        const int other = ps->startTag == sf.d[6] ? ps->endTag : ps->startTag;
+        const double sg = ps->startTag == sf.d[6] ? 1.0 : -1.0;
+        const std::array<double, 3> shift = {sg * ps->translation[0], sg * ps->translation[1],
+                                             sg * ps->translation[2]};
+        cn.periodic = true;
+        bool found = false;
+        for (int pbq = 0; pbq < (int)c.blocks.size() && !found; ++pbq) {
+          const HostBlock &B = c.blocks[pbq];
+          for (const auto &pf : B.surf) {
+            if (pf.type != "periodic" || pf.d[6] != other || &pf == &sf) continue;
+            cn.pb = pbq;
+            cn.ps = pf.SurfType();
+            cn.pdir = (cn.ps - 1) / 2;
+            const int e1 = (cn.pdir + 1) % 3, e2 = (cn.pdir + 2) % 3;
+            cn.pr3 = pf.d[2 * cn.pdir];
+            cn.pr1s = pf.d[2 * e1];
+            cn.pn1 = pf.d[2 * e1 + 1] - cn.pr1s;
+            cn.pr2s = pf.d[2 * e2];
+            cn.pn2 = pf.d[2 * e2 + 1] - cn.pr2s;
+            if (OrientPatch(cn, A, nodes[a], B, nodes[pbq], shift)) {
+              found = true;
+              break;
+            }
+          }
+        }
+        if (!found) return where + " has no matching periodic patch";
+        if (c.blocks[cn.pb].N(cn.pdir) < A.g) return where + ": partner block is thinner than the ghost layers";
+        c.conns.push_back(cn);
+        continue;
+      }
       cn.ps = TagPartnerSurface(sf.d[6]);
       cn.pb = sf.d[6] - 1000 * cn.ps;
       const std::string where = "block " + std::to_string(a) + ": interblock tag " + std::to_string(sf.d[6]);
@@ -736,6 +782,11 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
             return "viscousWall heat flux is not supported";
           if (p.count("wallTreatment") && p.at("wallTreatment") != "lowRe")
             return "wall laws are not supported";
+          if (p.count("startTag")) s.startTag = std::stoi(p.at("startTag"));
+          if (p.count("endTag")) s.endTag = std::stoi(p.at("endTag"));
+          if (p.count("translation")) s.translation = ParseVec(p.at("translation"));
+          if (s.kind == "periodic" && (p.count("axis") || p.count("rotation")))
+            return "rotational periodicity is not supported";
           c.bcs.push_back(s);
         }
       } else if (key == "boundaryConditions") {
@@ -790,6 +841,10 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     ic.p /= pScale;
   }
   for (auto &s : c.bcs) {
+    if (s.kind == "periodic") {  // src/inputStates.cpp:883-894
+      for (auto &v : s.translation) v /= c.lRef;
+      continue;
+    }
     if (s.kind == "viscousWall") {  // src/inputStates.cpp:775-783
       for (auto &v : s.vel) v /= c.aRef;
       s.twall /= c.tRef;
@@ -853,7 +908,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     }
     b.surf = surfs[bq];
     for (const auto &s : b.surf) {
-      if (s.type == "periodic") return "periodic connections are not supported";
+
       const int st = s.SurfType();
       if (st == 0) return "boundary surface is neither an i, j nor k surface";
       // the surface must lie on the block boundary and its patch inside it
@@ -940,7 +995,8 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     for (const auto &s : b.surf) {
       const auto &t = s.type;
       if (t != "slipWall" && t != "viscousWall" && t != "characteristic" && t != "supersonicInflow" &&
-          t != "supersonicOutflow" && t != "stagnationInlet" && t != "pressureOutlet" && t != "interblock")
+          t != "supersonicOutflow" && t != "stagnationInlet" && t != "pressureOutlet" && t != "interblock" &&
+          t != "periodic")
         return "unsupported boundary condition " + t;
       if (t == "characteristic" || t == "supersonicInflow" || t == "stagnationInlet" || t == "pressureOutlet") {
         bool found = false;

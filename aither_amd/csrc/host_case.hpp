@@ -40,6 +40,8 @@ struct BcState {  // boundaryStates entry, nondimensional
   std::array<double, 3> dir{0, 0, 0};
   bool isothermal = false;  // viscousWall with a given wall temperature
   double twall = 0;
+  int startTag = -1, endTag = -1;             // periodic: the paired surfaces' tags
+  std::array<double, 3> translation{0, 0, 0};  // periodic: start surface onto end surface
 };
 
 struct Surface {  // boundarySurface: bc type, i/j/k ranges, tag
@@ -92,6 +94,7 @@ struct Connection {
   int pb = 0, ps = 0, pdir = 0, pr3 = 0;  // partner block, surface, axis, face index
   int pr1s = 0, pn1 = 0, pr2s = 0, pn2 = 0;
   bool swap = false, rev1 = false, rev2 = false;
+  bool periodic = false;  // periodic: ghost geometry stays the mirror
 
   void Map(int p1, int p2, bool cells, int *q1, int *q2) const;
   // (i, j, k) of this block's ghost cell at layer l and of the partner's

@@ -570,7 +570,7 @@ int Upload(ath_ctx *c) {
     long total = 0;
     for (int layer = 1; layer <= h.g; ++layer)
       for (const auto &sf : h.surf) {
-        if (sf.type == "interblock") continue;  // filled by the halo gather
+        if (sf.type == "interblock" || sf.type == "periodic") continue;  // filled by the halo gather
         const int stp = sf.SurfType();
         const int dir = (stp - 1) / 2;
         const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;

@@ -74,6 +74,8 @@ typedef struct {
   int has_mf;
   int isothermal; /* viscousWall: wall temperature given */
   double twall;
+  int startTag, endTag; /* periodic: the two surfaces' tags and the start */
+  v3 trans;             /* surface's translation onto the end surface */
 } bcstate_t;
 
 /* src/boundaryConditions.cpp:2424-2450 */
@@ -130,6 +132,8 @@ typedef struct conn_s {
   int s, dir, r3, r1s, n1, r2s, n2;
   int pb, ps, pdir, pr3, pr1s, pn1, pr2s, pn2;
   int swap, rev1, rev2;
+  int periodic; /* periodic connection: no geometry swap */
+  v3 shift;     /* added to this patch's nodes before matching */
 } conn_t;
 
 struct orc_case {
@@ -852,6 +856,7 @@ static void cell_widths(blk_t *b) {
 
 static void seterr(char *err, int errlen, const char *fmt, ...);
 static void edge_ghosts(orc_case *c, blk_t *b, int viscous);
+static const bcstate_t *find_periodic(const orc_case *c, int tag);
 
 /* partner block and surface from the tag: src/boundaryConditions.cpp:2455-2495 */
 static int tag_partner_surface(int tag) {
@@ -899,7 +904,8 @@ static int orient_patch(conn_t *cn, const orc_case *c, const v3 *const *nodes, i
         for (int q = 0; q < 3 && ok; ++q) {
           int b1, b2;
           conn_map(cn, corners[q][0], corners[q][1], 0, &b1, &b2);
-          v3 pa = node_at(nodes[a], A, cn->dir, cn->r3, cn->r1s + corners[q][0], cn->r2s + corners[q][1]);
+          v3 pa = vadd(node_at(nodes[a], A, cn->dir, cn->r3, cn->r1s + corners[q][0], cn->r2s + corners[q][1]),
+                       cn->shift);
           v3 pbn = node_at(nodes[cn->pb], B, cn->pdir, cn->pr3, cn->pr1s + b1, cn->pr2s + b2);
           ok = same_point(pa, pbn);
         }
@@ -957,6 +963,7 @@ static void swap_geometry(orc_case *c) {
     blk_t *A = &c->blk[a];
     for (int q = 0; q < A->nconn; ++q) {
       const conn_t *cn = &A->conn[q];
+      if (cn->periodic) continue; /* periodic ghosts keep the mirrored geometry */
       const blk_t *B = &c->blk[cn->pb];
       const geom_copy *G = &gc[cn->pb];
       double f3 = ((cn->s + cn->ps) % 2 == 0) ? -1.0 : 1.0;
@@ -1027,7 +1034,8 @@ static int connect_blocks(orc_case *c, const v3 *const *nodes, char *err, int er
     }
     for (int s = 0; s < A->nsurf; ++s) {
       const surf_t *sf = &A->surf[s];
-      if (strcmp(sf->type, "interblock")) continue;
+      int isPer = !strcmp(sf->type, "periodic");
+      if (strcmp(sf->type, "interblock") && !isPer) continue;
       conn_t cn;
       memset(&cn, 0, sizeof cn);
       cn.s = surf_type(sf);
@@ -1038,6 +1046,46 @@ static int connect_blocks(orc_case *c, const v3 *const *nodes, char *err, int er
       cn.n1 = sf->d[2 * d1 + 1] - cn.r1s;
       cn.r2s = sf->d[2 * d2];
       cn.n2 = sf->d[2 * d2 + 1] - cn.r2s;
+      if (isPer) {
+        /* periodic pairs match by geometry once the start surface is moved
+         * by the translation (GetConnectionBCs, src/boundaryConditions.cpp:
+         * 543-593, patch::Transform :2223-2248) */
+        const bcstate_t *ps = find_periodic(c, sf->d[6]);
+        if (!ps) { seterr(err, errlen, "block %d: no periodic state for tag %d", a, sf->d[6]); return -1; }
+        int other = ps->startTag == sf->d[6] ? ps->endTag : ps->startTag;
+        cn.periodic = 1;
+        cn.shift = ps->startTag == sf->d[6] ? ps->trans : vscl(-1.0, ps->trans);
+        int found = 0;
+        for (int pbq = 0; pbq < c->nb && !found; ++pbq) {
+          const blk_t *B = &c->blk[pbq];
+          for (int t = 0; t < B->nsurf && !found; ++t) {
+            const surf_t *pf = &B->surf[t];
+            if (strcmp(pf->type, "periodic") || pf->d[6] != other || pf == sf) continue;
+            cn.pb = pbq;
+            cn.ps = surf_type(pf);
+            cn.pdir = (cn.ps - 1) / 2;
+            int e1 = (cn.pdir + 1) % 3, e2 = (cn.pdir + 2) % 3;
+            cn.pr3 = pf->d[2 * cn.pdir];
+            cn.pr1s = pf->d[2 * e1];
+            cn.pn1 = pf->d[2 * e1 + 1] - cn.pr1s;
+            cn.pr2s = pf->d[2 * e2];
+            cn.pn2 = pf->d[2 * e2 + 1] - cn.pr2s;
+            if (orient_patch(&cn, c, nodes, a) == 0) found = 1;
+          }
+        }
+        if (!found) { seterr(err, errlen, "block %d: no matching periodic patch for tag %d", a, sf->d[6]); return -1; }
+        if (ndir(&c->blk[cn.pb], cn.pdir) < A->g) {
+          seterr(err, errlen, "block %d: periodic partner is thinner than the ghost layers", a);
+          return -1;
+        }
+        A->conn = (conn_t *)realloc(A->conn, sizeof(conn_t) * (size_t)(A->nconn + 1));
+        A->conn[A->nconn++] = cn;
+        int n1full = d1 == 0 ? A->ni : d1 == 1 ? A->nj : A->nk;
+        for (int p2 = 0; p2 < cn.n2; ++p2)
+          for (int p1 = 0; p1 < cn.n1; ++p1)
+            A->isconn[cn.s - 1][(cn.r1s + p1) + (long)n1full * (cn.r2s + p2)] = 1;
+        continue;
+      }
       cn.ps = tag_partner_surface(sf->d[6]);
       cn.pb = sf->d[6] - 1000 * cn.ps;
       if (cn.ps < 0 || cn.pb < 0 || cn.pb >= c->nb) {
@@ -1402,6 +1450,13 @@ static int alloc_block(blk_t *b) {
   return (b->vol && b->W && b->order) ? 0 : -1;
 }
 
+static const bcstate_t *find_periodic(const orc_case *c, int tag) {
+  for (int q = 0; q < c->nbc; ++q)
+    if (!strcmp(c->bcs[q].kind, "periodic") && (c->bcs[q].startTag == tag || c->bcs[q].endTag == tag))
+      return &c->bcs[q];
+  return NULL;
+}
+
 static const bcstate_t *find_bc(const orc_case *c, int tag) {
   for (int q = 0; q < c->nbc; ++q)
     if (c->bcs[q].tag == tag) return &c->bcs[q];
@@ -1596,6 +1651,14 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
         if (get_param(ents[q], "direction", v, sizeof v)) s->dir = parse_vec(v);
         if (get_param(ents[q], "nonreflecting", v, sizeof v)) s->nonreflecting = !strcmp(v, "true");
         if (get_param(ents[q], "temperature", v, sizeof v)) { s->isothermal = 1; s->twall = strtod(v, NULL); }
+        if (get_param(ents[q], "startTag", v, sizeof v)) s->startTag = atoi(v);
+        if (get_param(ents[q], "endTag", v, sizeof v)) s->endTag = atoi(v);
+        if (get_param(ents[q], "translation", v, sizeof v)) s->trans = parse_vec(v);
+        if (!strcmp(s->kind, "periodic") && (get_param(ents[q], "axis", v, sizeof v) ||
+                                             get_param(ents[q], "rotation", v, sizeof v))) {
+          seterr(err, errlen, "rotational periodicity is not supported");
+          goto fail;
+        }
         if (get_param(ents[q], "heatFlux", v, sizeof v) && strtod(v, NULL) != 0.0) {
           seterr(err, errlen, "viscousWall heat flux is not supported");
           goto fail;
@@ -1666,6 +1729,10 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
   }
   for (int q = 0; q < c->nbc; ++q) {
     bcstate_t *s = &c->bcs[q];
+    if (!strcmp(s->kind, "periodic")) { /* src/inputStates.cpp:883-894 */
+      s->trans = vdiv(s->trans, c->lRef);
+      continue;
+    }
     if (!strcmp(s->kind, "viscousWall")) { /* src/inputStates.cpp:775-783 */
       s->vel = vdiv(s->vel, c->aRef);
       s->twall /= c->tRef;
@@ -1716,12 +1783,7 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
       b->nsurf = nsurfPer[bq];
       b->surf = surfsPer[bq];
       surfsPer[bq] = NULL;
-      for (int s = 0; s < b->nsurf; ++s) {
-        if (!strcmp(b->surf[s].type, "periodic")) {
-          seterr(err, errlen, "periodic connections are not supported");
-          goto fail2;
-        }
-      }
+
       qsort(b->surf, (size_t)b->nsurf, sizeof(surf_t), surf_cmp);
       if (build_geometry(b, coords[bq])) { seterr(err, errlen, "negative volume in block %d", bq); goto fail2; }
       ghost_geometry(b);
@@ -2003,7 +2065,7 @@ static void assign_ghosts(orc_case *c, blk_t *b) {
         aC = r3;
         if (iC >= n3) iC = n3 - 1;
       }
-      if (!strcmp(sf->type, "interblock")) continue; /* filled by swap_halo */
+      if (!strcmp(sf->type, "interblock") || !strcmp(sf->type, "periodic")) continue; /* swap_halo */
       char bcn[32];
       snprintf(bcn, sizeof bcn, "%s", sf->type);
       if (!strcmp(bcn, "viscousWall")) strcpy(bcn, "slipWall");

@@ -120,7 +120,7 @@ def test_residual_dt_diagonal_parity(case_paths, name):
 
 # cases whose free trajectory is not bit-identical (std::pow); their per-step
 # residual is compared from a common state
-RESYNC = {"subsonicCylinder", "multiblockCylinder", "shockTube", "viscousFlatPlate"}
+RESYNC = {"subsonicCylinder", "multiblockCylinder", "shockTube", "viscousFlatPlate", "couette"}
 
 
 def run_both(inp, iterations, precision=0, check_fields=True, step_tol=1e-12, resync=False):
@@ -462,6 +462,21 @@ def test_viscous_connections(tmp_path):
     path, _ = cc.write_pair(str(tmp_path), "cut", r0, r1, timeint="implicitEuler", eq="navierStokes", scale=1e-3)
     for order in ("PENCIL", "HYPERPLANE"):
         say("viscous connections", order)
+        s = Solver(path)
+        s.set_sweep_ordering(getattr(Solver, order))
+        o = pyoracle.OracleCase(path)
+        _parity_from_common_state(s, o, 4)
+        s.close()
+
+
+@pytest.mark.parametrize("eq", ["euler", "navierStokes"])
+def test_periodic_every_sweep_ordering(tmp_path, eq):
+    """A block periodic in i (ghost states and lagged du across the periodic
+    connection, mirrored ghost geometry) against the oracle."""
+    from test_oracle_connections import _periodic_pair
+    path = _periodic_pair(str(tmp_path), eq=eq)
+    for order in ("PENCIL", "TILED", "HYPERPLANE"):
+        say("periodic", eq, order)
         s = Solver(path)
         s.set_sweep_ordering(getattr(Solver, order))
         o = pyoracle.OracleCase(path)

@@ -34,6 +34,8 @@ def _cases(tmp):
            os.path.join(ROOT, "tests", "golden", "supersonicWedge", "supersonicWedge.inp"),
            os.path.join(ROOT, "tests", "golden", "shockTube", "shockTube.inp"),
            os.path.join(ROOT, "tests", "golden", "multiblockCylinder", "multiblockCylinder.inp"),
+           os.path.join(ROOT, "tests", "golden", "viscousFlatPlate", "viscousFlatPlate.inp"),
+           os.path.join(ROOT, "tests", "golden", "couette", "couette.inp"),
            cases.vortex_case(os.path.join(tmp, "v"), n=7, iterations=1),
            cases.bump_case(os.path.join(tmp, "b"), seed=3, n=9)[0]]
     # two-block cases with interblock connections in several orientations

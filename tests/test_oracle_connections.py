@@ -100,3 +100,85 @@ def test_unmatched_connection_is_an_error(tmp_path):
     open(p, "w").write(bad)
     with pytest.raises(RuntimeError, match="interblock"):
         pyoracle.OracleCase(p)
+
+
+def _periodic_pair(tmp, n=(5, 4, 3), copies=False, eq="euler"):
+    """A block periodic in i on a grid invariant under the i-translation
+    (x = i h + f(j, k)), or three copies of it joined by interblock
+    connections (copies=True); the state is periodic in x."""
+    from aither_amd import cases
+    os.makedirs(tmp, exist_ok=True)
+    ni, nj, nk = n
+    reps = 3 if copies else 1
+    h, Lx = 0.1, 0.1 * ni
+    i, j, k = np.meshgrid(np.arange(reps * ni + 1), np.arange(nj + 1), np.arange(nk + 1), indexing="ij")
+    x = i * h + 0.03 * np.sin(1.3 * j) * np.cos(0.9 * k)
+    y = j * 0.08 + 0.01 * np.sin(0.7 * k)
+    z = k * 0.07 + 0.012 * np.cos(0.8 * j)
+    x = x - (Lx if copies else 0.0)  # the centre copy sits where the single block does
+    xc, yc, zc = cases.cell_centres(x, y, z)
+    rho = 1.225 * (1.0 + 0.05 * np.sin(2 * np.pi * xc / Lx + 2.0 * yc))
+    u = 100.0 + 20.0 * np.cos(2 * np.pi * xc / Lx) * np.sin(3.0 * zc)
+    v = 15.0 * np.sin(2 * np.pi * xc / Lx + 1.0)
+    w = 8.0 * np.cos(2 * np.pi * xc / Lx + 2.0 * yc)
+    p = 101325.0 * (1.0 + 0.03 * np.cos(2 * np.pi * xc / Lx + zc))
+    cases.write_cloud(os.path.join(tmp, "per.cloud"), xc, yc, zc, rho, u, v, w, p)
+    fs = "characteristic(tag=1; pressure=101325.0; density=1.225; velocity=[100.0, 0.0, 0.0])"
+    if not copies:
+        cases.write_plot3d(os.path.join(tmp, "per.xyz"), [(x, y, z)])
+        types = {s: ("characteristic", 1) for s in ("jmin", "jmax", "kmin", "kmax")}
+        types["imin"] = ("periodic", 7)
+        types["imax"] = ("periodic", 8)
+        bcs = [cases._surfaces_box(ni, nj, nk, types)]
+        states = f"<{fs}, periodic(startTag=7; endTag=8; translation=[{Lx!r}, 0.0, 0.0])>"
+    else:
+        blocks, bcs = [], []
+        for b in range(3):
+            sl = slice(b * ni, (b + 1) * ni + 1)
+            blocks.append((x[sl], y[sl], z[sl]))
+            types = {s: ("characteristic", 1) for s in ("jmin", "jmax", "kmin", "kmax", "imin", "imax")}
+            if b > 0:
+                types["imin"] = ("interblock", 2000 + b - 1)
+            if b < 2:
+                types["imax"] = ("interblock", 1000 + b + 1)
+            bcs.append(cases._surfaces_box(ni, nj, nk, types))
+        cases.write_plot3d(os.path.join(tmp, "per.xyz"), blocks)
+        states = f"<{fs}>"
+    with open(os.path.join(tmp, "per.inp"), "w") as f:
+        f.write(f"""gridName: per
+equationSet: {eq}
+timeIntegration: implicitEuler
+cflStart: 2.0
+cflMax: 2.0
+faceReconstruction: thirdOrder
+limiter: vanAlbada
+iterations: 3
+referenceTemperature: 288.15
+referenceDensity: 1.225
+initialConditions: <icState(tag=-1; file=per.cloud)>
+matrixSolver: lusgs
+boundaryStates: {states}
+boundaryConditions: {len(bcs)}
+""" + "".join(bcs))
+    return os.path.join(tmp, "per.inp")
+
+
+def test_periodic_block_equals_centre_of_three_copies(tmp_path):
+    """On a grid invariant under the period's translation with a state
+    periodic in x, a block periodic in i has the residual of the centre block
+    of three copies joined by interblock connections: the periodic ghost
+    cells carry the partner's state and, by the invariance, the mirrored
+    ghost geometry equals the true neighbour's (src/procBlock.cpp:2200-2202)."""
+    a = pyoracle.OracleCase(_periodic_pair(str(tmp_path / "p")))
+    b = pyoracle.OracleCase(_periodic_pair(str(tmp_path / "c"), copies=True))
+    for o in (a, b):
+        o.bc()
+        o.calc_residual()
+    Ra, Rb = a.residual(0), b.residual(1)
+    assert np.abs(Ra - Rb).max() <= 1e-12 * np.abs(Rb).max()
+    # and a translation that does not map the start surface onto the end one
+    # finds no partner
+    bad = open(_periodic_pair(str(tmp_path / "q"))).read().replace("translation=[0.5", "translation=[0.45")
+    open(str(tmp_path / "q" / "per.inp"), "w").write(bad)
+    with pytest.raises(RuntimeError, match="periodic"):
+        pyoracle.OracleCase(str(tmp_path / "q" / "per.inp"))
