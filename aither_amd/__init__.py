@@ -45,6 +45,7 @@ class Info(ctypes.Structure):
         ("total_cells", ctypes.c_double), ("r_ref", ctypes.c_double),
         ("t_ref", ctypes.c_double), ("l_ref", ctypes.c_double), ("a_ref", ctypes.c_double),
         ("gas_constant", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("block_matrix", ctypes.c_int),
     ]
 
 
@@ -103,6 +104,7 @@ _SIGS = {
     "ath_get_spectral_radius": [_P, ctypes.c_int, _D],
     "ath_get_diagonal": [_P, ctypes.c_int, _D],
     "ath_get_update": [_P, ctypes.c_int, _D],
+    "ath_get_block_diagonal": [_P, ctypes.c_int, ctypes.c_int, _D],
     "ath_enable_timing": [_P, ctypes.c_int],
     "ath_get_timing": [_P, ctypes.POINTER(Timing)],
     "ath_inspect_info": [ctypes.c_char_p, ctypes.POINTER(Info)],
@@ -292,3 +294,10 @@ class Solver:
 
     def update_vector(self, b=0):
         return self._field(self._lib.ath_get_update, b, 5)
+
+    def block_diagonal(self, b=0, inverse=False):
+        """Block main diagonal after AddDiagonalTerms (or its inverse), one
+        5x5 row-major matrix per cell (blusgs / bdplur cases only)."""
+        out = np.empty(self.ncells(b) * 25, dtype=np.float64)
+        _check(self._lib.ath_get_block_diagonal(self._h, b, int(inverse), _ptr(out)))
+        return out.reshape(-1, 5, 5)

@@ -522,4 +522,142 @@ __device__ __forceinline__ void ScalarOffDiag(const GasT<T> &g, const T Wnb[5], 
   }
 }
 
+// ---------------------------------------------------------------------------
+// 5x5 blocks (matrixSolver blusgs / bdplur), row-major
+
+// fluxJacobian::InvFluxJacobian and RusanovFluxJacobian
+// (include/fluxJacobian.hpp:447-560) for one species (mass fraction 1):
+// 0.5 |A| dF/dU at the state, then the face spectral radius added to
+// (positive) or subtracted from each entry of the identity, the off-diagonal
+// entries +/- 0 included
+template <typename T>
+__device__ __forceinline__ void RusanovJacobian(const GasT<T> &g, const T W[5], T nx, T ny, T nz, T mag,
+                                                bool positive, T J[25]) {
+#pragma clang fp contract(off)
+  const T u = W[1], v = W[2], w = W[3];
+  const T vn = (u * nx + v * ny) + w * nz;
+  const T sr = T(0.5) * mag * (fabs(vn) + SoS(g, W[0], W[4]));
+  const T gamma = g.gamma, gm1 = gamma - T(1);
+  const T phi = T(0.5) * gm1 * ((u * u + v * v) + w * w);
+  const T a1 = gamma * TotalEnergy(g, W[0], u, v, w, W[4]) - phi;
+  const T a3 = gamma - T(2);
+  J[0] = vn * T(0);
+  J[1] = nx;
+  J[2] = ny;
+  J[3] = nz;
+  J[4] = T(0);
+  J[5] = phi * nx - u * vn;
+  J[10] = phi * ny - v * vn;
+  J[15] = phi * nz - w * vn;
+  J[20] = vn * (phi - a1);
+  J[6] = vn - a3 * nx * u;
+  J[11] = v * nx - gm1 * u * ny;
+  J[16] = w * nx - gm1 * u * nz;
+  J[21] = a1 * nx - gm1 * u * vn;
+  J[7] = u * ny - gm1 * v * nx;
+  J[12] = vn - a3 * ny * v;
+  J[17] = w * ny - gm1 * v * nz;
+  J[22] = a1 * ny - gm1 * v * vn;
+  J[8] = u * nz - gm1 * w * nx;
+  J[13] = v * nz - gm1 * w * ny;
+  J[18] = vn - a3 * nz * w;
+  J[23] = a1 * nz - gm1 * w * vn;
+  J[9] = gm1 * nx;
+  J[14] = gm1 * ny;
+  J[19] = gm1 * nz;
+  J[24] = gamma * vn;
+  const T f = T(0.5) * mag;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) {
+    const T dis = (q % 6 == 0) ? sr : T(0);
+    const T jq = J[q] * f;
+    J[q] = positive ? jq + dis : jq - dis;
+  }
+}
+
+// ArrayMultiplication, block branch (include/fluxJacobian.hpp:50-86):
+// out[r] = sum over c in order of M[r][c] v[c], from +0
+template <typename T>
+__device__ __forceinline__ void BlockMult(const T M[25], const T v[5], T out[5]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) s += M[r * 5 + c] * v[c];
+    out[r] = s;
+  }
+}
+
+// MatrixInverse (src/matrix.cpp:57-105) with FindMaxInColumn (:145-165),
+// in registers: row r's pivot is the largest |m[i][r]|, i >= r, taken before
+// row r's leading entries are eliminated (row 0 when all vanish); row r is
+// eliminated against rows 0..r-1 (LinearCombRow over the whole row),
+// normalised from the pivot column on, and the upper triangle is cleared from
+// the bottom row up.  The identity takes every row operation.  Returns false
+// on a zero pivot.
+template <typename T>
+__device__ __forceinline__ void SwapRows5(T m[25], int r1, int r2) {
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const T t = m[r1 * 5 + c];
+    m[r1 * 5 + c] = m[r2 * 5 + c];
+    m[r2 * 5 + c] = t;
+  }
+}
+template <typename T>
+__device__ __forceinline__ bool GJInverse5(T m[25], T I[25]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int q = 0; q < 25; ++q) I[q] = (q % 6 == 0) ? T(1) : T(0);
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    T mv = T(0);
+    int pr = 0;
+#pragma unroll
+    for (int i = r; i < 5; ++i) {
+      const T a = fabs(m[i * 5 + r]);
+      if (a > mv) {
+        mv = a;
+        pr = i;
+      }
+    }
+    // the swap with a runtime partner, as selects over the candidate rows
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      if (i != r && pr == i) {
+        SwapRows5(m, r, i);
+        SwapRows5(I, r, i);
+      }
+#pragma unroll
+    for (int i = 0; i < r; ++i) {
+      const T factor = m[r * 5 + i] / m[i * 5 + i];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        m[r * 5 + c] = m[r * 5 + c] - factor * m[i * 5 + c];
+        I[r * 5 + c] = I[r * 5 + c] - factor * I[i * 5 + c];
+      }
+    }
+    if (m[r * 5 + r] == T(0)) ok = false;
+    const T nf = T(1) / m[r * 5 + r];
+#pragma unroll
+    for (int c = r; c < 5; ++c) m[r * 5 + c] *= nf;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) I[r * 5 + c] *= nf;
+  }
+#pragma unroll
+  for (int r = 3; r >= 0; --r)
+#pragma unroll
+    for (int i = 4; i > r; --i) {
+      const T factor = m[r * 5 + i];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        m[r * 5 + c] = m[r * 5 + c] - factor * m[i * 5 + c];
+        I[r * 5 + c] = I[r * 5 + c] - factor * I[i * 5 + c];
+      }
+    }
+  return ok;
+}
+
 }  // namespace ath

@@ -726,8 +726,11 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
       } else if (key == "turbulenceModel") {
         if (val != "none") return "unsupported turbulenceModel " + val;
       } else if (key == "matrixSolver") {
-        if (val == "lusgs") c.solver = Solver::LUSGS;
-        else if (val == "dplur") c.solver = Solver::DPLUR;
+        // src/input.cpp:713-720, 845-856
+        if (val == "lusgs") c.solver = Solver::LUSGS, c.block = false;
+        else if (val == "dplur") c.solver = Solver::DPLUR, c.block = false;
+        else if (val == "blusgs") c.solver = Solver::LUSGS, c.block = true;
+        else if (val == "bdplur") c.solver = Solver::DPLUR, c.block = true;
         else return "unsupported matrixSolver " + val;
       } else if (key == "matrixSweeps") c.sweeps = std::stoi(val);
       else if (key == "matrixRelaxation") c.relax = std::stod(val);
@@ -801,6 +804,8 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   }
   if (c.rRef <= 0 || c.tRef <= 0) return "referenceDensity and referenceTemperature are required";
   if (c.timeint == TimeInt::ExplicitEuler) c.nonlinear = 1;  // src/input.cpp:928-934
+  c.block = c.block && c.Implicit();  // input::IsBlockMatrix (src/input.cpp:713-720)
+  if (c.block && c.viscous) return "block matrix solvers with navierStokes are not supported";
   double fn = 0, fm = 0;
   if (!FluidProps(c.dir, fluid, &fn, &fm, &c.suth)) return "unknown fluid " + fluid;
 

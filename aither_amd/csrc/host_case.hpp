@@ -116,6 +116,7 @@ struct HostCase {
   Flux flux = Flux::Roe;
   TimeInt timeint = TimeInt::ExplicitEuler;
   Solver solver = Solver::LUSGS;
+  bool block = false;  // blusgs / bdplur: 5x5 block main diagonal and off-diagonals
   double gasR = 0, gasN = 0, gamma = 0;
   // laminar viscous terms (equationSet navierStokes)
   bool viscous = false;

@@ -28,6 +28,9 @@ struct DevBlock {
   double *xold;          // 5 * ncell, DP-LUR Jacobi copy
   float *xold32;
   double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
+  // block matrix solvers: main diagonal after AddDiagonalTerms and its
+  // inverse, 25 * ncell each, entry (r, c) of cell p at (5 r + c) ncell + p
+  double *Ab, *Abi;
   // off-diagonal product of the ghost neighbour across each boundary face
   // that lies on an interblock connection (zero on other faces), per sweep
   // pass; null when the block has no connection.  5 * nbf, SoA.
@@ -60,6 +63,7 @@ struct Params {
   double theta, zeta, relax, dualCFL;
   double dtFixed;  // nondimensional global time step, <= 0 for local dt
   int implicit, multilevel, initMatrix, explicitEuler;
+  int block;  // matrixSolver blusgs / bdplur: 5x5 main diagonal and off-diagonals
 };
 
 // ---------------------------------------------------------------------------
@@ -293,12 +297,13 @@ constexpr int kNFJ = kTI * (kTJ + 1) * kTK;
 constexpr int kNFK = kTI * kTJ * (kTK + 1);
 constexpr int kNF = kNFI + kNFJ + kNFK;
 
-template <int RECON, int LIM, int FLUX>
-__device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &g, double kappa,
-                                         int d, int i, int j, int k, double F[5]) {
+// reconstructed lower (WL) and upper (WR) states of the face (i, j, k) of
+// direction d (src/procBlock.cpp:384-437)
+template <int RECON, int LIM>
+__device__ __forceinline__ void FaceStates(const DevBlock &b, double kappa, int d, int i, int j, int k,
+                                           double WL[5], double WR[5]) {
   const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
   const long c0 = PadIdx(b, i, j, k);  // upper cell of the face
-  double WL[5], WR[5];
   if (RECON == 0) {
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
@@ -337,6 +342,13 @@ __device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &
       WR[e] = Muscl<LIM>(up1, u0, um1, kappa, dPR, dMR);
     }
   }
+}
+
+template <int RECON, int LIM, int FLUX>
+__device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &g, double kappa,
+                                         int d, int i, int j, int k, double F[5]) {
+  double WL[5], WR[5];
+  FaceStates<RECON, LIM>(b, kappa, d, i, j, k, WL, WR);
   const double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
   if (FLUX == 0) RoeFlux(g, WL, WR, a.x, a.y, a.z, F);
   else AusmFlux(g, WL, WR, a.x, a.y, a.z, F);
@@ -352,7 +364,7 @@ __device__ __forceinline__ void CellTail(const DevBlock &b, const Params &P, lon
   b.sr[p] = sr;
   const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*cflp) * (vol / Dmax(sr, 0.0));
   b.dt[p] = dt;
-  if (P.implicit) {
+  if (P.implicit && !P.block) {
     double dvt = (vol * (1.0 + P.zeta)) / (dt * P.theta);
     if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
     double A = Ad * P.relax;
@@ -424,6 +436,72 @@ __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
     return;
   }
   CellTail(b, P, p, b.vol[p], sr, sr, cflp);
+}
+
+// Block main diagonal (src/procBlock.cpp:449-486, 588-625, 726-763):
+// every face adds the Rusanov Jacobian of its lower state to its lower cell
+// and subtracts that of its upper state from its upper cell, faces in i, j, k
+// order, so a cell sees -J(face i-) +J(face i+) -J(j-) +J(j+) -J(k-) +J(k+)
+// from a zero matrix.  Then AddDiagonalTerms (src/linearSolver.cpp:146-171)
+// and the Gauss-Jordan inverse (src/linearSolver.cpp:173-184).  One thread per
+// cell, after the residual kernel has written sr and dt.  A zero pivot sets
+// bit 4 of `flag`.
+template <int RECON, int LIM>
+__global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *flag) {
+#pragma clang fp contract(off)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<double> g = Cast<double>(P.gas);
+  double D[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) D[q] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double WL[5], WR[5], J[25];
+    FaceStates<RECON, LIM>(b, P.kappa, d, i, j, k, WL, WR);
+    double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+    RusanovJacobian<double>(g, WR, a.x, a.y, a.z, a.w, false, J);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
+    const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
+    FaceStates<RECON, LIM>(b, P.kappa, d, ui, uj, uk, WL, WR);
+    a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
+    RusanovJacobian<double>(g, WL, a.x, a.y, a.z, a.w, true, J);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
+  }
+  const double sr = b.sr[p];
+  double dvt = (b.vol[p] * (1.0 + P.zeta)) / (b.dt[p] * P.theta);
+  if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) D[q * 6] *= P.relax;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) D[q * 6] += dvt;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b.Ab[q * b.ncell + p] = D[q];
+  double I[25];
+  if (!GJInverse5<double>(D, I)) atomicOr(flag, 4);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b.Abi[q * b.ncell + p] = I[q];
+}
+
+// block product with the cell's stored matrix (Ab or Abi)
+template <typename T>
+__device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
+  T m[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) m[q] = T(M[q * b.ncell + p]);
+  BlockMult<T>(m, v, out);
+}
+
+// RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194), inviscid
+template <typename T>
+__device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], const T du[5], double4 a,
+                                             bool positive, T out[5]) {
+  T J[25];
+  RusanovJacobian<T>(g, W, T(a.x), T(a.y), T(a.z), T(a.w), positive, J);
+  BlockMult<T>(J, du, out);
 }
 
 
@@ -773,7 +851,8 @@ struct OpBuf<float> {
 // adding +0 leaves the sum unchanged (it starts at +0 and never becomes -0).
 template <typename T>
 __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const ViscParams &vp,
-                                           const T *x, int i, int j, int k, bool upper, T out[5]) {
+                                           const T *x, int i, int j, int k, bool upper, T out[5],
+                                           int block = 0) {
 #pragma unroll
   for (int e = 0; e < 5; ++e) out[e] = T(0);
   const T *bt = OpBuf<T>::Bt(b);
@@ -799,8 +878,12 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       W[e] = T(b.W[e * b.npad + c]);
       du[e] = x[e * b.ncell + q];
     }
-    const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
-    ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
+    if (block) {
+      BlockOffDiag<T>(g, W, du, a, !upper, o);
+    } else {
+      const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
+      ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
+    }
 #pragma unroll
     for (int e = 0; e < 5; ++e) out[e] += o[e];
   }
@@ -820,6 +903,15 @@ __global__ void __launch_bounds__(256) KInitUpdate(DevBlock b, Params P) {
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  if (P.block) {
+    T v[5], o[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) v[e] = T(rb[e]);
+    CellBlockMult<T>(b, b.Abi, p, v, o);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
+    return;
+  }
   const T ainv = OpBuf<T>::Ainv(b)[p];
 #pragma unroll
   for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = T(rb[e]) * ainv;
@@ -837,15 +929,24 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T L[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
   if (useUpper) {
     T U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
 #pragma unroll
     for (int e = 0; e < 5; ++e) L[e] -= U[e];
   }
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  if (P.block) {
+    T v[5], o[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + L[e];
+    CellBlockMult<T>(b, b.Abi, p, v, o);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
+    return;
+  }
   const T ainv = OpBuf<T>::Ainv(b)[p];
 #pragma unroll
   for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = (T(rb[e]) + L[e]) * ainv;
@@ -863,7 +964,26 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T U[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
+  if (P.block) {
+    T v[5], o[5];
+    if (full) {
+      T L[5];
+      OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
+      double rb[5];
+      Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + L[e]) - U[e];
+      CellBlockMult<T>(b, b.Abi, p, v, o);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
+    } else {
+      CellBlockMult<T>(b, b.Abi, p, U, o);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = x[e * b.ncell + p] - o[e];
+    }
+    return;
+  }
   const T ainv = OpBuf<T>::Ainv(b)[p];
   if (full) {
     T L[5];
@@ -888,12 +1008,21 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
   const GasT<T> g = Cast<T>(P.gas);
   const T *xo = OpBuf<T>::Xold(b);
   T L[5], U[5];
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L);
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L, P.block);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U, P.block);
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
-  const T ainv = OpBuf<T>::Ainv(b)[p];
   T *x = OpBuf<T>::X(b);
+  if (P.block) {
+    T v[5], o[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + T(0)) + (L[e] - U[e]);
+    CellBlockMult<T>(b, b.Abi, p, v, o);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
+    return;
+  }
+  const T ainv = OpBuf<T>::Ainv(b)[p];
 #pragma unroll
   for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = ((T(rb[e]) + T(0)) + (L[e] - U[e])) * ainv;
 }
@@ -939,14 +1068,21 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Param
     const GasT<T> g = Cast<T>(P.gas);
     const T *x = OpBuf<T>::X(b);
     T L[5], U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
-    const double A = b.A[p];
+    const double A = P.block ? 0.0 : b.A[p];
+    double axb[5] = {0, 0, 0, 0, 0};
+    if (P.block) {
+      double xv[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) xv[e] = (double)x[e * b.ncell + p];
+      CellBlockMult<double>(b, b.Ab, p, xv, axb);
+    }
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
-      const double ax = (double)x[e * b.ncell + p] * A;
+      const double ax = P.block ? axb[e] : (double)x[e * b.ncell + p] * A;
       const double axmb = (ax - (double)(L[e] - U[e])) - rb[e];
       const double r = 0.0 - axmb;
       acc += r * r;
@@ -1236,7 +1372,12 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
     Wn[e] = A.W[e * A.npad + c.gpad];
     dn[e] = xb[e * B.ncell + c.scell];
   }
-  if (P.visc.on)
+  if (P.block) {
+    T W[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) W[e] = T(Wn[e]);
+    BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
+  } else if (P.visc.on)
     NbOffDiag<T>(Cast<T>(P.gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.c2c[c.d][c.face],
                  A.mu[c.gpad]);
   else

@@ -385,6 +385,7 @@ int Upload(ath_ctx *c) {
   c->P.multilevel = hc.MultiLevel() ? 1 : 0;
   c->P.initMatrix = hc.InitMatrix() ? 1 : 0;
   c->P.explicitEuler = hc.timeint == TimeInt::ExplicitEuler ? 1 : 0;
+  c->P.block = hc.block ? 1 : 0;
   c->P.visc.on = hc.viscous ? 1 : 0;
   c->P.visc.recon4 = hc.viscRecon4 ? 1 : 0;
   c->P.visc.vscale = hc.vscale;
@@ -483,6 +484,11 @@ int Upload(ath_ctx *c) {
     if (int r = c->Alloc(&d.rhs, 5 * nc)) return r;
     HIP_OK(hipMemset(d.x, 0, 5 * nc * sizeof(double)));
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
+    d.Ab = d.Abi = nullptr;
+    if (hc.block) {
+      if (int r = c->Alloc(&d.Ab, 25 * nc)) return r;
+      if (int r = c->Alloc(&d.Abi, 25 * nc)) return r;
+    }
     d.Ainv32 = nullptr;
     d.x32 = nullptr;
     d.xold32 = nullptr;
@@ -676,6 +682,9 @@ void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
   dim3 grid((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
             (d.nk + ath::kTK - 1) / ath::kTK);
   hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+  if (c->P.block)  // inviscid only (navierStokes with a block solver is rejected at load)
+    hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
+                       c->dflag);
   if (!c->P.visc.on) return;
   // viscous ghost cells, temperature and viscosity, viscous fluxes
   // (src/procBlock.cpp:6126-6138)
@@ -728,7 +737,9 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
-    if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_PENCIL) {
+    // the block solvers sweep in hyperplane order whatever the ordering set
+    const int order = c->P.block ? ATH_SWEEP_HYPERPLANE : c->sweepOrder;
+    if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_PENCIL) {
       if (sw == 0)
         for (auto &br : c->blk)
           hipLaunchKernelGGL(ath::KLusgsRhs, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
@@ -750,7 +761,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         }
         mark(false);
       }
-    } else if (hc.solver == ath::Solver::LUSGS && c->sweepOrder == ATH_SWEEP_TILED) {
+    } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED) {
       for (int dir = 0; dir < 2; ++dir) {
         LaunchConnTerms<T>(c);
         mark(true);
@@ -974,6 +985,8 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
   if (!c || (precision != ATH_IMPLICIT_FP64 && precision != ATH_IMPLICIT_FP32))
     return Fail(ATH_ERR_ARG, "bad precision");
   HIP_OK(hipSetDevice(c->device));
+  if (precision == ATH_IMPLICIT_FP32 && c->P.block)
+    return Fail(ATH_ERR_ARG, "the FP32 operator is not available with a block matrix solver");
   if (precision == ATH_IMPLICIT_FP32)
     if (int r = EnsureFp32(c)) return r;
   c->fp32 = precision;
@@ -1251,7 +1264,24 @@ static int GetPhys(ath_ctx *c, int b, double *out, int which) {
 int ath_get_residual(ath_ctx *c, int b, double *r) { return GetPhys(c, b, r, 0); }
 int ath_get_dt(ath_ctx *c, int b, double *dt) { return GetPhys(c, b, dt, 1); }
 int ath_get_spectral_radius(ath_ctx *c, int b, double *sr) { return GetPhys(c, b, sr, 2); }
-int ath_get_diagonal(ath_ctx *c, int b, double *a) { return GetPhys(c, b, a, 3); }
+int ath_get_diagonal(ath_ctx *c, int b, double *a) {
+  if (c && c->P.block) return Fail(ATH_ERR_ARG, "the case uses a block matrix: use ath_get_block_diagonal");
+  return GetPhys(c, b, a, 3);
+}
+
+int ath_get_block_diagonal(ath_ctx *c, int b, int inverse, double *a) {
+  if (!c || !a || b < 0 || b >= (int)c->blk.size()) return Fail(ATH_ERR_ARG, "bad argument");
+  if (!c->P.block) return Fail(ATH_ERR_ARG, "the case does not use a block matrix");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const auto &d = c->blk[b].d;
+  const long n = d.ncell;
+  std::vector<double> t(25 * (size_t)n);
+  HIP_OK(hipMemcpy(t.data(), inverse ? d.Abi : d.Ab, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long p = 0; p < n; ++p)
+    for (int q = 0; q < 25; ++q) a[25 * p + q] = t[q * n + p];
+  return ATH_OK;
+}
 int ath_get_update(ath_ctx *c, int b, double *du) { return GetPhys(c, b, du, 4); }
 
 static void FillInfo(const ath::HostCase &hc, ath_info *o) {
@@ -1268,6 +1298,7 @@ static void FillInfo(const ath::HostCase &hc, ath_info *o) {
   o->a_ref = hc.aRef;
   o->gas_constant = hc.gasR;
   o->gamma = hc.gamma;
+  o->block_matrix = hc.block ? 1 : 0;
 }
 
 int ath_inspect_info(const char *inp_path, ath_info *out) {

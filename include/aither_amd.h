@@ -47,6 +47,7 @@ typedef struct {
   double r_ref, t_ref, l_ref, a_ref;
   double gas_constant;   /* nondimensional R of the single species */
   double gamma;
+  int block_matrix;      /* input::IsBlockMatrix, src/input.cpp:713-720 */
 } ath_info;
 
 /* per-iteration norms (what src/main.cpp:252-275 reduces and prints) */
@@ -139,6 +140,11 @@ int ath_get_dt(ath_ctx *ctx, int block, double *dt);
 int ath_get_spectral_radius(ath_ctx *ctx, int block, double *sr);
 int ath_get_diagonal(ath_ctx *ctx, int block, double *a);
 int ath_get_update(ath_ctx *ctx, int block, double *du);      /* neq/cell */
+/* block matrix solvers (blusgs, bdplur): linearSolver a_ after
+ * AddDiagonalTerms (inverse = 0) or aInv_ (inverse = 1), 25 per cell,
+ * row-major (src/linearSolver.cpp:146-184); ath_get_diagonal refuses such
+ * cases */
+int ath_get_block_diagonal(ath_ctx *ctx, int block, int inverse, double *a);
 
 /* ---- timing of the last ath_run (HIP events on the context's stream) ---- */
 typedef struct {

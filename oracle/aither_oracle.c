@@ -112,6 +112,7 @@ typedef struct {
   double *sr;     /* physical */
   double *dt;     /* physical */
   double *A, *Ainv;
+  double *Ab, *Abi; /* blusgs / bdplur: main diagonal and its inverse, 25 per physical cell, row-major */
   double *consN, *consNm1;
   double *x;      /* padded, NEQ per cell */
   double *T, *mu; /* padded: temperature, viscosity (viscous cases) */
@@ -150,6 +151,7 @@ struct orc_case {
   int flux;    /* 0 roe, 1 ausm */
   int timeint; /* 0 explicitEuler 1 implicitEuler 2 crankNicholson 3 bdf2 */
   int solver;  /* 0 lusgs, 1 dplur */
+  int block;   /* 5x5 block main diagonal and off-diagonals (blusgs, bdplur) */
   int implicit, multilevel, storeN, initMatrix;
   orc_gas gas;
   double gamma;
@@ -631,6 +633,132 @@ void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
   uv3 a = {V(area[0], area[1], area[2]), area[3]};
   scalar_offdiag(g, Wnb, du, a, positive, out);
 }
+
+/* ------------------------------------------------------------------------ */
+/* block matrices (blusgs, bdplur): 5x5, row-major                          */
+
+/* fluxJacobian::InvFluxJacobian and RusanovFluxJacobian
+ * (include/fluxJacobian.hpp:447-560) for one species: 0.5 |A| dF/dU at the
+ * state, then the face spectral radius added to (positive) or subtracted from
+ * every entry of the identity (off-diagonal entries +/- 0) */
+static void rusanov_jacobian(const orc_gas *g, const double *W, uv3 a, int positive, double *J) {
+  double sr = inv_face_specrad(g, W, a);
+  v3 n = a.u;
+  v3 vel = V(W[1], W[2], W[3]);
+  double u = W[1], v = W[2], w = W[3];
+  double vn = vdot(vel, n);
+  double mf = W[0] / (0.0 + W[0]);
+  double gamma = orc_gamma(g);
+  double gm1 = gamma - 1.0;
+  double phi = 0.5 * gm1 * vdot(vel, vel);
+  double a1 = gamma * energy(g, W) - phi;
+  double a3 = gamma - 2.0;
+  for (int q = 0; q < 25; ++q) J[q] = 0.0;
+  J[0] = vn * (1.0 - mf);
+  J[1] = mf * n.x;
+  J[2] = mf * n.y;
+  J[3] = mf * n.z;
+  J[5] = phi * n.x - u * vn;
+  J[10] = phi * n.y - v * vn;
+  J[15] = phi * n.z - w * vn;
+  J[20] = vn * (phi - a1);
+  J[6] = vn - a3 * n.x * u;
+  J[11] = v * n.x - gm1 * u * n.y;
+  J[16] = w * n.x - gm1 * u * n.z;
+  J[21] = a1 * n.x - gm1 * u * vn;
+  J[7] = u * n.y - gm1 * v * n.x;
+  J[12] = vn - a3 * n.y * v;
+  J[17] = w * n.y - gm1 * v * n.z;
+  J[22] = a1 * n.y - gm1 * v * vn;
+  J[8] = u * n.z - gm1 * w * n.x;
+  J[13] = v * n.z - gm1 * w * n.y;
+  J[18] = vn - a3 * n.z * w;
+  J[23] = a1 * n.z - gm1 * w * vn;
+  J[9] = gm1 * n.x;
+  J[14] = gm1 * n.y;
+  J[19] = gm1 * n.z;
+  J[24] = gamma * vn;
+  double f = 0.5 * a.m;
+  for (int q = 0; q < 25; ++q) J[q] *= f;
+  for (int r = 0; r < 5; ++r)
+    for (int cc = 0; cc < 5; ++cc) {
+      double dis = (r == cc) ? 1.0 * sr : 0.0;
+      J[r * 5 + cc] = positive ? J[r * 5 + cc] + dis : J[r * 5 + cc] - dis;
+    }
+}
+void orc_rusanov_jacobian(const orc_gas *g, const double *W, const double *area, int positive,
+                          double *J) {
+  uv3 a = {V(area[0], area[1], area[2]), area[3]};
+  rusanov_jacobian(g, W, a, positive, J);
+}
+
+/* ArrayMultiplication, block branch (include/fluxJacobian.hpp:50-86):
+ * out[r] = sum over c in order of M[r][c] v[c], from +0 */
+static void block_mult(const double *M, const double *v, double *out) {
+  for (int r = 0; r < 5; ++r) {
+    double s = 0.0;
+    for (int cc = 0; cc < 5; ++cc) s += M[r * 5 + cc] * v[cc];
+    out[r] = s;
+  }
+}
+
+/* LinearCombRow (src/matrix.cpp:119-132): row r2 -= factor * row r1 */
+static void lin_comb_row(double *m, int n, int r1, double factor, int r2) {
+  for (int cc = 0; cc < n; ++cc) m[r2 * n + cc] = m[r2 * n + cc] - factor * m[r1 * n + cc];
+}
+static void swap_rows(double *m, int n, int r1, int r2) {
+  if (r1 == r2) return;
+  for (int cc = 0; cc < n; ++cc) {
+    double t = m[r1 * n + cc];
+    m[r1 * n + cc] = m[r2 * n + cc];
+    m[r2 * n + cc] = t;
+  }
+}
+
+/* MatrixInverse (src/matrix.cpp:57-105) with FindMaxInColumn (:145-165):
+ * the pivot of row r is the largest |m[i][r]|, i >= r, taken before row r's
+ * leading entries are eliminated (row 0 when every candidate vanishes); row r
+ * is then eliminated against rows 0..r-1, normalised from the pivot column
+ * on, and the upper triangle is cleared from the bottom up.  Returns -1 on a
+ * zero pivot (the reference exits). */
+static int gj_inverse(double *m, int n) {
+  double I[25];
+  for (int r = 0; r < n; ++r)
+    for (int cc = 0; cc < n; ++cc) I[r * n + cc] = (r == cc) ? 1.0 : 0.0;
+  for (int r = 0; r < n; ++r) {
+    int cp = r;
+    double mv = 0.0;
+    int pr = 0;
+    for (int ii = cp; ii <= n - 1; ++ii)
+      if (fabs(m[ii * n + r]) > mv) {
+        mv = fabs(m[ii * n + r]);
+        pr = ii;
+      }
+    swap_rows(m, n, r, pr);
+    swap_rows(I, n, r, pr);
+    if (r != 0)
+      for (int ii = 0; ii < cp; ++ii) {
+        double factor = m[r * n + ii] / m[ii * n + ii];
+        lin_comb_row(m, n, ii, factor, r);
+        lin_comb_row(I, n, ii, factor, r);
+      }
+    if (m[r * n + cp] == 0.0) return -1;
+    double nf = 1.0 / m[r * n + cp];
+    for (int cc = cp; cc < n; ++cc) m[r * n + cc] *= nf;
+    for (int cc = 0; cc < n; ++cc) I[r * n + cc] *= nf;
+  }
+  for (int r = n - 2; r >= 0; --r) {
+    int cp = r;
+    for (int ii = n - 1; ii > cp; --ii) {
+      double factor = m[r * n + ii];
+      lin_comb_row(m, n, ii, factor, r);
+      lin_comb_row(I, n, ii, factor, r);
+    }
+  }
+  memcpy(m, I, sizeof(double) * (size_t)(n * n));
+  return 0;
+}
+int orc_block_inverse(double *m) { return gj_inverse(m, 5); }
 
 /* ------------------------------------------------------------------------ */
 /* geometry: src/plot3d.cpp:36-359, 490-498                                 */
@@ -1444,6 +1572,8 @@ static int alloc_block(blk_t *b) {
   b->dt = (double *)calloc((size_t)np, sizeof(double));
   b->A = (double *)calloc((size_t)np, sizeof(double));
   b->Ainv = (double *)calloc((size_t)np, sizeof(double));
+  b->Ab = (double *)calloc((size_t)np * 25, sizeof(double));
+  b->Abi = (double *)calloc((size_t)np * 25, sizeof(double));
   b->consN = (double *)calloc((size_t)np * NEQ, sizeof(double));
   b->consNm1 = (double *)calloc((size_t)np * NEQ, sizeof(double));
   b->order = (int *)malloc(sizeof(int) * 3 * (size_t)np);
@@ -1595,8 +1725,11 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
     } else if (!strcmp(key, "turbulenceModel")) {
       if (strcmp(val, "none")) { seterr(err, errlen, "unsupported turbulenceModel %s", val); goto fail; }
     } else if (!strcmp(key, "matrixSolver")) {
-      if (!strcmp(val, "lusgs")) c->solver = 0;
-      else if (!strcmp(val, "dplur")) c->solver = 1;
+      /* src/input.cpp:713-720, 845-856 */
+      if (!strcmp(val, "lusgs")) c->solver = 0, c->block = 0;
+      else if (!strcmp(val, "dplur")) c->solver = 1, c->block = 0;
+      else if (!strcmp(val, "blusgs")) c->solver = 0, c->block = 1;
+      else if (!strcmp(val, "bdplur")) c->solver = 1, c->block = 1;
       else { seterr(err, errlen, "unsupported matrixSolver %s", val); goto fail; }
     } else if (!strcmp(key, "matrixSweeps")) c->sweeps = atoi(val);
     else if (!strcmp(key, "matrixRelaxation")) c->relax = strtod(val, NULL);
@@ -1692,6 +1825,8 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
   c->multilevel = (c->timeint == 3);
   c->storeN = c->implicit;
   c->initMatrix = (c->solver == 1 || c->sweeps > 1);
+  c->block = c->block && c->implicit; /* IsBlockMatrix: src/input.cpp:713-720 */
+  if (c->block && c->viscous) { seterr(err, errlen, "block matrix solvers with navierStokes are not supported"); goto fail2; }
   if (c->timeint == 0) c->nonlinear = 1; /* src/input.cpp:928-934 */
 
   /* reference speed of sound and nondimensional gas constant:
@@ -1887,7 +2022,7 @@ static void free_case(orc_case *c) {
     free(b->vol); free(b->ctr);
     for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
     free(b->W); free(b->x); free(b->T); free(b->mu); free(b->R); free(b->sr); free(b->dt);
-    free(b->A); free(b->Ainv); free(b->consN); free(b->consNm1);
+    free(b->A); free(b->Ainv); free(b->Ab); free(b->Abi); free(b->consN); free(b->consNm1);
     free(b->order); free(b->surf); free(b->conn);
     for (int q = 0; q < 6; ++q) free(b->isconn[q]);
   }
@@ -2505,6 +2640,14 @@ static void inv_flux_dir(orc_case *c, blk_t *b, int d) {
         if (f3 > 0) {
           double *R = &b->R[pidx(b, i - ei, j - ej, k - ek) * NEQ];
           for (int e = 0; e < NEQ; ++e) R[e] += F[e] * a.m;
+          /* block main diagonal: the lower cell adds the Rusanov Jacobian of
+           * the face's lower state (src/procBlock.cpp:449-457) */
+          if (c->block) {
+            double J[25];
+            rusanov_jacobian(&c->gas, WL, a, 1, J);
+            double *Ab = &b->Ab[pidx(b, i - ei, j - ej, k - ek) * 25];
+            for (int q = 0; q < 25; ++q) Ab[q] = Ab[q] + J[q];
+          }
         }
         if (f3 < n3) {
           double *R = &b->R[pidx(b, i, j, k) * NEQ];
@@ -2512,7 +2655,15 @@ static void inv_flux_dir(orc_case *c, blk_t *b, int d) {
           uv3 aU = b->fa[d][fidx(b, d, i + ei, j + ej, k + ek)];
           double lam = inv_cell_specrad(&c->gas, W0, a, aU);
           b->sr[pidx(b, i, j, k)] += lam;
-          if (c->implicit) b->A[pidx(b, i, j, k)] += lam;
+          if (c->block) {
+            /* the upper cell subtracts that of the upper state (:480-486) */
+            double J[25];
+            rusanov_jacobian(&c->gas, WR, a, 0, J);
+            double *Ab = &b->Ab[pidx(b, i, j, k) * 25];
+            for (int q = 0; q < 25; ++q) Ab[q] = Ab[q] - J[q];
+          } else if (c->implicit) {
+            b->A[pidx(b, i, j, k)] += lam;
+          }
         }
       }
 }
@@ -2559,6 +2710,19 @@ void orc_invert_diagonal(orc_case *c) {
           long p = pidx(b, i, j, k);
           double dvt = (b->vol[cidx(b, i, j, k)] * (1.0 + c->zeta)) / (b->dt[p] * c->theta);
           if (c->dualCFL > 0.0) dvt += dmax(b->sr[p], 0.0) / c->dualCFL;
+          if (c->block) {
+            /* MultiplyFacOnDiagonal / AddFacOnDiagonal (src/matrix.cpp:
+             * 201-225), then MatrixInverse of a copy */
+            double *Ab = &b->Ab[p * 25], *Abi = &b->Abi[p * 25];
+            for (int q = 0; q < 5; ++q) Ab[q * 6] *= c->relax;
+            for (int q = 0; q < 5; ++q) Ab[q * 6] += dvt;
+            memcpy(Abi, Ab, sizeof(double) * 25);
+            if (gj_inverse(Abi, 5)) {
+              fprintf(stderr, "ERROR: Singular matrix in Gauss-Jordan elimination!\n");
+              abort();
+            }
+            continue;
+          }
           b->A[p] *= c->relax;
           b->A[p] += dvt;
           b->Ainv[p] = 1.0 * (1.0 / b->A[p]);
@@ -2567,8 +2731,10 @@ void orc_invert_diagonal(orc_case *c) {
 }
 
 void orc_reset_diagonal(orc_case *c) {
-  for (int bq = 0; bq < c->nb; ++bq)
+  for (int bq = 0; bq < c->nb; ++bq) {
     memset(c->blk[bq].A, 0, sizeof(double) * (size_t)ncells(&c->blk[bq]));
+    memset(c->blk[bq].Ab, 0, sizeof(double) * 25 * (size_t)ncells(&c->blk[bq]));
+  }
 }
 
 /* SolDeltaMmN / SolDeltaNm1: src/procBlock.cpp:1010-1034; the rhs
@@ -2618,6 +2784,14 @@ static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
     long cn = cidx(b, ni, nj, nk);
     /* FaceSpectralRadius with the viscous part (include/spectralRadius.hpp:
      * 192-203): the neighbour's state and viscosity, ProjC2CDist */
+    if (c->block) {
+      /* RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194) */
+      double J[25];
+      rusanov_jacobian(&c->gas, &b->W[cn * NEQ], a, !upper, J);
+      block_mult(J, &x[cn * NEQ], o);
+      for (int e = 0; e < NEQ; ++e) out[e] += o[e];
+      continue;
+    }
     double lv = c->viscous ? visc_face_specrad(c, &b->W[cn * NEQ], a, proj_c2c(b, d, fi, fj, fk), b->mu[cn])
                            : 0.0;
     scalar_offdiag_v(&c->gas, &b->W[cn * NEQ], &x[cn * NEQ], a, !upper, c->viscous, lv, o);
@@ -2637,7 +2811,9 @@ void orc_init_update(orc_case *c) {
           double rb[NEQ];
           rhs_b(c, b, i, j, k, 0, rb);
           long p = pidx(b, i, j, k);
-          for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = rb[e] * b->Ainv[p];
+          if (c->block) block_mult(&b->Abi[p * 25], rb, &b->x[cidx(b, i, j, k) * NEQ]);
+          else
+            for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = rb[e] * b->Ainv[p];
         }
   }
 }
@@ -2655,6 +2831,12 @@ static void lusgs_forward(orc_case *c, blk_t *b, int sweep) {
     }
     rhs_b(c, b, i, j, k, 1, rb);
     long p = pidx(b, i, j, k);
+    if (c->block) {
+      double v[NEQ];
+      for (int e = 0; e < NEQ; ++e) v[e] = rb[e] + L[e];
+      block_mult(&b->Abi[p * 25], v, &b->x[cidx(b, i, j, k) * NEQ]);
+      continue;
+    }
     for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = (rb[e] + L[e]) * b->Ainv[p];
   }
 }
@@ -2671,7 +2853,17 @@ static void lusgs_backward(orc_case *c, blk_t *b, int sweep) {
       double L[NEQ], rb[NEQ];
       implicit_lu(c, b, i, j, k, b->x, 0, L);
       rhs_b(c, b, i, j, k, 1, rb);
-      for (int e = 0; e < NEQ; ++e) xc[e] = ((rb[e] + L[e]) - U[e]) * b->Ainv[p];
+      if (c->block) {
+        double v[NEQ];
+        for (int e = 0; e < NEQ; ++e) v[e] = (rb[e] + L[e]) - U[e];
+        block_mult(&b->Abi[p * 25], v, xc);
+      } else {
+        for (int e = 0; e < NEQ; ++e) xc[e] = ((rb[e] + L[e]) - U[e]) * b->Ainv[p];
+      }
+    } else if (c->block) {
+      double t[NEQ];
+      block_mult(&b->Abi[p * 25], U, t);
+      for (int e = 0; e < NEQ; ++e) xc[e] = xc[e] - t[e];
     } else {
       for (int e = 0; e < NEQ; ++e) xc[e] = xc[e] - U[e] * b->Ainv[p];
     }
@@ -2692,6 +2884,12 @@ static void dplur_sweep(orc_case *c, blk_t *b) {
         for (int e = 0; e < NEQ; ++e) L[e] -= U[e];
         rhs_b(c, b, i, j, k, 0, rb);
         long p = pidx(b, i, j, k);
+        if (c->block) {
+          double v[NEQ];
+          for (int e = 0; e < NEQ; ++e) v[e] = (rb[e] + 0.0) + L[e];
+          block_mult(&b->Abi[p * 25], v, &b->x[cidx(b, i, j, k) * NEQ]);
+          continue;
+        }
         for (int e = 0; e < NEQ; ++e)
           b->x[cidx(b, i, j, k) * NEQ + e] = ((rb[e] + 0.0) + L[e]) * b->Ainv[p];
       }
@@ -2715,8 +2913,10 @@ static double matrix_residual(orc_case *c) {
           for (int e = 0; e < NEQ; ++e) L[e] -= U[e];
           rhs_b(c, b, i, j, k, 0, rb);
           long p = pidx(b, i, j, k);
+          double axb[NEQ];
+          if (c->block) block_mult(&b->Ab[p * 25], &b->x[cidx(b, i, j, k) * NEQ], axb);
           for (int e = 0; e < NEQ; ++e) {
-            double ax = b->x[cidx(b, i, j, k) * NEQ + e] * b->A[p];
+            double ax = c->block ? axb[e] : b->x[cidx(b, i, j, k) * NEQ + e] * b->A[p];
             double axmb = (ax - L[e]) - rb[e];
             double r = 0.0 - axmb;
             part += r * r;
@@ -2894,6 +3094,15 @@ void orc_get_specrad(const orc_case *c, int bq, double *out) {
 void orc_get_diag(const orc_case *c, int bq, double *out) {
   memcpy(out, c->blk[bq].A, sizeof(double) * (size_t)ncells(&c->blk[bq]));
 }
+void orc_get_block_diag(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->Ab, sizeof(double) * 25 * (size_t)ncells(b));
+}
+void orc_get_block_inverse(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->Abi, sizeof(double) * 25 * (size_t)ncells(b));
+}
+int orc_is_block(const orc_case *c) { return c->block; }
 void orc_get_update(const orc_case *c, int bq, double *out) {
   const blk_t *b = &c->blk[bq];
   for (int k = 0; k < b->nk; ++k)

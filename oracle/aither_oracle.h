@@ -89,6 +89,11 @@ void orc_get_dt(const orc_case *c, int b, double *out);
 void orc_get_specrad(const orc_case *c, int b, double *out);
 void orc_get_diag(const orc_case *c, int b, double *out);      /* A (scalar) */
 void orc_get_update(const orc_case *c, int b, double *out);    /* du, neq/cell */
+/* blusgs / bdplur: main diagonal after AddDiagonalTerms and its inverse,
+ * 25 per cell, row-major */
+void orc_get_block_diag(const orc_case *c, int b, double *out);
+void orc_get_block_inverse(const orc_case *c, int b, double *out);
+int orc_is_block(const orc_case *c);
 void orc_set_update(orc_case *c, int b, const double *in);
 void orc_get_volume(const orc_case *c, int b, double *out);
 void orc_get_center(const orc_case *c, int b, double *out);    /* 3/cell */
@@ -130,6 +135,12 @@ double orc_inv_cell_specrad(const orc_gas *g, const double *W,
 void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
                         const double *area /*ux,uy,uz,mag*/, int positive,
                         double *out);
+/* Rusanov flux Jacobian 0.5|A| dF/dU +/- spectral radius I, 25 row-major */
+void orc_rusanov_jacobian(const orc_gas *g, const double *W,
+                          const double *area /*ux,uy,uz,mag*/, int positive,
+                          double *J);
+/* Gauss-Jordan inverse of a 5x5 row-major matrix in place; -1 if singular */
+int orc_block_inverse(double *m);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,13 @@ def lib():
         L.orc_inv_cell_specrad.argtypes = [G, _D, _D, _D]
         L.orc_inv_cell_specrad.restype = ctypes.c_double
         L.orc_scalar_offdiag.argtypes = [G, _D, _D, _D, ctypes.c_int, _D]
+        L.orc_rusanov_jacobian.argtypes = [G, _D, _D, ctypes.c_int, _D]
+        L.orc_block_inverse.argtypes = [_D]
+        L.orc_block_inverse.restype = ctypes.c_int
+        for f in ("orc_get_block_diag", "orc_get_block_inverse"):
+            getattr(L, f).argtypes = [_V, ctypes.c_int, _D]
+        L.orc_is_block.argtypes = [_V]
+        L.orc_is_block.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -175,6 +182,15 @@ class OracleCase:
 
     def diagonal(self, b=0):
         return self._get(self.L.orc_get_diag, b, 1)
+
+    def block_diagonal(self, b=0):
+        return self._get(self.L.orc_get_block_diag, b, 25).reshape(-1, 5, 5)
+
+    def block_inverse(self, b=0):
+        return self._get(self.L.orc_get_block_inverse, b, 25).reshape(-1, 5, 5)
+
+    def is_block(self):
+        return bool(self.L.orc_is_block(self.h))
 
     def update_vector(self, b=0):
         return self._get(self.L.orc_get_update, b, 5)

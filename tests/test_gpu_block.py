@@ -1,0 +1,158 @@
+"""Block-implicit solvers (matrixSolver blusgs / bdplur) on the GPU against
+the FP64 oracle, through the C ABI.
+
+Bars: the block main diagonal and its Gauss-Jordan inverse equal the
+oracle's bit for bit where the reconstruction calls no std::pow (constant,
+MUSCL), else within 1e-12 relative; per-step residuals within 1e-12 (the
+bar of the scalar path) and the state after the run within 1e-12.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import aither_amd
+from aither_amd import Solver, cases
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _variant(path, tag, **repl):
+    text = open(path).read()
+    for a, b in repl.items():
+        assert a in text, a
+        text = text.replace(a, b)
+    out = path.replace(".inp", f"_{tag}.inp")
+    with open(out, "w") as f:
+        f.write(text)
+    return out
+
+
+def _bump(tmp, recon="thirdOrder", solver="blusgs", sweeps=1, timeint="implicitEuler", cfl=20.0, its=6):
+    p = cases.bump_case(str(tmp), seed=11, n=10, iterations=its, cfl=cfl, recon=recon)[0]
+    return _variant(p, f"{solver}{sweeps}{timeint}",
+                    **{"matrixSolver: lusgs": f"matrixSolver: {solver}",
+                       "matrixSweeps: 1": f"matrixSweeps: {sweeps}",
+                       "timeIntegration: implicitEuler": f"timeIntegration: {timeint}"})
+
+
+@pytest.mark.parametrize("recon,flux", [("constant", "roe"), ("thirdOrder", "roe"), ("thirdOrder", "ausm"),
+                                        ("weno", "roe")])
+def test_block_diagonal_and_inverse_match_oracle(tmp_path, recon, flux):
+    path = _bump(tmp_path, recon=recon)
+    if flux == "ausm":
+        path = _variant(path, "ausm", **{"limiter: vanAlbada": "limiter: vanAlbada\ninviscidFlux: ausm"})
+    say("block diagonal", recon, flux)
+    s = Solver(path)
+    assert s.info.block_matrix == 1
+    o = pyoracle.OracleCase(path)
+    assert o.is_block()
+    s.begin_step(0)
+    o.begin_step(0)
+    s.assign_ghost_cells()
+    s.calc_residual()
+    o.bc()
+    o.calc_residual()
+    o.time_step()
+    o.invert_diagonal()
+    A, Ai = s.block_diagonal(), s.block_diagonal(inverse=True)
+    Ao, Aio = o.block_diagonal(), o.block_inverse()
+    assert rel(s.residual(), o.residual()) <= 1e-12
+    if recon == "weno":
+        assert rel(A, Ao) <= 1e-12 and rel(Ai, Aio) <= 1e-12
+    else:
+        assert np.array_equal(A, Ao)
+        assert np.array_equal(Ai, Aio)
+    with pytest.raises(aither_amd.AitherError):
+        s.diagonal()
+    s.close()
+
+
+@pytest.mark.parametrize("solver,sweeps,timeint", [("blusgs", 1, "implicitEuler"), ("blusgs", 2, "implicitEuler"),
+                                                   ("bdplur", 3, "implicitEuler"), ("blusgs", 1, "bdf2"),
+                                                   ("bdplur", 2, "crankNicholson")])
+def test_block_solvers_step_parity(tmp_path, solver, sweeps, timeint):
+    path = _bump(tmp_path, solver=solver, sweeps=sweeps, timeint=timeint)
+    if timeint != "implicitEuler":
+        path = _variant(path, "nl", **{"matrixRelaxation: 1.0": "matrixRelaxation: 1.0\nnonlinearIterations: 2"})
+    say("block solver", solver, sweeps, timeint)
+    s = Solver(path)
+    s.set_matrix_residual(1)
+    o = pyoracle.OracleCase(path)
+    o.set_matrix_residual(1)
+    for nn in range(5):
+        s.begin_step(nn)
+        o.begin_step(nn)
+        for mm in range(o.nonlinear):
+            ns = s.iterate(nn, mm)
+            no = o.iterate(nn, mm)
+            assert rel(s.residual(), o.residual()) <= 1e-12, (nn, mm)
+            assert abs(ns.matrix_resid - no["matrix_resid"]) <= 1e-10 * no["matrix_resid"], (nn, mm)
+    assert rel(s.get_state(), o.state()) <= 1e-12
+    s.close()
+
+
+def test_block_solver_ignores_sweep_ordering_and_refuses_fp32(tmp_path):
+    path = _bump(tmp_path)
+    out = []
+    for order in ("PENCIL", "TILED", "HYPERPLANE"):
+        say("block ordering", order)
+        s = Solver(path)
+        s.set_sweep_ordering(getattr(Solver, order))
+        for nn in range(3):
+            s.begin_step(nn)
+            s.iterate(nn, 0)
+        out.append(s.get_state())
+        if order == "HYPERPLANE":
+            with pytest.raises(aither_amd.AitherError):
+                s.set_implicit_precision(1)
+        s.close()
+    assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[2])
+
+
+def test_block_graph_run_equals_iterate(tmp_path):
+    path = _bump(tmp_path, solver="blusgs", sweeps=2)
+    say("block graph replay")
+    a = Solver(path)
+    for nn in range(4):
+        a.begin_step(nn)
+        a.iterate(nn, 0)
+    b = Solver(path)
+    b.run(0, 4)
+    assert np.array_equal(a.get_state(), b.get_state())
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("solver,sweeps", [("blusgs", 1), ("bdplur", 2)])
+def test_block_solvers_across_connections(tmp_path, solver, sweeps):
+    import conn_cases as cc
+    r0, r1 = cc.orientation_set()[211]
+    path, _ = cc.write_pair(str(tmp_path), "cut", r0, r1, timeint="implicitEuler", solver=solver, sweeps=sweeps)
+    say("block connections", solver, sweeps)
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    for nn in range(4):
+        s.begin_step(nn)
+        o.begin_step(nn)
+        ns = s.iterate(nn, 0)
+        no = o.iterate(nn, 0)
+        Rs = np.concatenate([s.residual(b) for b in range(o.nb)])
+        Ro = np.concatenate([o.residual(b) for b in range(o.nb)])
+        assert rel(Rs, Ro) <= 1e-12, (nn, rel(Rs, Ro))
+    Ws = np.concatenate([s.get_state(b) for b in range(o.nb)])
+    Wo = np.concatenate([o.state(b) for b in range(o.nb)])
+    assert rel(Ws, Wo) <= 1e-12
+    s.close()

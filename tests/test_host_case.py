@@ -80,3 +80,28 @@ def test_errors_are_reported_not_fatal(tmp_path):
     with pytest.raises(aither_amd.AitherError) as e:
         inspect_info(bad)
     assert e.value.code == -2 and "rans" in str(e.value)
+
+
+@pytest.mark.parametrize("solver,timeint,block", [("blusgs", "implicitEuler", 1), ("bdplur", "bdf2", 1),
+                                                  ("blusgs", "explicitEuler", 0), ("lusgs", "implicitEuler", 0)])
+def test_block_matrix_selection(tmp_path, solver, timeint, block):
+    """input::IsBlockMatrix (src/input.cpp:713-720): blusgs and bdplur give a
+    block matrix only with an implicit time integration, like the oracle"""
+    import pyoracle
+    inp = cases.vortex_case(str(tmp_path), n=4, iterations=1, solver=solver)
+    text = open(inp).read().replace("timeIntegration: implicitEuler", f"timeIntegration: {timeint}")
+    open(inp, "w").write(text)
+    assert inspect_info(inp).block_matrix == block
+    assert pyoracle.OracleCase(inp).is_block() == bool(block)
+
+
+def test_block_matrix_with_viscous_terms_is_refused(tmp_path):
+    import pyoracle
+    inp = cases.bump_case(str(tmp_path), seed=1, n=4, iterations=1, viscous=True)[0]
+    text = open(inp).read().replace("matrixSolver: lusgs", "matrixSolver: blusgs")
+    open(inp, "w").write(text)
+    with pytest.raises(aither_amd.AitherError) as e:
+        inspect_info(inp)
+    assert e.value.code == -2 and "block" in str(e.value)
+    with pytest.raises(RuntimeError, match="block"):
+        pyoracle.OracleCase(inp)
