@@ -206,3 +206,109 @@ boundaryConditions: 1
     with open(path, "w") as f:
         f.write(inp)
     return path, dict(n=n, h_over_l=hl, mach=mach)
+
+
+def channel_case(outdir, name="channel", refine=1.0, steps=50, subits=20, dt=1.0e-5, seed=0,
+                 solver="blusgs", sweeps=1):
+    """Batched multiblock laminar channel: 8 point-matched blocks in a
+    4 (streamwise) x 2 (wall-normal) layout, columns of 96, 96, 96 and 48
+    cells, a 64-cell wall row under a 32-cell row, 32 cells spanwise
+    (1,032,192 cells at refine = 1; every count is scaled by `refine`).
+    Wall-normal geometric stretching 1.1 from a first cell of 1e-5 L, a 4 L
+    long, 0.5 L wide channel; no-slip adiabatic bottom wall, characteristic
+    top, inflow and outflow, spanwise periodic.  Mach 0.3, Re_L = 1e6 at
+    288.15 K with Sutherland air (the density follows from Re); thermal
+    conductivity refit so that k / mu = c_p / 0.72 (a fluid file beside the
+    input; the spectral radius keeps the reference's 4 gamma / (9 gamma - 5)).
+    AUSMPW+ with MUSCL (kappa = 1/3, van Albada), central viscous flux,
+    BLU-SGS, BDF2 with `subits` nonlinear iterations per physical step of
+    `dt` seconds; uniform free stream plus seeded 1e-3 random velocity
+    perturbations.  Returns the input path and a dict of the sizes."""
+    rng = np.random.default_rng(seed)
+    os.makedirs(outdir, exist_ok=True)
+    cols = [max(2, int(round(c * refine))) for c in (96, 96, 96, 48)]
+    rows = [max(2, int(round(r * refine))) for r in (64, 32)]
+    nk = max(2, int(round(32 * refine)))
+    Lx, Lz, dy1, ratio = 4.0, 0.5, 1.0e-5, 1.1
+    nx = sum(cols)
+    xs = Lx * np.arange(nx + 1) / nx
+    ys = np.concatenate([[0.0], np.cumsum(dy1 * ratio ** np.arange(sum(rows)))])
+    zs = Lz * np.arange(nk + 1) / nk
+    R = air_gas_constant()
+    gamma = (AIR_N + 1.0) / AIR_N
+    t_inf = 288.15
+    a_inf = math.sqrt(gamma * R * t_inf)
+    u_inf = 0.3 * a_inf
+    mu_inf = 1.458e-6 * t_inf ** 1.5 / (t_inf + 110.4)
+    rho_inf = 1.0e6 * mu_inf / (u_inf * 1.0)
+    p_inf = rho_inf * R * t_inf
+    cp = R * (AIR_N + 1.0)
+    with open(os.path.join(outdir, "air72.dat"), "w") as f:
+        f.write(f"""n: {AIR_N}
+molarMass: {AIR_MOLAR * 1000.0!r}
+sutherlandViscosityC1: 1.458e-6
+sutherlandViscosityS: 110.4
+sutherlandConductivityC1: {1.458e-6 * cp / 0.72!r}
+sutherlandConductivityS: 110.4
+""")
+    blocks, bcs, cents = [], [], []
+    x0s = np.concatenate([[0], np.cumsum(cols)])
+    y0s = np.concatenate([[0], np.cumsum(rows)])
+
+    def bid(c, r):
+        return c + 4 * r
+
+    for r in range(2):
+        for c in range(4):
+            ii = np.arange(x0s[c], x0s[c + 1] + 1)
+            jj = np.arange(y0s[r], y0s[r + 1] + 1)
+            X, Y, Z = np.meshgrid(xs[ii], ys[jj], zs, indexing="ij")
+            blocks.append((X, Y, Z))
+            cents.append(cell_centres(X, Y, Z))
+            types = {
+                "imin": ("characteristic", 1) if c == 0 else ("interblock", 2000 + bid(c - 1, r)),
+                "imax": ("characteristic", 1) if c == 3 else ("interblock", 1000 + bid(c + 1, r)),
+                "jmin": ("viscousWall", 3) if r == 0 else ("interblock", 4000 + bid(c, 0)),
+                "jmax": ("characteristic", 1) if r == 1 else ("interblock", 3000 + bid(c, 1)),
+                "kmin": ("periodic", 7), "kmax": ("periodic", 8),
+            }
+            bcs.append(_surfaces_box(cols[c], rows[r], nk, types))
+    write_plot3d(os.path.join(outdir, name + ".xyz"), blocks)
+    # cloud: every block's cell centres, block order, i fastest
+    xc = np.concatenate([cc[0].transpose(2, 1, 0).reshape(-1) for cc in cents])
+    yc = np.concatenate([cc[1].transpose(2, 1, 0).reshape(-1) for cc in cents])
+    zc = np.concatenate([cc[2].transpose(2, 1, 0).reshape(-1) for cc in cents])
+    n = xc.size
+    pert = 1.0e-3 * u_inf * rng.uniform(-1.0, 1.0, (3, n))
+    with open(os.path.join(outdir, name + ".cloud"), "w") as f:
+        f.write(f"{n}\nair72\n")
+        data = np.stack([xc, yc, zc, np.full(n, rho_inf), u_inf + pert[0], pert[1], pert[2], np.full(n, p_inf),
+                         np.zeros(n), np.zeros(n), np.ones(n)], axis=1)
+        np.savetxt(f, data, fmt="%.17g")
+    inp = f"""# multiblock laminar channel, refine {refine}, seed {seed}
+gridName: {name}
+equationSet: navierStokes
+timeIntegration: bdf2
+timeStep: {dt!r}
+nonlinearIterations: {subits}
+iterations: {steps}
+faceReconstruction: thirdOrder
+limiter: vanAlbada
+inviscidFlux: ausm
+viscousFaceReconstruction: central
+referenceTemperature: {t_inf!r}
+referenceDensity: {rho_inf!r}
+referenceLength: 1.0
+fluids: <fluid(name=air72; referenceMassFraction=1.0)>
+initialConditions: <icState(tag=-1; file={name}.cloud)>
+matrixSolver: {solver}
+matrixSweeps: {sweeps}
+matrixRelaxation: 1.0
+boundaryStates: <characteristic(tag=1; pressure={p_inf!r}; density={rho_inf!r}; velocity=[{u_inf!r}, 0.0, 0.0]), viscousWall(tag=3), periodic(startTag=7; endTag=8; translation=[0.0, 0.0, {Lz!r}])>
+boundaryConditions: 8
+""" + "".join(bcs)
+    path = os.path.join(outdir, name + ".inp")
+    with open(path, "w") as f:
+        f.write(inp)
+    cells = sum(cc * rr * nk for cc in cols for rr in rows)
+    return path, dict(cells=cells, blocks=8, cols=cols, rows=rows, nk=nk, steps=steps, subits=subits)

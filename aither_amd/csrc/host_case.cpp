@@ -805,7 +805,6 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (c.rRef <= 0 || c.tRef <= 0) return "referenceDensity and referenceTemperature are required";
   if (c.timeint == TimeInt::ExplicitEuler) c.nonlinear = 1;  // src/input.cpp:928-934
   c.block = c.block && c.Implicit();  // input::IsBlockMatrix (src/input.cpp:713-720)
-  if (c.block && c.viscous) return "block matrix solvers with navierStokes are not supported";
   double fn = 0, fm = 0;
   if (!FluidProps(c.dir, fluid, &fn, &fm, &c.suth)) return "unknown fluid " + fluid;
 

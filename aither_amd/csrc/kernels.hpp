@@ -43,6 +43,7 @@ struct DevBlock {
   const double *volP;           // padded volumes
   const double *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
   const unsigned char *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
+  double *vgrad;                // block solvers: padded cell velocity gradient, 9 * npad, d v_c / d x_r at (3 r + c) npad
   long cellBase;         // running cell offset of this block (L-inf index)
   int blockId;
 };
@@ -438,73 +439,6 @@ __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
   CellTail(b, P, p, b.vol[p], sr, sr, cflp);
 }
 
-// Block main diagonal (src/procBlock.cpp:449-486, 588-625, 726-763):
-// every face adds the Rusanov Jacobian of its lower state to its lower cell
-// and subtracts that of its upper state from its upper cell, faces in i, j, k
-// order, so a cell sees -J(face i-) +J(face i+) -J(j-) +J(j+) -J(k-) +J(k+)
-// from a zero matrix.  Then AddDiagonalTerms (src/linearSolver.cpp:146-171)
-// and the Gauss-Jordan inverse (src/linearSolver.cpp:173-184).  One thread per
-// cell, after the residual kernel has written sr and dt.  A zero pivot sets
-// bit 4 of `flag`.
-template <int RECON, int LIM>
-__global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *flag) {
-#pragma clang fp contract(off)
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= b.ncell) return;
-  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
-  const GasT<double> g = Cast<double>(P.gas);
-  double D[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) D[q] = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double WL[5], WR[5], J[25];
-    FaceStates<RECON, LIM>(b, P.kappa, d, i, j, k, WL, WR);
-    double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
-    RusanovJacobian<double>(g, WR, a.x, a.y, a.z, a.w, false, J);
-#pragma unroll
-    for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
-    const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
-    FaceStates<RECON, LIM>(b, P.kappa, d, ui, uj, uk, WL, WR);
-    a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
-    RusanovJacobian<double>(g, WL, a.x, a.y, a.z, a.w, true, J);
-#pragma unroll
-    for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
-  }
-  const double sr = b.sr[p];
-  double dvt = (b.vol[p] * (1.0 + P.zeta)) / (b.dt[p] * P.theta);
-  if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) D[q * 6] *= P.relax;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) D[q * 6] += dvt;
-#pragma unroll
-  for (int q = 0; q < 25; ++q) b.Ab[q * b.ncell + p] = D[q];
-  double I[25];
-  if (!GJInverse5<double>(D, I)) atomicOr(flag, 4);
-#pragma unroll
-  for (int q = 0; q < 25; ++q) b.Abi[q * b.ncell + p] = I[q];
-}
-
-// block product with the cell's stored matrix (Ab or Abi)
-template <typename T>
-__device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
-  T m[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) m[q] = T(M[q * b.ncell + p]);
-  BlockMult<T>(m, v, out);
-}
-
-// RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194), inviscid
-template <typename T>
-__device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], const T du[5], double4 a,
-                                             bool positive, T out[5]) {
-  T J[25];
-  RusanovJacobian<T>(g, W, T(a.x), T(a.y), T(a.z), T(a.w), positive, J);
-  BlockMult<T>(J, du, out);
-}
-
-
 // ---------------------------------------------------------------------------
 // laminar viscous residual: procBlock::CalcResidualNoSource's viscous branch
 // (src/procBlock.cpp:6126-6138)
@@ -665,19 +599,12 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
 // viscousFlux::CalcFlux / CalcWallFlux (src/viscousFlux.cpp:58-188),
 // TauNormal (src/utility.cpp:426-436); the wall form has no +0 species term
 template <bool R4>
-__device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P, int d, int i, int j, int k,
-                                             double F[5]) {
+__device__ __forceinline__ void ViscFaceState(const DevBlock &b, int d, int i, int j, int k, double Wf[5],
+                                              double &muF) {
 #pragma clang fp contract(off)
-  const GasT<double> g = Cast<double>(P.gas);
-  double vg[3][3], tg[3];
-  FaceGrads(b, d, i, j, k, vg, tg);
-  const int n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
-  const int f3 = d == 0 ? i : (d == 1 ? j : k);
-  const bool wall = (f3 == 0 || f3 == n3) && b.wallf[BtSlot(b, d, f3 == 0 ? 0 : 1, i, j, k)];
   const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
   const long cU = PadIdx(b, i, j, k), cL = cU - s;
   const double *cw = b.cw + d * b.npad;
-  double Wf[5], muF;
   if (!R4) {
     const double w[2] = {cw[cL], cw[cU]};
     const double c0 = LagrangeCoeffN<2, 1, 0, 0>(w, 0), c1 = LagrangeCoeffN<2, 1, 0, 0>(w, 1);
@@ -697,6 +624,20 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
     }
     muF = c[0] * b.mu[cLL] + c[1] * b.mu[cL] + c[2] * b.mu[cU] + c[3] * b.mu[cUU];
   }
+}
+
+template <bool R4>
+__device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P, int d, int i, int j, int k,
+                                             double F[5]) {
+#pragma clang fp contract(off)
+  const GasT<double> g = Cast<double>(P.gas);
+  double vg[3][3], tg[3];
+  FaceGrads(b, d, i, j, k, vg, tg);
+  const int n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
+  const int f3 = d == 0 ? i : (d == 1 ? j : k);
+  const bool wall = (f3 == 0 || f3 == n3) && b.wallf[BtSlot(b, d, f3 == 0 ? 0 : 1, i, j, k)];
+  double Wf[5], muF;
+  ViscFaceState<R4>(b, d, i, j, k, Wf, muF);
   const double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
   const double n[3] = {a.x, a.y, a.z};
   const double mu = P.visc.vscale * muF, mut = P.visc.vscale * 0.0;
@@ -719,6 +660,88 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
   F[2] = tau[1] * a.w;
   F[3] = tau[2] * a.w;
   F[4] = (wall ? tv + q : tv + q + 0.0) * a.w;
+}
+
+// fluxJacobian::ApproxTSLJacobian (include/fluxJacobian.hpp:666-758) for a
+// laminar single species (diffusion coefficient 0, mass fraction 1), times
+// DelprimitiveDelConservative (:612-662) on the right (MatrixMultiply,
+// src/matrix.cpp:186-199).  `left` gives the shear term the lower cell's
+// sign; vg is the velocity gradient d v_c / d x_r at [3 r + c].
+__device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscParams &vp, const double W[5],
+                                            double muLam, double4 a, double dist, bool left, const double vg[9],
+                                            double out[25]) {
+#pragma clang fp contract(off)
+  const double t = TempOf(g, W[0], W[4]);
+  const double mu = vp.vscale * muLam, mut = vp.vscale * 0.0;
+  const double nx = a.x, ny = a.y, nz = a.z;
+  const double u = W[1], v = W[2], w = W[3];
+  const double vn = (u * nx + v * ny) + w * nz;
+  const double rho = 0.0 + W[0];
+  const double k = SutherlandK(vp, t) * vp.vscale;
+  const double kt = mut * g.cps / 0.9;
+  // TauNormal (src/utility.cpp:426-436)
+  const double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
+  const double trace = vg[0] + vg[4] + vg[8];
+  const double n[3] = {nx, ny, nz};
+  double tau[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double s0 = vg[r * 3 + 0] + vg[r], s1 = vg[r * 3 + 1] + vg[3 + r], s2 = vg[r * 3 + 2] + vg[6 + r];
+    const double sa = s0 * n[0] + s1 * n[1] + s2 * n[2];
+    tau[r] = lambda * trace * n[r] + (mu + mut) * sa;
+  }
+  const double fac = left ? -1.0 : 1.0;
+  const double third = 1.0 / 3.0;
+  const double mr = (mu + mut) * rho;
+  double J[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) J[q] = 0.0;
+  J[0] = (0.0 + 0.0) * (1.0 - W[0] / rho) / mr;
+  J[20] = -(k + kt) * t / mr + J[0] * (g.cps * t);
+  const double sh = fac * 0.5 * dist / (mu + mut);
+  J[6] = third * nx * nx + 1.0;
+  J[11] = third * nx * ny;
+  J[16] = third * nx * nz;
+  J[21] = sh * tau[0] + third * nx * vn + u;
+  J[7] = third * ny * nx;
+  J[12] = third * ny * ny + 1.0;
+  J[17] = third * ny * nz;
+  J[22] = sh * tau[1] + third * ny * vn + v;
+  J[8] = third * nz * nx;
+  J[13] = third * nz * ny;
+  J[18] = third * nz * nz + 1.0;
+  J[23] = sh * tau[2] + third * nz * vn + w;
+  J[24] = (k + kt) / mr;
+  const double f = a.w * (mu + mut) / dist;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) J[q] *= f;
+  // DelprimitiveDelConservative
+  const double gm1 = g.gamma - 1.0;
+  const double invRho = 1.0 / rho;
+  double Pm[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) Pm[q] = 0.0;
+  Pm[0] = 1.0;
+  Pm[5] = -invRho * u;
+  Pm[10] = -invRho * v;
+  Pm[15] = -invRho * w;
+  Pm[20] = 0.5 * gm1 * ((u * u + v * v) + w * w);
+  Pm[6] = invRho;
+  Pm[21] = -gm1 * u;
+  Pm[12] = invRho;
+  Pm[22] = -gm1 * v;
+  Pm[18] = invRho;
+  Pm[23] = -gm1 * w;
+  Pm[24] = gm1;
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc += J[r * 5 + q] * Pm[q * 5 + c];
+      out[r * 5 + c] = acc;
+    }
 }
 
 // procBlock::CalcViscFluxI/J/K (src/procBlock.cpp:1233-2135), laminar.  The
@@ -826,6 +849,127 @@ __device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, 
   RhsFrom(P, W, b.vol[p], b.dt[p], R, cN, cNm1, out);
 }
 
+// Block main diagonal (src/procBlock.cpp:449-486, 588-625, 726-763):
+// every face adds the Rusanov Jacobian of its lower state to its lower cell
+// and subtracts that of its upper state from its upper cell, faces in i, j, k
+// order, so a cell sees -J(face i-) +J(face i+) -J(j-) +J(j+) -J(k-) +J(k+)
+// from a zero matrix.  Then AddDiagonalTerms (src/linearSolver.cpp:146-171)
+// and the Gauss-Jordan inverse (src/linearSolver.cpp:173-184).  One thread per
+// cell, after the residual kernels have written sr and dt.  A zero pivot sets
+// bit 4 of `flag`.
+//
+// MODE 0: the inviscid sum, then AddDiagonalTerms and the inverse.
+// With the viscous terms the inviscid sum must see the inviscid ghost states
+// (a viscousWall is a slip wall there), so it runs right after the inviscid
+// residual kernel and parks the sum in Ab (MODE 1); after the viscous pass
+// MODE 2 (central) or 3 (fourth-order face states) adds the viscous faces'
+// thin-shear-layer Jacobians, the cell as upper cell adding
+// (src/procBlock.cpp:1467-1474) and as lower cell subtracting (:1416-1424),
+// stores the cell velocity gradient, a sixth of each face's in the same order
+// (:1396-1414, 1427-1436), and finishes.
+template <int RECON, int LIM, int MODE>
+__global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *flag) {
+#pragma clang fp contract(off)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<double> g = Cast<double>(P.gas);
+  double D[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) D[q] = MODE >= 2 ? b.Ab[q * b.ncell + p] : 0.0;
+#pragma unroll
+  for (int d = 0; d < (MODE >= 2 ? 0 : 3); ++d) {
+    double WL[5], WR[5], J[25];
+    FaceStates<RECON, LIM>(b, P.kappa, d, i, j, k, WL, WR);
+    double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+    RusanovJacobian<double>(g, WR, a.x, a.y, a.z, a.w, false, J);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
+    const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
+    FaceStates<RECON, LIM>(b, P.kappa, d, ui, uj, uk, WL, WR);
+    a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
+    RusanovJacobian<double>(g, WL, a.x, a.y, a.z, a.w, true, J);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
+  }
+  if (MODE == 1) {
+#pragma unroll
+    for (int q = 0; q < 25; ++q) b.Ab[q * b.ncell + p] = D[q];
+    return;
+  }
+  if (MODE >= 2) {
+    constexpr double sixth = 1.0 / 6.0;
+    double G[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) G[q] = 0.0;
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int fi = i + (side && d == 0), fj = j + (side && d == 1), fk = k + (side && d == 2);
+        double vg[3][3], tg[3], Wf[5], muF, J[25];
+        FaceGrads(b, d, fi, fj, fk, vg, tg);
+        ViscFaceState<MODE == 3>(b, d, fi, fj, fk, Wf, muF);
+        const long fq = FaceIdx(b, d, fi, fj, fk);
+        const double vgr[9] = {vg[0][0], vg[0][1], vg[0][2], vg[1][0], vg[1][1], vg[1][2], vg[2][0], vg[2][1], vg[2][2]};
+        TSLJacobian(g, P.visc, Wf, muF, b.fa[d][fq], b.c2c[d][fq], side == 1, vgr, J);
+#pragma unroll
+        for (int q = 0; q < 25; ++q) D[q] = side ? D[q] - J[q] : D[q] + J[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) G[q] += sixth * vgr[q];
+      }
+    }
+    const long c = PadIdx(b, i, j, k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) b.vgrad[q * b.npad + c] = G[q];
+  }
+  const double sr = b.sr[p];
+  double dvt = (b.vol[p] * (1.0 + P.zeta)) / (b.dt[p] * P.theta);
+  if (P.dualCFL > 0.0) dvt += Dmax(sr, 0.0) / P.dualCFL;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) D[q * 6] *= P.relax;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) D[q * 6] += dvt;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b.Ab[q * b.ncell + p] = D[q];
+  double I[25];
+  if (!GJInverse5<double>(D, I)) atomicOr(flag, 4);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b.Abi[q * b.ncell + p] = I[q];
+}
+
+// RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194): the neighbour's
+// Rusanov Jacobian; with the viscous terms (vp != null) its thin-shear-layer
+// Jacobian from the neighbour's state, viscosity `mu` and cell velocity
+// gradient `vg`, at the face's projected centre distance, subtracted for a
+// lower (positive) and added for an upper neighbour
+template <typename T>
+__device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], const T du[5], double4 a,
+                                             bool positive, T out[5], const ViscParams *vp = nullptr,
+                                             double mu = 0.0, double dist = 1.0, const double *vg = nullptr) {
+  T J[25];
+  RusanovJacobian<T>(g, W, T(a.x), T(a.y), T(a.z), T(a.w), positive, J);
+  if (vp && vp->on) {
+    const GasT<double> gd = {double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
+    double Wd[5], V[25];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) Wd[e] = double(W[e]);
+    TSLJacobian(gd, *vp, Wd, mu, a, dist, positive, vg, V);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - T(V[q]) : J[q] + T(V[q]);
+  }
+  BlockMult<T>(J, du, out);
+}
+
+// block product with the cell's stored matrix (Ab or Abi)
+template <typename T>
+__device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
+  T m[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) m[q] = T(M[q * b.ncell + p]);
+  BlockMult<T>(m, v, out);
+}
+
 template <typename T>
 struct OpBuf;
 template <>
@@ -879,7 +1023,14 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       du[e] = x[e * b.ncell + q];
     }
     if (block) {
-      BlockOffDiag<T>(g, W, du, a, !upper, o);
+      if (vp.on) {
+        double vgn[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vgn[q] = b.vgrad[q * b.npad + c];
+        BlockOffDiag<T>(g, W, du, a, !upper, o, &vp, b.mu[c], b.c2c[d][fq], vgn);
+      } else {
+        BlockOffDiag<T>(g, W, du, a, !upper, o);
+      }
     } else {
       const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
       ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
@@ -1349,6 +1500,19 @@ __global__ void __launch_bounds__(256) KHaloState(const DevBlock *__restrict__ b
   for (int e = 0; e < 5; ++e) A.W[e * A.npad + h.dpad] = B.W[e * B.npad + h.spad];
 }
 
+// ghost cell velocity gradients <- partner interior cells
+// (SwapEddyViscAndGradients, src/gridLevel.cpp:385-387), block solvers with
+// the viscous terms
+__global__ void __launch_bounds__(256) KHaloGrad(const DevBlock *__restrict__ blocks,
+                                                 const HaloTask *__restrict__ tasks, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const HaloTask h = tasks[t];
+  const DevBlock &A = blocks[h.dst], &B = blocks[h.src];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) A.vgrad[q * A.npad + h.dpad] = B.vgrad[q * B.npad + h.spad];
+}
+
 // the off-diagonal product of the ghost neighbour across one connection
 // face, from the partner's current du (SwapImplicitUpdate before each pass,
 // src/linearSolver.cpp:444-463, then OffDiagonal with the ghost state)
@@ -1376,7 +1540,15 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
     T W[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) W[e] = T(Wn[e]);
-    BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
+    if (P.visc.on) {
+      double vgn[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) vgn[q] = A.vgrad[q * A.npad + c.gpad];
+      BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.mu[c.gpad],
+                      A.c2c[c.d][c.face], vgn);
+    } else {
+      BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
+    }
   } else if (P.visc.on)
     NbOffDiag<T>(Cast<T>(P.gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.c2c[c.d][c.face],
                  A.mu[c.gpad]);

@@ -485,9 +485,14 @@ int Upload(ath_ctx *c) {
     HIP_OK(hipMemset(d.x, 0, 5 * nc * sizeof(double)));
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
     d.Ab = d.Abi = nullptr;
+    d.vgrad = nullptr;
     if (hc.block) {
       if (int r = c->Alloc(&d.Ab, 25 * nc)) return r;
       if (int r = c->Alloc(&d.Abi, 25 * nc)) return r;
+      if (hc.viscous) {
+        if (int r = c->Alloc(&d.vgrad, 9 * np)) return r;
+        HIP_OK(hipMemset(d.vgrad, 0, 9 * np * sizeof(double)));
+      }
     }
     d.Ainv32 = nullptr;
     d.x32 = nullptr;
@@ -682,9 +687,16 @@ void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
   dim3 grid((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
             (d.nk + ath::kTK - 1) / ath::kTK);
   hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
-  if (c->P.block)  // inviscid only (navierStokes with a block solver is rejected at load)
-    hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
-                       c->dflag);
+  if (c->P.block) {
+    // inviscid block sum from the inviscid ghost states; finished here, or
+    // parked in Ab until the viscous pass has formed dt
+    if (c->P.visc.on)
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d,
+                         c->P, c->dflag);
+    else
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d,
+                         c->P, c->dflag);
+  }
   if (!c->P.visc.on) return;
   // viscous ghost cells, temperature and viscosity, viscous fluxes
   // (src/procBlock.cpp:6126-6138)
@@ -698,6 +710,23 @@ void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
     hipLaunchKernelGGL((ath::KViscFlux<true>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
   else
     hipLaunchKernelGGL((ath::KViscFlux<false>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+  // the block diagonal needs dt, which the viscous pass completes
+  if (c->P.block) {
+    if (c->P.visc.recon4)
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
+                         c->dflag);
+    else
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
+                         c->dflag);
+  }
+}
+
+// after every block's residual: the connections' ghost velocity gradients,
+// which the viscous block off-diagonals read
+void LaunchAfterResiduals(ath_ctx *c) {
+  if (c->P.block && c->P.visc.on && c->nhalo > 0)
+    hipLaunchKernelGGL(ath::KHaloGrad, dim3(Grid(c->nhalo)), dim3(kBlk), 0, c->stream, c->dblocks, c->dhalo,
+                       c->nhalo);
 }
 
 void LaunchResidual(ath_ctx *c, const BlockRes &br) {
@@ -1062,6 +1091,7 @@ int ath_calc_residual(ath_ctx *c) {
   if (!c) return Fail(ATH_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
   for (auto &br : c->blk) LaunchResidual(c, br);
+  LaunchAfterResiduals(c);
   if (int r = CheckLaunch()) return r;
   HIP_OK(hipStreamSynchronize(c->stream));
   return ATH_OK;
@@ -1124,6 +1154,7 @@ int ath_iterate(ath_ctx *c, int nn, int mm, ath_norms *norms) {
   if (slot < 0) return slot;
   LaunchBoundary(c);
   for (auto &br : c->blk) LaunchResidual(c, br);
+  LaunchAfterResiduals(c);
   LaunchTail(c, false, 0);
   if (c->P.multilevel && mm == c->hc.nonlinear - 1) LaunchCopyNm1(c);
   if (int r = CheckLaunch()) return r;
@@ -1169,6 +1200,7 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
         LaunchResidual(c, c->blk[q]);
         if (c->timing) hipEventRecord(c->evs[eb + 2 * q + 1], c->stream);
       }
+      LaunchAfterResiduals(c);
       if (c->timing) sweepLaunches += LaunchTail(c, true, eb + 2 * nflux);
       else HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
       if (c->P.multilevel && mm == nl - 1) LaunchCopyNm1(c);

@@ -116,6 +116,7 @@ typedef struct {
   double *consN, *consNm1;
   double *x;      /* padded, NEQ per cell */
   double *T, *mu; /* padded: temperature, viscosity (viscous cases) */
+  double *vg;     /* padded: cell velocity gradient, 9 per cell, row-major d v_c / d x_r */
   int *order;     /* 3 per physical cell */
   int nsurf;
   surf_t *surf;
@@ -983,6 +984,7 @@ static void cell_widths(blk_t *b) {
 /* interblock connections                                                   */
 
 static void seterr(char *err, int errlen, const char *fmt, ...);
+static double proj_c2c(const blk_t *b, int d, int i, int j, int k);
 static void edge_ghosts(orc_case *c, blk_t *b, int viscous);
 static const bcstate_t *find_periodic(const orc_case *c, int tag);
 
@@ -1261,19 +1263,23 @@ static int connect_blocks(orc_case *c, const v3 *const *nodes, char *err, int er
  * src/utility.cpp:400-423): ghost cells of every connection take the
  * partner's interior values; the sources are interior cells, so the order
  * of the swaps does not matter */
+/* which: 0 states, 1 implicit update, 2 cell velocity gradients
+ * (SwapEddyViscAndGradients, src/gridLevel.cpp:385-387; Cartesian
+ * components, so no rotation) */
 static void swap_halo(orc_case *c, int which) {
   for (int a = 0; a < c->nb; ++a) {
     blk_t *A = &c->blk[a];
     for (int q = 0; q < A->nconn; ++q) {
       const conn_t *cn = &A->conn[q];
       const blk_t *B = &c->blk[cn->pb];
-      double *dst = which ? A->x : A->W;
-      const double *src = which ? B->x : B->W;
+      double *dst = which == 2 ? A->vg : which ? A->x : A->W;
+      const double *src = which == 2 ? B->vg : which ? B->x : B->W;
+      int w = which == 2 ? 9 : NEQ;
       for (int l = 1; l <= A->g; ++l)
         for (int p2 = 0; p2 < cn->n2; ++p2)
           for (int p1 = 0; p1 < cn->n1; ++p1)
-            memcpy(&dst[conn_ghost(cn, A, l, p1, p2) * NEQ], &src[conn_source(cn, B, l, p1, p2) * NEQ],
-                   sizeof(double) * NEQ);
+            memcpy(&dst[conn_ghost(cn, A, l, p1, p2) * w], &src[conn_source(cn, B, l, p1, p2) * w],
+                   sizeof(double) * (size_t)w);
     }
   }
 }
@@ -1566,6 +1572,7 @@ static int alloc_block(blk_t *b) {
   b->W = (double *)calloc((size_t)nc * NEQ, sizeof(double));
   b->x = (double *)calloc((size_t)nc * NEQ, sizeof(double));
   b->T = (double *)calloc((size_t)nc, sizeof(double));
+  b->vg = (double *)calloc((size_t)nc * 9, sizeof(double));
   b->mu = (double *)calloc((size_t)nc, sizeof(double));
   b->R = (double *)calloc((size_t)np * NEQ, sizeof(double));
   b->sr = (double *)calloc((size_t)np, sizeof(double));
@@ -1826,7 +1833,6 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
   c->storeN = c->implicit;
   c->initMatrix = (c->solver == 1 || c->sweeps > 1);
   c->block = c->block && c->implicit; /* IsBlockMatrix: src/input.cpp:713-720 */
-  if (c->block && c->viscous) { seterr(err, errlen, "block matrix solvers with navierStokes are not supported"); goto fail2; }
   if (c->timeint == 0) c->nonlinear = 1; /* src/input.cpp:928-934 */
 
   /* reference speed of sound and nondimensional gas constant:
@@ -2021,7 +2027,7 @@ static void free_case(orc_case *c) {
     blk_t *b = &c->blk[bq];
     free(b->vol); free(b->ctr);
     for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
-    free(b->W); free(b->x); free(b->T); free(b->mu); free(b->R); free(b->sr); free(b->dt);
+    free(b->W); free(b->x); free(b->T); free(b->vg); free(b->mu); free(b->R); free(b->sr); free(b->dt);
     free(b->A); free(b->Ainv); free(b->Ab); free(b->Abi); free(b->consN); free(b->consNm1);
     free(b->order); free(b->surf); free(b->conn);
     for (int q = 0; q < 6; ++q) free(b->isconn[q]);
@@ -2515,6 +2521,96 @@ static void viscous_flux(const orc_case *c, const double vg[3][3], const double 
   F[4] = wall ? tv + q : tv + q + 0.0;
 }
 
+/* fluxJacobian::ApproxTSLJacobian (include/fluxJacobian.hpp:666-758) for a
+ * laminar single species (diffusion coefficient 0, mass fraction 1), times
+ * DelprimitiveDelConservative (:612-662) on the right (MatrixMultiply,
+ * src/matrix.cpp:186-199).  `left` selects the sign of the shear term;
+ * vgr is the velocity gradient, row-major d v_c / d x_r. */
+static void tsl_jacobian(const orc_case *c, const double *W, double muLam, uv3 a, double dist, int left,
+                         const double *vgr, double *out) {
+  const orc_gas *g = &c->gas;
+  double t = temperature(g, W);
+  double mu = c->vscale * muLam, mut = c->vscale * 0.0;
+  v3 n = a.u;
+  v3 vel = V(W[1], W[2], W[3]);
+  double vn = vdot(vel, n);
+  double mf = W[0] / (0.0 + W[0]);
+  double rho = 0.0 + W[0];
+  double k = cond_of_t(c, t) * c->vscale;
+  double kt = mut * cps(g) / 0.9;
+  /* TauNormal (src/utility.cpp:426-436) */
+  double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
+  double trace = vgr[0] + vgr[4] + vgr[8];
+  double nn[3] = {n.x, n.y, n.z}, tau[3];
+  for (int r = 0; r < 3; ++r) {
+    double s0 = vgr[r * 3 + 0] + vgr[0 * 3 + r], s1 = vgr[r * 3 + 1] + vgr[1 * 3 + r],
+           s2 = vgr[r * 3 + 2] + vgr[2 * 3 + r];
+    double sa = s0 * nn[0] + s1 * nn[1] + s2 * nn[2];
+    tau[r] = lambda * trace * nn[r] + (mu + mut) * sa;
+  }
+  double fac = left ? -1.0 : 1.0;
+  const double third = 1.0 / 3.0;
+  double J[25];
+  for (int q = 0; q < 25; ++q) J[q] = 0.0;
+  double diff = 0.0 + 0.0; /* diffNone: LaminarDiffCoeff + TurbDiffCoeff */
+  J[0] = diff * (1.0 - mf) / ((mu + mut) * rho);
+  double hs = cps(g) * t; /* SpeciesEnthalpy, hf = 0 */
+  double speciesEnthalpy = J[0] * hs;
+  J[20] = -(k + kt) * t / ((mu + mut) * rho) + speciesEnthalpy;
+  J[6] = third * n.x * n.x + 1.0;
+  J[11] = third * n.x * n.y;
+  J[16] = third * n.x * n.z;
+  J[21] = fac * 0.5 * dist / (mu + mut) * tau[0] + third * n.x * vn + W[1];
+  J[7] = third * n.y * n.x;
+  J[12] = third * n.y * n.y + 1.0;
+  J[17] = third * n.y * n.z;
+  J[22] = fac * 0.5 * dist / (mu + mut) * tau[1] + third * n.y * vn + W[2];
+  J[8] = third * n.z * n.x;
+  J[13] = third * n.z * n.y;
+  J[18] = third * n.z * n.z + 1.0;
+  J[23] = fac * 0.5 * dist / (mu + mut) * tau[2] + third * n.z * vn + W[3];
+  J[24] = (k + kt) / ((mu + mut) * rho);
+  double f = a.m * (mu + mut) / dist;
+  for (int q = 0; q < 25; ++q) J[q] *= f;
+  /* DelprimitiveDelConservative */
+  double gm1 = orc_gamma(g) - 1.0;
+  double invRho = 1.0 / (0.0 + W[0]);
+  double P[25];
+  for (int q = 0; q < 25; ++q) P[q] = 0.0;
+  P[0] = 1.0;
+  P[5] = -invRho * W[1];
+  P[10] = -invRho * W[2];
+  P[15] = -invRho * W[3];
+  P[20] = 0.5 * gm1 * vdot(vel, vel);
+  P[6] = invRho;
+  P[21] = -gm1 * W[1];
+  P[12] = invRho;
+  P[22] = -gm1 * W[2];
+  P[18] = invRho;
+  P[23] = -gm1 * W[3];
+  P[24] = gm1;
+  for (int r = 0; r < 5; ++r)
+    for (int ii = 0; ii < 5; ++ii) {
+      double sacc = 0.0;
+      for (int cc = 0; cc < 5; ++cc) sacc += J[r * 5 + cc] * P[cc * 5 + ii];
+      out[r * 5 + ii] = sacc;
+    }
+}
+void orc_viscous_flux(const orc_case *c, const double *vgr, const double *tg, const double *n, const double *Wf,
+                      double muFace, int wall, double *F) {
+  double vg[3][3] = {{vgr[0], vgr[1], vgr[2]}, {vgr[3], vgr[4], vgr[5]}, {vgr[6], vgr[7], vgr[8]}};
+  viscous_flux(c, vg, tg, V(n[0], n[1], n[2]), Wf, muFace, wall, F);
+}
+void orc_get_cell_grad(const orc_case *c, int bq, double *out) {
+  const blk_t *b = &c->blk[bq];
+  memcpy(out, b->vg, sizeof(double) * 9 * (size_t)ncells_padded(b));
+}
+void orc_tsl_jacobian(const orc_case *c, const double *W, double muLam, const double *area, double dist,
+                      int left, const double *vgr, double *out) {
+  uv3 a = {V(area[0], area[1], area[2]), area[3]};
+  tsl_jacobian(c, W, muLam, a, dist, left, vgr, out);
+}
+
 /* procBlock::CalcViscFluxI/J/K (src/procBlock.cpp:1233-2135), laminar */
 static void visc_flux_dir(orc_case *c, blk_t *b, int d) {
   int ei = (d == 0), ej = (d == 1), ek = (d == 2);
@@ -2552,18 +2648,40 @@ static void visc_flux_dir(orc_case *c, blk_t *b, int d) {
         uv3 a = b->fa[d][fidx(b, d, i, j, k)];
         double F[NEQ];
         viscous_flux(c, vg, tg, a.u, Wf, muF, wall, F);
+        const double sixth = 1.0 / 6.0;
+        double vgr[9] = {vg[0][0], vg[0][1], vg[0][2], vg[1][0], vg[1][1], vg[1][2], vg[2][0], vg[2][1], vg[2][2]};
+        double J[25];
         if (f3 > 0) {
           double *R = &b->R[pidx(b, i - ei, j - ej, k - ek) * NEQ];
           for (int e = 0; e < NEQ; ++e) R[e] -= F[e] * a.m;
+          /* cell velocity gradient: a sixth of each face's (:1396-1414) */
+          double *G = &b->vg[cL * 9];
+          for (int q = 0; q < 9; ++q) G[q] += sixth * vgr[q];
+          /* block diagonal: the lower cell subtracts the TSL Jacobian of
+           * the face state with the left sign (:1416-1424) */
+          if (c->block) {
+            tsl_jacobian(c, Wf, muF, a, proj_c2c(b, d, i, j, k), 1, vgr, J);
+            double *Ab = &b->Ab[pidx(b, i - ei, j - ej, k - ek) * 25];
+            for (int q = 0; q < 25; ++q) Ab[q] = Ab[q] - J[q];
+          }
         }
         if (f3 < n3) {
           long p = pidx(b, i, j, k);
           double *R = &b->R[p * NEQ];
           for (int e = 0; e < NEQ; ++e) R[e] += F[e] * a.m;
+          double *G = &b->vg[cU * 9];
+          for (int q = 0; q < 9; ++q) G[q] += sixth * vgr[q];
           uv3 aU = b->fa[d][fidx(b, d, i + ei, j + ej, k + ek)];
           double vsr = visc_cell_specrad(c, &b->W[cU * NEQ], a, aU, b->vol[cU], b->mu[cU]);
           b->sr[p] += vsr * c->viscCoeff;
-          if (c->implicit) b->A[p] += 2.0 * vsr;
+          if (c->block) {
+            /* the upper cell adds it with the right sign (:1467-1474) */
+            tsl_jacobian(c, Wf, muF, a, proj_c2c(b, d, i, j, k), 0, vgr, J);
+            double *Ab = &b->Ab[p * 25];
+            for (int q = 0; q < 25; ++q) Ab[q] = Ab[q] + J[q];
+          } else if (c->implicit) {
+            b->A[p] += 2.0 * vsr;
+          }
         }
       }
 }
@@ -2679,11 +2797,13 @@ void orc_residual(orc_case *c) {
     if (c->viscous) {
       viscous_ghosts(c, b);
       update_aux(c, b);
+      memset(b->vg, 0, sizeof(double) * 9 * (size_t)ncells_padded(b)); /* ResetGradients */
       visc_flux_dir(c, b, 0);
       visc_flux_dir(c, b, 1);
       visc_flux_dir(c, b, 2);
     }
   }
+  if (c->viscous) swap_halo(c, 2);
 }
 
 /* CalcBlockTimeStep / CalcCellDt: src/procBlock.cpp:782-820 */
@@ -2785,9 +2905,17 @@ static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
     /* FaceSpectralRadius with the viscous part (include/spectralRadius.hpp:
      * 192-203): the neighbour's state and viscosity, ProjC2CDist */
     if (c->block) {
-      /* RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194) */
+      /* RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194): with the
+       * viscous terms the neighbour's TSL Jacobian, from its state,
+       * viscosity and cell velocity gradient, is subtracted (lower) or
+       * added (upper) */
       double J[25];
       rusanov_jacobian(&c->gas, &b->W[cn * NEQ], a, !upper, J);
+      if (c->viscous) {
+        double Vj[25];
+        tsl_jacobian(c, &b->W[cn * NEQ], b->mu[cn], a, proj_c2c(b, d, fi, fj, fk), !upper, &b->vg[cn * 9], Vj);
+        for (int q = 0; q < 25; ++q) J[q] = !upper ? J[q] - Vj[q] : J[q] + Vj[q];
+      }
       block_mult(J, &x[cn * NEQ], o);
       for (int e = 0; e < NEQ; ++e) out[e] += o[e];
       continue;

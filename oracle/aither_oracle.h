@@ -139,6 +139,16 @@ void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
 void orc_rusanov_jacobian(const orc_gas *g, const double *W,
                           const double *area /*ux,uy,uz,mag*/, int positive,
                           double *J);
+/* laminar viscous face flux per unit area (velocity gradient row-major
+ * d v_c / d x_r, temperature gradient, unit normal, face state, face
+ * viscosity before scaling) and the thin-shear-layer Jacobian times
+ * d(prim)/d(cons), 25 row-major; left selects the lower cell's sign */
+void orc_viscous_flux(const orc_case *c, const double *vgr, const double *tg, const double *n,
+                      const double *Wf, double muFace, int wall, double *F);
+void orc_tsl_jacobian(const orc_case *c, const double *W, double muLam, const double *area,
+                      double dist, int left, const double *vgr, double *out);
+/* padded cell velocity gradients after the viscous residual, 9 per cell */
+void orc_get_cell_grad(const orc_case *c, int b, double *out);
 /* Gauss-Jordan inverse of a 5x5 row-major matrix in place; -1 if singular */
 int orc_block_inverse(double *m);
 

@@ -91,6 +91,9 @@ def lib():
         L.orc_block_inverse.restype = ctypes.c_int
         for f in ("orc_get_block_diag", "orc_get_block_inverse"):
             getattr(L, f).argtypes = [_V, ctypes.c_int, _D]
+        L.orc_viscous_flux.argtypes = [_V, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int, _D]
+        L.orc_tsl_jacobian.argtypes = [_V, _D, ctypes.c_double, _D, ctypes.c_double, ctypes.c_int, _D, _D]
+        L.orc_get_cell_grad.argtypes = [_V, ctypes.c_int, _D]
         L.orc_is_block.argtypes = [_V]
         L.orc_is_block.restype = ctypes.c_int
         _lib = L
@@ -188,6 +191,13 @@ class OracleCase:
 
     def block_inverse(self, b=0):
         return self._get(self.L.orc_get_block_inverse, b, 25).reshape(-1, 5, 5)
+
+    def cell_gradients(self, b=0):
+        ni, nj, nk = self.dims[b]
+        g = self.L.orc_num_ghosts(self.h)
+        out = np.empty((ni + 2 * g) * (nj + 2 * g) * (nk + 2 * g) * 9)
+        self.L.orc_get_cell_grad(self.h, b, P(out))
+        return out.reshape(nk + 2 * g, nj + 2 * g, ni + 2 * g, 3, 3)
 
     def is_block(self):
         return bool(self.L.orc_is_block(self.h))

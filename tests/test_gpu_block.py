@@ -156,3 +156,106 @@ def test_block_solvers_across_connections(tmp_path, solver, sweeps):
     Wo = np.concatenate([o.state(b) for b in range(o.nb)])
     assert rel(Ws, Wo) <= 1e-12
     s.close()
+
+
+def _common_state_parity(s, o, iterations, tol=1e-12):
+    """per-step residual and matrix residual from a common state (the oracle
+    restarts each step from the GPU state): Sutherland's T^1.5 is correctly
+    rounded on the device and not always in glibc"""
+    for nn in range(iterations):
+        for b in range(o.nb):
+            o.set_state(s.get_state(b), b)
+        s.begin_step(nn)
+        o.begin_step(nn)
+        for mm in range(o.nonlinear):
+            ns = s.iterate(nn, mm)
+            no = o.iterate(nn, mm)
+            Rs = np.concatenate([s.residual(b) for b in range(o.nb)])
+            Ro = np.concatenate([o.residual(b) for b in range(o.nb)])
+            assert rel(Rs, Ro) <= tol, (nn, mm, rel(Rs, Ro))
+            if no["matrix_resid"] > 0:
+                assert abs(ns.matrix_resid - no["matrix_resid"]) <= 1e-10 * no["matrix_resid"], (nn, mm)
+
+
+@pytest.mark.parametrize("visc_recon", ["central", "centralFourth"])
+def test_viscous_block_diagonal_matches_oracle(tmp_path, visc_recon):
+    p = cases.bump_case(str(tmp_path), seed=4, n=8, iterations=3, viscous=True, visc_recon=visc_recon)[0]
+    path = _variant(p, "blk", **{"matrixSolver: lusgs": "matrixSolver: blusgs"})
+    say("viscous block diagonal", visc_recon)
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    # from a non-uniform state: at the uniform initial state the limiter
+    # hides the wall ghost states from the inviscid face states
+    for nn in range(2):
+        s.begin_step(nn)
+        s.iterate(nn, 0)
+    o.set_state(s.get_state())
+    s.begin_step(2)
+    o.begin_step(2)
+    s.assign_ghost_cells()
+    s.calc_residual()
+    o.bc()
+    o.calc_residual()
+    o.time_step()
+    o.invert_diagonal()
+    assert rel(s.residual(), o.residual()) <= 1e-12
+    assert rel(s.block_diagonal(), o.block_diagonal()) <= 1e-12
+    assert rel(s.block_diagonal(inverse=True), o.block_inverse()) <= 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("solver,sweeps,timeint", [("blusgs", 1, "implicitEuler"), ("bdplur", 2, "implicitEuler"),
+                                                   ("blusgs", 2, "bdf2")])
+def test_viscous_block_solvers_step_parity(tmp_path, solver, sweeps, timeint):
+    p = cases.bump_case(str(tmp_path), seed=4, n=8, iterations=4, viscous=True)[0]
+    repl = {"matrixSolver: lusgs": f"matrixSolver: {solver}", "matrixSweeps: 1": f"matrixSweeps: {sweeps}",
+            "timeIntegration: implicitEuler": f"timeIntegration: {timeint}"}
+    if timeint == "bdf2":
+        repl["matrixRelaxation: 1.0"] = "matrixRelaxation: 1.0\nnonlinearIterations: 2"
+    path = _variant(p, f"{solver}{sweeps}{timeint}", **repl)
+    say("viscous block solver", solver, sweeps, timeint)
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    _common_state_parity(s, o, 4)
+    s.close()
+
+
+def test_viscous_block_across_connections(tmp_path):
+    """Viscous block off-diagonals across a rotated connection read the
+    partner's cell velocity gradients through the ghost cells."""
+    import conn_cases as cc
+    r0, r1 = cc.orientation_set()[350]
+    path, _ = cc.write_pair(str(tmp_path), "cut", r0, r1, timeint="implicitEuler", eq="navierStokes", scale=1e-3,
+                            solver="blusgs", sweeps=2)
+    say("viscous block connections")
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    _common_state_parity(s, o, 4)
+    s.close()
+
+
+@pytest.mark.parametrize("refine,solver", [(0.125, "blusgs"), (0.25, "blusgs"), (0.125, "bdplur")])
+def test_multiblock_channel_parity(tmp_path, refine, solver):
+    """The benchmark's multiblock laminar channel (eight blocks joined by
+    interblock connections, spanwise periodic, no-slip wall, AUSMPW+, BDF2
+    with block solver subiterations) at reduced resolution: per-step
+    residuals from a common state, then the free trajectory."""
+    path, meta = cases.channel_case(str(tmp_path), refine=refine, steps=2, subits=3, solver=solver,
+                                    sweeps=1 if solver == "blusgs" else 2)
+    say("multiblock channel", refine, solver, meta["cells"], "cells")
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    _common_state_parity(s, o, 2)
+    s.close()
+    s = Solver(path)
+    o = pyoracle.OracleCase(path)
+    for nn in range(2):
+        s.begin_step(nn)
+        o.begin_step(nn)
+        for mm in range(o.nonlinear):
+            s.iterate(nn, mm)
+            o.iterate(nn, mm)
+    Ws = np.concatenate([s.get_state(b) for b in range(o.nb)])
+    Wo = np.concatenate([o.state(b) for b in range(o.nb)])
+    assert rel(Ws, Wo) <= 1e-10
+    s.close()

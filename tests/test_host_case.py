@@ -95,13 +95,11 @@ def test_block_matrix_selection(tmp_path, solver, timeint, block):
     assert pyoracle.OracleCase(inp).is_block() == bool(block)
 
 
-def test_block_matrix_with_viscous_terms_is_refused(tmp_path):
+def test_block_matrix_with_viscous_terms(tmp_path):
+    """navierStokes with blusgs selects the block matrix in both readers"""
     import pyoracle
     inp = cases.bump_case(str(tmp_path), seed=1, n=4, iterations=1, viscous=True)[0]
     text = open(inp).read().replace("matrixSolver: lusgs", "matrixSolver: blusgs")
     open(inp, "w").write(text)
-    with pytest.raises(aither_amd.AitherError) as e:
-        inspect_info(inp)
-    assert e.value.code == -2 and "block" in str(e.value)
-    with pytest.raises(RuntimeError, match="block"):
-        pyoracle.OracleCase(inp)
+    assert inspect_info(inp).block_matrix == 1
+    assert pyoracle.OracleCase(inp).is_block()
