@@ -31,6 +31,12 @@ struct DevBlock {
   // block matrix solvers: main diagonal after AddDiagonalTerms and its
   // inverse, 25 * ncell each, entry (r, c) of cell p at (5 r + c) ncell + p
   double *Ab, *Abi;
+  // block matrix solvers: the off-diagonal blocks of every interior face of
+  // direction d, formed once per relax (they depend on the state only):
+  // Mlo = the lower cell's block as seen from the upper cell (positive),
+  // Mhi = the upper cell's block as seen from the lower cell; 25 * faces(d)
+  // each, entry q of face f at q * faces(d) + f
+  double *Mlo[3], *Mhi[3];
   // off-diagonal product of the ghost neighbour across each boundary face
   // that lies on an interblock connection (zero on other faces), per sweep
   // pass; null when the block has no connection.  5 * nbf, SoA.
@@ -961,13 +967,64 @@ __device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], con
   BlockMult<T>(J, du, out);
 }
 
+// the off-diagonal blocks of the interior faces of direction d: exactly the
+// matrices RusanovBlockOffDiagonal forms for each product, formed once per
+// relax instead of once per product (the state does not change during the
+// sweeps); one thread per face
+__global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d) {
+#pragma clang fp contract(off)
+  const int FI = b.ni + (d == 0), FJ = b.nj + (d == 1), FK = b.nk + (d == 2);
+  const long nf = (long)FI * FJ * FK;
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int i = (int)(f % FI), j = (int)((f / FI) % FJ), k = (int)(f / ((long)FI * FJ));
+  const int f3 = d == 0 ? i : (d == 1 ? j : k), n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
+  if (f3 == 0 || f3 == n3) return;  // boundary faces: connection terms, formed per pass
+  const GasT<double> g = Cast<double>(P.gas);
+  const double4 a = b.fa[d][f];
+  const long cU = PadIdx(b, i, j, k);
+  const long cL = PadIdx(b, i - (d == 0), j - (d == 1), k - (d == 2));
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const long c = side ? cU : cL;
+    double W[5], J[25];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) W[e] = b.W[e * b.npad + c];
+    const bool positive = side == 0;
+    RusanovJacobian<double>(g, W, a.x, a.y, a.z, a.w, positive, J);
+    if (P.visc.on) {
+      double vg[9], V[25];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) vg[q] = b.vgrad[q * b.npad + c];
+      TSLJacobian(g, P.visc, W, b.mu[c], a, b.c2c[d][f], positive, vg, V);
+#pragma unroll
+      for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - V[q] : J[q] + V[q];
+    }
+    double *M = side ? b.Mhi[d] : b.Mlo[d];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) M[q * nf + f] = J[q];
+  }
+}
+
+// BlockMult with the matrix read entry by entry from global memory (entry q
+// at M[q * stride + idx]), in the same order of operations
+template <typename T>
+__device__ __forceinline__ void BlockMultG(const double *__restrict__ M, long stride, long idx, const T v[5],
+                                           T out[5]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += T(M[(r * 5 + c) * stride + idx]) * v[c];
+    out[r] = acc;
+  }
+}
+
 // block product with the cell's stored matrix (Ab or Abi)
 template <typename T>
 __device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
-  T m[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) m[q] = T(M[q * b.ncell + p]);
-  BlockMult<T>(m, v, out);
+  BlockMultG<T>(M, b.ncell, p, v, out);
 }
 
 template <typename T>
@@ -1023,14 +1080,9 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       du[e] = x[e * b.ncell + q];
     }
     if (block) {
-      if (vp.on) {
-        double vgn[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) vgn[q] = b.vgrad[q * b.npad + c];
-        BlockOffDiag<T>(g, W, du, a, !upper, o, &vp, b.mu[c], b.c2c[d][fq], vgn);
-      } else {
-        BlockOffDiag<T>(g, W, du, a, !upper, o);
-      }
+      // the face's precomputed block of this neighbour
+      const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
+      BlockMultG<T>(upper ? b.Mhi[d] : b.Mlo[d], nf, fq, du, o);
     } else {
       const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
       ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
@@ -1770,6 +1822,214 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_store(&s.flags[TileId(s, tl.x, tl.y, tl.z)], epoch, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block LU-SGS sweep over all blocks in one launch (blusgs).
+//
+// The blocks of a pass are independent (a connection's partner update enters
+// lagged, through the precomputed connection terms), so one persistent launch
+// sweeps them together: the tiles of every block are queued by their tile
+// hyperplane (then block, then k, j, i), each workgroup takes the next ticket,
+// waits for the (at most three) tiles of the same block it depends on, and
+// walks the tile's cell hyperplanes.  Per cell, the dependency side's three
+// off-diagonal blocks (formed once per relax) and D^-1 are read before the
+// wait and stay in registers; the arithmetic of each cell is that of
+// KLusgsForward / KLusgsBackward, so every order gives the same bits.
+
+struct MultiSweepCtl {
+  int *ticket;          // reset to 0 before each sweep
+  int *epoch;           // incremented before each sweep
+  int *flags;           // one per tile of every block, the epoch of its publication
+  int *err;             // bit 1: a dependency wait exceeded the spin limit
+  const int4 *tiles;    // (a, b, c, block) in pass order
+  const int4 *grid;     // per block: tile grid dims and the block's first flag
+  int ntiles;
+};
+
+__global__ void KMultiSweepPrologue(MultiSweepCtl s) {
+  *s.ticket = 0;
+  *s.epoch = *s.epoch + 1;
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
+                                                        MultiSweepCtl s, int full) {
+  __shared__ T sX[5][kLT];
+  __shared__ T sHX[5][kHalo];  // halo du
+  __shared__ int sTicket;
+  __shared__ int sEpoch;
+
+  const int tid = threadIdx.x;
+  const int li = tid % kLTI, lj = (tid / kLTI) % kLTJ, lk = tid / (kLTI * kLTJ);
+  const GasT<T> g = Cast<T>(P.gas);
+  const int sgn = FWD ? -1 : 1;  // direction of the dependency neighbours
+
+  // thread 0 owns the ticket and the flag (see KLusgsTile)
+  if (tid == 0) {
+    sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sEpoch = __hip_atomic_load(s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const int epoch = sEpoch;
+  for (;;) {
+    const int t = sTicket;
+    if (t >= s.ntiles) break;
+    const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
+    const DevBlock &b = blocks[tl.w];
+    const int4 gr = s.grid[tl.w];
+    T *x = OpBuf<T>::X(b);
+    const int i = tl.x * kLTI + li, j = tl.y * kLTJ + lj, k = tl.z * kLTK + lk;
+    const bool valid = i < b.ni && j < b.nj && k < b.nk;
+    const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
+
+    // --- static data before the dependency wait
+    double rb[5] = {0, 0, 0, 0, 0};
+    double mdep[3][25], mai[25];
+    if (valid) {
+      if (FWD || full) Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+        const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk;
+        const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
+        const long fq = inside ? (FWD ? FaceIdx(b, d, i, j, k) : FaceIdx(b, d, gi, gj, gk)) : 0;
+        const double *M = FWD ? b.Mlo[d] : b.Mhi[d];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[q * nf + fq] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 25; ++q) mai[q] = b.Abi[q * b.ncell + p];
+    }
+
+    // --- wait for the tiles of the same block this one depends on
+    if (tid == 0) {
+      const int deps[3][3] = {{tl.x + sgn, tl.y, tl.z}, {tl.x, tl.y + sgn, tl.z}, {tl.x, tl.y, tl.z + sgn}};
+      for (int q = 0; q < 3; ++q) {
+        const int a = deps[q][0], bb = deps[q][1], cc = deps[q][2];
+        if (a < 0 || bb < 0 || cc < 0 || a >= gr.x || bb >= gr.y || cc >= gr.z) continue;
+        const int id = gr.w + a + gr.x * (bb + gr.y * cc);
+        long spins = 0;
+        while (__hip_atomic_load(&s.flags[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+          if ((++spins & 255) == 0 &&
+              (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2))
+            break;
+          if (spins > kSpinLimit) {
+            __hip_atomic_fetch_or(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // halo du, published by the dependency tiles
+    {
+      const int di = FWD ? -1 : kLTI, dj = FWD ? -1 : kLTJ, dk = FWD ? -1 : kLTK;
+      if (tid < kHalo) {
+        int hi, hj, hk;
+        if (tid < kHaloI) { hi = di; hj = tid % kLTJ; hk = tid / kLTJ; }
+        else if (tid < kHaloI + kHaloJ) { const int q = tid - kHaloI; hi = q % kLTI; hj = dj; hk = q / kLTI; }
+        else { const int q = tid - kHaloI - kHaloJ; hi = q % kLTI; hj = q / kLTI; hk = dk; }
+        const int gi = tl.x * kLTI + hi, gj = tl.y * kLTJ + hj, gk = tl.z * kLTK + hk;
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
+          const long q = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
+#pragma unroll
+          for (int e = 0; e < 5; ++e) sHX[e][tid] = x[e * b.ncell + q];
+        }
+      }
+    }
+    if (!FWD && valid) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sX[e][tid] = x[e * b.ncell + p];  // forward result
+    }
+    __syncthreads();
+
+    // --- intra-tile hyperplanes
+    const int plane = li + lj + lk;
+    for (int st = 0; st < kLTPlanes; ++st) {
+      const int pl = FWD ? st : kLTPlanes - 1 - st;
+      if (valid && plane == pl) {
+        T D[5];  // dependency-side sum: L (forward) or U (backward), i, j, k order
+#pragma unroll
+        for (int e = 0; e < 5; ++e) D[e] = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+          if (gi < 0 || gj < 0 || gk < 0 || gi >= b.ni || gj >= b.nj || gk >= b.nk) {
+            const T *bt = OpBuf<T>::Bt(b);
+            if (bt) {
+              const long q = BtSlot(b, d, FWD ? 0 : 1, i, j, k);
+#pragma unroll
+              for (int e = 0; e < 5; ++e) D[e] += bt[e * b.nbf + q];
+            }
+            continue;
+          }
+          const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
+          const int L3 = d == 0 ? kLTI : (d == 1 ? kLTJ : kLTK);
+          const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
+          T dn[5], o[5], m[25];
+          if (inTile) {
+            const int nt = tid + sgn * (d == 0 ? 1 : (d == 1 ? kLTI : kLTI * kLTJ));
+#pragma unroll
+            for (int e = 0; e < 5; ++e) dn[e] = sX[e][nt];
+          } else {
+            const int h = d == 0 ? lj + kLTJ * lk : (d == 1 ? kHaloI + li + kLTI * lk : kHaloI + kHaloJ + li + kLTI * lj);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) dn[e] = sHX[e][h];
+          }
+#pragma unroll
+          for (int q = 0; q < 25; ++q) m[q] = T(mdep[d][q]);
+          BlockMult<T>(m, dn, o);
+#pragma unroll
+          for (int e = 0; e < 5; ++e) D[e] += o[e];
+        }
+        T xn[5], v[5], ai[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) ai[q] = T(mai[q]);
+        if (full) {
+          // the other side, from the previous sweep's du in global memory
+          T O[5];
+          OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, O, 1);
+          if (FWD) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + (D[e] - O[e]);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + O[e]) - D[e];
+          }
+          BlockMult<T>(ai, v, xn);
+        } else if (FWD) {
+#pragma unroll
+          for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + D[e];
+          BlockMult<T>(ai, v, xn);
+        } else {
+          T o[5];
+          BlockMult<T>(ai, D, o);
+#pragma unroll
+          for (int e = 0; e < 5; ++e) xn[e] = sX[e][tid] - o[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+          sX[e][tid] = xn[e];
+          x[e * b.ncell + p] = xn[e];
+        }
+      }
+      __syncthreads();
+    }
+
+    // --- publish, then take the next ticket
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_AGENT);
       sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -81,6 +81,9 @@ struct ath_ctx {
   int nhalo = 0;
   ath::ConnTask *dconn = nullptr;
   int nconn = 0;
+  // block solvers: one tiled sweep over all blocks
+  ath::MultiSweepCtl msweep{};
+  int msweepGrid = 0;
   // timing
   int timing = 0;
   std::vector<hipEvent_t> evs;
@@ -121,7 +124,7 @@ long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) 
 // the device copy of the blocks' DevBlocks (pointers change when the FP32
 // operator buffers are created)
 int SyncBlocks(ath_ctx *c) {
-  if (c->hc.conns.empty()) return ATH_OK;
+  if (c->hc.conns.empty() && !c->hc.block) return ATH_OK;
   std::vector<ath::DevBlock> v;
   for (const auto &br : c->blk) v.push_back(br.d);
   if (!c->dblocks)
@@ -366,6 +369,49 @@ int UploadConnections(ath_ctx *c) {
   return SyncBlocks(c);
 }
 
+// the tiles of every block in pass order: tile hyperplane, then block, then
+// k, j, i; per block its tile grid and first flag
+int UploadMultiSweep(ath_ctx *c) {
+  using namespace ath;
+  std::vector<int4> grid, tiles;
+  int base = 0, maxPlane = 0;
+  for (auto &br : c->blk) {
+    const int ta = (br.d.ni + kLTI - 1) / kLTI, tb = (br.d.nj + kLTJ - 1) / kLTJ, tc = (br.d.nk + kLTK - 1) / kLTK;
+    grid.push_back(make_int4(ta, tb, tc, base));
+    base += ta * tb * tc;
+    maxPlane = std::max(maxPlane, ta + tb + tc - 2);
+  }
+  for (int pl = 0; pl < maxPlane; ++pl)
+    for (size_t q = 0; q < grid.size(); ++q)
+      for (int cc = 0; cc < grid[q].z; ++cc)
+        for (int bb = 0; bb < grid[q].y; ++bb)
+          for (int aa = 0; aa < grid[q].x; ++aa)
+            if (aa + bb + cc == pl) tiles.push_back(make_int4(aa, bb, cc, (int)q));
+  MultiSweepCtl &m = c->msweep;
+  m.ntiles = (int)tiles.size();
+  int4 *dt = nullptr, *dg = nullptr;
+  if (int r = c->Alloc(&dt, tiles.size())) return r;
+  if (int r = c->Alloc(&dg, grid.size())) return r;
+  HIP_OK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dg, grid.data(), grid.size() * sizeof(int4), hipMemcpyHostToDevice));
+  m.tiles = dt;
+  m.grid = dg;
+  if (int r = c->Alloc(&m.flags, base)) return r;
+  if (int r = c->Alloc(&m.ticket, 2)) return r;
+  HIP_OK(hipMemset(m.flags, 0, base * sizeof(int)));
+  HIP_OK(hipMemset(m.ticket, 0, 2 * sizeof(int)));
+  m.epoch = m.ticket + 1;
+  m.err = c->dflag;
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, c->device));
+  c->msweepGrid = std::max(1, std::min(m.ntiles, 2 * prop.multiProcessorCount));
+  // the sweep reads every block's DevBlock from device memory; a case
+  // without connections has not copied them yet
+  if (int r = SyncBlocks(c)) return r;
+  if (!c->dblocks) return Fail(ATH_ERR_HIP, "block sweep: device block table missing");
+  return ATH_OK;
+}
+
 int Upload(ath_ctx *c) {
   using namespace ath;
   const HostCase &hc = c->hc;
@@ -486,9 +532,15 @@ int Upload(ath_ctx *c) {
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
     d.Ab = d.Abi = nullptr;
     d.vgrad = nullptr;
+    for (int dd = 0; dd < 3; ++dd) d.Mlo[dd] = d.Mhi[dd] = nullptr;
     if (hc.block) {
       if (int r = c->Alloc(&d.Ab, 25 * nc)) return r;
       if (int r = c->Alloc(&d.Abi, 25 * nc)) return r;
+      for (int dd = 0; dd < 3; ++dd) {
+        const long nf = (long)(h.ni + (dd == 0)) * (h.nj + (dd == 1)) * (h.nk + (dd == 2));
+        if (int r = c->Alloc(&d.Mlo[dd], 25 * nf)) return r;
+        if (int r = c->Alloc(&d.Mhi[dd], 25 * nf)) return r;
+      }
       if (hc.viscous) {
         if (int r = c->Alloc(&d.vgrad, 9 * np)) return r;
         HIP_OK(hipMemset(d.vgrad, 0, 9 * np * sizeof(double)));
@@ -632,6 +684,8 @@ int Upload(ath_ctx *c) {
   c->histCap = 4096;
   if (int r = c->Alloc(&c->dhist, 8 * (size_t)c->histCap)) return r;
   if (int r = UploadConnections(c)) return r;
+  if (c->hc.block)
+    if (int r = UploadMultiSweep(c)) return r;
   return ATH_OK;
 }
 
@@ -762,12 +816,19 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     if (timed) hipEventRecord(c->evs[evBase + 2 * pass + (begin ? 0 : 1)], c->stream);
     if (!begin) ++pass;
   };
+  if (c->P.block)  // the off-diagonal blocks of the interior faces, once per relax
+    for (auto &br : c->blk)
+      for (int d = 0; d < 3; ++d) {
+        const long nf = (long)(br.d.ni + (d == 0)) * (br.d.nj + (d == 1)) * (br.d.nk + (d == 2));
+        hipLaunchKernelGGL(ath::KBlockOffMats, dim3(Grid(nf, 128)), dim3(128), 0, c->stream, br.d, c->P, d);
+      }
   for (auto &br : c->blk)
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
-    // the block solvers sweep in hyperplane order whatever the ordering set
-    const int order = c->P.block ? ATH_SWEEP_HYPERPLANE : c->sweepOrder;
+    // the block solvers have the tiled and hyperplane orders; the pencil
+    // order maps to the tiled one
+    const int order = c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL ? ATH_SWEEP_TILED : c->sweepOrder;
     if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_PENCIL) {
       if (sw == 0)
         for (auto &br : c->blk)
@@ -788,6 +849,20 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
             hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
           ++launches;
         }
+        mark(false);
+      }
+    } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED && c->P.block) {
+      for (int dir = 0; dir < 2; ++dir) {
+        LaunchConnTerms<T>(c);
+        mark(true);
+        hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
+        if (dir == 0)
+          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, true>), dim3(c->msweepGrid), dim3(ath::kLT), 0, c->stream,
+                             c->dblocks, c->P, c->msweep, full);
+        else
+          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, false>), dim3(c->msweepGrid), dim3(ath::kLT), 0, c->stream,
+                             c->dblocks, c->P, c->msweep, full);
+        ++launches;
         mark(false);
       }
     } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED) {

@@ -1,14 +1,25 @@
 """Benchmark: cell-updates/s of aither's hot path on one MI355X.
 
-Workload (a "step" is one nonlinear iteration of the whole hot path: ghost
-cells, fused face-flux residual + spectral radius + dt + diagonal, LU-SGS
-forward and backward sweeps, matrix residual, update and norm reductions):
-inviscid Euler, calorically perfect air, one Cartesian block 64^3 on
-[0,10]^3, isentropic vortex (strength 5, core radius 1) on a Mach 0.5
-freestream, characteristic farfield BCs, Roe flux, MUSCL (kappa = 1/3) with
-van Albada limiter, implicit Euler LU-SGS at CFL 5.  The timed region runs
-`--steps` iterations from the initial condition (200 = the configuration's
-full run); the state is resident in HBM before timing starts.
+A "step" is one nonlinear iteration of the whole hot path over every block:
+ghost cells (boundary patches and interblock/periodic connections), the
+residual (inviscid and viscous face fluxes, spectral radii, dt and the
+implicit main diagonal), the implicit sweeps, the matrix residual, the update
+and the norm reductions.  The state is resident in HBM before timing starts.
+
+Workloads (--workload):
+  * channel (default): batched multiblock laminar Navier-Stokes.  Eight
+    point-matched blocks of uneven sizes (48x32x32 to 96x64x32, 1,032,192
+    cells) forming a 3D channel: no-slip adiabatic bottom wall, wall-normal
+    geometric stretching 1.1 from a first cell of 1e-5 L, spanwise periodic,
+    characteristic inflow/outflow/top; Mach 0.3, Re_L = 1e6, Sutherland air
+    with k/mu = c_p/0.72; AUSMPW+ with MUSCL (kappa = 1/3, van Albada),
+    central viscous flux; BLU-SGS with 5x5 blocks; BDF2 dual time stepping,
+    20 nonlinear iterations per physical step, 50 physical steps (1000
+    steps here); uniform free stream plus seeded 1e-3 random velocity
+    perturbations.
+  * vortex: inviscid Euler, one Cartesian block 64^3 on [0,10]^3, isentropic
+    vortex on a Mach 0.5 free stream, characteristic farfield, Roe, MUSCL,
+    implicit Euler LU-SGS at CFL 5, 200 iterations (the CPU-runnable case).
 
 Prints one JSON line.  Run `python bench.py --help` for options.
 """
@@ -22,53 +33,79 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLUX_BYTES_PER_CELL = 224      # W 40 + geometry 128 + R 40 + lambda 8 + diag 8 (FP64)
-# LU-SGS, FP64 operator: forward pass W 40 + R 40 + U^n 40 + geometry 96 +
-# V 8 + dt 8 + D^-1 8 + x 40 = 280; backward pass W 40 + geometry 96 + D^-1 8
-# + x read 40 + x write 40 = 224; a pass is one forward or backward launch
-SWEEP_BYTES_PER_CELL = {"fwd": 280, "bwd": 224}
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch on the 64^3 workload (FP64), from rocprofv3
-# FETCH_SIZE / WRITE_SIZE scaled by the same counters on a 256 MiB device
-# copy (FETCH_SIZE reads half the copied bytes: factor 2.0; WRITE_SIZE 1.0);
-# see profiles/euler-vortex-64_hbm_counters.md
-FLUX_TRAFFIC_BYTES = 172.2e6
-SWEEP_TRAFFIC_BYTES = 0.5 * (150.7e6 + 132.2e6)   # pencil order, forward and backward
+
+# Algorithmic bytes per physical cell (FP64; each array element once, halos
+# and re-reads excluded).
+# vortex (inviscid, scalar LU-SGS, implicit Euler):
+#   residual kernel: W 40 + geometry 128 + R 40 + lambda 8 + diagonal 8 = 224
+#   forward pass: W 40 + R 40 + U^n 40 + geometry 96 + V 8 + dt 8 + D^-1 8 +
+#   x 40 = 280; backward pass: W 40 + geometry 96 + D^-1 8 + x read 40 + x
+#   write 40 = 224
+VORTEX_FLUX_BYTES = 224
+VORTEX_SWEEP_BYTES = {"fwd": 280, "bwd": 224}
+# HBM bytes per launch on the 64^3 vortex (FP64), from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE scaled by the same counters on a 256 MiB device copy (factor 2.0
+# for FETCH_SIZE, 1.0 for WRITE_SIZE); see profiles/euler-vortex-64_hbm_counters.md
+VORTEX_FLUX_TRAFFIC = 172.2e6
+VORTEX_SWEEP_TRAFFIC = 0.5 * (150.7e6 + 132.2e6)
+# channel (laminar, block LU-SGS, BDF2):
+#   residual stage (inviscid and viscous face fluxes, spectral radii, dt,
+#   block diagonal and inverse): W 40 + geometry 128 + R 40 + lambda 8 + dt 8
+#   + T and mu 16 + D 200 + D^-1 200 + cell velocity gradient 72 = 712
+#   forward pass: W 40 + R 40 + U^n 40 + U^n-1 40 + geometry 96 + V 8 + dt 8 +
+#   D^-1 200 + mu 8 + velocity gradient 72 + centre distances 24 + x 40 = 616
+#   backward pass (first sweep, no initialisation: x - D^-1 U): W 40 +
+#   geometry 96 + D^-1 200 + mu 8 + velocity gradient 72 + centre distances
+#   24 + x read 40 + x write 40 = 520
+CHANNEL_FLUX_BYTES = 712
+CHANNEL_SWEEP_BYTES = {"fwd": 616, "bwd": 520}
 
 
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
-    p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--n", type=int, default=64, help="cells per direction")
+    p.add_argument("--workload", choices=["channel", "vortex"], default="channel")
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed nonlinear iterations (channel: a multiple of 20; default 1000, vortex 200)")
+    p.add_argument("--warmup", type=int, default=None, help="untimed nonlinear iterations (default 20 / 10)")
+    p.add_argument("--refine", type=float, default=1.0, help="channel: scale every cell count")
+    p.add_argument("--n", type=int, default=64, help="vortex: cells per direction")
     p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
-                   help="implicit-operator precision (residual is always FP64)")
+                   help="vortex: implicit-operator precision (residual is always FP64)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-iters", type=int, default=12)
+    p.add_argument("--cpu-iters", type=int, default=None,
+                   help="oracle iterations timed for cpu_baseline (default 2 channel, 12 vortex)")
     p.add_argument("--no-fp32-leg", action="store_true")
     p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
-                   help="LU-SGS execution order (all give identical results)")
+                   help="scalar LU-SGS execution order (all give identical results)")
     p.add_argument("--hbm-calibration", action="store_true",
                    help="only run a device-to-device copy of known size (to scale HBM counters)")
     return p.parse_args()
 
 
-def cpu_baseline(inp, iters):
+def cpu_baseline(inp, iters, what):
     """The oracle (single-threaded FP64 C restatement) on the first `iters`
-    iterations of the same case."""
+    nonlinear iterations of the same case."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     c = pyoracle.OracleCase(inp)
-    ncell = c.ncells()
+    ncell = sum(c.ncells(b) for b in range(c.nb))
     t0 = time.perf_counter()
-    for nn in range(iters):
+    done = 0
+    nn = 0
+    while done < iters:
         c.begin_step(nn)
-        c.iterate(nn, 0)
+        for mm in range(c.nonlinear):
+            if done == iters:
+                break
+            c.iterate(nn, mm)
+            done += 1
+        nn += 1
     dt = time.perf_counter() - t0
     return {"value": ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"first {iters} iterations of the same {c.dims[0][0]}^3 case, oracle C restatement, "
-                      f"1 thread ({dt:.1f} s)"}
+            "sample": f"first {iters} nonlinear iterations of the same {what} ({ncell} cells), "
+                      f"oracle C restatement, 1 thread ({dt:.1f} s)"}
 
 
 def hbm_calibration():
@@ -87,6 +124,174 @@ def hbm_calibration():
                       "bytes_written_per_dispatch": n, "dispatches": 5}), flush=True)
 
 
+def roofline(name, nbytes, avg_s, share, traffic=None, **extra):
+    gbs = nbytes / avg_s / 1e9
+    out = {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
+           "algorithmic_bytes_per_launch": nbytes, "share_of_step": share}
+    out.update(extra)
+    return out
+
+
+def timed_run(s, states0, nphys, warm_phys, prof_phys=None):
+    """warmup (graph capture, caches), restart from the initial condition,
+    then `nphys` physical steps replaying the captured graph, timed by the
+    wall clock with the device synchronised on both sides (ath_run returns
+    after a stream synchronisation); then, from the initial condition again,
+    `prof_phys` physical steps launched eagerly with a HIP event pair around
+    every residual stage and sweep pass (the per-kernel breakdown)"""
+    if warm_phys > 0:
+        s.run(0, warm_phys)
+    for b, W in enumerate(states0):
+        s.set_state(W, b)
+    s.enable_timing(False)
+    t0 = time.perf_counter()
+    last = s.run(0, nphys)
+    wall = time.perf_counter() - t0
+    for b, W in enumerate(states0):
+        s.set_state(W, b)
+    s.enable_timing(True)
+    s.run(0, prof_phys or nphys)
+    tm = s.timing()
+    s.enable_timing(False)
+    return last, wall, tm
+
+
+def bench_channel(args, Solver, cases):
+    steps = args.steps if args.steps is not None else 1000
+    warm = args.warmup if args.warmup is not None else 20
+    work = tempfile.mkdtemp(prefix="aither_bench_")
+    inp, meta = cases.channel_case(work, refine=args.refine, steps=max(1, steps // 20), subits=20)
+    sub = meta["subits"]
+    nphys = max(1, steps // sub)
+    steps = nphys * sub
+    s = Solver(inp)
+    nb = s.info.num_blocks
+    states0 = [s.get_state(b) for b in range(nb)]
+    ncell = sum(s.ncells(b) for b in range(nb))
+    last, wall, tm = timed_run(s, states0, nphys, max(0, warm // sub), prof_phys=min(nphys, 2))
+    value = ncell * steps / wall
+    # one residual stage per block per iteration, one pass = a forward or a
+    # backward sweep over all blocks
+    flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
+    pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
+    flux_share = tm.flux_ms / max(tm.run_ms, 1e-30)
+    sweep_share = tm.sweep_ms / max(tm.run_ms, 1e-30)
+    pass_bytes = 0.5 * (CHANNEL_SWEEP_BYTES["fwd"] + CHANNEL_SWEEP_BYTES["bwd"]) * ncell
+    sweep = roofline("block LU-SGS sweep pass over the eight blocks (KLusgsTile<double, FWD, BLK>, one launch "
+                     "per block), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
+                     launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1))
+    flux = roofline("residual stage of one block (inviscid + viscous fluxes, spectral radii, dt, block diagonal "
+                    "and inverse)", CHANNEL_FLUX_BYTES * ncell / nb, flux_avg_s, flux_share)
+    dominant, other = (sweep, flux) if sweep_share >= flux_share else (flux, sweep)
+    out = {
+        "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": (max(0, warm // sub)) * sub,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated stretched channel grid, uniform free stream plus seeded 1e-3 random "
+                "velocity perturbations, seed 0)",
+        "config": {
+            "workload": "multiblock laminar channel: 8 blocks 48x32x32..96x64x32, no-slip wall, stretching 1.1 "
+                        "from 1e-5 L, spanwise periodic, Mach 0.3, Re_L 1e6, Sutherland air (k/mu = cp/0.72), "
+                        "AUSMPW+ MUSCL, central viscous flux, BLU-SGS 5x5, BDF2 20 subiterations per step",
+            "cells": ncell, "blocks": nb, "physical_steps": nphys, "subiterations": sub,
+            "parallelism": "single GPU",
+        },
+        "roofline": dominant,
+        "roofline_other": other,
+        "final_normalized_residual": list(last.normalized[:5]),
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters or 2, "multiblock channel")
+    return out
+
+
+def bench_vortex(args, Solver, cases):
+    steps = args.steps if args.steps is not None else 200
+    warm = args.warmup if args.warmup is not None else 10
+    work = tempfile.mkdtemp(prefix="aither_bench_")
+    inp = cases.vortex_case(work, n=args.n, iterations=steps)
+    s = Solver(inp)
+    prec = Solver.FP32 if args.precision == "fp32" else Solver.FP64
+    s.set_implicit_precision(prec)
+    order = None if args.sweep == "default" else getattr(Solver, args.sweep.upper())
+    if order is not None:
+        s.set_sweep_ordering(order)
+    W0 = s.get_state()
+    ncell = s.ncells(0)
+    last, wall, tm = timed_run(s, [W0], steps, warm, prof_phys=min(steps, 50))
+    value = ncell * steps / wall
+    flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
+    pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
+    pass_bytes = 0.5 * (VORTEX_SWEEP_BYTES["fwd"] + VORTEX_SWEEP_BYTES["bwd"]) * ncell
+    if prec == Solver.FP32:
+        pass_bytes -= 0.5 * (4 + 20 + 4 + 20 + 20) * ncell  # D^-1 and x words in FP32
+    order_name = args.sweep if args.sweep != "default" else "pencil"
+    out = {
+        "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == Solver.FP64 else "f64-residual/f32-implicit",
+        "data": "synthetic (seed-free analytic isentropic vortex)",
+        "config": {
+            "workload": f"euler-vortex-{args.n}^3: 1 block {args.n}^3 cells, Roe, MUSCL kappa=1/3 "
+                        "van Albada, implicit Euler LU-SGS CFL 5, characteristic farfield",
+            "cells": ncell, "iterations": steps, "parallelism": "single GPU",
+        },
+        "roofline": roofline(
+            f"LU-SGS sweep pass ({order_name} order), average of forward and backward", pass_bytes, pass_avg_s,
+            tm.sweep_ms / max(tm.run_ms, 1e-30),
+            traffic=VORTEX_SWEEP_TRAFFIC if (order_name == "pencil" and args.n == 64 and prec == Solver.FP64)
+            else None, launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1)),
+        "roofline_other": roofline(
+            "KResidual (fused face flux + spectral radius + dt + diagonal)", VORTEX_FLUX_BYTES * ncell,
+            flux_avg_s, tm.flux_ms / max(tm.run_ms, 1e-30),
+            traffic=VORTEX_FLUX_TRAFFIC if args.n == 64 else None),
+        "final_normalized_residual": list(last.normalized[:5]),
+    }
+    s.close()
+    if not args.no_fp32_leg and prec == Solver.FP64:
+        # same run with the FP32 implicit operator: throughput and the
+        # difference of the final state from the FP64-operator run
+        import numpy as np
+        s64 = Solver(inp)
+        if order is not None:
+            s64.set_sweep_ordering(order)
+        s64.run(0, steps)
+        ref = s64.get_state()
+        s64.close()
+        s32 = Solver(inp)
+        s32.set_implicit_precision(Solver.FP32)
+        if order is not None:
+            s32.set_sweep_ordering(order)
+        _, w32, _ = timed_run(s32, [W0], steps, min(warm, 5), prof_phys=1)
+        st = s32.get_state()
+        s32.close()
+        out["fp32_operator"] = {
+            "value": ncell * steps / w32,
+            "state_rel_l2_vs_fp64": float(np.linalg.norm(st - ref) / np.linalg.norm(ref)),
+        }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters or 12, f"{args.n}^3 vortex")
+    return out
+
+
 def main():
     args = parse()
     if args.hbm_calibration:
@@ -97,116 +302,7 @@ def main():
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
     from aither_amd import Solver, cases
-
-    work = tempfile.mkdtemp(prefix="aither_bench_")
-    inp = cases.vortex_case(work, n=args.n, iterations=args.steps)
-    s = Solver(inp)
-    prec = Solver.FP32 if args.precision == "fp32" else Solver.FP64
-    s.set_implicit_precision(prec)
-    order = None if args.sweep == "default" else getattr(Solver, args.sweep.upper())
-    if order is not None:
-        s.set_sweep_ordering(order)
-    W0 = s.get_state()
-    ncell = s.ncells(0)
-
-    # warmup (graph capture, caches), then restart from the initial condition
-    if args.warmup > 0:
-        s.run(0, args.warmup)
-    s.set_state(W0)
-    s.enable_timing(True)
-    t0 = time.perf_counter()
-    last = s.run(0, args.steps)
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    tm = s.timing()
-    value = ncell * args.steps / wall
-
-    flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
-    pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
-    flux_gbs = FLUX_BYTES_PER_CELL * ncell / flux_avg_s / 1e9
-    pass_bytes = 0.5 * (SWEEP_BYTES_PER_CELL["fwd"] + SWEEP_BYTES_PER_CELL["bwd"]) * ncell
-    if prec == Solver.FP32:
-        pass_bytes -= 0.5 * (4 + 20 + 4 + 20 + 20) * ncell  # D^-1 and x words in FP32
-    sweep_gbs = pass_bytes / pass_avg_s / 1e9
-    order_name = args.sweep if args.sweep != "default" else "pencil"
-    out = {
-        "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if prec == Solver.FP64 else "f64-residual/f32-implicit",
-        "data": "synthetic (seed-free analytic isentropic vortex)",
-        "config": {
-            "workload": f"euler-vortex-{args.n}^3: 1 block {args.n}^3 cells, Roe, MUSCL kappa=1/3 "
-                        "van Albada, implicit Euler LU-SGS CFL 5, characteristic farfield",
-            "cells": ncell, "iterations": args.steps, "parallelism": "single GPU",
-        },
-        # the dominant kernel: one LU-SGS sweep pass (forward or backward)
-        "roofline": {
-            "kernel": f"LU-SGS sweep pass ({order_name} order), average of forward and backward",
-            "bound": "hbm",
-            "achieved": sweep_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": sweep_gbs / HBM_PEAK_GBS,
-            "traffic": SWEEP_TRAFFIC_BYTES if (order_name == "pencil" and args.n == 64 and
-                                               prec == Solver.FP64) else None,
-            "avg_launch_us": pass_avg_s * 1e6,
-            "algorithmic_bytes_per_launch": pass_bytes,
-            "launches_per_pass": tm.sweep_launches / max(tm.sweep_passes, 1),
-            "share_of_step": tm.sweep_ms / max(tm.run_ms, 1e-30),
-        },
-        "roofline_flux": {
-            "kernel": "KResidual (fused face flux + spectral radius + dt + diagonal)",
-            "bound": "hbm",
-            "achieved": flux_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": flux_gbs / HBM_PEAK_GBS,
-            "traffic": FLUX_TRAFFIC_BYTES if args.n == 64 else None,
-            "avg_launch_us": flux_avg_s * 1e6,
-            "algorithmic_bytes_per_launch": FLUX_BYTES_PER_CELL * ncell,
-            "share_of_step": tm.flux_ms / max(tm.run_ms, 1e-30),
-        },
-        "final_normalized_residual": list(last.normalized[:5]),
-    }
-    s.close()
-
-    if not args.no_fp32_leg and prec == Solver.FP64:
-        # same run with the FP32 implicit operator: throughput and the
-        # difference of the final state from the FP64-operator run
-        import numpy as np
-        ref = None
-        s64 = Solver(inp)
-        if order is not None:
-            s64.set_sweep_ordering(order)
-        s64.run(0, args.steps)
-        ref = s64.get_state()
-        s64.close()
-        s32 = Solver(inp)
-        s32.set_implicit_precision(Solver.FP32)
-        if order is not None:
-            s32.set_sweep_ordering(order)
-        s32.run(0, min(args.warmup, 5))
-        s32.set_state(W0)
-        t0 = time.perf_counter()
-        s32.run(0, args.steps)
-        w32 = time.perf_counter() - t0
-        st = s32.get_state()
-        s32.close()
-        out["fp32_operator"] = {
-            "value": ncell * args.steps / w32,
-            "state_rel_l2_vs_fp64": float(np.linalg.norm(st - ref) / np.linalg.norm(ref)),
-        }
-
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters)
+    out = bench_channel(args, Solver, cases) if args.workload == "channel" else bench_vortex(args, Solver, cases)
     print(json.dumps(out), flush=True)
 
 

@@ -104,7 +104,7 @@ def test_block_solvers_step_parity(tmp_path, solver, sweeps, timeint):
     s.close()
 
 
-def test_block_solver_ignores_sweep_ordering_and_refuses_fp32(tmp_path):
+def test_block_sweep_orderings_agree_and_fp32_is_refused(tmp_path):
     path = _bump(tmp_path)
     out = []
     for order in ("PENCIL", "TILED", "HYPERPLANE"):
