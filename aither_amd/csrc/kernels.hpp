@@ -37,6 +37,12 @@ struct DevBlock {
   // Mhi = the upper cell's block as seen from the lower cell; 25 * faces(d)
   // each, entry q of face f at q * faces(d) + f
   double *Mlo[3], *Mhi[3];
+  // block solvers with the viscous terms: what the viscous pass computed at
+  // every physical face of direction d (velocity gradient 9, face state 5,
+  // face viscosity 1, face conductivity before scaling 1), 16 * faces(d),
+  // entry q of face f at q * faces(d) + f; the thin-shear-layer part of the
+  // block diagonal reads it back
+  double *fvis[3];
   // off-diagonal product of the ghost neighbour across each boundary face
   // that lies on an interblock connection (zero on other faces), per sweep
   // pass; null when the block has no connection.  5 * nbf, SoA.
@@ -45,6 +51,7 @@ struct DevBlock {
   long nbf;              // boundary faces: 2 (nj nk + nk ni + ni nj)
   // viscous cases only
   double *T, *mu;               // padded: temperature, viscosity
+  double *kc;                   // padded: conductivity before scaling (block solvers only)
   const double4 *faP[3];        // padded faces (ghost faces included)
   const double *volP;           // padded volumes
   const double *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
@@ -531,6 +538,7 @@ __global__ void __launch_bounds__(256) KAux(DevBlock b, Params P) {
   const double t = TempOf(Cast<double>(P.gas), b.W[q], b.W[4 * b.npad + q]);
   b.T[q] = t;
   b.mu[q] = SutherlandMu(P.visc, t);
+  if (b.kc) b.kc[q] = SutherlandK(P.visc, t);  // read by the block Jacobians
 }
 
 // CalcGradsI/J/K at a face (src/procBlock.cpp:5173-5788) with VectorGradGG /
@@ -644,7 +652,20 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
   const bool wall = (f3 == 0 || f3 == n3) && b.wallf[BtSlot(b, d, f3 == 0 ? 0 : 1, i, j, k)];
   double Wf[5], muF;
   ViscFaceState<R4>(b, d, i, j, k, Wf, muF);
-  const double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+  const long fq = FaceIdx(b, d, i, j, k);
+  if (b.fvis[0]) {
+    const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
+    double *o = b.fvis[d];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[(r * 3 + c) * nf + fq] = vg[r][c];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o[(9 + e) * nf + fq] = Wf[e];
+    o[14 * nf + fq] = muF;
+    o[15 * nf + fq] = SutherlandK(P.visc, TempOf(g, Wf[0], Wf[4]));
+  }
+  const double4 a = b.fa[d][fq];
   const double n[3] = {a.x, a.y, a.z};
   const double mu = P.visc.vscale * muF, mut = P.visc.vscale * 0.0;
   const double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
@@ -673,9 +694,12 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
 // DelprimitiveDelConservative (:612-662) on the right (MatrixMultiply,
 // src/matrix.cpp:186-199).  `left` gives the shear term the lower cell's
 // sign; vg is the velocity gradient d v_c / d x_r at [3 r + c].
+// kLam is Sutherland's conductivity at the state's temperature before the
+// nondimensional scaling (formed once per cell or face by the kernels that
+// hold it).
 __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscParams &vp, const double W[5],
-                                            double muLam, double4 a, double dist, bool left, const double vg[9],
-                                            double out[25]) {
+                                            double muLam, double kLam, double4 a, double dist, bool left,
+                                            const double vg[9], double out[25]) {
 #pragma clang fp contract(off)
   const double t = TempOf(g, W[0], W[4]);
   const double mu = vp.vscale * muLam, mut = vp.vscale * 0.0;
@@ -683,7 +707,7 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
   const double u = W[1], v = W[2], w = W[3];
   const double vn = (u * nx + v * ny) + w * nz;
   const double rho = 0.0 + W[0];
-  const double k = SutherlandK(vp, t) * vp.vscale;
+  const double k = kLam * vp.vscale;
   const double kt = mut * g.cps / 0.9;
   // TauNormal (src/utility.cpp:426-436)
   const double lambda = 0.0 - (2.0 / 3.0) * (mu + mut);
@@ -913,12 +937,17 @@ __global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *fla
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
         const int fi = i + (side && d == 0), fj = j + (side && d == 1), fk = k + (side && d == 2);
-        double vg[3][3], tg[3], Wf[5], muF, J[25];
-        FaceGrads(b, d, fi, fj, fk, vg, tg);
-        ViscFaceState<MODE == 3>(b, d, fi, fj, fk, Wf, muF);
         const long fq = FaceIdx(b, d, fi, fj, fk);
-        const double vgr[9] = {vg[0][0], vg[0][1], vg[0][2], vg[1][0], vg[1][1], vg[1][2], vg[2][0], vg[2][1], vg[2][2]};
-        TSLJacobian(g, P.visc, Wf, muF, b.fa[d][fq], b.c2c[d][fq], side == 1, vgr, J);
+        const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
+        // the face's gradient, state and viscosity as the viscous pass formed them
+        const double *fv = b.fvis[d];
+        double vgr[9], Wf[5], J[25];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vgr[q] = fv[q * nf + fq];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) Wf[e] = fv[(9 + e) * nf + fq];
+        const double muF = fv[14 * nf + fq], kF = fv[15 * nf + fq];
+        TSLJacobian(g, P.visc, Wf, muF, kF, b.fa[d][fq], b.c2c[d][fq], side == 1, vgr, J);
 #pragma unroll
         for (int q = 0; q < 25; ++q) D[q] = side ? D[q] - J[q] : D[q] + J[q];
 #pragma unroll
@@ -952,7 +981,8 @@ __global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *fla
 template <typename T>
 __device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], const T du[5], double4 a,
                                              bool positive, T out[5], const ViscParams *vp = nullptr,
-                                             double mu = 0.0, double dist = 1.0, const double *vg = nullptr) {
+                                             double mu = 0.0, double kLam = 0.0, double dist = 1.0,
+                                             const double *vg = nullptr) {
   T J[25];
   RusanovJacobian<T>(g, W, T(a.x), T(a.y), T(a.z), T(a.w), positive, J);
   if (vp && vp->on) {
@@ -960,7 +990,7 @@ __device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], con
     double Wd[5], V[25];
 #pragma unroll
     for (int e = 0; e < 5; ++e) Wd[e] = double(W[e]);
-    TSLJacobian(gd, *vp, Wd, mu, a, dist, positive, vg, V);
+    TSLJacobian(gd, *vp, Wd, mu, kLam, a, dist, positive, vg, V);
 #pragma unroll
     for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - T(V[q]) : J[q] + T(V[q]);
   }
@@ -970,23 +1000,22 @@ __device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], con
 // the off-diagonal blocks of the interior faces of direction d: exactly the
 // matrices RusanovBlockOffDiagonal forms for each product, formed once per
 // relax instead of once per product (the state does not change during the
-// sweeps); one thread per face
+// sweeps); one thread per face and side (lower cell's block, then upper's)
 __global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d) {
 #pragma clang fp contract(off)
   const int FI = b.ni + (d == 0), FJ = b.nj + (d == 1), FK = b.nk + (d == 2);
   const long nf = (long)FI * FJ * FK;
-  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nf) return;
+  const int side = t >= nf ? 1 : 0;
+  const long f = t - side * nf;
   const int i = (int)(f % FI), j = (int)((f / FI) % FJ), k = (int)(f / ((long)FI * FJ));
   const int f3 = d == 0 ? i : (d == 1 ? j : k), n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
   if (f3 == 0 || f3 == n3) return;  // boundary faces: connection terms, formed per pass
   const GasT<double> g = Cast<double>(P.gas);
   const double4 a = b.fa[d][f];
-  const long cU = PadIdx(b, i, j, k);
-  const long cL = PadIdx(b, i - (d == 0), j - (d == 1), k - (d == 2));
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const long c = side ? cU : cL;
+  {
+    const long c = side ? PadIdx(b, i, j, k) : PadIdx(b, i - (d == 0), j - (d == 1), k - (d == 2));
     double W[5], J[25];
 #pragma unroll
     for (int e = 0; e < 5; ++e) W[e] = b.W[e * b.npad + c];
@@ -996,7 +1025,7 @@ __global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d
       double vg[9], V[25];
 #pragma unroll
       for (int q = 0; q < 9; ++q) vg[q] = b.vgrad[q * b.npad + c];
-      TSLJacobian(g, P.visc, W, b.mu[c], a, b.c2c[d][f], positive, vg, V);
+      TSLJacobian(g, P.visc, W, b.mu[c], b.kc[c], a, b.c2c[d][f], positive, vg, V);
 #pragma unroll
       for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - V[q] : J[q] + V[q];
     }
@@ -1597,7 +1626,7 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
 #pragma unroll
       for (int q = 0; q < 9; ++q) vgn[q] = A.vgrad[q * A.npad + c.gpad];
       BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.mu[c.gpad],
-                      A.c2c[c.d][c.face], vgn);
+                      A.kc[c.gpad], A.c2c[c.d][c.face], vgn);
     } else {
       BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
     }

@@ -532,7 +532,8 @@ int Upload(ath_ctx *c) {
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
     d.Ab = d.Abi = nullptr;
     d.vgrad = nullptr;
-    for (int dd = 0; dd < 3; ++dd) d.Mlo[dd] = d.Mhi[dd] = nullptr;
+    d.kc = nullptr;
+    for (int dd = 0; dd < 3; ++dd) d.Mlo[dd] = d.Mhi[dd] = d.fvis[dd] = nullptr;
     if (hc.block) {
       if (int r = c->Alloc(&d.Ab, 25 * nc)) return r;
       if (int r = c->Alloc(&d.Abi, 25 * nc)) return r;
@@ -540,10 +541,14 @@ int Upload(ath_ctx *c) {
         const long nf = (long)(h.ni + (dd == 0)) * (h.nj + (dd == 1)) * (h.nk + (dd == 2));
         if (int r = c->Alloc(&d.Mlo[dd], 25 * nf)) return r;
         if (int r = c->Alloc(&d.Mhi[dd], 25 * nf)) return r;
+        if (hc.viscous)
+          if (int r = c->Alloc(&d.fvis[dd], 16 * nf)) return r;
       }
       if (hc.viscous) {
         if (int r = c->Alloc(&d.vgrad, 9 * np)) return r;
         HIP_OK(hipMemset(d.vgrad, 0, 9 * np * sizeof(double)));
+        if (int r = c->Alloc(&d.kc, np)) return r;
+        HIP_OK(hipMemset(d.kc, 0, np * sizeof(double)));
       }
     }
     d.Ainv32 = nullptr;
@@ -820,7 +825,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     for (auto &br : c->blk)
       for (int d = 0; d < 3; ++d) {
         const long nf = (long)(br.d.ni + (d == 0)) * (br.d.nj + (d == 1)) * (br.d.nk + (d == 2));
-        hipLaunchKernelGGL(ath::KBlockOffMats, dim3(Grid(nf, 128)), dim3(128), 0, c->stream, br.d, c->P, d);
+        hipLaunchKernelGGL(ath::KBlockOffMats, dim3(Grid(2 * nf, 128)), dim3(128), 0, c->stream, br.d, c->P, d);
       }
   for (auto &br : c->blk)
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
