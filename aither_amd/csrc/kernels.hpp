@@ -653,18 +653,6 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
   double Wf[5], muF;
   ViscFaceState<R4>(b, d, i, j, k, Wf, muF);
   const long fq = FaceIdx(b, d, i, j, k);
-  if (b.fvis[0]) {
-    const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
-    double *o = b.fvis[d];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o[(r * 3 + c) * nf + fq] = vg[r][c];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) o[(9 + e) * nf + fq] = Wf[e];
-    o[14 * nf + fq] = muF;
-    o[15 * nf + fq] = SutherlandK(P.visc, TempOf(g, Wf[0], Wf[4]));
-  }
   const double4 a = b.fa[d][fq];
   const double n[3] = {a.x, a.y, a.z};
   const double mu = P.visc.vscale * muF, mut = P.visc.vscale * 0.0;
@@ -678,8 +666,21 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
     tau[r] = lambda * trace * n[r] + (mu + mut) * sa;
   }
   const double t = TempOf(g, Wf[0], Wf[4]);
-  const double kk = SutherlandK(P.visc, t) * P.visc.vscale;
+  const double kLam = SutherlandK(P.visc, t);
+  const double kk = kLam * P.visc.vscale;
   const double kt = mut * g.cps / 0.9;
+  if (b.fvis[0]) {
+    const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
+    double *o = b.fvis[d];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[(r * 3 + c) * nf + fq] = vg[r][c];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o[(9 + e) * nf + fq] = Wf[e];
+    o[14 * nf + fq] = muF;
+    o[15 * nf + fq] = kLam;
+  }
   const double tv = tau[0] * Wf[1] + tau[1] * Wf[2] + tau[2] * Wf[3];
   const double q = (kk + kt) * (tg[0] * n[0] + tg[1] * n[1] + tg[2] * n[2]);
   F[0] = 0.0 * a.w;
@@ -1886,16 +1887,25 @@ __global__ void KMultiSweepPrologue(MultiSweepCtl s) {
   *s.epoch = *s.epoch + 1;
 }
 
+// tiles of 8 x 8 x 4 cells (four wavefronts): smaller tiles shorten the
+// tile's own hyperplane walk but multiply the ticket and flag traffic, which
+// costs more (measured: 4 x 4 x 4 tiles take 1.3 times as long per pass)
+constexpr int kBTI = 8, kBTJ = 8, kBTK = 4;
+constexpr int kBT = kBTI * kBTJ * kBTK;
+constexpr int kBTPlanes = kBTI + kBTJ + kBTK - 2;
+constexpr int kBHaloI = kBTJ * kBTK, kBHaloJ = kBTI * kBTK, kBHaloK = kBTI * kBTJ;
+constexpr int kBHalo = kBHaloI + kBHaloJ + kBHaloK;
+
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
+__global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
                                                         MultiSweepCtl s, int full) {
-  __shared__ T sX[5][kLT];
-  __shared__ T sHX[5][kHalo];  // halo du
+  __shared__ T sX[5][kBT];
+  __shared__ T sHX[5][kBHalo];  // halo du
   __shared__ int sTicket;
   __shared__ int sEpoch;
 
   const int tid = threadIdx.x;
-  const int li = tid % kLTI, lj = (tid / kLTI) % kLTJ, lk = tid / (kLTI * kLTJ);
+  const int li = tid % kBTI, lj = (tid / kBTI) % kBTJ, lk = tid / (kBTI * kBTJ);
   const GasT<T> g = Cast<T>(P.gas);
   const int sgn = FWD ? -1 : 1;  // direction of the dependency neighbours
 
@@ -1913,7 +1923,7 @@ __global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restri
     const DevBlock &b = blocks[tl.w];
     const int4 gr = s.grid[tl.w];
     T *x = OpBuf<T>::X(b);
-    const int i = tl.x * kLTI + li, j = tl.y * kLTJ + lj, k = tl.z * kLTK + lk;
+    const int i = tl.x * kBTI + li, j = tl.y * kBTJ + lj, k = tl.z * kBTK + lk;
     const bool valid = i < b.ni && j < b.nj && k < b.nk;
     const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
 
@@ -1961,13 +1971,13 @@ __global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restri
 
     // halo du, published by the dependency tiles
     {
-      const int di = FWD ? -1 : kLTI, dj = FWD ? -1 : kLTJ, dk = FWD ? -1 : kLTK;
-      if (tid < kHalo) {
+      const int di = FWD ? -1 : kBTI, dj = FWD ? -1 : kBTJ, dk = FWD ? -1 : kBTK;
+      if (tid < kBHalo) {
         int hi, hj, hk;
-        if (tid < kHaloI) { hi = di; hj = tid % kLTJ; hk = tid / kLTJ; }
-        else if (tid < kHaloI + kHaloJ) { const int q = tid - kHaloI; hi = q % kLTI; hj = dj; hk = q / kLTI; }
-        else { const int q = tid - kHaloI - kHaloJ; hi = q % kLTI; hj = q / kLTI; hk = dk; }
-        const int gi = tl.x * kLTI + hi, gj = tl.y * kLTJ + hj, gk = tl.z * kLTK + hk;
+        if (tid < kBHaloI) { hi = di; hj = tid % kBTJ; hk = tid / kBTJ; }
+        else if (tid < kBHaloI + kBHaloJ) { const int q = tid - kBHaloI; hi = q % kBTI; hj = dj; hk = q / kBTI; }
+        else { const int q = tid - kBHaloI - kBHaloJ; hi = q % kBTI; hj = q / kBTI; hk = dk; }
+        const int gi = tl.x * kBTI + hi, gj = tl.y * kBTJ + hj, gk = tl.z * kBTK + hk;
         if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
           const long q = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
 #pragma unroll
@@ -1983,8 +1993,8 @@ __global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restri
 
     // --- intra-tile hyperplanes
     const int plane = li + lj + lk;
-    for (int st = 0; st < kLTPlanes; ++st) {
-      const int pl = FWD ? st : kLTPlanes - 1 - st;
+    for (int st = 0; st < kBTPlanes; ++st) {
+      const int pl = FWD ? st : kBTPlanes - 1 - st;
       if (valid && plane == pl) {
         T D[5];  // dependency-side sum: L (forward) or U (backward), i, j, k order
 #pragma unroll
@@ -2002,15 +2012,15 @@ __global__ void __launch_bounds__(kLT) KBlockSweepTiles(const DevBlock *__restri
             continue;
           }
           const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
-          const int L3 = d == 0 ? kLTI : (d == 1 ? kLTJ : kLTK);
+          const int L3 = d == 0 ? kBTI : (d == 1 ? kBTJ : kBTK);
           const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
           T dn[5], o[5], m[25];
           if (inTile) {
-            const int nt = tid + sgn * (d == 0 ? 1 : (d == 1 ? kLTI : kLTI * kLTJ));
+            const int nt = tid + sgn * (d == 0 ? 1 : (d == 1 ? kBTI : kBTI * kBTJ));
 #pragma unroll
             for (int e = 0; e < 5; ++e) dn[e] = sX[e][nt];
           } else {
-            const int h = d == 0 ? lj + kLTJ * lk : (d == 1 ? kHaloI + li + kLTI * lk : kHaloI + kHaloJ + li + kLTI * lj);
+            const int h = d == 0 ? lj + kBTJ * lk : (d == 1 ? kBHaloI + li + kBTI * lk : kBHaloI + kBHaloJ + li + kBTI * lj);
 #pragma unroll
             for (int e = 0; e < 5; ++e) dn[e] = sHX[e][h];
           }

@@ -376,7 +376,7 @@ int UploadMultiSweep(ath_ctx *c) {
   std::vector<int4> grid, tiles;
   int base = 0, maxPlane = 0;
   for (auto &br : c->blk) {
-    const int ta = (br.d.ni + kLTI - 1) / kLTI, tb = (br.d.nj + kLTJ - 1) / kLTJ, tc = (br.d.nk + kLTK - 1) / kLTK;
+    const int ta = (br.d.ni + kBTI - 1) / kBTI, tb = (br.d.nj + kBTJ - 1) / kBTJ, tc = (br.d.nk + kBTK - 1) / kBTK;
     grid.push_back(make_int4(ta, tb, tc, base));
     base += ta * tb * tc;
     maxPlane = std::max(maxPlane, ta + tb + tc - 2);
@@ -862,10 +862,10 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         mark(true);
         hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
         if (dir == 0)
-          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, true>), dim3(c->msweepGrid), dim3(ath::kLT), 0, c->stream,
+          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, true>), dim3(c->msweepGrid), dim3(ath::kBT), 0, c->stream,
                              c->dblocks, c->P, c->msweep, full);
         else
-          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, false>), dim3(c->msweepGrid), dim3(ath::kLT), 0, c->stream,
+          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, false>), dim3(c->msweepGrid), dim3(ath::kBT), 0, c->stream,
                              c->dblocks, c->P, c->msweep, full);
         ++launches;
         mark(false);
