@@ -1887,9 +1887,10 @@ __global__ void KMultiSweepPrologue(MultiSweepCtl s) {
   *s.epoch = *s.epoch + 1;
 }
 
-// tiles of 8 x 8 x 4 cells (four wavefronts): smaller tiles shorten the
-// tile's own hyperplane walk but multiply the ticket and flag traffic, which
-// costs more (measured: 4 x 4 x 4 tiles take 1.3 times as long per pass)
+// tiles of 8 x 8 x 4 cells (four wavefronts).  Measured on the multiblock
+// channel: 4 x 4 x 4 tiles (more tickets and flags) take 1.3 times as long
+// per pass, 16 x 4 x 4 tiles (whole cache lines per tile row, but a longer
+// walk of 22 cell hyperplanes) 1.12 times as long.
 constexpr int kBTI = 8, kBTJ = 8, kBTK = 4;
 constexpr int kBT = kBTI * kBTJ * kBTK;
 constexpr int kBTPlanes = kBTI + kBTJ + kBTK - 2;
