@@ -60,6 +60,11 @@ VORTEX_SWEEP_TRAFFIC = 0.5 * (150.7e6 + 132.2e6)
 #   24 + x read 40 + x write 40 = 520
 CHANNEL_FLUX_BYTES = 712
 CHANNEL_SWEEP_BYTES = {"fwd": 616, "bwd": 520}
+# HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
+# see profiles/channel-multiblock-1M_hbm_counters.md
+CHANNEL_SWEEP_TRAFFIC = 0.5 * (2427.5e6 + 2065.6e6)   # one pass, forward and backward averaged
+CHANNEL_FLUX_TRAFFIC = 650.9e6                        # residual stage of one block
 
 
 def parse():
@@ -178,11 +183,14 @@ def bench_channel(args, Solver, cases):
     flux_share = tm.flux_ms / max(tm.run_ms, 1e-30)
     sweep_share = tm.sweep_ms / max(tm.run_ms, 1e-30)
     pass_bytes = 0.5 * (CHANNEL_SWEEP_BYTES["fwd"] + CHANNEL_SWEEP_BYTES["bwd"]) * ncell
-    sweep = roofline("block LU-SGS sweep pass over the eight blocks (KLusgsTile<double, FWD, BLK>, one launch "
-                     "per block), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
+    full = args.refine == 1.0
+    sweep = roofline("block LU-SGS sweep pass over all eight blocks (KBlockSweepTiles<double, FWD>, one "
+                     "launch), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
+                     traffic=CHANNEL_SWEEP_TRAFFIC if full else None,
                      launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1))
     flux = roofline("residual stage of one block (inviscid + viscous fluxes, spectral radii, dt, block diagonal "
-                    "and inverse)", CHANNEL_FLUX_BYTES * ncell / nb, flux_avg_s, flux_share)
+                    "and inverse)", CHANNEL_FLUX_BYTES * ncell / nb, flux_avg_s, flux_share,
+                    traffic=CHANNEL_FLUX_TRAFFIC if full else None)
     dominant, other = (sweep, flux) if sweep_share >= flux_share else (flux, sweep)
     out = {
         "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
