@@ -1018,6 +1018,7 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
   for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (c->l2first[e] + eps);
   if (out) *out = n;
   if (flag & 2) return Fail(ATH_ERR_DEVICE, "a sweep dependency wait exceeded its spin limit");
+  if (flag & 4) return Fail(ATH_ERR_DEVICE, "singular block main diagonal in Gauss-Jordan elimination");
   if (flag & 1) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
   return ATH_OK;
 }
