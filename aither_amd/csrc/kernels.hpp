@@ -92,12 +92,12 @@ __device__ __forceinline__ double TempOf(const GasT<double> &g, double rho, doub
 // 97-117), nondimensional; t^1.5 correctly rounded
 __device__ __forceinline__ double SutherlandMu(const ViscParams &v, double t) {
   const double temp = t * v.tRef;
-  const double mu = (v.c1 * PowCR(temp, 1.5)) / (temp + v.s1);
+  const double mu = (v.c1 * Pow15CR(temp)) / (temp + v.s1);
   return mu / v.muRef;
 }
 __device__ __forceinline__ double SutherlandK(const ViscParams &v, double t) {
   const double temp = t * v.tRef;
-  const double k = (v.c2 * PowCR(temp, 1.5)) / (temp + v.s2);
+  const double k = (v.c2 * Pow15CR(temp)) / (temp + v.s2);
   return k / v.kNonDim;
 }
 // max(4/(3 rho), gamma/rho) and NondimScaling (mu / Pr + mut / Prt), mut = 0
