@@ -28,15 +28,18 @@ struct DevBlock {
   double *xold;          // 5 * ncell, DP-LUR Jacobi copy
   float *xold32;
   double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
-  // block matrix solvers: main diagonal after AddDiagonalTerms and its
-  // inverse, 25 * ncell each, entry (r, c) of cell p at (5 r + c) ncell + p
+  // block matrix solvers: main diagonal after AddDiagonalTerms, 25 * ncell,
+  // entry (r, c) of cell p at (5 r + c) ncell + p; its inverse and the
+  // off-diagonal blocks in the tile-blocked layout of TileBase (the sweep's
+  // 8 x 8 x 4 tiles: one tile's values of one entry are 256 consecutive
+  // doubles, so a wavefront reads whole cache lines no other tile shares)
   double *Ab, *Abi;
-  // block matrix solvers: the off-diagonal blocks of every interior face of
-  // direction d, formed once per relax (they depend on the state only):
-  // Mlo = the lower cell's block as seen from the upper cell (positive),
-  // Mhi = the upper cell's block as seen from the lower cell; 25 * faces(d)
-  // each, entry q of face f at q * faces(d) + f
-  double *Mlo[3], *Mhi[3];
+  // the off-diagonal blocks, formed once per relax (they depend on the state
+  // only), stored at the cell whose row uses them: Lc[d] = the lower
+  // neighbour's block across the cell's lower face of direction d
+  // (positive), Uc[d] = the upper neighbour's block across its upper face
+  double *Lc[3], *Uc[3];
+  int tni, tnj, tnk;  // tile grid of the layout
   // block solvers with the viscous terms: what the viscous pass computed at
   // every physical face of direction d (velocity gradient 9, face state 5,
   // face viscosity 1, face conductivity before scaling 1), 16 * faces(d),
@@ -148,6 +151,12 @@ __device__ __forceinline__ long BtSlot(const DevBlock &b, int d, int upper, int 
   if (d == 0) return upper * sI + j + (long)b.nj * k;
   if (d == 1) return 2 * sI + upper * sJ + k + (long)b.nk * i;
   return 2 * sI + 2 * sJ + upper * sK + i + (long)b.ni * j;
+}
+// tile-blocked layout of the block solvers' per-cell 5x5 blocks: entry q of
+// cell (i, j, k) at TileBase(b, i, j, k) + 256 q
+__device__ __forceinline__ long TileBase(const DevBlock &b, int i, int j, int k) {
+  const long tile = (long)(i >> 3) + (long)b.tni * ((j >> 3) + (long)b.tnj * (k >> 2));
+  return tile * (25 * 256) + ((i & 7) + 8 * ((j & 7) + 8 * (k & 3)));
 }
 __device__ __forceinline__ void Map3(int dir, int d3, int d1, int d2, int &i, int &j, int &k) {
   if (dir == 0) { i = d3; j = d1; k = d2; }
@@ -971,7 +980,7 @@ __global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *fla
   double I[25];
   if (!GJInverse5<double>(D, I)) atomicOr(flag, 4);
 #pragma unroll
-  for (int q = 0; q < 25; ++q) b.Abi[q * b.ncell + p] = I[q];
+  for (int q = 0; q < 25; ++q) b.Abi[TileBase(b, i, j, k) + 256 * q] = I[q];
 }
 
 // RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194): the neighbour's
@@ -1030,9 +1039,11 @@ __global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d
 #pragma unroll
       for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - V[q] : J[q] + V[q];
     }
-    double *M = side ? b.Mhi[d] : b.Mlo[d];
+    // the lower cell's block is used by the upper cell (its Lc), and vice versa
+    const long base = side ? TileBase(b, i - (d == 0), j - (d == 1), k - (d == 2)) : TileBase(b, i, j, k);
+    double *M = side ? b.Uc[d] : b.Lc[d];
 #pragma unroll
-    for (int q = 0; q < 25; ++q) M[q * nf + f] = J[q];
+    for (int q = 0; q < 25; ++q) M[base + 256 * q] = J[q];
   }
 }
 
@@ -1051,10 +1062,16 @@ __device__ __forceinline__ void BlockMultG(const double *__restrict__ M, long st
   }
 }
 
-// block product with the cell's stored matrix (Ab or Abi)
+// block product with the cell's main diagonal Ab (cell-major layout)
 template <typename T>
 __device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
   BlockMultG<T>(M, b.ncell, p, v, out);
+}
+// block product with a tile-blocked per-cell block (Abi, Lc, Uc)
+template <typename T>
+__device__ __forceinline__ void TileBlockMult(const DevBlock &b, const double *M, int i, int j, int k, const T v[5],
+                                              T out[5]) {
+  BlockMultG<T>(M, 256, TileBase(b, i, j, k), v, out);
 }
 
 template <typename T>
@@ -1110,9 +1127,8 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       du[e] = x[e * b.ncell + q];
     }
     if (block) {
-      // the face's precomputed block of this neighbour
-      const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
-      BlockMultG<T>(upper ? b.Mhi[d] : b.Mlo[d], nf, fq, du, o);
+      // the precomputed block of this neighbour, kept at the cell
+      TileBlockMult<T>(b, upper ? b.Uc[d] : b.Lc[d], i, j, k, du, o);
     } else {
       const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
       ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
@@ -1140,7 +1156,7 @@ __global__ void __launch_bounds__(256) KInitUpdate(DevBlock b, Params P) {
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = T(rb[e]);
-    CellBlockMult<T>(b, b.Abi, p, v, o);
+    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1175,7 +1191,7 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + L[e];
-    CellBlockMult<T>(b, b.Abi, p, v, o);
+    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1207,11 +1223,11 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
       Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
       for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + L[e]) - U[e];
-      CellBlockMult<T>(b, b.Abi, p, v, o);
+      TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
 #pragma unroll
       for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     } else {
-      CellBlockMult<T>(b, b.Abi, p, U, o);
+      TileBlockMult<T>(b, b.Abi, i, j, k, U, o);
 #pragma unroll
       for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = x[e * b.ncell + p] - o[e];
     }
@@ -1250,7 +1266,7 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + T(0)) + (L[e] - U[e]);
-    CellBlockMult<T>(b, b.Abi, p, v, o);
+    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1937,14 +1953,13 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
       for (int d = 0; d < 3; ++d) {
         const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
         const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk;
-        const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
-        const long fq = inside ? (FWD ? FaceIdx(b, d, i, j, k) : FaceIdx(b, d, gi, gj, gk)) : 0;
-        const double *M = FWD ? b.Mlo[d] : b.Mhi[d];
+        const double *M = (FWD ? b.Lc[d] : b.Uc[d]) + TileBase(b, i, j, k);
 #pragma unroll
-        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[q * nf + fq] : 0.0;
+        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[256 * q] : 0.0;
       }
+      const double *Mi = b.Abi + TileBase(b, i, j, k);
 #pragma unroll
-      for (int q = 0; q < 25; ++q) mai[q] = b.Abi[q * b.ncell + p];
+      for (int q = 0; q < 25; ++q) mai[q] = Mi[256 * q];
     }
 
     // --- wait for the tiles of the same block this one depends on

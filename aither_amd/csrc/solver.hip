@@ -533,14 +533,19 @@ int Upload(ath_ctx *c) {
     d.Ab = d.Abi = nullptr;
     d.vgrad = nullptr;
     d.kc = nullptr;
-    for (int dd = 0; dd < 3; ++dd) d.Mlo[dd] = d.Mhi[dd] = d.fvis[dd] = nullptr;
+    for (int dd = 0; dd < 3; ++dd) d.Lc[dd] = d.Uc[dd] = d.fvis[dd] = nullptr;
+    d.tni = (h.ni + 7) / 8;
+    d.tnj = (h.nj + 7) / 8;
+    d.tnk = (h.nk + 3) / 4;
     if (hc.block) {
+      // tile-blocked per-cell blocks: 25 entries of 256 cells per 8x8x4 tile
+      const long tb = 25L * 256 * d.tni * d.tnj * d.tnk;
       if (int r = c->Alloc(&d.Ab, 25 * nc)) return r;
-      if (int r = c->Alloc(&d.Abi, 25 * nc)) return r;
+      if (int r = c->Alloc(&d.Abi, tb)) return r;
       for (int dd = 0; dd < 3; ++dd) {
         const long nf = (long)(h.ni + (dd == 0)) * (h.nj + (dd == 1)) * (h.nk + (dd == 2));
-        if (int r = c->Alloc(&d.Mlo[dd], 25 * nf)) return r;
-        if (int r = c->Alloc(&d.Mhi[dd], 25 * nf)) return r;
+        if (int r = c->Alloc(&d.Lc[dd], tb)) return r;
+        if (int r = c->Alloc(&d.Uc[dd], tb)) return r;
         if (hc.viscous)
           if (int r = c->Alloc(&d.fvis[dd], 16 * nf)) return r;
       }
@@ -1389,10 +1394,24 @@ int ath_get_block_diagonal(ath_ctx *c, int b, int inverse, double *a) {
   HIP_OK(hipStreamSynchronize(c->stream));
   const auto &d = c->blk[b].d;
   const long n = d.ncell;
-  std::vector<double> t(25 * (size_t)n);
-  HIP_OK(hipMemcpy(t.data(), inverse ? d.Abi : d.Ab, t.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (long p = 0; p < n; ++p)
-    for (int q = 0; q < 25; ++q) a[25 * p + q] = t[q * n + p];
+  if (!inverse) {
+    std::vector<double> t(25 * (size_t)n);
+    HIP_OK(hipMemcpy(t.data(), d.Ab, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long p = 0; p < n; ++p)
+      for (int q = 0; q < 25; ++q) a[25 * p + q] = t[q * n + p];
+    return ATH_OK;
+  }
+  // the inverse is kept in the sweep's tile-blocked layout
+  std::vector<double> t(25L * 256 * d.tni * d.tnj * d.tnk);
+  HIP_OK(hipMemcpy(t.data(), d.Abi, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < d.nk; ++k)
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.ni; ++i) {
+        const long tile = (long)(i >> 3) + (long)d.tni * ((j >> 3) + (long)d.tnj * (k >> 2));
+        const long base = tile * (25 * 256) + ((i & 7) + 8 * ((j & 7) + 8 * (k & 3)));
+        const long p = i + (long)j * d.ni + (long)k * d.ni * d.nj;
+        for (int q = 0; q < 25; ++q) a[25 * p + q] = t[base + 256 * q];
+      }
   return ATH_OK;
 }
 int ath_get_update(ath_ctx *c, int b, double *du) { return GetPhys(c, b, du, 4); }
