@@ -1918,7 +1918,7 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
                                                         MultiSweepCtl s, int full) {
   __shared__ T sX[5][kBT];
   __shared__ T sHX[5][kBHalo];  // halo du
-  __shared__ int sTicket;
+  __shared__ int sTicket, sNext;
   __shared__ int sEpoch;
 
   const int tid = threadIdx.x;
@@ -1982,6 +1982,9 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
         }
       }
     }
+    // the next ticket now, so that its atomic overlaps this tile's walk (a
+    // held ticket only ever waits on smaller ones, so no cycle can form)
+    if (tid == 0) sNext = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
@@ -2071,13 +2074,16 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
 #pragma unroll
           for (int e = 0; e < 5; ++e) xn[e] = sX[e][tid] - o[e];
         }
+        // in LDS only: a global store here would make every step's barrier
+        // wait for it
 #pragma unroll
-        for (int e = 0; e < 5; ++e) {
-          sX[e][tid] = xn[e];
-          x[e * b.ncell + p] = xn[e];
-        }
+        for (int e = 0; e < 5; ++e) sX[e][tid] = xn[e];
       }
       __syncthreads();
+    }
+    if (valid) {
+#pragma unroll
+      for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = sX[e][tid];
     }
 
     // --- publish, then take the next ticket
@@ -2086,7 +2092,7 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
     if (tid == 0) {
       __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_AGENT);
-      sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sTicket = sNext;
     }
     __syncthreads();
   }
