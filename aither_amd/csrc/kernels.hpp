@@ -1052,14 +1052,11 @@ __global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d
 template <typename T>
 __device__ __forceinline__ void BlockMultG(const double *__restrict__ M, long stride, long idx, const T v[5],
                                            T out[5]) {
-#pragma clang fp contract(off)
+  // all 25 loads issued before the arithmetic
+  T m[25];
 #pragma unroll
-  for (int r = 0; r < 5; ++r) {
-    T acc = T(0);
-#pragma unroll
-    for (int c = 0; c < 5; ++c) acc += T(M[(r * 5 + c) * stride + idx]) * v[c];
-    out[r] = acc;
-  }
+  for (int q = 0; q < 25; ++q) m[q] = T(M[q * stride + idx]);
+  BlockMult<T>(m, v, out);
 }
 
 // block product with the cell's main diagonal Ab (cell-major layout)

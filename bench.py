@@ -63,8 +63,8 @@ CHANNEL_SWEEP_BYTES = {"fwd": 616, "bwd": 520}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
-CHANNEL_SWEEP_TRAFFIC = 0.5 * (2427.5e6 + 2065.6e6)   # one pass, forward and backward averaged
-CHANNEL_FLUX_TRAFFIC = 650.9e6                        # residual stage of one block
+CHANNEL_SWEEP_TRAFFIC = 0.5 * (1363.6e6 + 1000.9e6)   # one pass, forward and backward averaged
+CHANNEL_FLUX_TRAFFIC = 649.6e6                        # residual stage of one block
 
 
 def parse():
