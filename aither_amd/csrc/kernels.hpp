@@ -1913,8 +1913,7 @@ constexpr int kBHalo = kBHaloI + kBHaloJ + kBHaloK;
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
                                                         MultiSweepCtl s, int full) {
-  __shared__ T sX[5][kBT];
-  __shared__ T sHX[5][kBHalo];  // halo du
+  __shared__ T sA[5][kBT + kBHalo];  // du of the tile's cells, then of the halo columns
   __shared__ int sTicket, sNext;
   __shared__ int sEpoch;
 
@@ -1942,6 +1941,26 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
     const long p = (long)i + (long)j * b.ni + (long)k * b.ni * b.nj;
 
     // --- static data before the dependency wait
+    const int ni = b.ni, nj = b.nj, nk = b.nk;
+    const long ncell = b.ncell, nbf = b.nbf;
+    const T *bt = OpBuf<T>::Bt(b);
+    // connection term (static during the pass) of the face of cell (ci, cj,
+    // ck) in direction d on the dependency side; zero without connections
+    auto conn = [&](int d, int ci, int cj, int ck, int e) -> T {
+      return bt ? bt[e * nbf + BtSlot(b, d, FWD ? 0 : 1, ci, cj, ck)] : T(0);
+    };
+    // LDS column of the dependency neighbour in each direction: a cell of
+    // the tile or a halo column
+    int nb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
+      const int L3 = d == 0 ? kBTI : (d == 1 ? kBTJ : kBTK);
+      const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
+      nb[d] = inTile ? tid + sgn * (d == 0 ? 1 : (d == 1 ? kBTI : kBTI * kBTJ))
+                     : kBT + (d == 0 ? lj + kBTJ * lk
+                                     : (d == 1 ? kBHaloI + li + kBTI * lk : kBHaloI + kBHaloJ + li + kBTI * lj));
+    }
     double rb[5] = {0, 0, 0, 0, 0};
     double mdep[3][25], mai[25];
     if (valid) {
@@ -1949,14 +1968,50 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-        const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk;
+        const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < ni && gj < nj && gk < nk;
+        // a neighbour outside the block enters through its connection term,
+        // which its LDS column holds; the identity block passes it on
+        // unchanged (x + 0 y = x, and the sum never starts from -0)
         const double *M = (FWD ? b.Lc[d] : b.Uc[d]) + TileBase(b, i, j, k);
 #pragma unroll
-        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[256 * q] : 0.0;
+        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[256 * q] : (q % 6 == 0 ? 1.0 : 0.0);
       }
       const double *Mi = b.Abi + TileBase(b, i, j, k);
 #pragma unroll
       for (int q = 0; q < 25; ++q) mai[q] = Mi[256 * q];
+      // backward: the forward result, which only this tile overwrites
+      if (!FWD) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) sA[e][tid] = x[e * ncell + p];
+      }
+    } else if (!FWD) {
+      // a partial tile's position just past the block's upper face along one
+      // axis is the dependency neighbour of the face cell below it
+      const int oi = i >= ni, oj = j >= nj, ok = k >= nk;
+      const int d = oi ? 0 : (oj ? 1 : 2);
+      const bool edge = oi + oj + ok == 1 && (oi ? i == ni : (oj ? j == nj : k == nk));
+#pragma unroll
+      for (int e = 0; e < 5; ++e)
+        sA[e][tid] = edge ? conn(d, i - (d == 0), j - (d == 1), k - (d == 2), e) : T(0);
+    }
+    // halo columns: positions outside the block hold connection terms now,
+    // those inside the du of the dependency tiles after the wait
+    int hi = 0, hj = 0, hk = 0, hd = 0;
+    bool hin = false;
+    if (tid < kBHalo) {
+      const int di = FWD ? -1 : kBTI, dj = FWD ? -1 : kBTJ, dk = FWD ? -1 : kBTK;
+      if (tid < kBHaloI) { hd = 0; hi = di; hj = tid % kBTJ; hk = tid / kBTJ; }
+      else if (tid < kBHaloI + kBHaloJ) { const int q = tid - kBHaloI; hd = 1; hi = q % kBTI; hj = dj; hk = q / kBTI; }
+      else { const int q = tid - kBHaloI - kBHaloJ; hd = 2; hi = q % kBTI; hj = q / kBTI; hk = dk; }
+      hi += tl.x * kBTI; hj += tl.y * kBTJ; hk += tl.z * kBTK;
+      hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
+      if (!hin) {
+        // the tile cell next to it along hd
+        const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
+        const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) sA[e][kBT + tid] = face ? conn(hd, ci, cj, ck, e) : T(0);
+      }
     }
 
     // --- wait for the tiles of the same block this one depends on
@@ -1986,24 +2041,10 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
     // halo du, published by the dependency tiles
-    {
-      const int di = FWD ? -1 : kBTI, dj = FWD ? -1 : kBTJ, dk = FWD ? -1 : kBTK;
-      if (tid < kBHalo) {
-        int hi, hj, hk;
-        if (tid < kBHaloI) { hi = di; hj = tid % kBTJ; hk = tid / kBTJ; }
-        else if (tid < kBHaloI + kBHaloJ) { const int q = tid - kBHaloI; hi = q % kBTI; hj = dj; hk = q / kBTI; }
-        else { const int q = tid - kBHaloI - kBHaloJ; hi = q % kBTI; hj = q / kBTI; hk = dk; }
-        const int gi = tl.x * kBTI + hi, gj = tl.y * kBTJ + hj, gk = tl.z * kBTK + hk;
-        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < b.ni && gj < b.nj && gk < b.nk) {
-          const long q = (long)gi + (long)gj * b.ni + (long)gk * b.ni * b.nj;
+    if (hin) {
+      const long q = (long)hi + (long)hj * ni + (long)hk * ni * nj;
 #pragma unroll
-          for (int e = 0; e < 5; ++e) sHX[e][tid] = x[e * b.ncell + q];
-        }
-      }
-    }
-    if (!FWD && valid) {
-#pragma unroll
-      for (int e = 0; e < 5; ++e) sX[e][tid] = x[e * b.ncell + p];  // forward result
+      for (int e = 0; e < 5; ++e) sA[e][kBT + tid] = x[e * ncell + q];
     }
     __syncthreads();
 
@@ -2017,29 +2058,9 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
         for (int e = 0; e < 5; ++e) D[e] = T(0);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-          if (gi < 0 || gj < 0 || gk < 0 || gi >= b.ni || gj >= b.nj || gk >= b.nk) {
-            const T *bt = OpBuf<T>::Bt(b);
-            if (bt) {
-              const long q = BtSlot(b, d, FWD ? 0 : 1, i, j, k);
-#pragma unroll
-              for (int e = 0; e < 5; ++e) D[e] += bt[e * b.nbf + q];
-            }
-            continue;
-          }
-          const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
-          const int L3 = d == 0 ? kBTI : (d == 1 ? kBTJ : kBTK);
-          const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
           T dn[5], o[5], m[25];
-          if (inTile) {
-            const int nt = tid + sgn * (d == 0 ? 1 : (d == 1 ? kBTI : kBTI * kBTJ));
 #pragma unroll
-            for (int e = 0; e < 5; ++e) dn[e] = sX[e][nt];
-          } else {
-            const int h = d == 0 ? lj + kBTJ * lk : (d == 1 ? kBHaloI + li + kBTI * lk : kBHaloI + kBHaloJ + li + kBTI * lj);
-#pragma unroll
-            for (int e = 0; e < 5; ++e) dn[e] = sHX[e][h];
-          }
+          for (int e = 0; e < 5; ++e) dn[e] = sA[e][nb[d]];
 #pragma unroll
           for (int q = 0; q < 25; ++q) m[q] = T(mdep[d][q]);
           BlockMult<T>(m, dn, o);
@@ -2069,18 +2090,18 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
           T o[5];
           BlockMult<T>(ai, D, o);
 #pragma unroll
-          for (int e = 0; e < 5; ++e) xn[e] = sX[e][tid] - o[e];
+          for (int e = 0; e < 5; ++e) xn[e] = sA[e][tid] - o[e];
         }
-        // in LDS only: a global store here would make every step's barrier
-        // wait for it
+        // in LDS only during the walk: per-step global stores of scattered
+        // cells would write partial cache lines
 #pragma unroll
-        for (int e = 0; e < 5; ++e) sX[e][tid] = xn[e];
+        for (int e = 0; e < 5; ++e) sA[e][tid] = xn[e];
       }
       __syncthreads();
     }
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = sX[e][tid];
+      for (int e = 0; e < 5; ++e) x[e * ncell + p] = sA[e][tid];
     }
 
     // --- publish, then take the next ticket
