@@ -1753,9 +1753,11 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
           __builtin_amdgcn_s_sleep(1);
         }
       }
+      // one acquire for the workgroup, ordered before every wave's halo
+      // loads by the barrier
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
     // halo du, published by the dependency tiles
     {
@@ -1859,9 +1861,8 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
       __syncthreads();
     }
 
-    // --- publish: every thread releases its stores, then one flag store;
-    // the same thread takes the next ticket
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // --- publish: the barrier orders every wave's du stores before thread
+    // 0's release store; the same thread takes the next ticket
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_store(&s.flags[TileId(s, tl.x, tl.y, tl.z)], epoch, __ATOMIC_RELEASE,
@@ -2033,12 +2034,15 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
           __builtin_amdgcn_s_sleep(1);
         }
       }
+      // one acquire for the workgroup: the barrier below orders it before
+      // every wave's halo loads (each wave's own fence would invalidate the
+      // XCD's L2 once more)
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // the next ticket now, so that its atomic overlaps this tile's walk (a
+      // held ticket only ever waits on smaller ones, so no cycle can form)
+      sNext = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // the next ticket now, so that its atomic overlaps this tile's walk (a
-    // held ticket only ever waits on smaller ones, so no cycle can form)
-    if (tid == 0) sNext = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
     // halo du, published by the dependency tiles
     if (hin) {
@@ -2104,8 +2108,8 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
       for (int e = 0; e < 5; ++e) x[e * ncell + p] = sA[e][tid];
     }
 
-    // --- publish, then take the next ticket
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // --- publish: the barrier orders every wave's du stores before thread
+    // 0's release store (one L2 write-back per tile), then the next ticket
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
