@@ -29,16 +29,12 @@ struct DevBlock {
   float *xold32;
   double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
   // block matrix solvers: main diagonal after AddDiagonalTerms, 25 * ncell,
-  // entry (r, c) of cell p at (5 r + c) ncell + p; its inverse and the
-  // off-diagonal blocks in the tile-blocked layout of TileBase (the sweep's
-  // 8 x 8 x 4 tiles: one tile's values of one entry are 256 consecutive
-  // doubles, so a wavefront reads whole cache lines no other tile shares)
+  // entry (r, c) of cell p at (5 r + c) ncell + p; its inverse in the
+  // tile-blocked layout of TileBase (the sweep's 8 x 8 x 4 tiles: one tile's
+  // values of one entry are 256 consecutive doubles, so a wavefront reads
+  // whole cache lines no other tile shares).  The off-diagonal blocks are
+  // formed where they are used (NeighbourBlock).
   double *Ab, *Abi;
-  // the off-diagonal blocks, formed once per relax (they depend on the state
-  // only), stored at the cell whose row uses them: Lc[d] = the lower
-  // neighbour's block across the cell's lower face of direction d
-  // (positive), Uc[d] = the upper neighbour's block across its upper face
-  double *Lc[3], *Uc[3];
   int tni, tnj, tnk;  // tile grid of the layout
   // block solvers with the viscous terms: what the viscous pass computed at
   // every physical face of direction d (velocity gradient 9, face state 5,
@@ -399,11 +395,28 @@ __device__ __forceinline__ void CellTail(const DevBlock &b, const Params &P, lon
   }
 }
 
+// Workgroups are dealt to the eight XCDs in turn (workgroup n to XCD n mod
+// 8), and each XCD has its own L2.  Renumbering them so that every XCD
+// receives one contiguous range of the work keeps neighbouring tiles, which
+// read the same stencil data, in one L2.  A bijection of [0, N) for any N.
+__device__ __forceinline__ unsigned XcdRemap(unsigned n, unsigned N) {
+  const unsigned q = N / 8u, r = N % 8u, x = n % 8u, l = n / 8u;
+  return x * q + min(x, r) + l;
+}
+// the remapped tile of a three-dimensional grid (x fastest)
+__device__ __forceinline__ int3 XcdTile3() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned n = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned m = XcdRemap(n, gx * gy * gridDim.z);
+  return make_int3((int)(m % gx), (int)((m / gx) % gy), (int)(m / (gx * gy)));
+}
+
 template <int RECON, int LIM, int FLUX>
 __global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
                                                  const double *__restrict__ cflp) {
   __shared__ double sF[5][kNF];
-  const int i0 = blockIdx.x * kTI, j0 = blockIdx.y * kTJ, k0 = blockIdx.z * kTK;
+  const int3 tb = XcdTile3();
+  const int i0 = tb.x * kTI, j0 = tb.y * kTJ, k0 = tb.z * kTK;
   const GasT<double> g = Cast<double>(P.gas);
   for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
     int d, li, lj, lk;
@@ -793,7 +806,8 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
 template <bool R4>
 __global__ void __launch_bounds__(256) KViscFlux(DevBlock b, Params P, const double *__restrict__ cflp) {
   __shared__ double sF[5][kNF];
-  const int i0 = blockIdx.x * kTI, j0 = blockIdx.y * kTJ, k0 = blockIdx.z * kTK;
+  const int3 tb = XcdTile3();
+  const int i0 = tb.x * kTI, j0 = tb.y * kTJ, k0 = tb.z * kTK;
   for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
     int d, li, lj, lk;
     if (f < kNFI) {
@@ -910,7 +924,7 @@ __device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, 
 template <int RECON, int LIM, int MODE>
 __global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *flag) {
 #pragma clang fp contract(off)
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long p = (long)XcdRemap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   const GasT<double> g = Cast<double>(P.gas);
@@ -1007,43 +1021,34 @@ __device__ __forceinline__ void BlockOffDiag(const GasT<T> &g, const T W[5], con
   BlockMult<T>(J, du, out);
 }
 
-// the off-diagonal blocks of the interior faces of direction d: exactly the
-// matrices RusanovBlockOffDiagonal forms for each product, formed once per
-// relax instead of once per product (the state does not change during the
-// sweeps); one thread per face and side (lower cell's block, then upper's)
-__global__ void __launch_bounds__(128) KBlockOffMats(DevBlock b, Params P, int d) {
+// The off-diagonal block of cell (i, j, k) for its neighbour across
+// direction d (upper: the neighbour at +1, else at -1), the matrix that
+// RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194) forms: the
+// neighbour's Rusanov Jacobian at the shared face, with the neighbour's
+// thin-shear-layer Jacobian (its cell velocity gradient, viscosity and
+// conductivity, the face's projected centre distance) subtracted for a lower
+// and added for an upper neighbour.  Formed where it is used, from the same
+// inputs in the same order, so every user sees the same bits.
+__device__ __forceinline__ void NeighbourBlock(const DevBlock &b, const GasT<double> &g, const ViscParams &vp,
+                                               int d, bool upper, int i, int j, int k, double J[25]) {
 #pragma clang fp contract(off)
-  const int FI = b.ni + (d == 0), FJ = b.nj + (d == 1), FK = b.nk + (d == 2);
-  const long nf = (long)FI * FJ * FK;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * nf) return;
-  const int side = t >= nf ? 1 : 0;
-  const long f = t - side * nf;
-  const int i = (int)(f % FI), j = (int)((f / FI) % FJ), k = (int)(f / ((long)FI * FJ));
-  const int f3 = d == 0 ? i : (d == 1 ? j : k), n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
-  if (f3 == 0 || f3 == n3) return;  // boundary faces: connection terms, formed per pass
-  const GasT<double> g = Cast<double>(P.gas);
+  const int s = upper ? 1 : -1;
+  const int ni = i + (d == 0 ? s : 0), nj = j + (d == 1 ? s : 0), nk = k + (d == 2 ? s : 0);
+  const long f = upper ? FaceIdx(b, d, ni, nj, nk) : FaceIdx(b, d, i, j, k);
   const double4 a = b.fa[d][f];
-  {
-    const long c = side ? PadIdx(b, i, j, k) : PadIdx(b, i - (d == 0), j - (d == 1), k - (d == 2));
-    double W[5], J[25];
+  const long c = PadIdx(b, ni, nj, nk);
+  double W[5];
 #pragma unroll
-    for (int e = 0; e < 5; ++e) W[e] = b.W[e * b.npad + c];
-    const bool positive = side == 0;
-    RusanovJacobian<double>(g, W, a.x, a.y, a.z, a.w, positive, J);
-    if (P.visc.on) {
-      double vg[9], V[25];
+  for (int e = 0; e < 5; ++e) W[e] = b.W[e * b.npad + c];
+  const bool positive = !upper;
+  RusanovJacobian<double>(g, W, a.x, a.y, a.z, a.w, positive, J);
+  if (vp.on) {
+    double vg[9], V[25];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) vg[q] = b.vgrad[q * b.npad + c];
-      TSLJacobian(g, P.visc, W, b.mu[c], b.kc[c], a, b.c2c[d][f], positive, vg, V);
+    for (int q = 0; q < 9; ++q) vg[q] = b.vgrad[q * b.npad + c];
+    TSLJacobian(g, vp, W, b.mu[c], b.kc[c], a, b.c2c[d][f], positive, vg, V);
 #pragma unroll
-      for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - V[q] : J[q] + V[q];
-    }
-    // the lower cell's block is used by the upper cell (its Lc), and vice versa
-    const long base = side ? TileBase(b, i - (d == 0), j - (d == 1), k - (d == 2)) : TileBase(b, i, j, k);
-    double *M = side ? b.Uc[d] : b.Lc[d];
-#pragma unroll
-    for (int q = 0; q < 25; ++q) M[base + 256 * q] = J[q];
+    for (int q = 0; q < 25; ++q) J[q] = positive ? J[q] - V[q] : J[q] + V[q];
   }
 }
 
@@ -1064,7 +1069,7 @@ template <typename T>
 __device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
   BlockMultG<T>(M, b.ncell, p, v, out);
 }
-// block product with a tile-blocked per-cell block (Abi, Lc, Uc)
+// block product with a tile-blocked per-cell block (Abi)
 template <typename T>
 __device__ __forceinline__ void TileBlockMult(const DevBlock &b, const double *M, int i, int j, int k, const T v[5],
                                               T out[5]) {
@@ -1124,8 +1129,13 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       du[e] = x[e * b.ncell + q];
     }
     if (block) {
-      // the precomputed block of this neighbour, kept at the cell
-      TileBlockMult<T>(b, upper ? b.Uc[d] : b.Lc[d], i, j, k, du, o);
+      const GasT<double> gd = {double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
+      double J[25];
+      NeighbourBlock(b, gd, vp, d, upper, i, j, k, J);
+      T m[25];
+#pragma unroll
+      for (int q = 0; q < 25; ++q) m[q] = T(J[q]);
+      BlockMult<T>(m, du, o);
     } else {
       const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
       ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
@@ -1307,7 +1317,9 @@ template <typename T>
 __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Params P,
                                                                Partial *part) {
   __shared__ double s[kRedThreads];
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // partial n covers cells [n blockDim, (n + 1) blockDim) whichever XCD runs it
+  const unsigned n = XcdRemap(blockIdx.x, gridDim.x);
+  const long p = (long)n * blockDim.x + threadIdx.x;
   double acc = 0.0;
   if (p < b.ncell) {
     const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
@@ -1340,7 +1352,7 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Param
     if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x].mres = s[0];
+  if (threadIdx.x == 0) part[n].mres = s[0];
 }
 
 // procBlock::UpdateBlock (src/procBlock.cpp:826-871) with
@@ -1964,8 +1976,10 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
     }
     double rb[5] = {0, 0, 0, 0, 0};
     double mdep[3][25], mai[25];
+    T Ost[5];  // full sweeps: the other side's sum, from the previous sweep's du
     if (valid) {
       if (FWD || full) Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+      const GasT<double> gd = Cast<double>(P.gas);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
@@ -1973,10 +1987,17 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
         // a neighbour outside the block enters through its connection term,
         // which its LDS column holds; the identity block passes it on
         // unchanged (x + 0 y = x, and the sum never starts from -0)
-        const double *M = (FWD ? b.Lc[d] : b.Uc[d]) + TileBase(b, i, j, k);
+        if (inside) {
+          NeighbourBlock(b, gd, P.visc, d, !FWD, i, j, k, mdep[d]);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 25; ++q) mdep[d][q] = inside ? M[256 * q] : (q % 6 == 0 ? 1.0 : 0.0);
+          for (int q = 0; q < 25; ++q) mdep[d][q] = q % 6 == 0 ? 1.0 : 0.0;
+        }
       }
+      // the other side's du is not written before this tile publishes: its
+      // cells in other tiles wait for this one, those in it are stored after
+      // the walk
+      if (full) OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, Ost, 1);
       const double *Mi = b.Abi + TileBase(b, i, j, k);
 #pragma unroll
       for (int q = 0; q < 25; ++q) mai[q] = Mi[256 * q];
@@ -2075,15 +2096,12 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
 #pragma unroll
         for (int q = 0; q < 25; ++q) ai[q] = T(mai[q]);
         if (full) {
-          // the other side, from the previous sweep's du in global memory
-          T O[5];
-          OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, O, 1);
           if (FWD) {
 #pragma unroll
-            for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + (D[e] - O[e]);
+            for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + (D[e] - Ost[e]);
           } else {
 #pragma unroll
-            for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + O[e]) - D[e];
+            for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + Ost[e]) - D[e];
           }
           BlockMult<T>(ai, v, xn);
         } else if (FWD) {

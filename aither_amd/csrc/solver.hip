@@ -533,7 +533,7 @@ int Upload(ath_ctx *c) {
     d.Ab = d.Abi = nullptr;
     d.vgrad = nullptr;
     d.kc = nullptr;
-    for (int dd = 0; dd < 3; ++dd) d.Lc[dd] = d.Uc[dd] = d.fvis[dd] = nullptr;
+    for (int dd = 0; dd < 3; ++dd) d.fvis[dd] = nullptr;
     d.tni = (h.ni + 7) / 8;
     d.tnj = (h.nj + 7) / 8;
     d.tnk = (h.nk + 3) / 4;
@@ -544,8 +544,6 @@ int Upload(ath_ctx *c) {
       if (int r = c->Alloc(&d.Abi, tb)) return r;
       for (int dd = 0; dd < 3; ++dd) {
         const long nf = (long)(h.ni + (dd == 0)) * (h.nj + (dd == 1)) * (h.nk + (dd == 2));
-        if (int r = c->Alloc(&d.Lc[dd], tb)) return r;
-        if (int r = c->Alloc(&d.Uc[dd], tb)) return r;
         if (hc.viscous)
           if (int r = c->Alloc(&d.fvis[dd], 16 * nf)) return r;
       }
@@ -826,12 +824,6 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     if (timed) hipEventRecord(c->evs[evBase + 2 * pass + (begin ? 0 : 1)], c->stream);
     if (!begin) ++pass;
   };
-  if (c->P.block)  // the off-diagonal blocks of the interior faces, once per relax
-    for (auto &br : c->blk)
-      for (int d = 0; d < 3; ++d) {
-        const long nf = (long)(br.d.ni + (d == 0)) * (br.d.nj + (d == 1)) * (br.d.nk + (d == 2));
-        hipLaunchKernelGGL(ath::KBlockOffMats, dim3(Grid(2 * nf, 128)), dim3(128), 0, c->stream, br.d, c->P, d);
-      }
   for (auto &br : c->blk)
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
   for (int sw = 0; sw < hc.sweeps; ++sw) {

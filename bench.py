@@ -53,13 +53,15 @@ VORTEX_SWEEP_TRAFFIC = 0.5 * (150.7e6 + 132.2e6)
 #   residual stage (inviscid and viscous face fluxes, spectral radii, dt,
 #   block diagonal and inverse): W 40 + geometry 128 + R 40 + lambda 8 + dt 8
 #   + T and mu 16 + D 200 + D^-1 200 + cell velocity gradient 72 = 712
-#   forward pass: W 40 + R 40 + U^n 40 + U^n-1 40 + geometry 96 + V 8 + dt 8 +
-#   D^-1 200 + mu 8 + velocity gradient 72 + centre distances 24 + x 40 = 616
+#   forward pass (the off-diagonal blocks formed from the neighbours' state,
+#   velocity gradient, viscosity and conductivity): W 40 + R 40 + U^n 40 +
+#   U^n-1 40 + geometry 96 + V 8 + dt 8 + D^-1 200 + mu 8 + k 8 + velocity
+#   gradient 72 + centre distances 24 + x 40 = 624
 #   backward pass (first sweep, no initialisation: x - D^-1 U): W 40 +
-#   geometry 96 + D^-1 200 + mu 8 + velocity gradient 72 + centre distances
-#   24 + x read 40 + x write 40 = 520
+#   geometry 96 + D^-1 200 + mu 8 + k 8 + velocity gradient 72 + centre
+#   distances 24 + x read 40 + x write 40 = 528
 CHANNEL_FLUX_BYTES = 712
-CHANNEL_SWEEP_BYTES = {"fwd": 616, "bwd": 520}
+CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
