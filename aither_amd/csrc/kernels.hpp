@@ -77,6 +77,7 @@ struct Params {
   double dtFixed;  // nondimensional global time step, <= 0 for local dt
   int implicit, multilevel, initMatrix, explicitEuler;
   int block;  // matrixSolver blusgs / bdplur: 5x5 main diagonal and off-diagonals
+  int fp32;   // the scalar implicit operator runs in FP32: the residual stage writes Ainv32, else Ainv
 };
 
 // ---------------------------------------------------------------------------
@@ -390,8 +391,11 @@ __device__ __forceinline__ void CellTail(const DevBlock &b, const Params &P, lon
     A += dvt;
     b.A[p] = A;
     const double ainv = 1.0 / A;
-    if (b.Ainv) b.Ainv[p] = ainv;
-    if (b.Ainv32) b.Ainv32[p] = (float)ainv;
+    if (!P.fp32) {
+      if (b.Ainv) b.Ainv[p] = ainv;
+    } else if (b.Ainv32) {
+      b.Ainv32[p] = (float)ainv;
+    }
   }
 }
 
@@ -403,20 +407,43 @@ __device__ __forceinline__ unsigned XcdRemap(unsigned n, unsigned N) {
   const unsigned q = N / 8u, r = N % 8u, x = n % 8u, l = n / 8u;
   return x * q + min(x, r) + l;
 }
-// the remapped tile of a three-dimensional grid (x fastest)
-__device__ __forceinline__ int3 XcdTile3() {
-  const unsigned gx = gridDim.x, gy = gridDim.y;
-  const unsigned n = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const unsigned m = XcdRemap(n, gx * gy * gridDim.z);
-  return make_int3((int)(m % gx), (int)((m / gx) % gy), (int)(m / (gx * gy)));
+
+// One launch over every block.  Per block, its grid of work units (tiles or
+// ranges of cells) and the global index of its first unit; workgroup n, after
+// the XCD renumbering, takes unit n of the concatenation, so each XCD works
+// on one contiguous range of units across the blocks.
+struct BlockMap {
+  const DevBlock *blocks;
+  const int4 *units;  // per block: (ux, uy, uz, first unit), ux uy uz units
+  int nb;
+};
+struct Unit {
+  const DevBlock *b;
+  int x, y, z;      // the unit within its block's grid
+  unsigned local;   // index within the block (x fastest)
+  unsigned global;  // index in the concatenation
+};
+__device__ __forceinline__ Unit MapUnit(const BlockMap &m) {
+  const unsigned n = XcdRemap(blockIdx.x, gridDim.x);
+  int lo = 0, hi = m.nb - 1;  // the last block whose first unit is <= n
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((unsigned)m.units[mid].w <= n) lo = mid;
+    else hi = mid - 1;
+  }
+  const int4 u = m.units[lo];
+  const unsigned l = n - (unsigned)u.w;
+  return {m.blocks + lo, (int)(l % (unsigned)u.x), (int)((l / (unsigned)u.x) % (unsigned)u.y),
+          (int)(l / ((unsigned)u.x * (unsigned)u.y)), l, n};
 }
 
 template <int RECON, int LIM, int FLUX>
-__global__ void __launch_bounds__(256) KResidual(DevBlock b, Params P,
+__global__ void __launch_bounds__(256) KResidual(BlockMap M, Params P,
                                                  const double *__restrict__ cflp) {
   __shared__ double sF[5][kNF];
-  const int3 tb = XcdTile3();
-  const int i0 = tb.x * kTI, j0 = tb.y * kTJ, k0 = tb.z * kTK;
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const int i0 = U.x * kTI, j0 = U.y * kTJ, k0 = U.z * kTK;
   const GasT<double> g = Cast<double>(P.gas);
   for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
     int d, li, lj, lk;
@@ -551,8 +578,10 @@ __global__ void __launch_bounds__(256) KViscWall(DevBlock b, const WallTask *__r
 
 // UpdateAuxillaryVariables (src/procBlock.cpp:6171-6189): temperature and
 // viscosity of every cell but the block corners
-__global__ void __launch_bounds__(256) KAux(DevBlock b, Params P) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) KAux(BlockMap M, Params P) {
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long q = (long)U.local * blockDim.x + threadIdx.x;
   if (q >= b.npad) return;
   const int i = (int)(q % b.NI) - b.g, j = (int)((q / b.NI) % b.NJ) - b.g, k = (int)(q / ((long)b.NI * b.NJ)) - b.g;
   const bool corner = (i < 0 || i >= b.ni) && (j < 0 || j >= b.nj) && (k < 0 || k >= b.nk);
@@ -768,33 +797,33 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
   const double f = a.w * (mu + mut) / dist;
 #pragma unroll
   for (int q = 0; q < 25; ++q) J[q] *= f;
-  // DelprimitiveDelConservative
+  // DelprimitiveDelConservative: its nonzero entries (row-major index)
   const double gm1 = g.gamma - 1.0;
   const double invRho = 1.0 / rho;
-  double Pm[25];
+  const double p5 = -invRho * u, p10 = -invRho * v, p15 = -invRho * w;
+  const double p20 = 0.5 * gm1 * ((u * u + v * v) + w * w);
+  const double p6 = invRho, p12 = invRho, p18 = invRho;
+  const double p21 = -gm1 * u, p22 = -gm1 * v, p23 = -gm1 * w, p24 = gm1;
+  // J times it, the row-by-column sums from +0 of the 5x5x5 product with the
+  // products of structural zeros left out: row 0 of J is +-0, so is its
+  // first and last column in rows 1-3, and the product matrix is zero off
+  // its first column, diagonal and last row.  A sum from +0 of finite terms
+  // is never -0 and is unchanged by adding +-0, so each entry equals the
+  // full product's bit for bit.
 #pragma unroll
-  for (int q = 0; q < 25; ++q) Pm[q] = 0.0;
-  Pm[0] = 1.0;
-  Pm[5] = -invRho * u;
-  Pm[10] = -invRho * v;
-  Pm[15] = -invRho * w;
-  Pm[20] = 0.5 * gm1 * ((u * u + v * v) + w * w);
-  Pm[6] = invRho;
-  Pm[21] = -gm1 * u;
-  Pm[12] = invRho;
-  Pm[22] = -gm1 * v;
-  Pm[18] = invRho;
-  Pm[23] = -gm1 * w;
-  Pm[24] = gm1;
+  for (int q = 0; q < 25; ++q) out[q] = 0.0;
 #pragma unroll
-  for (int r = 0; r < 5; ++r)
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double acc = 0.0;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc += J[r * 5 + q] * Pm[q * 5 + c];
-      out[r * 5 + c] = acc;
-    }
+  for (int r = 1; r < 4; ++r) {
+    out[r * 5 + 0] = (J[r * 5 + 1] * p5 + J[r * 5 + 2] * p10) + J[r * 5 + 3] * p15;
+    out[r * 5 + 1] = J[r * 5 + 1] * p6;
+    out[r * 5 + 2] = J[r * 5 + 2] * p12;
+    out[r * 5 + 3] = J[r * 5 + 3] * p18;
+  }
+  out[20] = (((J[20] + J[21] * p5) + J[22] * p10) + J[23] * p15) + J[24] * p20;
+  out[21] = J[21] * p6 + J[24] * p21;
+  out[22] = J[22] * p12 + J[24] * p22;
+  out[23] = J[23] * p18 + J[24] * p23;
+  out[24] = J[24] * p24;
 }
 
 // procBlock::CalcViscFluxI/J/K (src/procBlock.cpp:1233-2135), laminar.  The
@@ -804,10 +833,11 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
 // spectral radii (i, j, k) to its spectral radius (times the viscous CFL
 // coefficient) and twice them to its diagonal, then forms dt and A.
 template <bool R4>
-__global__ void __launch_bounds__(256) KViscFlux(DevBlock b, Params P, const double *__restrict__ cflp) {
+__global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P, const double *__restrict__ cflp) {
   __shared__ double sF[5][kNF];
-  const int3 tb = XcdTile3();
-  const int i0 = tb.x * kTI, j0 = tb.y * kTJ, k0 = tb.z * kTK;
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const int i0 = U.x * kTI, j0 = U.y * kTJ, k0 = U.z * kTK;
   for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
     int d, li, lj, lk;
     if (f < kNFI) {
@@ -922,9 +952,11 @@ __device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, 
 // stores the cell velocity gradient, a sixth of each face's in the same order
 // (:1396-1414, 1427-1436), and finishes.
 template <int RECON, int LIM, int MODE>
-__global__ void __launch_bounds__(128) KBlockDiag(DevBlock b, Params P, int *flag) {
+__global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P, int *flag) {
 #pragma clang fp contract(off)
-  const long p = (long)XcdRemap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   const GasT<double> g = Cast<double>(P.gas);
@@ -1147,8 +1179,10 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
 
 // InitializeMatrixUpdate (src/linearSolver.cpp:111-144)
 template <typename T>
-__global__ void __launch_bounds__(256) KInitUpdate(DevBlock b, Params P) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) KInitUpdate(BlockMap M, Params P) {
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   T *x = OpBuf<T>::X(b);
   if (!P.initMatrix) {
@@ -1314,12 +1348,14 @@ __device__ __forceinline__ bool LinfBetter(double v, long long idx, double bv, l
 // linearSolver::Residual / AXmB (src/linearSolver.cpp:58-109), squared and
 // summed as in mgSolution::CycleAtLevel (src/mgSolution.cpp:197-206)
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Params P,
+__global__ void __launch_bounds__(kRedThreads) KMatrixResidual(BlockMap M, Params P,
                                                                Partial *part) {
   __shared__ double s[kRedThreads];
-  // partial n covers cells [n blockDim, (n + 1) blockDim) whichever XCD runs it
-  const unsigned n = XcdRemap(blockIdx.x, gridDim.x);
-  const long p = (long)n * blockDim.x + threadIdx.x;
+  // the partial of unit n covers the same cells whichever XCD runs it
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const unsigned n = U.global;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
   double acc = 0.0;
   if (p < b.ncell) {
     const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
@@ -1358,12 +1394,14 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(DevBlock b, Param
 // procBlock::UpdateBlock (src/procBlock.cpp:826-871) with
 // ImplicitTimeAdvance (:902-916) or ExplicitEulerTimeAdvance (:880-896)
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) KUpdate(DevBlock b, Params P, Partial *part,
+__global__ void __launch_bounds__(kRedThreads) KUpdate(BlockMap M, Params P, Partial *part,
                                                        int *flag) {
   __shared__ double sl2[5][kRedThreads];
   __shared__ double sv[kRedThreads];
   __shared__ long long si[kRedThreads];
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
   double l2[5] = {0, 0, 0, 0, 0};
   double bv = 0.0;
   long long bi = -1;
@@ -1420,7 +1458,7 @@ __global__ void __launch_bounds__(kRedThreads) KUpdate(DevBlock b, Params P, Par
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    Partial &o = part[blockIdx.x];
+    Partial &o = part[U.global];
 #pragma unroll
     for (int e = 0; e < 5; ++e) o.l2[e] = sl2[e][0];
     o.linf = sv[0];

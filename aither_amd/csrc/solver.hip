@@ -84,6 +84,11 @@ struct ath_ctx {
   // block solvers: one tiled sweep over all blocks
   ath::MultiSweepCtl msweep{};
   int msweepGrid = 0;
+  // unit grids of the all-block launches (ath::BlockMap), kNumMaps of them,
+  // nb entries each, and each one's total unit count
+  int4 *dmaps = nullptr;
+  int mapTotal[4] = {0, 0, 0, 0};
+  ath::BlockMap Map(int q) const { return {dblocks, dmaps + (size_t)q * blk.size(), (int)blk.size()}; }
   // timing
   int timing = 0;
   std::vector<hipEvent_t> evs;
@@ -113,6 +118,12 @@ constexpr int kBlk = 256;
 
 int Grid(long n, int b = kBlk) { return (int)((n + b - 1) / b); }
 
+// the unit grids of the all-block launches: 8x8x4 tiles (residual and
+// viscous flux), ranges of 128 cells (block diagonal), of 256 cells (matrix
+// residual, update, initialisation; their partials are the reductions'
+// slots) and of 256 padded cells (auxiliary variables)
+enum { kMapTile = 0, kMapCell128 = 1, kMapCell256 = 2, kMapPad256 = 3, kNumMaps = 4 };
+
 // boundary-face slot, as BtSlot in the kernels
 long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) {
   const long sI = (long)h.nj * h.nk, sJ = (long)h.nk * h.ni, sK = (long)h.ni * h.nj;
@@ -124,7 +135,6 @@ long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) 
 // the device copy of the blocks' DevBlocks (pointers change when the FP32
 // operator buffers are created)
 int SyncBlocks(ath_ctx *c) {
-  if (c->hc.conns.empty() && !c->hc.block) return ATH_OK;
   std::vector<ath::DevBlock> v;
   for (const auto &br : c->blk) v.push_back(br.d);
   if (!c->dblocks)
@@ -371,6 +381,38 @@ int UploadConnections(ath_ctx *c) {
 
 // the tiles of every block in pass order: tile hyperplane, then block, then
 // k, j, i; per block its tile grid and first flag
+int UploadMaps(ath_ctx *c) {
+  const int nb = (int)c->blk.size();
+  std::vector<int4> v((size_t)kNumMaps * nb);
+  for (int q = 0; q < kNumMaps; ++q) {
+    int first = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto &d = c->blk[b].d;
+      int4 u;
+      if (q == kMapTile)
+        u = make_int4((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
+                      (d.nk + ath::kTK - 1) / ath::kTK, first);
+      else if (q == kMapCell128) u = make_int4(Grid(d.ncell, 128), 1, 1, first);
+      else if (q == kMapCell256) u = make_int4(Grid(d.ncell, ath::kRedThreads), 1, 1, first);
+      else u = make_int4(Grid(d.npad, 256), 1, 1, first);
+      v[(size_t)q * nb + b] = u;
+      first += u.x * u.y * u.z;
+    }
+    c->mapTotal[q] = first;
+  }
+  // the reductions' partial slots are the 256-cell units
+  for (int b = 0; b < nb; ++b)
+    if (v[(size_t)kMapCell256 * nb + b].w != c->blk[b].partBase)
+      return Fail(ATH_ERR_HIP, "unit map and reduction partials disagree");
+  if (!c->dmaps)
+    if (int r = c->Alloc(&c->dmaps, v.size())) return r;
+  HIP_OK(hipMemcpy(c->dmaps, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // the all-block launches read every block's DevBlock from device memory
+  if (int r = SyncBlocks(c)) return r;
+  if (!c->dblocks) return Fail(ATH_ERR_HIP, "device block table missing");
+  return ATH_OK;
+}
+
 int UploadMultiSweep(ath_ctx *c) {
   using namespace ath;
   std::vector<int4> grid, tiles;
@@ -694,7 +736,7 @@ int Upload(ath_ctx *c) {
   if (int r = UploadConnections(c)) return r;
   if (c->hc.block)
     if (int r = UploadMultiSweep(c)) return r;
-  return ATH_OK;
+  return UploadMaps(c);
 }
 
 int EnsureFp32(ath_ctx *c) {
@@ -744,42 +786,42 @@ void LaunchConnTerms(ath_ctx *c) {
 }
 
 template <int RECON, int LIM, int FLUX>
-void LaunchResidualT(ath_ctx *c, const BlockRes &br) {
-  const auto &d = br.d;
-  dim3 grid((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
-            (d.nk + ath::kTK - 1) / ath::kTK);
-  hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
-  if (c->P.block) {
+void LaunchResidualT(ath_ctx *c) {
+  const ath::Params &P = c->P;
+  const ath::BlockMap mt = c->Map(kMapTile), mc = c->Map(kMapCell128), mp = c->Map(kMapPad256);
+  const dim3 tiles(c->mapTotal[kMapTile]), cells(c->mapTotal[kMapCell128]);
+  hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
+  if (P.block) {
     // inviscid block sum from the inviscid ghost states; finished here, or
     // parked in Ab until the viscous pass has formed dt
-    if (c->P.visc.on)
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d,
-                         c->P, c->dflag);
+    if (P.visc.on)
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d,
-                         c->P, c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
   }
-  if (!c->P.visc.on) return;
+  if (!P.visc.on) return;
   // viscous ghost cells, temperature and viscosity, viscous fluxes
-  // (src/procBlock.cpp:6126-6138)
-  if (br.nwall > 0)
-    hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(br.nwall)), dim3(kBlk), 0, c->stream, d, br.wall, br.nwall,
-                       c->dstates, c->P.gas);
-  hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, d, br.edgeVisc,
-                     br.edgeLines, br.edgePerLine, c->P.gas);
-  hipLaunchKernelGGL(ath::KAux, dim3(Grid(d.npad)), dim3(kBlk), 0, c->stream, d, c->P);
-  if (c->P.visc.recon4)
-    hipLaunchKernelGGL((ath::KViscFlux<true>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+  // (src/procBlock.cpp:6126-6138); every block's ghost cells are its own, so
+  // the blocks' passes need no order among them
+  for (auto &br : c->blk) {
+    const auto &d = br.d;
+    if (br.nwall > 0)
+      hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(br.nwall)), dim3(kBlk), 0, c->stream, d, br.wall, br.nwall,
+                         c->dstates, P.gas);
+    hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, d, br.edgeVisc,
+                       br.edgeLines, br.edgePerLine, P.gas);
+  }
+  hipLaunchKernelGGL(ath::KAux, dim3(c->mapTotal[kMapPad256]), dim3(kBlk), 0, c->stream, mp, P);
+  if (P.visc.recon4)
+    hipLaunchKernelGGL((ath::KViscFlux<true>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
   else
-    hipLaunchKernelGGL((ath::KViscFlux<false>), grid, dim3(256), 0, c->stream, d, c->P, c->dcfl);
+    hipLaunchKernelGGL((ath::KViscFlux<false>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
   // the block diagonal needs dt, which the viscous pass completes
-  if (c->P.block) {
-    if (c->P.visc.recon4)
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
-                         c->dflag);
+  if (P.block) {
+    if (P.visc.recon4)
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), dim3(Grid(d.ncell, 128)), dim3(128), 0, c->stream, d, c->P,
-                         c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
   }
 }
 
@@ -791,25 +833,21 @@ void LaunchAfterResiduals(ath_ctx *c) {
                        c->nhalo);
 }
 
-void LaunchResidual(ath_ctx *c, const BlockRes &br) {
+// the residual stage of every block: one launch per kernel over all blocks
+void LaunchResidual(ath_ctx *c) {
   const auto &hc = c->hc;
   const int recon = (int)hc.recon;
   const int lim = (int)hc.limiter;
   const int flux = (int)hc.flux;
-  ath::DevBlock d = br.d;
-  if (!c->fp32) d.Ainv32 = nullptr;
-  else d.Ainv = nullptr;
-  BlockRes tmp = br;
-  tmp.d = d;
-#define ATH_RES(R, L, F) if (recon == R && lim == L && flux == F) return LaunchResidualT<R, L, F>(c, tmp)
+#define ATH_RES(R, L, F) if (recon == R && lim == L && flux == F) return LaunchResidualT<R, L, F>(c)
   ATH_RES(0, 0, 0); ATH_RES(0, 0, 1);
   ATH_RES(1, 0, 0); ATH_RES(1, 1, 0); ATH_RES(1, 2, 0);
   ATH_RES(1, 0, 1); ATH_RES(1, 1, 1); ATH_RES(1, 2, 1);
   // WENO ignores the limiter
-  if (recon == 2) return flux == 0 ? LaunchResidualT<2, 0, 0>(c, tmp) : LaunchResidualT<2, 0, 1>(c, tmp);
-  if (recon == 3) return flux == 0 ? LaunchResidualT<3, 0, 0>(c, tmp) : LaunchResidualT<3, 0, 1>(c, tmp);
+  if (recon == 2) return flux == 0 ? LaunchResidualT<2, 0, 0>(c) : LaunchResidualT<2, 0, 1>(c);
+  if (recon == 3) return flux == 0 ? LaunchResidualT<3, 0, 0>(c) : LaunchResidualT<3, 0, 1>(c);
   // constant reconstruction ignores the limiter
-  if (recon == 0) return flux == 0 ? LaunchResidualT<0, 0, 0>(c, tmp) : LaunchResidualT<0, 0, 1>(c, tmp);
+  if (recon == 0) return flux == 0 ? LaunchResidualT<0, 0, 0>(c) : LaunchResidualT<0, 0, 1>(c);
 #undef ATH_RES
 }
 
@@ -824,8 +862,8 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     if (timed) hipEventRecord(c->evs[evBase + 2 * pass + (begin ? 0 : 1)], c->stream);
     if (!begin) ++pass;
   };
-  for (auto &br : c->blk)
-    hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
+  hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                     c->Map(kMapCell256), c->P);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     // the block solvers have the tiled and hyperplane orders; the pencil
@@ -919,18 +957,16 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
   }
   if (c->matres) {
     LaunchConnTerms<T>(c);
-    for (auto &br : c->blk)
-      hipLaunchKernelGGL(ath::KMatrixResidual<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d,
-                         c->P, c->dpart + br.partBase);
+    hipLaunchKernelGGL(ath::KMatrixResidual<T>, dim3(c->mapTotal[kMapCell256]), dim3(ath::kRedThreads), 0,
+                       c->stream, c->Map(kMapCell256), c->P, c->dpart);
   }
   return launches;
 }
 
 template <typename T>
 void LaunchUpdateT(ath_ctx *c) {
-  for (auto &br : c->blk)
-    hipLaunchKernelGGL(ath::KUpdate<T>, dim3(br.nparts), dim3(ath::kRedThreads), 0, c->stream, br.d, c->P,
-                       c->dpart + br.partBase, c->dflag);
+  hipLaunchKernelGGL(ath::KUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(ath::kRedThreads), 0, c->stream,
+                     c->Map(kMapCell256), c->P, c->dpart, c->dflag);
   hipLaunchKernelGGL(ath::KReduceNorms, dim3(1), dim3(ath::kRedThreads), 0, c->stream, c->dpart, c->nparts,
                      c->dhist, c->dcounter, (c->P.implicit && c->matres) ? 1 : 0);
 }
@@ -1097,6 +1133,7 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
   if (precision == ATH_IMPLICIT_FP32)
     if (int r = EnsureFp32(c)) return r;
   c->fp32 = precision;
+  c->P.fp32 = precision == ATH_IMPLICIT_FP32 ? 1 : 0;
   return ATH_OK;
 }
 
@@ -1168,7 +1205,7 @@ int ath_assign_ghost_cells(ath_ctx *c) {
 int ath_calc_residual(ath_ctx *c) {
   if (!c) return Fail(ATH_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
-  for (auto &br : c->blk) LaunchResidual(c, br);
+  LaunchResidual(c);
   LaunchAfterResiduals(c);
   if (int r = CheckLaunch()) return r;
   HIP_OK(hipStreamSynchronize(c->stream));
@@ -1231,7 +1268,7 @@ int ath_iterate(ath_ctx *c, int nn, int mm, ath_norms *norms) {
   const int slot = ResetCounterIfFull(c, 1);
   if (slot < 0) return slot;
   LaunchBoundary(c);
-  for (auto &br : c->blk) LaunchResidual(c, br);
+  LaunchResidual(c);
   LaunchAfterResiduals(c);
   LaunchTail(c, false, 0);
   if (c->P.multilevel && mm == c->hc.nonlinear - 1) LaunchCopyNm1(c);
@@ -1250,7 +1287,7 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   const long iters = (long)nsteps * nl;
   if (iters == 0) return ATH_OK;
   if (int r = BuildGraph(c)) return r;
-  const int nflux = (int)c->blk.size();
+  const int nflux = 1;  // one residual stage over all blocks
   // timed runs launch the tail eagerly with an event pair around every sweep
   // pass; untimed runs replay the captured graph
   const int npass = SweepPasses(c);
@@ -1273,11 +1310,9 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
     for (int mm = 0; mm < nl; ++mm, ++it) {
       LaunchBoundary(c);
       const size_t eb = 2 + (size_t)it * perIt;
-      for (size_t q = 0; q < c->blk.size(); ++q) {
-        if (c->timing) hipEventRecord(c->evs[eb + 2 * q], c->stream);
-        LaunchResidual(c, c->blk[q]);
-        if (c->timing) hipEventRecord(c->evs[eb + 2 * q + 1], c->stream);
-      }
+      if (c->timing) hipEventRecord(c->evs[eb], c->stream);
+      LaunchResidual(c);
+      if (c->timing) hipEventRecord(c->evs[eb + 1], c->stream);
       LaunchAfterResiduals(c);
       if (c->timing) sweepLaunches += LaunchTail(c, true, eb + 2 * nflux);
       else HIP_OK(hipGraphLaunch(c->graphExec, c->stream));

@@ -66,7 +66,7 @@ CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
 CHANNEL_SWEEP_TRAFFIC = 0.5 * (1342.8e6 + 1118.8e6)   # one pass, forward and backward averaged
-CHANNEL_FLUX_TRAFFIC = 433.0e6                        # residual stage of one block
+CHANNEL_FLUX_TRAFFIC = None                           # residual stage over all blocks
 
 
 def parse():
@@ -178,8 +178,8 @@ def bench_channel(args, Solver, cases):
     ncell = sum(s.ncells(b) for b in range(nb))
     last, wall, tm = timed_run(s, states0, nphys, max(0, warm // sub), prof_phys=min(nphys, 2))
     value = ncell * steps / wall
-    # one residual stage per block per iteration, one pass = a forward or a
-    # backward sweep over all blocks
+    # one residual stage (every kernel launched once over all blocks) and two
+    # passes (a forward and a backward sweep over all blocks) per iteration
     flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
     pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
     flux_share = tm.flux_ms / max(tm.run_ms, 1e-30)
@@ -190,9 +190,9 @@ def bench_channel(args, Solver, cases):
                      "launch), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
                      traffic=CHANNEL_SWEEP_TRAFFIC if full else None,
                      launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1))
-    flux = roofline("residual stage of one block (inviscid + viscous fluxes, spectral radii, dt, block diagonal "
-                    "and inverse)", CHANNEL_FLUX_BYTES * ncell / nb, flux_avg_s, flux_share,
-                    traffic=CHANNEL_FLUX_TRAFFIC if full else None)
+    flux = roofline("residual stage over all eight blocks (inviscid + viscous fluxes, spectral radii, dt, block "
+                    "diagonal and inverse; one launch per kernel)", CHANNEL_FLUX_BYTES * ncell, flux_avg_s,
+                    flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full else None)
     dominant, other = (sweep, flux) if sweep_share >= flux_share else (flux, sweep)
     out = {
         "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",

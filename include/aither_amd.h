@@ -149,8 +149,9 @@ int ath_get_block_diagonal(ath_ctx *ctx, int block, int inverse, double *a);
 /* ---- timing of the last ath_run (HIP events on the context's stream) ---- */
 typedef struct {
   double run_ms;            /* whole ath_run */
-  double flux_ms;           /* summed duration of the fused flux kernels */
-  int flux_launches;
+  double flux_ms;           /* summed residual stages: every residual kernel
+                               launched once over all blocks */
+  int flux_launches;        /* residual stages timed (one per iteration) */
   double sweep_ms;          /* summed sweep passes: each forward or backward
                                LU-SGS pass (or DP-LUR sweep) over all blocks,
                                including its one-thread prologue */
