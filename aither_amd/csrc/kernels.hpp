@@ -276,11 +276,36 @@ __device__ inline void GhostState(const GasT<double> &g, const double in[5], int
   }
 }
 
-__global__ void __launch_bounds__(256) KBoundary(DevBlock b, const BcTask *__restrict__ tasks,
-                                                 int ntasks, long total,
+// Ghost-cell work of every block in one launch: per block, its task list,
+// the global index of its first item (a ghost cell, an edge line or a wall
+// cell) and its item count; thread t works on the item of the last block
+// whose first item is at or before t.
+struct BlockTasks {
+  const void *tasks;
+  long first, count;
+  int ntasks;  // tasks in the list (boundary patches)
+  int per;     // tasks per item (edge lines)
+};
+__device__ __forceinline__ int FindTaskBlock(const BlockTasks *bt, int nb, long t) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bt[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) KBoundary(const DevBlock *__restrict__ blocks,
+                                                 const BlockTasks *__restrict__ bt, int nb, long total,
                                                  const BcStateDev *__restrict__ states, Gas gas) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
+  const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tg >= total) return;
+  const int wb = FindTaskBlock(bt, nb, tg);
+  const DevBlock &b = blocks[wb];
+  const BcTask *__restrict__ tasks = static_cast<const BcTask *>(bt[wb].tasks);
+  const int ntasks = bt[wb].ntasks;
+  const long t = tg - bt[wb].first;
   int q = 0;
   while (q + 1 < ntasks && tasks[q + 1].base <= t) ++q;
   const BcTask tk = tasks[q];
@@ -521,10 +546,15 @@ struct EdgeTask {
 };
 
 // one thread per edge line; its tasks run in the reference's layer order
-__global__ void __launch_bounds__(64) KEdgeGhosts(DevBlock b, const EdgeTask *__restrict__ tasks, int nlines,
-                                                  int perLine, Gas gas) {
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= nlines) return;
+__global__ void __launch_bounds__(64) KEdgeGhosts(const DevBlock *__restrict__ blocks,
+                                                  const BlockTasks *__restrict__ bt, int nb, long total, Gas gas) {
+  const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tg >= total) return;
+  const int wb = FindTaskBlock(bt, nb, tg);
+  const DevBlock &b = blocks[wb];
+  const EdgeTask *__restrict__ tasks = static_cast<const EdgeTask *>(bt[wb].tasks);
+  const int perLine = bt[wb].per;
+  const long line = tg - bt[wb].first;
   const GasT<double> g = Cast<double>(gas);
   for (int q = 0; q < perLine; ++q) {
     const EdgeTask t = tasks[(long)line * perLine + q];
@@ -553,11 +583,15 @@ struct WallTask {
   int state, pad;
 };
 
-__global__ void __launch_bounds__(256) KViscWall(DevBlock b, const WallTask *__restrict__ tasks, int n,
+__global__ void __launch_bounds__(256) KViscWall(const DevBlock *__restrict__ blocks,
+                                                 const BlockTasks *__restrict__ bt, int nb, long total,
                                                  const BcStateDev *__restrict__ states, Gas gas) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const WallTask w = tasks[t];
+  const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tg >= total) return;
+  const int wb = FindTaskBlock(bt, nb, tg);
+  const DevBlock &b = blocks[wb];
+  const WallTask *__restrict__ tasks = static_cast<const WallTask *>(bt[wb].tasks);
+  const WallTask w = tasks[tg - bt[wb].first];
   const BcStateDev *s = w.state >= 0 ? &states[w.state] : nullptr;
   double in[5], gh[5];
 #pragma unroll

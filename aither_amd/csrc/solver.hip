@@ -88,6 +88,11 @@ struct ath_ctx {
   // nb entries each, and each one's total unit count
   int4 *dmaps = nullptr;
   int mapTotal[4] = {0, 0, 0, 0};
+  // the ghost-cell task lists of every block (ath::BlockTasks), kNumTaskMaps
+  // tables of nb entries, and each one's item total
+  ath::BlockTasks *dbt = nullptr;
+  long taskTotal[4] = {0, 0, 0, 0};
+  const ath::BlockTasks *Tasks(int q) const { return dbt + (size_t)q * blk.size(); }
   ath::BlockMap Map(int q) const { return {dblocks, dmaps + (size_t)q * blk.size(), (int)blk.size()}; }
   // timing
   int timing = 0;
@@ -123,6 +128,9 @@ int Grid(long n, int b = kBlk) { return (int)((n + b - 1) / b); }
 // residual, update, initialisation; their partials are the reductions'
 // slots) and of 256 padded cells (auxiliary variables)
 enum { kMapTile = 0, kMapCell128 = 1, kMapCell256 = 2, kMapPad256 = 3, kNumMaps = 4 };
+// the ghost-cell task tables: boundary patches, edge lines of the inviscid
+// and of the viscous pass, no-slip wall cells
+enum { kTaskBc = 0, kTaskEdgeInv = 1, kTaskEdgeVisc = 2, kTaskWall = 3, kNumTaskMaps = 4 };
 
 // boundary-face slot, as BtSlot in the kernels
 long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) {
@@ -407,6 +415,26 @@ int UploadMaps(ath_ctx *c) {
   if (!c->dmaps)
     if (int r = c->Alloc(&c->dmaps, v.size())) return r;
   HIP_OK(hipMemcpy(c->dmaps, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
+  std::vector<ath::BlockTasks> bt((size_t)kNumTaskMaps * nb);
+  for (int q = 0; q < kNumTaskMaps; ++q) {
+    long first = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto &br = c->blk[b];
+      ath::BlockTasks e{};
+      e.first = first;
+      if (q == kTaskBc) { e.tasks = br.dtasks; e.count = br.bcTotal; e.ntasks = (int)br.tasks.size(); }
+      else if (q == kTaskEdgeInv) { e.tasks = br.edgeInv; e.count = br.edgeLines; e.per = br.edgePerLine; }
+      else if (q == kTaskEdgeVisc) { e.tasks = br.edgeVisc; e.count = br.edgeLines; e.per = br.edgePerLine; }
+      else { e.tasks = br.wall; e.count = br.nwall; }
+      if (!e.tasks) e.count = 0;
+      bt[(size_t)q * nb + b] = e;
+      first += e.count;
+    }
+    c->taskTotal[q] = first;
+  }
+  if (!c->dbt)
+    if (int r = c->Alloc(&c->dbt, bt.size())) return r;
+  HIP_OK(hipMemcpy(c->dbt, bt.data(), bt.size() * sizeof(ath::BlockTasks), hipMemcpyHostToDevice));
   // the all-block launches read every block's DevBlock from device memory
   if (int r = SyncBlocks(c)) return r;
   if (!c->dblocks) return Fail(ATH_ERR_HIP, "device block table missing");
@@ -763,18 +791,17 @@ int EnsureFp32(ath_ctx *c) {
 // states of the boundary patches, then the connections' ghost cells (edge
 // ghost cells, which only viscous terms read, are not formed)
 void LaunchBoundary(ath_ctx *c) {
-  for (auto &br : c->blk)
-    if (br.bcTotal > 0)
-      hipLaunchKernelGGL(ath::KBoundary, dim3(Grid(br.bcTotal)), dim3(kBlk), 0, c->stream, br.d,
-                         br.dtasks, (int)br.tasks.size(), br.bcTotal, c->dstates, c->P.gas);
+  const int nb = (int)c->blk.size();
+  if (c->taskTotal[kTaskBc] > 0)
+    hipLaunchKernelGGL(ath::KBoundary, dim3(Grid(c->taskTotal[kTaskBc])), dim3(kBlk), 0, c->stream, c->dblocks,
+                       c->Tasks(kTaskBc), nb, c->taskTotal[kTaskBc], c->dstates, c->P.gas);
   if (c->nhalo > 0)
     hipLaunchKernelGGL(ath::KHaloState, dim3(Grid(c->nhalo)), dim3(kBlk), 0, c->stream, c->dblocks, c->dhalo,
                        c->nhalo);
   // edge ghost cells: read only by the viscous gradients
-  if (c->P.visc.on)
-    for (auto &br : c->blk)
-      hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, br.d, br.edgeInv,
-                         br.edgeLines, br.edgePerLine, c->P.gas);
+  if (c->P.visc.on && c->taskTotal[kTaskEdgeInv] > 0)
+    hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(c->taskTotal[kTaskEdgeInv], 64)), dim3(64), 0, c->stream,
+                       c->dblocks, c->Tasks(kTaskEdgeInv), nb, c->taskTotal[kTaskEdgeInv], c->P.gas);
 }
 
 // connection terms from the partners' current du, before each pass
@@ -803,14 +830,13 @@ void LaunchResidualT(ath_ctx *c) {
   // viscous ghost cells, temperature and viscosity, viscous fluxes
   // (src/procBlock.cpp:6126-6138); every block's ghost cells are its own, so
   // the blocks' passes need no order among them
-  for (auto &br : c->blk) {
-    const auto &d = br.d;
-    if (br.nwall > 0)
-      hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(br.nwall)), dim3(kBlk), 0, c->stream, d, br.wall, br.nwall,
-                         c->dstates, P.gas);
-    hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(br.edgeLines, 64)), dim3(64), 0, c->stream, d, br.edgeVisc,
-                       br.edgeLines, br.edgePerLine, P.gas);
-  }
+  const int nb = (int)c->blk.size();
+  if (c->taskTotal[kTaskWall] > 0)
+    hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(c->taskTotal[kTaskWall])), dim3(kBlk), 0, c->stream, c->dblocks,
+                       c->Tasks(kTaskWall), nb, c->taskTotal[kTaskWall], c->dstates, P.gas);
+  if (c->taskTotal[kTaskEdgeVisc] > 0)
+    hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(c->taskTotal[kTaskEdgeVisc], 64)), dim3(64), 0, c->stream,
+                       c->dblocks, c->Tasks(kTaskEdgeVisc), nb, c->taskTotal[kTaskEdgeVisc], P.gas);
   hipLaunchKernelGGL(ath::KAux, dim3(c->mapTotal[kMapPad256]), dim3(kBlk), 0, c->stream, mp, P);
   if (P.visc.recon4)
     hipLaunchKernelGGL((ath::KViscFlux<true>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
