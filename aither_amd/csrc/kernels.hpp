@@ -633,7 +633,12 @@ __global__ void __launch_bounds__(256) KAux(BlockMap M, Params P) {
 __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j, int k, double vg[3][3],
                                           double tg[3]) {
   const int L[3] = {i - (d == 0), j - (d == 1), k - (d == 2)}, U[3] = {i, j, k};
-  double au[3][3], al[3][3];
+  const long cL = PadIdx(b, L[0], L[1], L[2]), cU = PadIdx(b, U[0], U[1], U[2]);
+  const double vol = 0.5 * (b.volP[cL] + b.volP[cU]);
+  // Green-Gauss over the control volume's three face pairs: each entry is
+  // the sum, left to right, of +vu au - vl al for the pairs q = 0, 1, 2, so
+  // it is accumulated one pair at a time
+  double acc[3][4];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int eq[3] = {q == 0, q == 1, q == 2};
@@ -650,20 +655,14 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
       f3 = F[FacePadIdx(b, q, U[0], U[1], U[2])];
       f4 = F[FacePadIdx(b, q, L[0], L[1], L[2])];
     }
-    au[q][0] = 0.5 * (f1.x * f1.w + f2.x * f2.w);
-    au[q][1] = 0.5 * (f1.y * f1.w + f2.y * f2.w);
-    au[q][2] = 0.5 * (f1.z * f1.w + f2.z * f2.w);
-    al[q][0] = 0.5 * (f3.x * f3.w + f4.x * f4.w);
-    al[q][1] = 0.5 * (f3.y * f3.w + f4.y * f4.w);
-    al[q][2] = 0.5 * (f3.z * f3.w + f4.z * f4.w);
-  }
-  const long cL = PadIdx(b, L[0], L[1], L[2]), cU = PadIdx(b, U[0], U[1], U[2]);
-  const double vol = 0.5 * (b.volP[cL] + b.volP[cU]);
-  // face values: three velocity components, then temperature
-  double vu[3][4], vl[3][4];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int eq[3] = {q == 0, q == 1, q == 2};
+    double au[3], al[3];
+    au[0] = 0.5 * (f1.x * f1.w + f2.x * f2.w);
+    au[1] = 0.5 * (f1.y * f1.w + f2.y * f2.w);
+    au[2] = 0.5 * (f1.z * f1.w + f2.z * f2.w);
+    al[0] = 0.5 * (f3.x * f3.w + f4.x * f4.w);
+    al[1] = 0.5 * (f3.y * f3.w + f4.y * f4.w);
+    al[2] = 0.5 * (f3.z * f3.w + f4.z * f4.w);
+    // face values of the pair: three velocity components, then temperature
     const long uP = PadIdx(b, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2]);
     const long lP = PadIdx(b, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2]);
     const long uM = PadIdx(b, U[0] - eq[0], U[1] - eq[1], U[2] - eq[2]);
@@ -672,12 +671,18 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
     for (int v = 0; v < 4; ++v) {
       const double *A = v < 3 ? b.W + (v + 1) * b.npad : b.T;
       const double xl = A[cL], xu = A[cU];
+      double vu, vl;
       if (q == d) {
-        vu[q][v] = xu;
-        vl[q][v] = xl;
+        vu = xu;
+        vl = xl;
       } else {
-        vu[q][v] = 0.25 * (xl + xu + A[uP] + A[lP]);
-        vl[q][v] = 0.25 * (xl + xu + A[uM] + A[lM]);
+        vu = 0.25 * (xl + xu + A[uP] + A[lP]);
+        vl = 0.25 * (xl + xu + A[uM] + A[lM]);
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        if (q == 0) acc[r][v] = vu * au[r] - vl * al[r];
+        else acc[r][v] = acc[r][v] + vu * au[r] - vl * al[r];
       }
     }
   }
@@ -685,11 +690,8 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      vg[r][c] = (vu[0][c] * au[0][r] - vl[0][c] * al[0][r] + vu[1][c] * au[1][r] - vl[1][c] * al[1][r] +
-                  vu[2][c] * au[2][r] - vl[2][c] * al[2][r]) * invVol;
-    tg[r] = (vu[0][3] * au[0][r] - vl[0][3] * al[0][r] + vu[1][3] * au[1][r] - vl[1][3] * al[1][r] +
-             vu[2][3] * au[2][r] - vl[2][3] * al[2][r]) * invVol;
+    for (int c = 0; c < 3; ++c) vg[r][c] = acc[r][c] * invVol;
+    tg[r] = acc[r][3] * invVol;
   }
 }
 
@@ -2030,9 +2032,12 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
     const long ncell = b.ncell, nbf = b.nbf;
     const T *bt = OpBuf<T>::Bt(b);
     // connection term (static during the pass) of the face of cell (ci, cj,
-    // ck) in direction d on the dependency side; zero without connections
+    // ck) in direction d on the dependency side; zero without connections,
+    // and zero in the first forward pass, where every du starts at zero (J 0
+    // sums to +0), so the connection terms are not formed for it
+    const bool connZero = FWD && !full;
     auto conn = [&](int d, int ci, int cj, int ck, int e) -> T {
-      return bt ? bt[e * nbf + BtSlot(b, d, FWD ? 0 : 1, ci, cj, ck)] : T(0);
+      return (bt && !connZero) ? bt[e * nbf + BtSlot(b, d, FWD ? 0 : 1, ci, cj, ck)] : T(0);
     };
     // LDS column of the dependency neighbour in each direction: a cell of
     // the tile or a halo column

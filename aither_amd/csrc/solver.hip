@@ -919,7 +919,9 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
       }
     } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED && c->P.block) {
       for (int dir = 0; dir < 2; ++dir) {
-        LaunchConnTerms<T>(c);
+        // the first forward pass starts from du = 0: its connection terms
+        // are zero and the sweep does not read them
+        if (dir > 0 || full) LaunchConnTerms<T>(c);
         mark(true);
         hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
         if (dir == 0)
