@@ -512,6 +512,28 @@ __device__ __forceinline__ double InvCellSpecRad(const GasT<double> &g, double r
   return (fabs((u * nx + v * ny) + w * nz) + SoS(g, rho, p)) * fMag;
 }
 
+// RoeOffDiagonal for the inviscid equations (src/fluxJacobian.cpp:240-290):
+// the change of the Roe flux through the face when the neighbour's state
+// moves by du.  The old flux always takes the neighbour as its left state and
+// the cell (Wd) as its right one; the new flux takes the updated neighbour as
+// the left state for a lower (positive) and as the right state for an upper
+// neighbour.  The spectral radius term is zero without the viscous terms,
+// applied as +-(du 0).
+__device__ __forceinline__ void RoeOffDiag(const GasT<double> &g, const double Wnb[5], const double Wd[5],
+                                           const double du[5], double4 a, bool positive, double out[5]) {
+  double Fo[5], Fn[5], Wu[5];
+  RoeFlux(g, Wnb, Wd, a.x, a.y, a.z, Fo);
+  UpdatePrim(g, Wnb, du, Wu);
+  if (positive) RoeFlux(g, Wu, Wd, a.x, a.y, a.z, Fn);
+  else RoeFlux(g, Wd, Wu, a.x, a.y, a.z, Fn);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const double fc = a.w * (Fn[e] - Fo[e]);
+    const double sd = du[e] * 0.0;
+    out[e] = positive ? fc + sd : fc - sd;
+  }
+}
+
 // RusanovScalarOffDiagonal for the inviscid equations
 // (src/fluxJacobian.cpp:122-162): 0.5|A| (F(U_nb + du) - F(U_nb)) +/- lambda du
 // with lambda = InvFaceSpectralRadius (include/spectralRadius.hpp:66-80)

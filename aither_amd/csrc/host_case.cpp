@@ -744,7 +744,10 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
         else if (val == "ausm") c.flux = Flux::AUSM;
         else return "unsupported inviscidFlux " + val;
       } else if (key == "inviscidFluxJacobian") {
-        if (val != "rusanov") return "unsupported inviscidFluxJacobian " + val;
+        // src/input.cpp:82, 372-373; OffDiagonal (src/fluxJacobian.cpp:196-238)
+        if (val == "rusanov") c.roeJac = false;
+        else if (val == "approximateRoe") c.roeJac = true;
+        else return "unsupported inviscidFluxJacobian " + val;
       } else if (key == "multigridLevels") {
         if (std::stoi(val) != 1) return "multigrid is not supported";
       } else if (key == "fluids") {
@@ -805,6 +808,12 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (c.rRef <= 0 || c.tRef <= 0) return "referenceDensity and referenceTemperature are required";
   if (c.timeint == TimeInt::ExplicitEuler) c.nonlinear = 1;  // src/input.cpp:928-934
   c.block = c.block && c.Implicit();  // input::IsBlockMatrix (src/input.cpp:713-720)
+  // RoeOffDiagonal forms a flux change, not a matrix, so the block solvers
+  // cannot use it; with the viscous terms the reference's call passes the
+  // blending coefficient (0 in laminar flow) as the centre distance
+  // (src/fluxJacobian.cpp:229-230 against :240-245)
+  if (c.roeJac && c.block) return "approximateRoe with a block matrix solver is not supported";
+  if (c.roeJac && c.viscous) return "approximateRoe with the viscous equations is not supported";
   double fn = 0, fm = 0;
   if (!FluidProps(c.dir, fluid, &fn, &fm, &c.suth)) return "unknown fluid " + fluid;
 

@@ -117,6 +117,7 @@ struct HostCase {
   TimeInt timeint = TimeInt::ExplicitEuler;
   Solver solver = Solver::LUSGS;
   bool block = false;  // blusgs / bdplur: 5x5 block main diagonal and off-diagonals
+  bool roeJac = false;  // inviscidFluxJacobian approximateRoe: RoeOffDiagonal off-diagonals
   double gasR = 0, gasN = 0, gamma = 0;
   // laminar viscous terms (equationSet navierStokes)
   bool viscous = false;

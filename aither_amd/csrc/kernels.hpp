@@ -78,6 +78,7 @@ struct Params {
   int implicit, multilevel, initMatrix, explicitEuler;
   int block;  // matrixSolver blusgs / bdplur: 5x5 main diagonal and off-diagonals
   int fp32;   // the scalar implicit operator runs in FP32: the residual stage writes Ainv32, else Ainv
+  int roeJac; // inviscidFluxJacobian approximateRoe: the off-diagonals are Roe flux changes (FP64)
 };
 
 // ---------------------------------------------------------------------------
@@ -1170,7 +1171,7 @@ struct OpBuf<float> {
 template <typename T>
 __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const ViscParams &vp,
                                            const T *x, int i, int j, int k, bool upper, T out[5],
-                                           int block = 0) {
+                                           int block = 0, int roe = 0) {
 #pragma unroll
   for (int e = 0; e < 5; ++e) out[e] = T(0);
   const T *bt = OpBuf<T>::Bt(b);
@@ -1204,6 +1205,20 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
 #pragma unroll
       for (int q = 0; q < 25; ++q) m[q] = T(J[q]);
       BlockMult<T>(m, du, o);
+    } else if (roe) {
+      // RoeOffDiagonal: the cell's own state is the other side of the face
+      const GasT<double> gd = {double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
+      const long cc = PadIdx(b, i, j, k);
+      double Wd[5], Wn[5], dd[5], od[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        Wd[e] = b.W[e * b.npad + cc];
+        Wn[e] = double(W[e]);
+        dd[e] = double(du[e]);
+      }
+      RoeOffDiag(gd, Wn, Wd, dd, a, !upper, od);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) o[e] = T(od[e]);
     } else {
       const T lv = vp.on ? ViscFaceSR<T>(g, W, T(a.w), T(b.c2c[d][fq]), T(b.mu[c]), vp) : T(0);
       ScalarOffDiag<T>(g, W, du, T(a.x), T(a.y), T(a.z), T(a.w), !upper, o, vp.on != 0, lv);
@@ -1255,10 +1270,10 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T L[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
   if (useUpper) {
     T U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
 #pragma unroll
     for (int e = 0; e < 5; ++e) L[e] -= U[e];
   }
@@ -1290,12 +1305,12 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T U[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
   if (P.block) {
     T v[5], o[5];
     if (full) {
       T L[5];
-      OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
+      OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
       double rb[5];
       Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
@@ -1313,7 +1328,7 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
   const T ainv = OpBuf<T>::Ainv(b)[p];
   if (full) {
     T L[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, 0, P.roeJac);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
@@ -1334,8 +1349,8 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
   const GasT<T> g = Cast<T>(P.gas);
   const T *xo = OpBuf<T>::Xold(b);
   T L[5], U[5];
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L, P.block);
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U, P.block);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L, P.block, P.roeJac);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U, P.block, P.roeJac);
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
   T *x = OpBuf<T>::X(b);
@@ -1398,8 +1413,8 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(BlockMap M, Param
     const GasT<T> g = Cast<T>(P.gas);
     const T *x = OpBuf<T>::X(b);
     T L[5], U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block);
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
     const double A = P.block ? 0.0 : b.A[p];
@@ -1730,6 +1745,20 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
     } else {
       BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
     }
+  } else if (P.roeJac) {
+    // RoeOffDiagonal with the interior cell next to the ghost as the other
+    // side (a lower ghost lies below it, an upper ghost above)
+    const long s = c.d == 0 ? 1 : (c.d == 1 ? (long)A.NI : (long)A.NI * A.NJ);
+    const long ci = c.upper ? c.gpad - s : c.gpad + s;
+    double Wd[5], dd[5], od[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      Wd[e] = A.W[e * A.npad + ci];
+      dd[e] = double(dn[e]);
+    }
+    RoeOffDiag(Cast<double>(P.gas), Wn, Wd, dd, A.fa[c.d][c.face], c.upper == 0, od);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o[e] = T(od[e]);
   } else if (P.visc.on)
     NbOffDiag<T>(Cast<T>(P.gas), Wn, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.c2c[c.d][c.face],
                  A.mu[c.gpad]);

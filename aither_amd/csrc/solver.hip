@@ -501,6 +501,7 @@ int Upload(ath_ctx *c) {
   c->P.multilevel = hc.MultiLevel() ? 1 : 0;
   c->P.initMatrix = hc.InitMatrix() ? 1 : 0;
   c->P.explicitEuler = hc.timeint == TimeInt::ExplicitEuler ? 1 : 0;
+  c->P.roeJac = hc.roeJac ? 1 : 0;
   c->P.block = hc.block ? 1 : 0;
   c->P.visc.on = hc.viscous ? 1 : 0;
   c->P.visc.recon4 = hc.viscRecon4 ? 1 : 0;
@@ -893,8 +894,10 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     // the block solvers have the tiled and hyperplane orders; the pencil
-    // order maps to the tiled one
-    const int order = c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL ? ATH_SWEEP_TILED : c->sweepOrder;
+    // order maps to the tiled one.  The Roe flux-change off-diagonals
+    // (approximateRoe) run in the hyperplane order only.
+    const int order = c->P.roeJac ? ATH_SWEEP_HYPERPLANE
+                      : (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL ? ATH_SWEEP_TILED : c->sweepOrder);
     if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_PENCIL) {
       if (sw == 0)
         for (auto &br : c->blk)
@@ -1158,6 +1161,8 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
   HIP_OK(hipSetDevice(c->device));
   if (precision == ATH_IMPLICIT_FP32 && c->P.block)
     return Fail(ATH_ERR_ARG, "the FP32 operator is not available with a block matrix solver");
+  if (precision == ATH_IMPLICIT_FP32 && c->P.roeJac)
+    return Fail(ATH_ERR_ARG, "the FP32 operator is not available with the approximateRoe flux Jacobian");
   if (precision == ATH_IMPLICIT_FP32)
     if (int r = EnsureFp32(c)) return r;
   c->fp32 = precision;

@@ -153,6 +153,7 @@ struct orc_case {
   int timeint; /* 0 explicitEuler 1 implicitEuler 2 crankNicholson 3 bdf2 */
   int solver;  /* 0 lusgs, 1 dplur */
   int block;   /* 5x5 block main diagonal and off-diagonals (blusgs, bdplur) */
+  int roeJac;  /* inviscidFluxJacobian approximateRoe: RoeOffDiagonal */
   int implicit, multilevel, storeN, initMatrix;
   orc_gas gas;
   double gamma;
@@ -633,6 +634,33 @@ void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
                         const double *area, int positive, double *out) {
   uv3 a = {V(area[0], area[1], area[2]), area[3]};
   scalar_offdiag(g, Wnb, du, a, positive, out);
+}
+
+/* RoeOffDiagonal, inviscid (src/fluxJacobian.cpp:240-290): the change of the
+ * Roe flux through the face when the neighbour's state moves by du.  The old
+ * flux always takes the neighbour as its left state and the cell (`Wd`) as
+ * its right one; the new flux takes the updated neighbour as the left state
+ * for a lower (positive) and as the right state for an upper neighbour.
+ * The spectral radius term is zero without the viscous terms, applied as
+ * +-(du * 0). */
+static void roe_offdiag(const orc_gas *g, const double *Wnb, const double *Wd, const double *du, uv3 a,
+                        int positive, double *out) {
+  double n[3] = {a.u.x, a.u.y, a.u.z};
+  double Fold[NEQ], Fnew[NEQ], Wu[NEQ];
+  orc_roe_flux(g, Wnb, Wd, n, Fold);
+  update_prim(g, Wnb, du, Wu);
+  if (positive) orc_roe_flux(g, Wu, Wd, n, Fnew);
+  else orc_roe_flux(g, Wd, Wu, n, Fnew);
+  for (int e = 0; e < NEQ; ++e) {
+    double fc = a.m * (Fnew[e] - Fold[e]);
+    double sd = du[e] * 0.0;
+    out[e] = positive ? fc + sd : fc - sd;
+  }
+}
+void orc_roe_offdiag(const orc_gas *g, const double *Wnb, const double *Wd, const double *du,
+                     const double *area, int positive, double *out) {
+  uv3 a = {V(area[0], area[1], area[2]), area[3]};
+  roe_offdiag(g, Wnb, Wd, du, a, positive, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1750,7 +1778,10 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
       else if (!strcmp(val, "ausm")) c->flux = 1;
       else { seterr(err, errlen, "unsupported inviscidFlux %s", val); goto fail; }
     } else if (!strcmp(key, "inviscidFluxJacobian")) {
-      if (strcmp(val, "rusanov")) { seterr(err, errlen, "unsupported inviscidFluxJacobian %s", val); goto fail; }
+      /* src/input.cpp:82, 372-373; OffDiagonal src/fluxJacobian.cpp:196-238 */
+      if (!strcmp(val, "rusanov")) c->roeJac = 0;
+      else if (!strcmp(val, "approximateRoe")) c->roeJac = 1;
+      else { seterr(err, errlen, "unsupported inviscidFluxJacobian %s", val); goto fail; }
     } else if (!strcmp(key, "multigridLevels")) {
       if (atoi(val) != 1) { seterr(err, errlen, "multigrid is not supported"); goto fail; }
     } else if (!strcmp(key, "fluids")) {
@@ -1833,6 +1864,12 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
   c->storeN = c->implicit;
   c->initMatrix = (c->solver == 1 || c->sweeps > 1);
   c->block = c->block && c->implicit; /* IsBlockMatrix: src/input.cpp:713-720 */
+  /* RoeOffDiagonal forms a flux change, not a matrix, so the block solvers
+   * cannot use it; with the viscous terms the reference's call passes the
+   * blending coefficient (0 in laminar flow) as the centre distance
+   * (src/fluxJacobian.cpp:229-230 against :240-245) */
+  if (c->roeJac && c->block) { seterr(err, errlen, "approximateRoe with a block matrix solver is not supported"); goto fail2; }
+  if (c->roeJac && c->viscous) { seterr(err, errlen, "approximateRoe with the viscous equations is not supported"); goto fail2; }
   if (c->timeint == 0) c->nonlinear = 1; /* src/input.cpp:928-934 */
 
   /* reference speed of sound and nondimensional gas constant:
@@ -2917,6 +2954,11 @@ static void implicit_lu(const orc_case *c, const blk_t *b, int i, int j, int k,
         for (int q = 0; q < 25; ++q) J[q] = !upper ? J[q] - Vj[q] : J[q] + Vj[q];
       }
       block_mult(J, &x[cn * NEQ], o);
+      for (int e = 0; e < NEQ; ++e) out[e] += o[e];
+      continue;
+    }
+    if (c->roeJac) {
+      roe_offdiag(&c->gas, &b->W[cn * NEQ], &b->W[cidx(b, i, j, k) * NEQ], &x[cn * NEQ], a, !upper, o);
       for (int e = 0; e < NEQ; ++e) out[e] += o[e];
       continue;
     }

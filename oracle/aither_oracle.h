@@ -135,6 +135,9 @@ double orc_inv_cell_specrad(const orc_gas *g, const double *W,
 void orc_scalar_offdiag(const orc_gas *g, const double *Wnb, const double *du,
                         const double *area /*ux,uy,uz,mag*/, int positive,
                         double *out);
+/* RoeOffDiagonal, inviscid (src/fluxJacobian.cpp:240-290); area = (nx, ny, nz, |A|) */
+void orc_roe_offdiag(const orc_gas *g, const double *Wnb, const double *Wd, const double *du,
+                     const double *area, int positive, double *out);
 /* Rusanov flux Jacobian 0.5|A| dF/dU +/- spectral radius I, 25 row-major */
 void orc_rusanov_jacobian(const orc_gas *g, const double *W,
                           const double *area /*ux,uy,uz,mag*/, int positive,

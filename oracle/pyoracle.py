@@ -86,6 +86,7 @@ def lib():
         L.orc_inv_cell_specrad.argtypes = [G, _D, _D, _D]
         L.orc_inv_cell_specrad.restype = ctypes.c_double
         L.orc_scalar_offdiag.argtypes = [G, _D, _D, _D, ctypes.c_int, _D]
+        L.orc_roe_offdiag.argtypes = [G, _D, _D, _D, _D, ctypes.c_int, _D]
         L.orc_rusanov_jacobian.argtypes = [G, _D, _D, ctypes.c_int, _D]
         L.orc_block_inverse.argtypes = [_D]
         L.orc_block_inverse.restype = ctypes.c_int
