@@ -784,8 +784,12 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
             s.isothermal = true;
             s.twall = std::stod(p.at("temperature"));
           }
-          if (p.count("heatFlux") && std::stod(p.at("heatFlux")) != 0.0)
-            return "viscousWall heat flux is not supported";
+          // viscousWall::IsConstantHeatFlux (include/inputStates.hpp:379-381)
+          if (p.count("heatFlux")) {
+            if (p.count("temperature")) return "viscousWall: give a temperature or a heat flux, not both";
+            s.qwall = std::stod(p.at("heatFlux"));
+            s.heatFlux = s.qwall != 0.0;
+          }
           if (p.count("wallTreatment") && p.at("wallTreatment") != "lowRe")
             return "wall laws are not supported";
           if (p.count("startTag")) s.startTag = std::stoi(p.at("startTag"));
@@ -861,6 +865,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     if (s.kind == "viscousWall") {  // src/inputStates.cpp:775-783
       for (auto &v : s.vel) v /= c.aRef;
       s.twall /= c.tRef;
+      s.qwall /= std::pow(c.aRef / c.lRef, 3.0);  // src/inputStates.cpp:775-783
       continue;
     }
     if (s.kind == "stagnationInlet") {

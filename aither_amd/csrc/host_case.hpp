@@ -40,6 +40,8 @@ struct BcState {  // boundaryStates entry, nondimensional
   std::array<double, 3> dir{0, 0, 0};
   bool isothermal = false;  // viscousWall with a given wall temperature
   double twall = 0;
+  bool heatFlux = false;    // viscousWall with a given nonzero heat flux
+  double qwall = 0;
   int startTag = -1, endTag = -1;             // periodic: the paired surfaces' tags
   std::array<double, 3> translation{0, 0, 0};  // periodic: start surface onto end surface
 };

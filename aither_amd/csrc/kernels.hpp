@@ -126,6 +126,8 @@ struct BcStateDev {
   double p, rho, u, v, w, p0, t0, dx, dy, dz;
   double tw;  // viscousWall: wall temperature when iso
   int iso;
+  int flux;   // viscousWall: a given heat flux qw
+  double qw;
 };
 
 struct BcTask {
@@ -582,11 +584,13 @@ __global__ void __launch_bounds__(64) KEdgeGhosts(const DevBlock *__restrict__ b
 struct WallTask {
   long src, dst;
   int state, pad;
+  double wd;  // wall distance of the wall-adjacent cell (heat-flux walls)
 };
 
 __global__ void __launch_bounds__(256) KViscWall(const DevBlock *__restrict__ blocks,
                                                  const BlockTasks *__restrict__ bt, int nb, long total,
-                                                 const BcStateDev *__restrict__ states, Gas gas) {
+                                                 const BcStateDev *__restrict__ states, Gas gas,
+                                                 ViscParams vp) {
   const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tg >= total) return;
   const int wb = FindTaskBlock(bt, nb, tg);
@@ -604,6 +608,16 @@ __global__ void __launch_bounds__(256) KViscWall(const DevBlock *__restrict__ bl
   if (s && s->iso) {
     const GasT<double> g = Cast<double>(gas);
     const double tGhost = 2.0 * s->tw - TempOf(g, in[0], in[4]);
+    const double R = 0.0 + 1.0 * g.R;
+    gh[0] = (gh[4] / (R * tGhost)) * (in[0] / (0.0 + in[0]));
+  } else if (s && s->flux) {
+    // constant heat flux, low-Re treatment (src/ghostStates.cpp:228-238):
+    // the interior temperature less q / k times twice the wall distance of
+    // the wall-adjacent cell, k = Sutherland conductivity times the scaling
+    const GasT<double> g = Cast<double>(gas);
+    const double t = TempOf(g, in[0], in[4]);
+    const double kappa = SutherlandK(vp, t) * vp.vscale;
+    const double tGhost = TempOf(g, in[0], in[4]) - s->qw / kappa * 2.0 * w.wd;
     const double R = 0.0 + 1.0 * g.R;
     gh[0] = (gh[4] / (R * tGhost)) * (in[0] / (0.0 + in[0]));
   }

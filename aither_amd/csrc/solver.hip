@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -171,6 +172,31 @@ void HostMap3(int dir, int d3, int d1, int d2, int *i, int *j, int *k) {
   else { *i = d1; *j = d2; *k = d3; }
 }
 
+// procBlock::CalcWallDistance through kdtree::NearestNeighbor
+// (src/procBlock.cpp:6030-6041, src/kdtree.cpp:123-224): the distance from a
+// point to the nearest centre of a viscousWall face of any block
+// (GetViscousFaceCenters, src/utility.cpp:310-366); the minimum of the
+// squared distances is the same whatever the search order
+double WallDistance(const ath::HostCase &hc, const ath::Vec3 &pt) {
+  double best = std::numeric_limits<double>::max();
+  for (const auto &h : hc.blocks)
+    for (const auto &sf : h.surf) {
+      if (sf.type != "viscousWall") continue;
+      const int st = sf.SurfType(), dir = (st - 1) / 2;
+      const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
+        for (int q1 = sf.d[2 * d1]; q1 < sf.d[2 * d1 + 1]; ++q1) {
+          int i, j, k;
+          HostMap3(dir, sf.d[2 * dir], q1, q2, &i, &j, &k);
+          const ath::Vec3 &f = h.fc[dir][h.Face(dir, i, j, k)];
+          const double dx = pt.x - f.x, dy = pt.y - f.y, dz = pt.z - f.z;
+          const double d2s = ((0.0 + dx * dx) + dy * dy) + dz * dz;
+          if (d2s < best) best = d2s;
+        }
+    }
+  return std::sqrt(best);
+}
+
 // padded geometry, projected centre distances, wall flags and the ghost-cell
 // task lists of a viscous case
 int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
@@ -319,12 +345,17 @@ int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
         int state = -1;
         for (size_t q = 0; q < c->hc.bcs.size(); ++q)
           if (c->hc.bcs[q].tag == sf.d[6]) { state = (int)q; break; }
+        const bool flux = state >= 0 && c->hc.bcs[state].heatFlux;
+        // the cell next to the wall, whatever the layer
+        const int aC = st % 2 == 0 ? r3 - 1 : r3;
         for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
           for (int q1 = sf.d[2 * d1]; q1 < sf.d[2 * d1 + 1]; ++q1) {
-            int ii, ij, ik, gi, gj, gk;
+            int ii, ij, ik, gi, gj, gk, ai, aj, ak;
             HostMap3(dir, iC, q1, q2, &ii, &ij, &ik);
             HostMap3(dir, gC, q1, q2, &gi, &gj, &gk);
-            wt.push_back({h.Cell(ii, ij, ik), h.Cell(gi, gj, gk), state, 0});
+            HostMap3(dir, aC, q1, q2, &ai, &aj, &ak);
+            const double wd = flux ? WallDistance(c->hc, h.ctr[h.Cell(ai, aj, ak)]) : 0.0;
+            wt.push_back({h.Cell(ii, ij, ik), h.Cell(gi, gj, gk), state, 0, wd});
           }
       }
     br.nwall = (int)wt.size();
@@ -520,7 +551,7 @@ int Upload(ath_ctx *c) {
   std::vector<BcStateDev> st;
   for (const auto &s : hc.bcs)
     st.push_back({s.p, s.rho, s.vel[0], s.vel[1], s.vel[2], s.p0, s.t0, s.dir[0], s.dir[1], s.dir[2], s.twall,
-                  s.isothermal ? 1 : 0});
+                  s.isothermal ? 1 : 0, s.heatFlux ? 1 : 0, s.qwall});
   if (int r = c->Alloc(&c->dstates, st.size())) return r;
   if (!st.empty())
     HIP_OK(hipMemcpy(c->dstates, st.data(), st.size() * sizeof(BcStateDev), hipMemcpyHostToDevice));
@@ -834,7 +865,7 @@ void LaunchResidualT(ath_ctx *c) {
   const int nb = (int)c->blk.size();
   if (c->taskTotal[kTaskWall] > 0)
     hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(c->taskTotal[kTaskWall])), dim3(kBlk), 0, c->stream, c->dblocks,
-                       c->Tasks(kTaskWall), nb, c->taskTotal[kTaskWall], c->dstates, P.gas);
+                       c->Tasks(kTaskWall), nb, c->taskTotal[kTaskWall], c->dstates, P.gas, P.visc);
   if (c->taskTotal[kTaskEdgeVisc] > 0)
     hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(c->taskTotal[kTaskEdgeVisc], 64)), dim3(64), 0, c->stream,
                        c->dblocks, c->Tasks(kTaskEdgeVisc), nb, c->taskTotal[kTaskEdgeVisc], P.gas);

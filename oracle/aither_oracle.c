@@ -14,6 +14,7 @@
 #include "aither_oracle.h"
 
 #include <ctype.h>
+#include <float.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -74,6 +75,8 @@ typedef struct {
   int has_mf;
   int isothermal; /* viscousWall: wall temperature given */
   double twall;
+  int heatflux;   /* viscousWall: a nonzero heat flux given (include/inputStates.hpp:379-381) */
+  double qwall;
   int startTag, endTag; /* periodic: the two surfaces' tags and the start */
   v3 trans;             /* surface's translation onto the end surface */
 } bcstate_t;
@@ -1822,16 +1825,17 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
         if (get_param(ents[q], "direction", v, sizeof v)) s->dir = parse_vec(v);
         if (get_param(ents[q], "nonreflecting", v, sizeof v)) s->nonreflecting = !strcmp(v, "true");
         if (get_param(ents[q], "temperature", v, sizeof v)) { s->isothermal = 1; s->twall = strtod(v, NULL); }
+        if (get_param(ents[q], "heatFlux", v, sizeof v)) {
+          if (s->isothermal) { seterr(err, errlen, "viscousWall: give a temperature or a heat flux, not both"); goto fail; }
+          s->qwall = strtod(v, NULL);
+          s->heatflux = s->qwall != 0.0;
+        }
         if (get_param(ents[q], "startTag", v, sizeof v)) s->startTag = atoi(v);
         if (get_param(ents[q], "endTag", v, sizeof v)) s->endTag = atoi(v);
         if (get_param(ents[q], "translation", v, sizeof v)) s->trans = parse_vec(v);
         if (!strcmp(s->kind, "periodic") && (get_param(ents[q], "axis", v, sizeof v) ||
                                              get_param(ents[q], "rotation", v, sizeof v))) {
           seterr(err, errlen, "rotational periodicity is not supported");
-          goto fail;
-        }
-        if (get_param(ents[q], "heatFlux", v, sizeof v) && strtod(v, NULL) != 0.0) {
-          seterr(err, errlen, "viscousWall heat flux is not supported");
           goto fail;
         }
         if (get_param(ents[q], "wallTreatment", v, sizeof v) && strcmp(v, "lowRe")) {
@@ -1914,6 +1918,7 @@ orc_case *orc_load(const char *inp_path, char *err, int errlen) {
     if (!strcmp(s->kind, "viscousWall")) { /* src/inputStates.cpp:775-783 */
       s->vel = vdiv(s->vel, c->aRef);
       s->twall /= c->tRef;
+      s->qwall /= pow(c->aRef / c->lRef, 3.0);
       continue;
     }
     if (!strcmp(s->kind, "stagnationInlet")) {
@@ -2418,7 +2423,8 @@ static void edge_ghosts(orc_case *c, blk_t *b, int viscous) {
  * (src/ghostStates.cpp:126-145, 185-190, adiabatic default): the velocity
  * mirrors about the wall velocity; an isothermal wall sets the density from
  * the mirrored temperature at the interior pressure */
-static void viscous_wall_ghost(const orc_case *c, const double *in, const bcstate_t *s, double *ghost) {
+static double cond_of_t(const orc_case *c, double t);
+static void viscous_wall_ghost(const orc_case *c, const double *in, const bcstate_t *s, double wd, double *ghost) {
   memcpy(ghost, in, sizeof(double) * NEQ);
   v3 vw = s ? s->vel : V(0, 0, 0);
   ghost[1] = 2.0 * vw.x - in[1];
@@ -2429,7 +2435,46 @@ static void viscous_wall_ghost(const orc_case *c, const double *in, const bcstat
     double R = 0.0 + 1.0 * c->gas.R;
     double rho = ghost[4] / (R * tGhost);
     ghost[0] = rho * (in[0] / (0.0 + in[0]));
+  } else if (s && s->heatflux) {
+    /* constant heat flux, low-Re (src/ghostStates.cpp:228-238): the
+     * interior temperature less q / k times twice the wall distance of the
+     * wall-adjacent cell; k = EffectiveConductivity = Sutherland k times
+     * the nondimensional scaling (src/transport.cpp:192-196) */
+    double t = temperature(&c->gas, in);
+    double kappa = cond_of_t(c, t) * c->vscale;
+    double tGhost = temperature(&c->gas, in) - s->qwall / kappa * 2.0 * wd;
+    double R = 0.0 + 1.0 * c->gas.R;
+    double rho = ghost[4] / (R * tGhost);
+    ghost[0] = rho * (in[0] / (0.0 + in[0]));
   }
+}
+
+/* procBlock::CalcWallDistance via kdtree::NearestNeighbor
+ * (src/procBlock.cpp:6030-6041, src/kdtree.cpp:123-224): the distance from
+ * a point to the nearest viscousWall face centre of any block
+ * (GetViscousFaceCenters, src/utility.cpp:310-366); brute force gives the
+ * same minimum of the squared distances */
+static double wall_distance(const orc_case *c, v3 pt) {
+  double best = DBL_MAX;
+  for (int bq = 0; bq < c->nb; ++bq) {
+    const blk_t *b = &c->blk[bq];
+    for (int s = 0; s < b->nsurf; ++s) {
+      const surf_t *sf = &b->surf[s];
+      if (strcmp(sf->type, "viscousWall")) continue;
+      int st = surf_type(sf), dir = (st - 1) / 2;
+      int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      for (int q2 = sf->d[2 * d2]; q2 < sf->d[2 * d2 + 1]; ++q2)
+        for (int q1 = sf->d[2 * d1]; q1 < sf->d[2 * d1 + 1]; ++q1) {
+          int i, j, k;
+          map3(dir, sf->d[2 * dir], q1, q2, &i, &j, &k);
+          v3 f = b->fc[dir][fidx(b, dir, i, j, k)];
+          double dx = pt.x - f.x, dy = pt.y - f.y, dz = pt.z - f.z;
+          double d2s = ((0.0 + dx * dx) + dy * dy) + dz * dz;
+          if (d2s < best) best = d2s;
+        }
+    }
+  }
+  return sqrt(best);
 }
 
 /* procBlock::AssignViscousGhostCells (src/procBlock.cpp:2760-2836) */
@@ -2452,12 +2497,16 @@ static void viscous_ghosts(orc_case *c, blk_t *b) {
         if (iC >= n3) iC = n3 - 1;
       }
       const bcstate_t *bs = find_bc(c, sf->d[6]);
+      /* the wall distance is the wall-adjacent cell's, whatever the layer */
+      int aC = st % 2 == 0 ? r3 - 1 : r3;
       for (int q2 = sf->d[2 * d2]; q2 < sf->d[2 * d2 + 1]; ++q2)
         for (int q1 = sf->d[2 * d1]; q1 < sf->d[2 * d1 + 1]; ++q1) {
-          int ii, ij, ik, gi, gj, gk;
+          int ii, ij, ik, gi, gj, gk, ai, aj, ak;
           map3(dir, iC, q1, q2, &ii, &ij, &ik);
           map3(dir, gC, q1, q2, &gi, &gj, &gk);
-          viscous_wall_ghost(c, &b->W[cidx(b, ii, ij, ik) * NEQ], bs, &b->W[cidx(b, gi, gj, gk) * NEQ]);
+          map3(dir, aC, q1, q2, &ai, &aj, &ak);
+          double wd = (bs && bs->heatflux) ? wall_distance(c, b->ctr[cidx(b, ai, aj, ak)]) : 0.0;
+          viscous_wall_ghost(c, &b->W[cidx(b, ii, ij, ik) * NEQ], bs, wd, &b->W[cidx(b, gi, gj, gk) * NEQ]);
         }
     }
   edge_ghosts(c, b, 1);

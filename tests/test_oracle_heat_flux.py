@@ -1,0 +1,56 @@
+"""viscousWall with a prescribed heat flux (low-Re treatment,
+src/ghostStates.cpp:192-238) in the oracle.  No reference test case uses a
+laminar heat-flux wall (its one case, wallLaw, needs turbulence), so the
+checks are structural: a zero heat flux is the adiabatic wall bit for bit,
+a nonzero one changes only the cells next to that wall and their gradient
+stencil, the change grows with the flux, and a wall given both a temperature
+and a heat flux is refused."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from aither_amd import cases
+
+
+def _case(tmp_path, q):
+    p, _ = cases.bump_case(str(tmp_path), seed=4, n=8, iterations=3, viscous=True)
+    text = open(p).read()
+    assert "viscousWall(tag=3)" in text
+    if q is not None:
+        text = text.replace("viscousWall(tag=3)", f"viscousWall(tag=3; heatFlux={q})")
+    path = os.path.join(str(tmp_path), f"hf_{q}.inp")
+    open(path, "w").write(text)
+    return path
+
+
+def _residual(path):
+    o = pyoracle.OracleCase(path)
+    o.begin_step(0)
+    o.bc()
+    o.calc_residual()
+    return o.residual().reshape(8, 8, 8, 5)  # k, j, i
+
+
+def test_heat_flux_wall(tmp_path):
+    r0 = _residual(_case(tmp_path, None))
+    assert np.array_equal(_residual(_case(tmp_path, 0.0)), r0)
+    d1 = _residual(_case(tmp_path, 2000.0)) - r0
+    d2 = _residual(_case(tmp_path, 4000.0)) - r0
+    # the tagged wall is the lower j surface: only the first two cell layers
+    # (the wall face flux and the gradient stencils next to it) change
+    assert np.abs(d1[:, 2:, :, :]).max() == 0.0
+    assert np.abs(d1[:, 0, :, 4]).min() > 0.0
+    # a larger flux moves the wall cells' energy residual further, the same way
+    e1, e2 = d1[:, 0, :, 4], d2[:, 0, :, 4]
+    assert np.all(np.sign(e1) == np.sign(e2)) and np.all(np.abs(e2) > np.abs(e1))
+
+
+def test_heat_flux_with_temperature_is_refused(tmp_path):
+    p, _ = cases.bump_case(str(tmp_path), seed=4, n=6, iterations=1, viscous=True)
+    text = open(p).read().replace("viscousWall(tag=3)", "viscousWall(tag=3; temperature=300.0; heatFlux=10.0)")
+    path = os.path.join(str(tmp_path), "both.inp")
+    open(path, "w").write(text)
+    with pytest.raises(Exception):
+        pyoracle.OracleCase(path)
