@@ -36,11 +36,11 @@ struct DevBlock {
   // formed where they are used (NeighbourBlock).
   double *Ab, *Abi;
   int tni, tnj, tnk;  // tile grid of the layout
-  // block solvers with the viscous terms: what the viscous pass computed at
-  // every physical face of direction d (velocity gradient 9, face state 5,
-  // face viscosity 1, face conductivity before scaling 1), 16 * faces(d),
-  // entry q of face f at q * faces(d) + f; the thin-shear-layer part of the
-  // block diagonal reads it back
+  // block solvers with the viscous terms: the velocity gradient the viscous
+  // pass formed at every physical face of direction d, 9 * faces(d), entry q
+  // of face f at q * faces(d) + f; the thin-shear-layer part of the block
+  // diagonal reads it back (and forms the face state, viscosity and
+  // conductivity again from the cells, with the same functions)
   double *fvis[3];
   // off-diagonal product of the ghost neighbour across each boundary face
   // that lies on an interblock connection (zero on other faces), per sweep
@@ -778,10 +778,6 @@ __device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P,
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) o[(r * 3 + c) * nf + fq] = vg[r][c];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) o[(9 + e) * nf + fq] = Wf[e];
-    o[14 * nf + fq] = muF;
-    o[15 * nf + fq] = kLam;
   }
   const double tv = tau[0] * Wf[1] + tau[1] * Wf[2] + tau[2] * Wf[3];
   const double q = (kk + kt) * (tg[0] * n[0] + tg[1] * n[1] + tg[2] * n[2]);
@@ -1046,14 +1042,14 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P, int *fla
         const int fi = i + (side && d == 0), fj = j + (side && d == 1), fk = k + (side && d == 2);
         const long fq = FaceIdx(b, d, fi, fj, fk);
         const long nf = (long)(b.ni + (d == 0)) * (b.nj + (d == 1)) * (b.nk + (d == 2));
-        // the face's gradient, state and viscosity as the viscous pass formed them
+        // the face's gradient as the viscous pass formed it; its state,
+        // viscosity and conductivity formed again, as that pass forms them
         const double *fv = b.fvis[d];
-        double vgr[9], Wf[5], J[25];
+        double vgr[9], Wf[5], J[25], muF;
 #pragma unroll
         for (int q = 0; q < 9; ++q) vgr[q] = fv[q * nf + fq];
-#pragma unroll
-        for (int e = 0; e < 5; ++e) Wf[e] = fv[(9 + e) * nf + fq];
-        const double muF = fv[14 * nf + fq], kF = fv[15 * nf + fq];
+        ViscFaceState<MODE == 3>(b, d, fi, fj, fk, Wf, muF);
+        const double kF = SutherlandK(P.visc, TempOf(g, Wf[0], Wf[4]));
         TSLJacobian(g, P.visc, Wf, muF, kF, b.fa[d][fq], b.c2c[d][fq], side == 1, vgr, J);
 #pragma unroll
         for (int q = 0; q < 25; ++q) D[q] = side ? D[q] - J[q] : D[q] + J[q];

@@ -647,7 +647,7 @@ int Upload(ath_ctx *c) {
       for (int dd = 0; dd < 3; ++dd) {
         const long nf = (long)(h.ni + (dd == 0)) * (h.nj + (dd == 1)) * (h.nk + (dd == 2));
         if (hc.viscous)
-          if (int r = c->Alloc(&d.fvis[dd], 16 * nf)) return r;
+          if (int r = c->Alloc(&d.fvis[dd], 9 * nf)) return r;
       }
       if (hc.viscous) {
         if (int r = c->Alloc(&d.vgrad, 9 * np)) return r;
