@@ -65,8 +65,8 @@ CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
-CHANNEL_SWEEP_TRAFFIC = 0.5 * (1337.4e6 + 1119.6e6)   # one pass, forward and backward averaged
-CHANNEL_FLUX_TRAFFIC = 3680.0e6                       # residual stage over all blocks
+CHANNEL_SWEEP_TRAFFIC = 0.5 * (1336.7e6 + 1118.7e6)   # one pass, forward and backward averaged
+CHANNEL_FLUX_TRAFFIC = 3372.3e6                       # residual stage over all blocks
 
 
 def parse():
