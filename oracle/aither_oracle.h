@@ -11,20 +11,32 @@
  * -ffp-contract=off so that each floating-point operation is evaluated in the
  * order the reference writes it.
  *
- * Scope: single-species calorically perfect ideal gas, Euler equations,
- * multi-block input without interblock connections.  Reconstruction: constant,
- * MUSCL (kappa family) with none / vanAlbada / minmod limiters.  Inviscid flux:
- * Roe, AUSMPW+.  Time integration: explicitEuler, implicitEuler,
- * crankNicholson, bdf2 (dual time).  Implicit operators: scalar LU-SGS and
- * scalar DP-LUR (Rusanov off-diagonals).  Boundary conditions: slipWall
- * (viscousWall in the inviscid ghost pass), characteristic, supersonicInflow,
+ * Scope: single-species calorically perfect ideal gas; Euler and laminar
+ * Navier-Stokes equations (Sutherland transport, central or fourth-order
+ * viscous face states); any number of blocks joined by point-matched
+ * interblock and translation-periodic connections.  Reconstruction:
+ * constant, MUSCL (kappa family) with none / vanAlbada / minmod limiters,
+ * WENO5 and WENO-Z.  Inviscid flux: Roe, AUSMPW+.  Time integration:
+ * explicitEuler, implicitEuler, crankNicholson, bdf2 (dual time).  Implicit
+ * operators: scalar and 5x5 block LU-SGS and DP-LUR; Rusanov off-diagonals,
+ * and Roe flux-change off-diagonals (approximateRoe, Euler, scalar solvers).
+ * Boundary conditions: slipWall, viscousWall (low-Re: adiabatic,
+ * isothermal, prescribed heat flux), characteristic, supersonicInflow,
  * supersonicOutflow, stagnationInlet, pressureOutlet (reflecting).
  *
  * Parity pins (see tests/test_oracle_*.py):
  *   - reference-held regression known answers (testCases/regressionTests.py)
- *     for subsonicCylinder and supersonicWedge, run on the reference's own
- *     grids and input files copied as fixtures under tests/golden/;
- *   - closed forms and invariants for every kernel-level function.
+ *     run on the reference's own grids and input files copied as fixtures
+ *     under tests/golden/;
+ *   - closed forms and invariants for the kernel-level functions.
+ * Parity unpinned (no reference vector; checked only as stated):
+ *   - crankNicholson time integration (agrees with the GPU);
+ *   - the temperature columns of the thin-shear-layer Jacobian (form only);
+ *   - approximateRoe off-diagonals (linearisation and du = 0 terms pinned by
+ *     mathematics; no reference case uses them);
+ *   - viscousWall with a prescribed heat flux (zero flux equals the adiabatic
+ *     wall, the change stays next to the wall and grows with the flux; the
+ *     sign convention and the heat-flux scaling are not pinned).
  */
 #ifndef AITHER_ORACLE_H
 #define AITHER_ORACLE_H
