@@ -54,3 +54,25 @@ def test_heat_flux_with_temperature_is_refused(tmp_path):
     open(path, "w").write(text)
     with pytest.raises(Exception):
         pyoracle.OracleCase(path)
+
+
+def test_heat_flux_sign_and_scale(tmp_path):
+    """On the wall face the ghost temperature makes dT/dn = q / k, so the
+    wall cells' energy residual (flux times area, + for a cell's lower face)
+    grows by q_nd |A| k_face / k_cell, with q_nd = q / (a_ref / L_ref)^3 and
+    k_face / k_cell within a few per cent of one: a wrong sign or scaling of
+    the heat flux fails this."""
+    q = 2000.0
+    path = _case(tmp_path, q)
+    r0 = _residual(_case(tmp_path, None))
+    dE = (_residual(path) - r0)[:, 0, :, 4]  # k, i of the wall cells
+    o = pyoracle.OracleCase(path)
+    gh = o.L.orc_num_ghosts(o.h)
+    n = 8
+    NI = n + 2 * gh
+    fa = o.face_area(1).reshape(NI, NI + 1, NI, 4)
+    area = fa[gh:gh + n, gh, gh:gh + n, 3]  # the wall faces (j = 0), k, i
+    a_ref = np.sqrt(1.4 * 8.3144598 / 0.02897 * 288.15)  # air, T_ref 288.15 K, L_ref 1
+    q_nd = q / a_ref ** 3
+    ratio = dE / (q_nd * area)
+    assert np.all(np.abs(ratio - 1.0) < 0.1), (ratio.min(), ratio.max())
