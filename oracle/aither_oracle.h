@@ -34,9 +34,10 @@
  *   - the temperature columns of the thin-shear-layer Jacobian (form only);
  *   - approximateRoe off-diagonals (linearisation and du = 0 terms pinned by
  *     mathematics; no reference case uses them);
- *   - viscousWall with a prescribed heat flux (zero flux equals the adiabatic
- *     wall, the change stays next to the wall and grows with the flux; the
- *     sign convention and the heat-flux scaling are not pinned).
+ *   - viscousWall with a prescribed heat flux against the reference itself
+ *     (its one heat-flux case needs turbulence); pinned by zero flux equal to
+ *     the adiabatic wall, locality, and the wall cells' energy residual
+ *     changing by q |A| (sign and (a_ref/L_ref)^3 scaling).
  */
 #ifndef AITHER_ORACLE_H
 #define AITHER_ORACLE_H
