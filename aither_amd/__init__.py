@@ -90,6 +90,7 @@ _SIGS = {
     "ath_set_implicit_precision": [_P, ctypes.c_int],
     "ath_set_matrix_residual": [_P, ctypes.c_int],
     "ath_set_sweep_ordering": [_P, ctypes.c_int],
+    "ath_get_sweep_ordering": [_P, ctypes.POINTER(ctypes.c_int)],
     "ath_get_state": [_P, ctypes.c_int, _D],
     "ath_set_state": [_P, ctypes.c_int, _D],
     "ath_begin_step": [_P, ctypes.c_int],
@@ -222,6 +223,13 @@ class Solver:
 
     def set_sweep_ordering(self, ordering):
         _check(self._lib.ath_set_sweep_ordering(self._h, ordering))
+
+    def sweep_ordering(self):
+        """the order the sweeps actually run in (block solvers and the
+        approximateRoe Jacobian map some requests to another order)"""
+        v = ctypes.c_int()
+        _check(self._lib.ath_get_sweep_ordering(self._h, ctypes.byref(v)))
+        return v.value
 
     def set_matrix_residual(self, on):
         _check(self._lib.ath_set_matrix_residual(self._h, 1 if on else 0))

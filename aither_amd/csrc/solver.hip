@@ -909,6 +909,15 @@ void LaunchResidual(ath_ctx *c) {
 #undef ATH_RES
 }
 
+// the order the sweeps run in: the block solvers have the tiled and
+// hyperplane orders (the pencil order maps to the tiled one); the Roe
+// flux-change off-diagonals (approximateRoe) run in the hyperplane order only
+int EffectiveOrder(const ath_ctx *c) {
+  if (c->P.roeJac) return ATH_SWEEP_HYPERPLANE;
+  if (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL) return ATH_SWEEP_TILED;
+  return c->sweepOrder;
+}
+
 // One relax.  When `timed`, an event pair brackets every sweep pass (a
 // forward or backward LU-SGS pass, or one DP-LUR sweep, over all blocks)
 // from c->evs[evBase]; returns the number of sweep kernel launches.
@@ -924,11 +933,7 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
                      c->Map(kMapCell256), c->P);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
-    // the block solvers have the tiled and hyperplane orders; the pencil
-    // order maps to the tiled one.  The Roe flux-change off-diagonals
-    // (approximateRoe) run in the hyperplane order only.
-    const int order = c->P.roeJac ? ATH_SWEEP_HYPERPLANE
-                      : (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL ? ATH_SWEEP_TILED : c->sweepOrder);
+    const int order = EffectiveOrder(c);
     if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_PENCIL) {
       if (sw == 0)
         for (auto &br : c->blk)
@@ -1112,6 +1117,12 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
   }
   for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (c->l2first[e] + eps);
   if (out) *out = n;
+  // a device error is reported once: the flag is cleared so that the
+  // context can go on (for example after ath_set_state restores a state)
+  if (flag) {
+    const int zero = 0;
+    HIP_OK(hipMemcpy(c->dflag, &zero, sizeof zero, hipMemcpyHostToDevice));
+  }
   if (flag & 2) return Fail(ATH_ERR_DEVICE, "a sweep dependency wait exceeded its spin limit");
   if (flag & 4) return Fail(ATH_ERR_DEVICE, "singular block main diagonal in Gauss-Jordan elimination");
   if (flag & 1) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
@@ -1206,6 +1217,12 @@ int ath_set_sweep_ordering(ath_ctx *c, int ordering) {
              ordering != ATH_SWEEP_PENCIL))
     return Fail(ATH_ERR_ARG, "bad sweep ordering");
   c->sweepOrder = ordering;
+  return ATH_OK;
+}
+
+int ath_get_sweep_ordering(const ath_ctx *c, int *ordering) {
+  if (!c || !ordering) return Fail(ATH_ERR_ARG, "null argument");
+  *ordering = EffectiveOrder(c);
   return ATH_OK;
 }
 
@@ -1349,8 +1366,9 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   HIP_OK(hipSetDevice(c->device));
   const int nl = c->hc.nonlinear;
   const long iters = (long)nsteps * nl;
-  if (iters == 0) return ATH_OK;
+  // nsteps == 0 only captures and instantiates the iteration's graph
   if (int r = BuildGraph(c)) return r;
+  if (iters == 0) return ATH_OK;
   const int nflux = 1;  // one residual stage over all blocks
   // timed runs launch the tail eagerly with an event pair around every sweep
   // pass; untimed runs replay the captured graph

@@ -93,7 +93,10 @@ def parse():
 
 def cpu_baseline(inp, iters, what):
     """The oracle (single-threaded FP64 C restatement) on the first `iters`
-    nonlinear iterations of the same case."""
+    nonlinear iterations of the same case.  Returns the baseline record and
+    the oracle's residual after each iteration and final state, which the
+    parity check compares with the GPU."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     c = pyoracle.OracleCase(inp)
@@ -101,18 +104,83 @@ def cpu_baseline(inp, iters, what):
     t0 = time.perf_counter()
     done = 0
     nn = 0
+    resid = []
+    tres = 0.0
     while done < iters:
         c.begin_step(nn)
         for mm in range(c.nonlinear):
             if done == iters:
                 break
             c.iterate(nn, mm)
+            t1 = time.perf_counter()
+            resid.append(np.concatenate([c.residual(b).reshape(-1) for b in range(c.nb)]))
+            tres += time.perf_counter() - t1
             done += 1
         nn += 1
-    dt = time.perf_counter() - t0
-    return {"value": ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"first {iters} nonlinear iterations of the same {what} ({ncell} cells), "
-                      f"oracle C restatement, 1 thread ({dt:.1f} s)"}
+    dt = time.perf_counter() - t0 - tres
+    state = np.concatenate([c.state(b).reshape(-1) for b in range(c.nb)])
+    rec = {"value": ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+           "sample": f"first {iters} nonlinear iterations of the same {what} ({ncell} cells), "
+                     f"oracle C restatement, 1 thread ({dt:.1f} s)"}
+    return rec, resid, state
+
+
+def gpu_parity(s, states0, resid_ref, state_ref):
+    """The GPU path over the same first iterations from the same initial
+    state as the oracle leg: per-iteration residual and final-state relative
+    L2 differences (the metric's bound is 1e-12 on the residual and 1e-10 on
+    the state)."""
+    import numpy as np
+    nb = len(states0)
+    for b, W in enumerate(states0):
+        s.set_state(W, b)
+    nl = s.info.nonlinear_iterations
+    rr = []
+    it = 0
+    nn = 0
+    while it < len(resid_ref):
+        s.begin_step(nn)
+        for mm in range(nl):
+            if it == len(resid_ref):
+                break
+            s.iterate(nn, mm)
+            R = np.concatenate([s.residual(b).reshape(-1) for b in range(nb)])
+            rr.append(float(np.linalg.norm(R - resid_ref[it]) / np.linalg.norm(resid_ref[it])))
+            it += 1
+        nn += 1
+    W = np.concatenate([s.get_state(b).reshape(-1) for b in range(nb)])
+    sd = float(np.linalg.norm(W - state_ref) / np.linalg.norm(state_ref))
+    ok = max(rr) <= 1e-12 and sd <= 1e-10
+    return {"iterations": len(rr), "residual_rel_l2": rr, "state_rel_l2": sd,
+            "bound": "residual <= 1e-12, state <= 1e-10", "ok": ok}
+
+
+def warm_up(s, warm):
+    """`warm` untimed nonlinear iterations: whole physical steps replayed
+    from the captured graph, the rest launched eagerly; the graph is captured
+    first, so no capture or instantiation falls in the timed region."""
+    s.run(0, 0)
+    sub = s.info.nonlinear_iterations
+    nphys, rest = divmod(warm, sub)
+    if nphys:
+        s.run(0, nphys)
+    if rest:
+        s.begin_step(nphys)
+        for mm in range(rest):
+            s.iterate(nphys, mm)
+
+
+def run_iterations(s, steps):
+    """exactly `steps` nonlinear iterations from outer step 0: whole physical
+    steps from the graph, a partial last step eagerly"""
+    sub = s.info.nonlinear_iterations
+    nphys, rest = divmod(steps, sub)
+    last = s.run(0, nphys) if nphys else None
+    if rest:
+        s.begin_step(nphys)
+        for mm in range(rest):
+            last = s.iterate(nphys, mm)
+    return last
 
 
 def hbm_calibration():
@@ -140,25 +208,25 @@ def roofline(name, nbytes, avg_s, share, traffic=None, **extra):
     return out
 
 
-def timed_run(s, states0, nphys, warm_phys, prof_phys=None):
-    """warmup (graph capture, caches), restart from the initial condition,
-    then `nphys` physical steps replaying the captured graph, timed by the
-    wall clock with the device synchronised on both sides (ath_run returns
+def timed_run(s, states0, steps, warm, prof_phys):
+    """`warm` untimed iterations (graph capture, caches), restart from the
+    initial condition, then exactly `steps` nonlinear iterations (whole
+    physical steps replaying the captured graph), timed by the wall clock with
+    the device synchronised on both sides (ath_run and ath_iterate return
     after a stream synchronisation); then, from the initial condition again,
     `prof_phys` physical steps launched eagerly with a HIP event pair around
     every residual stage and sweep pass (the per-kernel breakdown)"""
-    if warm_phys > 0:
-        s.run(0, warm_phys)
+    warm_up(s, warm)
     for b, W in enumerate(states0):
         s.set_state(W, b)
     s.enable_timing(False)
     t0 = time.perf_counter()
-    last = s.run(0, nphys)
+    last = run_iterations(s, steps)
     wall = time.perf_counter() - t0
     for b, W in enumerate(states0):
         s.set_state(W, b)
     s.enable_timing(True)
-    s.run(0, prof_phys or nphys)
+    s.run(0, prof_phys)
     tm = s.timing()
     s.enable_timing(False)
     return last, wall, tm
@@ -168,15 +236,14 @@ def bench_channel(args, Solver, cases):
     steps = args.steps if args.steps is not None else 1000
     warm = args.warmup if args.warmup is not None else 20
     work = tempfile.mkdtemp(prefix="aither_bench_")
-    inp, meta = cases.channel_case(work, refine=args.refine, steps=max(1, steps // 20), subits=20)
+    inp, meta = cases.channel_case(work, refine=args.refine, steps=max(1, -(-steps // 20)), subits=20)
     sub = meta["subits"]
-    nphys = max(1, steps // sub)
-    steps = nphys * sub
+    nphys = -(-steps // sub)
     s = Solver(inp)
     nb = s.info.num_blocks
     states0 = [s.get_state(b) for b in range(nb)]
     ncell = sum(s.ncells(b) for b in range(nb))
-    last, wall, tm = timed_run(s, states0, nphys, max(0, warm // sub), prof_phys=min(nphys, 2))
+    last, wall, tm = timed_run(s, states0, steps, warm, prof_phys=min(nphys, 2))
     value = ncell * steps / wall
     # one residual stage (every kernel launched once over all blocks) and two
     # passes (a forward and a backward sweep over all blocks) per iteration
@@ -200,7 +267,7 @@ def bench_channel(args, Solver, cases):
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": (max(0, warm // sub)) * sub,
+        "warmup": warm,
         "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -219,9 +286,10 @@ def bench_channel(args, Solver, cases):
         "roofline_other": other,
         "final_normalized_residual": list(last.normalized[:5]),
     }
-    s.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters or 2, "multiblock channel")
+        out["cpu_baseline"], resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 2, "multiblock channel")
+        out["parity"] = gpu_parity(s, states0, resid_ref, state_ref)
+    s.close()
     return out
 
 
@@ -239,6 +307,10 @@ def bench_vortex(args, Solver, cases):
     W0 = s.get_state()
     ncell = s.ncells(0)
     last, wall, tm = timed_run(s, [W0], steps, warm, prof_phys=min(steps, 50))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 12, f"{args.n}^3 vortex")
+        parity = gpu_parity(s, [W0], resid_ref, state_ref)
     value = ncell * steps / wall
     flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3
     pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
@@ -297,8 +369,9 @@ def bench_vortex(args, Solver, cases):
             "value": ncell * steps / w32,
             "state_rel_l2_vs_fp64": float(np.linalg.norm(st - ref) / np.linalg.norm(ref)),
         }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(inp, args.cpu_iters or 12, f"{args.n}^3 vortex")
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["parity"] = parity
     return out
 
 

@@ -12,6 +12,9 @@
  *
  * Every call returns ATH_OK (0) or a negative ATH_ERR_* status; the message of
  * the last failure on the calling thread is available from ath_last_error().
+ * A device-side error (ATH_ERR_NONPHYSICAL, ATH_ERR_DEVICE) is reported by
+ * the call that reads the norms after it and then cleared: the context stays
+ * usable, with the state as the failing iteration left it.
  * The library never calls exit().  One context is driven by one host thread
  * at a time and owns one HIP stream.
  */
@@ -96,6 +99,10 @@ int ath_set_implicit_precision(ath_ctx *ctx, int precision);
 #define ATH_SWEEP_HYPERPLANE 1
 #define ATH_SWEEP_PENCIL 2
 int ath_set_sweep_ordering(ath_ctx *ctx, int ordering);
+/* the order the sweeps actually run in: block solvers (blusgs, bdplur) map
+ * ATH_SWEEP_PENCIL to ATH_SWEEP_TILED, and the approximateRoe flux Jacobian
+ * always runs ATH_SWEEP_HYPERPLANE, whatever was set */
+int ath_get_sweep_ordering(const ath_ctx *ctx, int *ordering);
 /* compute the matrix residual each iteration (default 1), as
  * linearSolver::Residual does (src/linearSolver.cpp:95-109) */
 int ath_set_matrix_residual(ath_ctx *ctx, int on);
@@ -117,7 +124,9 @@ int ath_iterate(ath_ctx *ctx, int nn, int mm, ath_norms *norms);
 
 /* The main loop of src/main.cpp:231-302 for outer steps [nn0, nn0+nsteps),
  * each with the input's nonlinear iterations, replayed from a captured HIP
- * graph.  Norms of the last iteration go to `last` (may be NULL). */
+ * graph.  Norms of the last iteration go to `last` (may be NULL).  With
+ * nsteps == 0 it only captures and instantiates the graph (no iteration
+ * runs), so a caller can keep that cost out of a timed loop. */
 int ath_run(ath_ctx *ctx, int nn0, int nsteps, ath_norms *last);
 
 /* ---- granular steps (gridLevel / procBlock mirror) ---------------------- */

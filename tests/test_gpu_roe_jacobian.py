@@ -50,7 +50,9 @@ def test_roe_offdiagonal_step_parity(tmp_path, solver, sweeps):
     s.close()
 
 
-def test_roe_offdiagonal_orderings_agree(tmp_path):
+def test_roe_offdiagonal_runs_in_hyperplane_order(tmp_path):
+    """approximateRoe runs in the hyperplane order whatever order is set;
+    the order actually used is read back from the library."""
     p = cases.bump_case(str(tmp_path), seed=11, n=10, iterations=3, cfl=5.0)[0]
     path = _roe(p)
     out = []
@@ -58,6 +60,7 @@ def test_roe_offdiagonal_orderings_agree(tmp_path):
         print("approximateRoe ordering", order, flush=True)
         s = Solver(path)
         s.set_sweep_ordering(getattr(Solver, order))
+        assert s.sweep_ordering() == Solver.HYPERPLANE
         for nn in range(3):
             s.begin_step(nn)
             s.iterate(nn, 0)
