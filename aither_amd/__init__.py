@@ -18,8 +18,8 @@ import os
 
 import numpy as np
 
-__all__ = ["Solver", "Info", "Norms", "Timing", "AitherError", "load_library", "eval_face_flux",
-           "inspect_info", "inspect_field", "LIB_PATH"]
+__all__ = ["Solver", "Server", "SlotInfo", "Info", "Norms", "Timing", "AitherError", "load_library",
+           "eval_face_flux", "inspect_info", "inspect_field", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libaither_amd.so")
 MAX_EQ = 8
@@ -69,6 +69,14 @@ class Norms(ctypes.Structure):
         }
 
 
+class SlotInfo(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int), ("error", ctypes.c_int), ("iterations", ctypes.c_int),
+        ("num_blocks", ctypes.c_int), ("cells", ctypes.c_double), ("device_bytes", ctypes.c_double),
+        ("drop", ctypes.c_double), ("last", Norms),
+    ]
+
+
 class Timing(ctypes.Structure):
     _fields_ = [
         ("run_ms", ctypes.c_double), ("flux_ms", ctypes.c_double),
@@ -113,6 +121,17 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_long)],
     "ath_eval_face_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                            ctypes.c_int, _D],
+    "ath_server_open": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)],
+    "ath_server_close": [_P],
+    "ath_server_memory": [_P, _D, _D],
+    "ath_admit": [_P, ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "ath_step_batch": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "ath_get_slot_info": [_P, ctypes.c_int, ctypes.POINTER(SlotInfo)],
+    "ath_slot_block_dims": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "ath_get_slot_state": [_P, ctypes.c_int, ctypes.c_int, _D],
+    "ath_set_slot_state": [_P, ctypes.c_int, ctypes.c_int, _D],
+    "ath_retire": [_P, ctypes.c_int],
 }
 
 
@@ -127,7 +146,7 @@ def load_library(path=LIB_PATH):
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = None if name == "ath_close" else ctypes.c_int
+        fn.restype = None if name in ("ath_close", "ath_server_close") else ctypes.c_int
     lib.ath_last_error.restype = ctypes.c_char_p
     lib.ath_last_error.argtypes = []
     _lib = lib
@@ -309,3 +328,81 @@ class Solver:
         out = np.empty(self.ncells(b) * 25, dtype=np.float64)
         _check(self._lib.ath_get_block_diagonal(self._h, b, int(inverse), _ptr(out)))
         return out.reshape(-1, 5, 5)
+
+
+class Server:
+    """Many steady cases resident on one GPU, advanced together (an
+    ``ath_server``): ``admit`` uploads a case if memory and a slot allow
+    (else returns None), ``step`` runs every running case for some
+    iterations in shared launches, ``info``/``state`` read a case and
+    ``retire`` frees it."""
+
+    RUNNING, CONVERGED, FAILED, LIMIT = 0, 1, 2, 3
+
+    def __init__(self, device=0, hbm_budget=0.0, max_slots=64, precision=0):
+        self._lib = load_library()
+        h = _P()
+        _check(self._lib.ath_server_open(device, float(hbm_budget), int(max_slots), int(precision),
+                                         ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ath_server_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def memory(self):
+        used, free = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ath_server_memory(self._h, ctypes.byref(used), ctypes.byref(free)))
+        return used.value, free.value
+
+    def admit(self, inp_path, residual_drop=1e-8, max_iterations=10000):
+        """slot id, or None when the case does not fit now (ATH_ERR_NOMEM)"""
+        slot = ctypes.c_int(-1)
+        rc = self._lib.ath_admit(self._h, os.fsencode(inp_path), float(residual_drop), int(max_iterations),
+                                 ctypes.byref(slot))
+        if rc == -4:
+            return None
+        _check(rc)
+        return slot.value
+
+    def step(self, iterations):
+        running = ctypes.c_int(0)
+        _check(self._lib.ath_step_batch(self._h, int(iterations), ctypes.byref(running)))
+        return running.value
+
+    def info(self, slot):
+        o = SlotInfo()
+        _check(self._lib.ath_get_slot_info(self._h, slot, ctypes.byref(o)))
+        return o
+
+    def dims(self, slot, block=0):
+        ni, nj, nk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.ath_slot_block_dims(self._h, slot, block, ctypes.byref(ni), ctypes.byref(nj),
+                                             ctypes.byref(nk)))
+        return ni.value, nj.value, nk.value
+
+    def state(self, slot, block=0):
+        ni, nj, nk = self.dims(slot, block)
+        out = np.empty(ni * nj * nk * 5, dtype=np.float64)
+        _check(self._lib.ath_get_slot_state(self._h, slot, block, _ptr(out)))
+        return out.reshape(-1, 5)
+
+    def set_state(self, slot, W, block=0):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1)
+        _check(self._lib.ath_set_slot_state(self._h, slot, block, _ptr(W)))
+
+    def retire(self, slot):
+        _check(self._lib.ath_retire(self._h, slot))

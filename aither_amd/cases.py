@@ -148,7 +148,7 @@ boundaryConditions: 1
 
 
 def bump_case(outdir, seed, n=None, name=None, iterations=100, cfl=5.0, recon="thirdOrder", viscous=False,
-              visc_recon="central"):
+              visc_recon="central", limiter="vanAlbada"):
     """3D channel with a sin^2 bump on the lower wall; parameters seeded."""
     rng = np.random.default_rng(seed)
     n = n or int(rng.integers(24, 97))
@@ -191,7 +191,7 @@ timeIntegration: implicitEuler
 cflStart: {cfl}
 cflMax: {cfl}
 faceReconstruction: {recon}
-limiter: vanAlbada
+limiter: {limiter}
 iterations: {iterations}
 referenceTemperature: {t_inf!r}
 referenceDensity: {rho_inf!r}

@@ -10,6 +10,8 @@
 
 namespace ath {
 
+struct BcStateDev;
+
 struct DevBlock {
   int ni, nj, nk, g;
   int NI, NJ;
@@ -56,8 +58,14 @@ struct DevBlock {
   const double *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
   const unsigned char *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
   double *vgrad;                // block solvers: padded cell velocity gradient, 9 * npad, d v_c / d x_r at (3 r + c) npad
-  long cellBase;         // running cell offset of this block (L-inf index)
-  int blockId;
+  long cellBase;         // cell offset of this block within its case (L-inf index)
+  int blockId;           // block index within its case
+  // per case: the CFL number of the current iteration, the device error bits
+  // (1 nonphysical state, 4 singular block diagonal) and the boundary states
+  // by tag
+  const double *cfl;
+  int *flag;
+  const BcStateDev *states;
 };
 
 // laminar viscous terms: Sutherland transport (src/transport.cpp:29-176),
@@ -301,7 +309,7 @@ __device__ __forceinline__ int FindTaskBlock(const BlockTasks *bt, int nb, long 
 
 __global__ void __launch_bounds__(256) KBoundary(const DevBlock *__restrict__ blocks,
                                                  const BlockTasks *__restrict__ bt, int nb, long total,
-                                                 const BcStateDev *__restrict__ states, Gas gas) {
+                                                 Gas gas) {
   const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tg >= total) return;
   const int wb = FindTaskBlock(bt, nb, tg);
@@ -324,7 +332,7 @@ __global__ void __launch_bounds__(256) KBoundary(const DevBlock *__restrict__ bl
   for (int e = 0; e < 5; ++e) in[e] = b.W[e * b.npad + src];
   const GasT<double> g = Cast<double>(gas);
   GhostState(g, in, tk.bctype, a.x, a.y, a.z, tk.stype, tk.layer,
-             tk.state >= 0 ? &states[tk.state] : nullptr, gh);
+             tk.state >= 0 ? &b.states[tk.state] : nullptr, gh);
   for (int e = 0; e < 5; ++e) b.W[e * b.npad + dst] = gh[e];
 }
 
@@ -408,9 +416,9 @@ __device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &
 // (src/linearSolver.cpp:146-188) for one cell; `sr` is the cell spectral
 // radius, `Ad` the accumulated main diagonal
 __device__ __forceinline__ void CellTail(const DevBlock &b, const Params &P, long p, double vol, double sr,
-                                         double Ad, const double *cflp) {
+                                         double Ad) {
   b.sr[p] = sr;
-  const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*cflp) * (vol / Dmax(sr, 0.0));
+  const double dt = P.dtFixed > 0.0 ? P.dtFixed : (*b.cfl) * (vol / Dmax(sr, 0.0));
   b.dt[p] = dt;
   if (P.implicit && !P.block) {
     double dvt = (vol * (1.0 + P.zeta)) / (dt * P.theta);
@@ -466,8 +474,7 @@ __device__ __forceinline__ Unit MapUnit(const BlockMap &m) {
 }
 
 template <int RECON, int LIM, int FLUX>
-__global__ void __launch_bounds__(256) KResidual(BlockMap M, Params P,
-                                                 const double *__restrict__ cflp) {
+__global__ void __launch_bounds__(256) KResidual(BlockMap M, Params P) {
   __shared__ double sF[5][kNF];
   const Unit U = MapUnit(M);
   const DevBlock &b = *U.b;
@@ -526,7 +533,7 @@ __global__ void __launch_bounds__(256) KResidual(BlockMap M, Params P,
     b.sr[p] = sr;
     return;
   }
-  CellTail(b, P, p, b.vol[p], sr, sr, cflp);
+  CellTail(b, P, p, b.vol[p], sr, sr);
 }
 
 // ---------------------------------------------------------------------------
@@ -589,15 +596,14 @@ struct WallTask {
 
 __global__ void __launch_bounds__(256) KViscWall(const DevBlock *__restrict__ blocks,
                                                  const BlockTasks *__restrict__ bt, int nb, long total,
-                                                 const BcStateDev *__restrict__ states, Gas gas,
-                                                 ViscParams vp) {
+                                                 Gas gas, ViscParams vp) {
   const long tg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tg >= total) return;
   const int wb = FindTaskBlock(bt, nb, tg);
   const DevBlock &b = blocks[wb];
   const WallTask *__restrict__ tasks = static_cast<const WallTask *>(bt[wb].tasks);
   const WallTask w = tasks[tg - bt[wb].first];
-  const BcStateDev *s = w.state >= 0 ? &states[w.state] : nullptr;
+  const BcStateDev *s = w.state >= 0 ? &b.states[w.state] : nullptr;
   double in[5], gh[5];
 #pragma unroll
   for (int e = 0; e < 5; ++e) gh[e] = in[e] = b.W[e * b.npad + w.src];
@@ -880,7 +886,7 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
 // spectral radii (i, j, k) to its spectral radius (times the viscous CFL
 // coefficient) and twice them to its diagonal, then forms dt and A.
 template <bool R4>
-__global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P, const double *__restrict__ cflp) {
+__global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P) {
   __shared__ double sF[5][kNF];
   const Unit U = MapUnit(M);
   const DevBlock &b = *U.b;
@@ -939,7 +945,7 @@ __global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P, const dou
     sr += v * P.visc.coeff;
     if (P.implicit) Ad += 2.0 * v;
   }
-  CellTail(b, P, p, vol, sr, Ad, cflp);
+  CellTail(b, P, p, vol, sr, Ad);
 }
 // ---------------------------------------------------------------------------
 // implicit operator
@@ -999,7 +1005,7 @@ __device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, 
 // stores the cell velocity gradient, a sixth of each face's in the same order
 // (:1396-1414, 1427-1436), and finishes.
 template <int RECON, int LIM, int MODE>
-__global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P, int *flag) {
+__global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P) {
 #pragma clang fp contract(off)
   const Unit U = MapUnit(M);
   const DevBlock &b = *U.b;
@@ -1071,7 +1077,7 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P, int *fla
 #pragma unroll
   for (int q = 0; q < 25; ++q) b.Ab[q * b.ncell + p] = D[q];
   double I[25];
-  if (!GJInverse5<double>(D, I)) atomicOr(flag, 4);
+  if (!GJInverse5<double>(D, I)) atomicOr(b.flag, 4);
 #pragma unroll
   for (int q = 0; q < 25; ++q) b.Abi[TileBase(b, i, j, k) + 256 * q] = I[q];
 }
@@ -1455,8 +1461,7 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(BlockMap M, Param
 // procBlock::UpdateBlock (src/procBlock.cpp:826-871) with
 // ImplicitTimeAdvance (:902-916) or ExplicitEulerTimeAdvance (:880-896)
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) KUpdate(BlockMap M, Params P, Partial *part,
-                                                       int *flag) {
+__global__ void __launch_bounds__(kRedThreads) KUpdate(BlockMap M, Params P, Partial *part) {
   __shared__ double sl2[5][kRedThreads];
   __shared__ double sv[kRedThreads];
   __shared__ long long si[kRedThreads];
@@ -1492,7 +1497,7 @@ __global__ void __launch_bounds__(kRedThreads) KUpdate(BlockMap M, Params P, Par
     }
 #pragma unroll
     for (int e = 0; e < 5; ++e) b.W[e * b.npad + c] = Wn[e];
-    if (!(Wn[0] > 0.0) || !(Wn[4] > 0.0)) atomicOr(flag, 1);
+    if (!(Wn[0] > 0.0) || !(Wn[4] > 0.0)) atomicOr(b.flag, 1);
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
       l2[e] = R[e] * R[e];
@@ -1527,11 +1532,26 @@ __global__ void __launch_bounds__(kRedThreads) KUpdate(BlockMap M, Params P, Par
   }
 }
 
-// sums partials [0, n) in index order; writes slot `out` =
-// {l2[5], linf, linfIdx (as bits), mres}
-__global__ void __launch_bounds__(kRedThreads) KReduceNorms(const Partial *part, int n,
-                                                           double *hist, int *counter,
+// the norms of one case: its partials are [first, first + n) (its blocks'
+// 256-cell units, in block order), written to its history ring
+struct NormCase {
+  int first, n;
+  double *hist;
+  int *counter;
+};
+
+// one workgroup per case: sums the case's partials in index order (a tree
+// that depends only on the case's own partial count, so a case's norms do
+// not depend on which other cases share the launch); writes the next slot
+// of the case's ring = {l2[5], linf, linfIdx (as bits), mres}
+__global__ void __launch_bounds__(kRedThreads) KReduceNorms(const Partial *__restrict__ allPart,
+                                                           const NormCase *__restrict__ cases,
                                                            int withMres) {
+  const NormCase nc = cases[blockIdx.x];
+  const Partial *part = allPart + nc.first;
+  const int n = nc.n;
+  double *hist = nc.hist;
+  int *counter = nc.counter;
   __shared__ double sl2[5][kRedThreads];
   __shared__ double sm[kRedThreads];
   __shared__ double sv[kRedThreads];
@@ -1580,10 +1600,35 @@ __global__ void __launch_bounds__(kRedThreads) KReduceNorms(const Partial *part,
 __global__ void KSetScalar(double *dst, double v) { *dst = v; }
 __global__ void KSetInt(int *dst, int v) { *dst = v; }
 
+// input::CalcCFL (src/input.cpp:647-649) of every case of a group at its own
+// outer step, which then advances by one
+struct CflCase {
+  double start, step, max;
+  double *cfl;
+  int *nn;
+};
+// restart every case's norm ring at slot 0
+__global__ void KZeroCounters(const NormCase *__restrict__ cs, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) *cs[q].counter = 0;
+}
+
+__global__ void KCaseCfl(const CflCase *__restrict__ cs, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const CflCase c = cs[q];
+  const int nn = *c.nn;
+  const double v = c.start + nn * c.step;
+  *c.cfl = (c.max < v) ? c.max : v;
+  *c.nn = nn + 1;
+}
+
 // StoreOldSolution (src/procBlock.cpp:1037-1053)
-__global__ void __launch_bounds__(256) KBeginStep(DevBlock b, Params P, int copyNm1) {
+__global__ void __launch_bounds__(256) KBeginStep(BlockMap M, Params P, int copyNm1) {
 #pragma clang fp contract(off)
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const Unit Un = MapUnit(M);
+  const DevBlock &b = *Un.b;
+  const long p = (long)Un.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   const long c = PadIdx(b, i, j, k);
@@ -2295,18 +2340,32 @@ constexpr int kPW = 8;
 constexpr int kPenLanes = kPW * kPW;
 
 
+// one block's view of a pencil sweep
 struct PencilCtl {
-  int *ticket;        // reset to 0 before each sweep
-  int *prog;          // per pencil: steps completed in the current sweep
+  int *prog;          // per pencil of the block: steps completed in the current sweep
   int *err;           // bit 1: a dependency wait exceeded the spin limit
-  const int2 *order;  // pencils (a, b) in a+b order
-  int npen, pa, pb;   // pencil count and pencil grid
+  int pa, pb;         // pencil grid of the block
   void *sinkv;        // per pencil 5 x 64 operator words: stores of idle lanes
 };
 
-__global__ void __launch_bounds__(256) KPencilPrologue(PencilCtl s) {
+// the pencils of every block in one launch: entry t = (a, b, block, first
+// pencil of the block), ordered by pencil hyperplane a + b, then block, then
+// b, a.  A pencil depends only on pencils of its own block one hyperplane
+// lower (forward; higher in the backward pass, which takes the entries in
+// reverse), all of which hold earlier tickets.
+struct PencilGroup {
+  int *ticket;          // reset to 0 before each sweep
+  int *prog;            // per pencil of every block
+  int *err;
+  const int4 *entries;
+  int nent;             // pencils of all blocks
+  void *sinkv;          // 5 x 64 operator words per pencil
+  const DevBlock *blocks;
+};
+
+__global__ void __launch_bounds__(256) KPencilPrologue(PencilGroup s) {
   if (threadIdx.x == 0) *s.ticket = 0;
-  for (int q = threadIdx.x; q < s.npen; q += blockDim.x) s.prog[q] = 0;
+  for (int q = threadIdx.x; q < s.nent; q += blockDim.x) s.prog[q] = 0;
 }
 
 // pencil width (cells) in i or j
@@ -2346,8 +2405,10 @@ struct PenLane {
 
 // The right-hand side b of every cell, computed before the pencil sweeps
 // (it depends only on data the sweeps do not modify)
-__global__ void __launch_bounds__(256) KLusgsRhs(DevBlock b, Params P) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) KLusgsRhs(BlockMap M, Params P) {
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   double rb[5];
@@ -2552,17 +2613,25 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
 }
 
 template <typename T, bool FWD, bool FULL>
-__global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(DevBlock b, Params P, PencilCtl s) {
+__global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(PencilGroup G, Params P) {
   __shared__ PenShared<T, FULL> sh;
   const int tid = threadIdx.x, wave = tid / kPenLanes, lane = tid % kPenLanes;
-  if (tid == 0) sh.ticket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) sh.ticket = __hip_atomic_fetch_add(G.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (tid < kStagers) sh.stageReady[tid] = 0;
   if (tid < 3) sh.oReady[tid] = 0;
   if (tid == 0) sh.xReady = 0;
   __syncthreads();
-  const int t = sh.ticket;
-  if (t >= s.npen) return;
-  const int2 pc = s.order[FWD ? t : s.npen - 1 - t];
+  const int t = __builtin_amdgcn_readfirstlane(sh.ticket);
+  if (t >= G.nent) return;
+  const int4 ent = G.entries[FWD ? t : G.nent - 1 - t];
+  const DevBlock b = G.blocks[ent.z];
+  PencilCtl s;
+  s.prog = G.prog + ent.w;
+  s.err = G.err;
+  s.pa = (b.ni + kPW - 1) / kPW;
+  s.pb = (b.nj + kPW - 1) / kPW;
+  s.sinkv = static_cast<T *>(G.sinkv) + (size_t)ent.w * 5 * kPenLanes;
+  const int2 pc = make_int2(ent.x, ent.y);
   PenLane L;
   L.li = lane % kPW;
   L.lj = lane / kPW;

@@ -41,7 +41,6 @@ struct BlockRes {
   long bcTotal = 0;
   ath::SweepCtl sweep{};  // tiled LU-SGS dataflow state
   int sweepGrid = 0;
-  ath::PencilCtl pencil{};  // pencil-wavefront LU-SGS state
   // viscous cases: edge ghost tasks (inviscid and viscous pass) and wall tasks
   ath::EdgeTask *edgeInv = nullptr, *edgeVisc = nullptr;
   int edgeLines = 0, edgePerLine = 0;
@@ -49,23 +48,59 @@ struct BlockRes {
   int nwall = 0;
 };
 
+// One case resident on the device: its blocks, its device memory and the
+// per-case scalars every kernel reaches through its blocks (CFL, error bits,
+// boundary states), its norm history and the PrintResiduals normalisation.
+struct CaseRes {
+  ath::HostCase own;                  // a served case (geometry dropped after upload)
+  const ath::HostCase *hc = nullptr;  // the case description (own, or the context's)
+  std::vector<BlockRes> blocks;
+  std::vector<void *> allocs;         // device memory owned by this case
+  size_t bytes = 0;
+  int partFirst = 0, nparts = 0;      // reduction partials in the current group
+  ath::BcStateDev *dstates = nullptr;
+  double *dcfl = nullptr;
+  int *dflag = nullptr;               // bit 1 nonphysical, bit 4 singular block diagonal
+  int *dnn = nullptr;                 // served cases: outer step of the next iteration
+  double *dhist = nullptr;            // per-iteration norms ring
+  int *dcounter = nullptr;
+  int histCap = 0;
+  double l2first[ATH_MAX_EQ]{};
+  double cfl = 0;
+  std::vector<ath::HaloTask> halo;  // interblock connections, block indices within the case
+  std::vector<ath::ConnTask> conn;
+  // served cases
+  int id = -1, state = 0, error = 0, iters = 0, maxIters = 0;
+  double drop = 0;
+  ath_norms last{};
+  ~CaseRes() {
+    for (void *p : allocs) hipFree(p);
+  }
+};
+
 }  // namespace
 
 struct ath_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  ath::HostCase hc;
+  ath::HostCase hc;  // the case (a single-case context) or the numerics every case shares (a server)
   ath::Params P{};
+  // the resident cases; blk concatenates the blocks of the cases that
+  // iterate (all of them in a single-case context)
+  std::vector<std::unique_ptr<CaseRes>> cases;
+  std::vector<CaseRes *> active;
   std::vector<BlockRes> blk;
-  std::vector<void *> allocs;
-  ath::BcStateDev *dstates = nullptr;
-  double *dcfl = nullptr;
+  std::vector<void *> allocs;       // device memory of the context itself
+  std::vector<void *> groupAllocs;  // tables of the current set of active cases
+  std::vector<void *> *allocTo = nullptr;  // where Alloc records (default allocs)
+  size_t *bytesTo = nullptr;
   ath::Partial *dpart = nullptr;
   int nparts = 0;
-  double *dhist = nullptr;  // per-iteration norms ring
-  int histCap = 0;
-  int *dcounter = nullptr;
-  int *dflag = nullptr;
+  ath::NormCase *dnorm = nullptr;   // per active case: its partials and norm ring
+  ath::CflCase *dcflc = nullptr;    // per active served case: its CFL ramp
+  int *dflag = nullptr;             // bit 2: a sweep dependency wait exceeded its spin limit
+  // all-block pencil sweep
+  ath::PencilGroup pen{};
   int fp32 = 0;
   int matres = 1;
   int sweepOrder = ATH_SWEEP_PENCIL;  // ATH_SWEEP_*
@@ -101,18 +136,27 @@ struct ath_ctx {
   ath_timing tm{};
 
   ~ath_ctx() {
-    if (graphExec) hipGraphExecDestroy(graphExec);
-    if (graph) hipGraphDestroy(graph);
+    DropGraph();
     for (auto e : evs) hipEventDestroy(e);
+    cases.clear();
+    for (void *p : groupAllocs) hipFree(p);
     for (void *p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
+  }
+  void DropGraph() {
+    if (graphExec) hipGraphExecDestroy(graphExec);
+    if (graph) hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
   }
   template <typename T>
   int Alloc(T **p, size_t n) {
     void *q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (hipMalloc(&q, bytes) != hipSuccess)
       return Fail(ATH_ERR_NOMEM, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
-    allocs.push_back(q);
+    (allocTo ? allocTo : &allocs)->push_back(q);
+    if (bytesTo) *bytesTo += bytes;
     *p = static_cast<T *>(q);
     return ATH_OK;
   }
@@ -139,17 +183,6 @@ long HostBtSlot(const ath::HostBlock &h, int d, int upper, int i, int j, int k) 
   if (d == 0) return upper * sI + j + (long)h.nj * k;
   if (d == 1) return 2 * sI + upper * sJ + k + (long)h.nk * i;
   return 2 * sI + 2 * sJ + upper * sK + i + (long)h.ni * j;
-}
-
-// the device copy of the blocks' DevBlocks (pointers change when the FP32
-// operator buffers are created)
-int SyncBlocks(ath_ctx *c) {
-  std::vector<ath::DevBlock> v;
-  for (const auto &br : c->blk) v.push_back(br.d);
-  if (!c->dblocks)
-    if (int r = c->Alloc(&c->dblocks, v.size())) return r;
-  HIP_OK(hipMemcpy(c->dblocks, v.data(), v.size() * sizeof(ath::DevBlock), hipMemcpyHostToDevice));
-  return ATH_OK;
 }
 
 // boundaryConditions::GetBCSurface (src/boundaryConditions.cpp:109-190): the
@@ -199,7 +232,7 @@ double WallDistance(const ath::HostCase &hc, const ath::Vec3 &pt) {
 
 // padded geometry, projected centre distances, wall flags and the ghost-cell
 // task lists of a viscous case
-int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
+int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, BlockRes &br) {
   using namespace ath;
   DevBlock &d = br.d;
   const long np = d.npad;
@@ -343,9 +376,9 @@ int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
         const int gC = st % 2 == 0 ? r3 + layer - 1 : r3 - layer;
         const int iC = st % 2 == 0 ? std::max(r3 - layer, 0) : std::min(r3 + layer - 1, n3 - 1);
         int state = -1;
-        for (size_t q = 0; q < c->hc.bcs.size(); ++q)
-          if (c->hc.bcs[q].tag == sf.d[6]) { state = (int)q; break; }
-        const bool flux = state >= 0 && c->hc.bcs[state].heatFlux;
+        for (size_t q = 0; q < hc.bcs.size(); ++q)
+          if (hc.bcs[q].tag == sf.d[6]) { state = (int)q; break; }
+        const bool flux = state >= 0 && hc.bcs[state].heatFlux;
         // the cell next to the wall, whatever the layer
         const int aC = st % 2 == 0 ? r3 - 1 : r3;
         for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
@@ -354,7 +387,7 @@ int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
             HostMap3(dir, iC, q1, q2, &ii, &ij, &ik);
             HostMap3(dir, gC, q1, q2, &gi, &gj, &gk);
             HostMap3(dir, aC, q1, q2, &ai, &aj, &ak);
-            const double wd = flux ? WallDistance(c->hc, h.ctr[h.Cell(ai, aj, ak)]) : 0.0;
+            const double wd = flux ? WallDistance(hc, h.ctr[h.Cell(ai, aj, ak)]) : 0.0;
             wt.push_back({h.Cell(ii, ij, ik), h.Cell(gi, gj, gk), state, 0, wd});
           }
       }
@@ -367,13 +400,14 @@ int UploadViscous(ath_ctx *c, const ath::HostBlock &h, BlockRes &br) {
   return ATH_OK;
 }
 
-// gather lists of the interblock connections
-int UploadConnections(ath_ctx *c) {
+// gather lists of a case's interblock connections (block indices within the
+// case; the group tables add the case's first block)
+void CaseConnections(const ath::HostCase &hc, CaseRes &cr) {
   using namespace ath;
-  const HostCase &hc = c->hc;
-  if (hc.conns.empty()) return ATH_OK;
-  std::vector<HaloTask> halo;
-  std::vector<ConnTask> conn;
+  auto &halo = cr.halo;
+  auto &conn = cr.conn;
+  halo.clear();
+  conn.clear();
   for (const auto &cn : hc.conns) {
     const HostBlock &A = hc.blocks[cn.block], &B = hc.blocks[cn.pb];
     const bool upper = cn.s % 2 == 0;
@@ -409,165 +443,86 @@ int UploadConnections(ath_ctx *c) {
         }
       }
   }
-  c->nhalo = (int)halo.size();
-  c->nconn = (int)conn.size();
-  if (int r = c->Alloc(&c->dhalo, halo.size())) return r;
-  if (int r = c->Alloc(&c->dconn, conn.size())) return r;
-  HIP_OK(hipMemcpy(c->dhalo, halo.data(), halo.size() * sizeof(HaloTask), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->dconn, conn.data(), conn.size() * sizeof(ConnTask), hipMemcpyHostToDevice));
-  return SyncBlocks(c);
 }
 
-// the tiles of every block in pass order: tile hyperplane, then block, then
-// k, j, i; per block its tile grid and first flag
-int UploadMaps(ath_ctx *c) {
-  const int nb = (int)c->blk.size();
-  std::vector<int4> v((size_t)kNumMaps * nb);
-  for (int q = 0; q < kNumMaps; ++q) {
-    int first = 0;
-    for (int b = 0; b < nb; ++b) {
-      const auto &d = c->blk[b].d;
-      int4 u;
-      if (q == kMapTile)
-        u = make_int4((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
-                      (d.nk + ath::kTK - 1) / ath::kTK, first);
-      else if (q == kMapCell128) u = make_int4(Grid(d.ncell, 128), 1, 1, first);
-      else if (q == kMapCell256) u = make_int4(Grid(d.ncell, ath::kRedThreads), 1, 1, first);
-      else u = make_int4(Grid(d.npad, 256), 1, 1, first);
-      v[(size_t)q * nb + b] = u;
-      first += u.x * u.y * u.z;
-    }
-    c->mapTotal[q] = first;
+int RebuildGroup(ath_ctx *c);
+
+int AllocFp32(ath_ctx *c, ath::DevBlock &d) {
+  if (d.x32) return ATH_OK;
+  if (int r = c->Alloc(&d.Ainv32, d.ncell)) return r;
+  if (int r = c->Alloc(&d.x32, 5 * d.ncell)) return r;
+  if (int r = c->Alloc(&d.xold32, 5 * d.ncell)) return r;
+  HIP_OK(hipMemset(d.x32, 0, 5 * d.ncell * sizeof(float)));
+  if (d.bt) {
+    if (int r = c->Alloc(&d.bt32, 5 * d.nbf)) return r;
+    HIP_OK(hipMemset(d.bt32, 0, 5 * d.nbf * sizeof(float)));
   }
-  // the reductions' partial slots are the 256-cell units
-  for (int b = 0; b < nb; ++b)
-    if (v[(size_t)kMapCell256 * nb + b].w != c->blk[b].partBase)
-      return Fail(ATH_ERR_HIP, "unit map and reduction partials disagree");
-  if (!c->dmaps)
-    if (int r = c->Alloc(&c->dmaps, v.size())) return r;
-  HIP_OK(hipMemcpy(c->dmaps, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
-  std::vector<ath::BlockTasks> bt((size_t)kNumTaskMaps * nb);
-  for (int q = 0; q < kNumTaskMaps; ++q) {
-    long first = 0;
-    for (int b = 0; b < nb; ++b) {
-      const auto &br = c->blk[b];
-      ath::BlockTasks e{};
-      e.first = first;
-      if (q == kTaskBc) { e.tasks = br.dtasks; e.count = br.bcTotal; e.ntasks = (int)br.tasks.size(); }
-      else if (q == kTaskEdgeInv) { e.tasks = br.edgeInv; e.count = br.edgeLines; e.per = br.edgePerLine; }
-      else if (q == kTaskEdgeVisc) { e.tasks = br.edgeVisc; e.count = br.edgeLines; e.per = br.edgePerLine; }
-      else { e.tasks = br.wall; e.count = br.nwall; }
-      if (!e.tasks) e.count = 0;
-      bt[(size_t)q * nb + b] = e;
-      first += e.count;
-    }
-    c->taskTotal[q] = first;
-  }
-  if (!c->dbt)
-    if (int r = c->Alloc(&c->dbt, bt.size())) return r;
-  HIP_OK(hipMemcpy(c->dbt, bt.data(), bt.size() * sizeof(ath::BlockTasks), hipMemcpyHostToDevice));
-  // the all-block launches read every block's DevBlock from device memory
-  if (int r = SyncBlocks(c)) return r;
-  if (!c->dblocks) return Fail(ATH_ERR_HIP, "device block table missing");
   return ATH_OK;
 }
 
-int UploadMultiSweep(ath_ctx *c) {
-  using namespace ath;
-  std::vector<int4> grid, tiles;
-  int base = 0, maxPlane = 0;
-  for (auto &br : c->blk) {
-    const int ta = (br.d.ni + kBTI - 1) / kBTI, tb = (br.d.nj + kBTJ - 1) / kBTJ, tc = (br.d.nk + kBTK - 1) / kBTK;
-    grid.push_back(make_int4(ta, tb, tc, base));
-    base += ta * tb * tc;
-    maxPlane = std::max(maxPlane, ta + tb + tc - 2);
+int EnsureFp32(ath_ctx *c) {
+  for (auto &cp : c->cases) {
+    c->allocTo = &cp->allocs;
+    c->bytesTo = &cp->bytes;
+    for (auto &br : cp->blocks) {
+      const int r = AllocFp32(c, br.d);
+      if (r) {
+        c->allocTo = nullptr;
+        c->bytesTo = nullptr;
+        return r;
+      }
+    }
   }
-  for (int pl = 0; pl < maxPlane; ++pl)
-    for (size_t q = 0; q < grid.size(); ++q)
-      for (int cc = 0; cc < grid[q].z; ++cc)
-        for (int bb = 0; bb < grid[q].y; ++bb)
-          for (int aa = 0; aa < grid[q].x; ++aa)
-            if (aa + bb + cc == pl) tiles.push_back(make_int4(aa, bb, cc, (int)q));
-  MultiSweepCtl &m = c->msweep;
-  m.ntiles = (int)tiles.size();
-  int4 *dt = nullptr, *dg = nullptr;
-  if (int r = c->Alloc(&dt, tiles.size())) return r;
-  if (int r = c->Alloc(&dg, grid.size())) return r;
-  HIP_OK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dg, grid.data(), grid.size() * sizeof(int4), hipMemcpyHostToDevice));
-  m.tiles = dt;
-  m.grid = dg;
-  if (int r = c->Alloc(&m.flags, base)) return r;
-  if (int r = c->Alloc(&m.ticket, 2)) return r;
-  HIP_OK(hipMemset(m.flags, 0, base * sizeof(int)));
-  HIP_OK(hipMemset(m.ticket, 0, 2 * sizeof(int)));
-  m.epoch = m.ticket + 1;
-  m.err = c->dflag;
-  hipDeviceProp_t prop;
-  HIP_OK(hipGetDeviceProperties(&prop, c->device));
-  c->msweepGrid = std::max(1, std::min(m.ntiles, 2 * prop.multiProcessorCount));
-  // the sweep reads every block's DevBlock from device memory; a case
-  // without connections has not copied them yet
-  if (int r = SyncBlocks(c)) return r;
-  if (!c->dblocks) return Fail(ATH_ERR_HIP, "block sweep: device block table missing");
+  c->allocTo = nullptr;
+  c->bytesTo = nullptr;
+  if (int r = RebuildGroup(c)) return r;
+  HIP_OK(hipDeviceSynchronize());
   return ATH_OK;
 }
 
-int Upload(ath_ctx *c) {
+// Everything a case owns on the device, recorded in cr.allocs: per block the
+// state, geometry, residual and operator arrays, task lists and sweep state;
+// per case the boundary states by tag, the CFL number, the error bits and
+// the norm history.  The tables that span the active cases are built by
+// RebuildGroup.
+int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
   using namespace ath;
-  const HostCase &hc = c->hc;
-  // physics / numerics parameters
-  c->P.gas.R = hc.gasR;
-  c->P.gas.n = hc.gasN;
-  c->P.gas.gamma = hc.gamma;
-  c->P.gas.cps = hc.gasR * (hc.gasN + 1.0);
-  c->P.gas.cvs = hc.gasR * hc.gasN;
-  c->P.kappa = hc.kappa;
-  c->P.theta = hc.theta;
-  c->P.zeta = hc.zeta;
-  c->P.relax = hc.relax;
-  c->P.dualCFL = hc.dualCFL;
-  c->P.dtFixed = hc.dtIn > 0.0 ? hc.dtIn * hc.aRef / hc.lRef : -1.0;
-  c->P.implicit = hc.Implicit() ? 1 : 0;
-  c->P.multilevel = hc.MultiLevel() ? 1 : 0;
-  c->P.initMatrix = hc.InitMatrix() ? 1 : 0;
-  c->P.explicitEuler = hc.timeint == TimeInt::ExplicitEuler ? 1 : 0;
-  c->P.roeJac = hc.roeJac ? 1 : 0;
-  c->P.block = hc.block ? 1 : 0;
-  c->P.visc.on = hc.viscous ? 1 : 0;
-  c->P.visc.recon4 = hc.viscRecon4 ? 1 : 0;
-  c->P.visc.vscale = hc.vscale;
-  c->P.visc.prandtl = hc.prandtl;
-  c->P.visc.coeff = hc.viscCoeff;
-  c->P.visc.tRef = hc.tRef;
-  c->P.visc.muRef = hc.muRef;
-  c->P.visc.kNonDim = hc.kNonDim;
-  c->P.visc.c1 = hc.suth[0];
-  c->P.visc.s1 = hc.suth[1];
-  c->P.visc.c2 = hc.suth[2];
-  c->P.visc.s2 = hc.suth[3];
-
+  const HostCase &hc = *cr.hc;
+  c->allocTo = &cr.allocs;
+  c->bytesTo = &cr.bytes;
+  struct Restore {
+    ath_ctx *c;
+    ~Restore() { c->allocTo = nullptr; c->bytesTo = nullptr; }
+  } restore{c};
   // boundary states by tag
   std::vector<BcStateDev> st;
   for (const auto &s : hc.bcs)
     st.push_back({s.p, s.rho, s.vel[0], s.vel[1], s.vel[2], s.p0, s.t0, s.dir[0], s.dir[1], s.dir[2], s.twall,
                   s.isothermal ? 1 : 0, s.heatFlux ? 1 : 0, s.qwall});
-  if (int r = c->Alloc(&c->dstates, st.size())) return r;
+  if (int r = c->Alloc(&cr.dstates, st.size())) return r;
   if (!st.empty())
-    HIP_OK(hipMemcpy(c->dstates, st.data(), st.size() * sizeof(BcStateDev), hipMemcpyHostToDevice));
-  if (int r = c->Alloc(&c->dcfl, 1)) return r;
-  if (int r = c->Alloc(&c->dcounter, 1)) return r;
-  if (int r = c->Alloc(&c->dflag, 1)) return r;
-  HIP_OK(hipMemset(c->dcounter, 0, sizeof(int)));
-  HIP_OK(hipMemset(c->dflag, 0, sizeof(int)));
+    HIP_OK(hipMemcpy(cr.dstates, st.data(), st.size() * sizeof(BcStateDev), hipMemcpyHostToDevice));
+  if (int r = c->Alloc(&cr.dcfl, 1)) return r;
+  if (int r = c->Alloc(&cr.dflag, 1)) return r;
+  if (int r = c->Alloc(&cr.dnn, 1)) return r;
+  if (int r = c->Alloc(&cr.dcounter, 1)) return r;
+  HIP_OK(hipMemset(cr.dflag, 0, sizeof(int)));
+  HIP_OK(hipMemset(cr.dnn, 0, sizeof(int)));
+  HIP_OK(hipMemset(cr.dcounter, 0, sizeof(int)));
+  cr.histCap = 4096;
+  if (int r = c->Alloc(&cr.dhist, 8 * (size_t)cr.histCap)) return r;
+  cr.cfl = hc.Cfl(0);
+  HIP_OK(hipMemcpy(cr.dcfl, &cr.cfl, sizeof(double), hipMemcpyHostToDevice));
 
   long cellBase = 0;
-  int partBase = 0;
-  c->blk.resize(hc.blocks.size());
+  cr.blocks.resize(hc.blocks.size());
   for (size_t bq = 0; bq < hc.blocks.size(); ++bq) {
     const HostBlock &h = hc.blocks[bq];
-    BlockRes &br = c->blk[bq];
+    BlockRes &br = cr.blocks[bq];
     DevBlock &d = br.d;
+    d.cfl = cr.dcfl;
+    d.flag = cr.dflag;
+    d.states = cr.dstates;
     d.ni = h.ni; d.nj = h.nj; d.nk = h.nk; d.g = h.g;
     d.NI = h.NI; d.NJ = h.NJ;
     d.npad = h.NumPadded();
@@ -676,7 +631,7 @@ int Upload(ath_ctx *c) {
       d.c2c[dd] = nullptr;
     }
     if (hc.viscous)
-      if (int r = UploadViscous(c, h, br)) return r;
+      if (int r = UploadViscous(c, hc, h, br)) return r;
     // hyperplane order
     std::vector<int> cells;
     HyperplaneOrder(h.ni, h.nj, h.nk, &cells, &br.planeOff);
@@ -710,35 +665,8 @@ int Upload(ath_ctx *c) {
       HIP_OK(hipGetDeviceProperties(&prop, c->device));
       br.sweepGrid = std::max(1, std::min(sc.ntiles, 2 * prop.multiProcessorCount));
     }
-    // pencil sweep: kPW x kPW columns in a+b order, progress counters, ticket
-    {
-      PencilCtl &pc = br.pencil;
-      pc.pa = (h.ni + kPW - 1) / kPW;
-      pc.pb = (h.nj + kPW - 1) / kPW;
-      pc.npen = pc.pa * pc.pb;
-      std::vector<int2> order;
-      order.reserve(pc.npen);
-      for (int pl = 0; pl < pc.pa + pc.pb - 1; ++pl)
-        for (int bb = 0; bb < pc.pb; ++bb)
-          for (int aa = 0; aa < pc.pa; ++aa)
-            if (aa + bb == pl) order.push_back(make_int2(aa, bb));
-      int2 *dord = nullptr;
-      if (int r = c->Alloc(&dord, order.size())) return r;
-      HIP_OK(hipMemcpy(dord, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
-      pc.order = dord;
-      if (int r = c->Alloc(&pc.prog, pc.npen)) return r;
-      if (int r = c->Alloc(&pc.ticket, 1)) return r;
-      double *sink = nullptr;
-      if (int r = c->Alloc(&sink, (size_t)pc.npen * 5 * kPenLanes)) return r;
-      pc.sinkv = sink;
-      HIP_OK(hipMemset(pc.prog, 0, pc.npen * sizeof(int)));
-      HIP_OK(hipMemset(pc.ticket, 0, sizeof(int)));
-      pc.err = c->dflag;
-    }
-    // reduction partials
+    // reduction partials: the block's 256-cell units
     br.nparts = Grid(nc, kRedThreads);
-    br.partBase = partBase;
-    partBase += br.nparts;
     // boundary tasks (procBlock::AssignInviscidGhostCells, src/procBlock.cpp:2449-2529)
     long total = 0;
     for (int layer = 1; layer <= h.g; ++layer)
@@ -789,32 +717,273 @@ int Upload(ath_ctx *c) {
       HIP_OK(hipMemcpy(br.dtasks, br.tasks.data(), br.tasks.size() * sizeof(BcTask), hipMemcpyHostToDevice));
     }
   }
-  c->nparts = partBase;
-  if (int r = c->Alloc(&c->dpart, c->nparts)) return r;
-  c->histCap = 4096;
-  if (int r = c->Alloc(&c->dhist, 8 * (size_t)c->histCap)) return r;
-  if (int r = UploadConnections(c)) return r;
-  if (c->hc.block)
-    if (int r = UploadMultiSweep(c)) return r;
-  return UploadMaps(c);
+  CaseConnections(hc, cr);
+  if (fp32)
+    for (auto &br : cr.blocks)
+      if (int r = AllocFp32(c, br.d)) return r;
+  return ATH_OK;
 }
 
-int EnsureFp32(ath_ctx *c) {
-  for (auto &br : c->blk) {
-    auto &d = br.d;
-    if (d.x32) continue;
-    if (int r = c->Alloc(&d.Ainv32, d.ncell)) return r;
-    if (int r = c->Alloc(&d.x32, 5 * d.ncell)) return r;
-    if (int r = c->Alloc(&d.xold32, 5 * d.ncell)) return r;
-    HIP_OK(hipMemset(d.x32, 0, 5 * d.ncell * sizeof(float)));
-    if (d.bt) {
-      if (int r = c->Alloc(&d.bt32, 5 * d.nbf)) return r;
-      HIP_OK(hipMemset(d.bt32, 0, 5 * d.nbf * sizeof(float)));
+// The tables of the active cases, rebuilt whenever that set or a block's
+// buffers change: the concatenated block list and its device copy, the
+// connection gather lists, the unit grids of the all-block launches and the
+// reduction partials (each case's 256-cell units are one contiguous range),
+// the ghost-cell task tables, the pencil and block sweep queues and the
+// per-case norm and CFL tables.  Captured graphs are dropped.
+int RebuildGroup(ath_ctx *c) {
+  using namespace ath;
+  c->DropGraph();
+  for (void *p : c->groupAllocs) hipFree(p);
+  c->groupAllocs.clear();
+  c->allocTo = &c->groupAllocs;
+  struct Restore {
+    ath_ctx *c;
+    ~Restore() { c->allocTo = nullptr; }
+  } restore{c};
+  c->blk.clear();
+  std::vector<int> first;  // first block of each active case
+  for (CaseRes *cr : c->active) {
+    first.push_back((int)c->blk.size());
+    for (const auto &br : cr->blocks) c->blk.push_back(br);
+  }
+  const int nb = (int)c->blk.size();
+  c->dblocks = nullptr;
+  c->dmaps = nullptr;
+  c->dbt = nullptr;
+  c->dhalo = nullptr;
+  c->dconn = nullptr;
+  c->dpart = nullptr;
+  c->dnorm = nullptr;
+  c->dcflc = nullptr;
+  c->nhalo = c->nconn = 0;
+  c->nparts = 0;
+  c->pen = PencilGroup{};
+  c->msweep = MultiSweepCtl{};
+  for (int q = 0; q < kNumMaps; ++q) c->mapTotal[q] = 0;
+  for (int q = 0; q < kNumTaskMaps; ++q) c->taskTotal[q] = 0;
+  if (nb == 0) return ATH_OK;
+  // the device block table
+  {
+    std::vector<DevBlock> v;
+    for (const auto &br : c->blk) v.push_back(br.d);
+    if (int r = c->Alloc(&c->dblocks, v.size())) return r;
+    HIP_OK(hipMemcpy(c->dblocks, v.data(), v.size() * sizeof(DevBlock), hipMemcpyHostToDevice));
+  }
+  // connections
+  {
+    std::vector<HaloTask> halo;
+    std::vector<ConnTask> conn;
+    for (size_t q = 0; q < c->active.size(); ++q) {
+      const int f = first[q];
+      for (HaloTask t : c->active[q]->halo) {
+        t.dst += f;
+        t.src += f;
+        halo.push_back(t);
+      }
+      for (ConnTask t : c->active[q]->conn) {
+        t.blk += f;
+        t.src += f;
+        conn.push_back(t);
+      }
+    }
+    c->nhalo = (int)halo.size();
+    c->nconn = (int)conn.size();
+    if (!halo.empty()) {
+      if (int r = c->Alloc(&c->dhalo, halo.size())) return r;
+      HIP_OK(hipMemcpy(c->dhalo, halo.data(), halo.size() * sizeof(HaloTask), hipMemcpyHostToDevice));
+    }
+    if (!conn.empty()) {
+      if (int r = c->Alloc(&c->dconn, conn.size())) return r;
+      HIP_OK(hipMemcpy(c->dconn, conn.data(), conn.size() * sizeof(ConnTask), hipMemcpyHostToDevice));
     }
   }
-  if (int r = SyncBlocks(c)) return r;
-  HIP_OK(hipDeviceSynchronize());
+  // unit grids of the all-block launches
+  std::vector<int4> v((size_t)kNumMaps * nb);
+  for (int q = 0; q < kNumMaps; ++q) {
+    int f = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto &d = c->blk[b].d;
+      int4 u;
+      if (q == kMapTile)
+        u = make_int4((d.ni + ath::kTI - 1) / ath::kTI, (d.nj + ath::kTJ - 1) / ath::kTJ,
+                      (d.nk + ath::kTK - 1) / ath::kTK, f);
+      else if (q == kMapCell128) u = make_int4(Grid(d.ncell, 128), 1, 1, f);
+      else if (q == kMapCell256) u = make_int4(Grid(d.ncell, ath::kRedThreads), 1, 1, f);
+      else u = make_int4(Grid(d.npad, 256), 1, 1, f);
+      v[(size_t)q * nb + b] = u;
+      f += u.x * u.y * u.z;
+    }
+    c->mapTotal[q] = f;
+  }
+  if (int r = c->Alloc(&c->dmaps, v.size())) return r;
+  HIP_OK(hipMemcpy(c->dmaps, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // the reductions' partial slots are the 256-cell units; each case's are
+  // contiguous
+  for (int b = 0; b < nb; ++b) c->blk[b].partBase = v[(size_t)kMapCell256 * nb + b].w;
+  c->nparts = c->mapTotal[kMapCell256];
+  if (int r = c->Alloc(&c->dpart, c->nparts)) return r;
+  {
+    std::vector<NormCase> nc;
+    for (size_t q = 0; q < c->active.size(); ++q) {
+      CaseRes &cr = *c->active[q];
+      cr.partFirst = c->blk[first[q]].partBase;
+      cr.nparts = 0;
+      for (int b = first[q]; b < first[q] + (int)cr.blocks.size(); ++b) cr.nparts += c->blk[b].nparts;
+      nc.push_back({cr.partFirst, cr.nparts, cr.dhist, cr.dcounter});
+    }
+    if (int r = c->Alloc(&c->dnorm, nc.size())) return r;
+    HIP_OK(hipMemcpy(c->dnorm, nc.data(), nc.size() * sizeof(NormCase), hipMemcpyHostToDevice));
+    std::vector<CflCase> cc;
+    for (CaseRes *cr : c->active)
+      cc.push_back({cr->hc->cflStart, cr->hc->cflStep, cr->hc->cflMax, cr->dcfl, cr->dnn});
+    if (int r = c->Alloc(&c->dcflc, cc.size())) return r;
+    HIP_OK(hipMemcpy(c->dcflc, cc.data(), cc.size() * sizeof(CflCase), hipMemcpyHostToDevice));
+  }
+  // ghost-cell task tables
+  std::vector<ath::BlockTasks> bt((size_t)kNumTaskMaps * nb);
+  for (int q = 0; q < kNumTaskMaps; ++q) {
+    long f = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto &br = c->blk[b];
+      ath::BlockTasks e{};
+      e.first = f;
+      if (q == kTaskBc) { e.tasks = br.dtasks; e.count = br.bcTotal; e.ntasks = (int)br.tasks.size(); }
+      else if (q == kTaskEdgeInv) { e.tasks = br.edgeInv; e.count = br.edgeLines; e.per = br.edgePerLine; }
+      else if (q == kTaskEdgeVisc) { e.tasks = br.edgeVisc; e.count = br.edgeLines; e.per = br.edgePerLine; }
+      else { e.tasks = br.wall; e.count = br.nwall; }
+      if (!e.tasks) e.count = 0;
+      bt[(size_t)q * nb + b] = e;
+      f += e.count;
+    }
+    c->taskTotal[q] = f;
+  }
+  if (int r = c->Alloc(&c->dbt, bt.size())) return r;
+  HIP_OK(hipMemcpy(c->dbt, bt.data(), bt.size() * sizeof(ath::BlockTasks), hipMemcpyHostToDevice));
+  // pencil sweep queue: every block's kPW x kPW columns by column hyperplane
+  // a + b, then block, then b, a
+  {
+    std::vector<int4> ent;
+    std::vector<int> base(nb);
+    int total = 0, maxPlane = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto &d = c->blk[b].d;
+      const int pa = (d.ni + kPW - 1) / kPW, pb = (d.nj + kPW - 1) / kPW;
+      base[b] = total;
+      total += pa * pb;
+      maxPlane = std::max(maxPlane, pa + pb - 1);
+    }
+    for (int pl = 0; pl < maxPlane; ++pl)
+      for (int b = 0; b < nb; ++b) {
+        const auto &d = c->blk[b].d;
+        const int pa = (d.ni + kPW - 1) / kPW, pb = (d.nj + kPW - 1) / kPW;
+        for (int bb = 0; bb < pb; ++bb)
+          for (int aa = 0; aa < pa; ++aa)
+            if (aa + bb == pl) ent.push_back(make_int4(aa, bb, b, base[b]));
+      }
+    int4 *de = nullptr;
+    if (int r = c->Alloc(&de, ent.size())) return r;
+    HIP_OK(hipMemcpy(de, ent.data(), ent.size() * sizeof(int4), hipMemcpyHostToDevice));
+    PencilGroup &g = c->pen;
+    g.entries = de;
+    g.nent = (int)ent.size();
+    if (int r = c->Alloc(&g.prog, total)) return r;
+    if (int r = c->Alloc(&g.ticket, 1)) return r;
+    double *sink = nullptr;
+    if (int r = c->Alloc(&sink, (size_t)total * 5 * kPenLanes)) return r;
+    g.sinkv = sink;
+    HIP_OK(hipMemset(g.prog, 0, total * sizeof(int)));
+    HIP_OK(hipMemset(g.ticket, 0, sizeof(int)));
+    g.err = c->dflag;
+    g.blocks = c->dblocks;
+  }
+  // block solvers: one tiled sweep over all blocks, tiles by tile hyperplane,
+  // then block, then k, j, i; per block its tile grid and first flag
+  if (c->hc.block) {
+    std::vector<int4> grid, tiles;
+    int base = 0, maxPlane = 0;
+    for (auto &br : c->blk) {
+      const int ta = (br.d.ni + kBTI - 1) / kBTI, tb = (br.d.nj + kBTJ - 1) / kBTJ,
+                tc = (br.d.nk + kBTK - 1) / kBTK;
+      grid.push_back(make_int4(ta, tb, tc, base));
+      base += ta * tb * tc;
+      maxPlane = std::max(maxPlane, ta + tb + tc - 2);
+    }
+    for (int pl = 0; pl < maxPlane; ++pl)
+      for (size_t q = 0; q < grid.size(); ++q)
+        for (int cc = 0; cc < grid[q].z; ++cc)
+          for (int bb = 0; bb < grid[q].y; ++bb)
+            for (int aa = 0; aa < grid[q].x; ++aa)
+              if (aa + bb + cc == pl) tiles.push_back(make_int4(aa, bb, cc, (int)q));
+    MultiSweepCtl &m = c->msweep;
+    m.ntiles = (int)tiles.size();
+    int4 *dt = nullptr, *dg = nullptr;
+    if (int r = c->Alloc(&dt, tiles.size())) return r;
+    if (int r = c->Alloc(&dg, grid.size())) return r;
+    HIP_OK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dg, grid.data(), grid.size() * sizeof(int4), hipMemcpyHostToDevice));
+    m.tiles = dt;
+    m.grid = dg;
+    if (int r = c->Alloc(&m.flags, base)) return r;
+    if (int r = c->Alloc(&m.ticket, 2)) return r;
+    HIP_OK(hipMemset(m.flags, 0, base * sizeof(int)));
+    HIP_OK(hipMemset(m.ticket, 0, 2 * sizeof(int)));
+    m.epoch = m.ticket + 1;
+    m.err = c->dflag;
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, c->device));
+    c->msweepGrid = std::max(1, std::min(m.ntiles, 2 * prop.multiProcessorCount));
+  }
   return ATH_OK;
+}
+
+// kernel parameters of a case (every case of a server must produce the same)
+void FillParams(const ath::HostCase &hc, ath::Params *out) {
+  ath::Params P;
+  std::memset(&P, 0, sizeof P);
+  P.gas.R = hc.gasR;
+  P.gas.n = hc.gasN;
+  P.gas.gamma = hc.gamma;
+  P.gas.cps = hc.gasR * (hc.gasN + 1.0);
+  P.gas.cvs = hc.gasR * hc.gasN;
+  P.kappa = hc.kappa;
+  P.theta = hc.theta;
+  P.zeta = hc.zeta;
+  P.relax = hc.relax;
+  P.dualCFL = hc.dualCFL;
+  P.dtFixed = hc.dtIn > 0.0 ? hc.dtIn * hc.aRef / hc.lRef : -1.0;
+  P.implicit = hc.Implicit() ? 1 : 0;
+  P.multilevel = hc.MultiLevel() ? 1 : 0;
+  P.initMatrix = hc.InitMatrix() ? 1 : 0;
+  P.explicitEuler = hc.timeint == ath::TimeInt::ExplicitEuler ? 1 : 0;
+  P.roeJac = hc.roeJac ? 1 : 0;
+  P.block = hc.block ? 1 : 0;
+  P.visc.on = hc.viscous ? 1 : 0;
+  P.visc.recon4 = hc.viscRecon4 ? 1 : 0;
+  P.visc.vscale = hc.vscale;
+  P.visc.prandtl = hc.prandtl;
+  P.visc.coeff = hc.viscCoeff;
+  P.visc.tRef = hc.tRef;
+  P.visc.muRef = hc.muRef;
+  P.visc.kNonDim = hc.kNonDim;
+  P.visc.c1 = hc.suth[0];
+  P.visc.s1 = hc.suth[1];
+  P.visc.c2 = hc.suth[2];
+  P.visc.s2 = hc.suth[3];
+  *out = P;
+}
+
+// a single-case context: the case is the context's own
+int Upload(ath_ctx *c) {
+  FillParams(c->hc, &c->P);
+  if (int r = c->Alloc(&c->dflag, 1)) return r;
+  HIP_OK(hipMemset(c->dflag, 0, sizeof(int)));
+  auto cr = std::make_unique<CaseRes>();
+  cr->hc = &c->hc;
+  c->cases.push_back(std::move(cr));
+  CaseRes &k = *c->cases.back();
+  if (int r = UploadCase(c, k, false)) return r;
+  c->active.assign(1, &k);
+  return RebuildGroup(c);
 }
 
 // --- launch helpers (all on c->stream) -------------------------------------
@@ -826,7 +995,7 @@ void LaunchBoundary(ath_ctx *c) {
   const int nb = (int)c->blk.size();
   if (c->taskTotal[kTaskBc] > 0)
     hipLaunchKernelGGL(ath::KBoundary, dim3(Grid(c->taskTotal[kTaskBc])), dim3(kBlk), 0, c->stream, c->dblocks,
-                       c->Tasks(kTaskBc), nb, c->taskTotal[kTaskBc], c->dstates, c->P.gas);
+                       c->Tasks(kTaskBc), nb, c->taskTotal[kTaskBc], c->P.gas);
   if (c->nhalo > 0)
     hipLaunchKernelGGL(ath::KHaloState, dim3(Grid(c->nhalo)), dim3(kBlk), 0, c->stream, c->dblocks, c->dhalo,
                        c->nhalo);
@@ -849,14 +1018,14 @@ void LaunchResidualT(ath_ctx *c) {
   const ath::Params &P = c->P;
   const ath::BlockMap mt = c->Map(kMapTile), mc = c->Map(kMapCell128), mp = c->Map(kMapPad256);
   const dim3 tiles(c->mapTotal[kMapTile]), cells(c->mapTotal[kMapCell128]);
-  hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
+  hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), tiles, dim3(256), 0, c->stream, mt, P);
   if (P.block) {
     // inviscid block sum from the inviscid ghost states; finished here, or
     // parked in Ab until the viscous pass has formed dt
     if (P.visc.on)
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), cells, dim3(128), 0, c->stream, mc, P);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), cells, dim3(128), 0, c->stream, mc, P);
   }
   if (!P.visc.on) return;
   // viscous ghost cells, temperature and viscosity, viscous fluxes
@@ -865,21 +1034,21 @@ void LaunchResidualT(ath_ctx *c) {
   const int nb = (int)c->blk.size();
   if (c->taskTotal[kTaskWall] > 0)
     hipLaunchKernelGGL(ath::KViscWall, dim3(Grid(c->taskTotal[kTaskWall])), dim3(kBlk), 0, c->stream, c->dblocks,
-                       c->Tasks(kTaskWall), nb, c->taskTotal[kTaskWall], c->dstates, P.gas, P.visc);
+                       c->Tasks(kTaskWall), nb, c->taskTotal[kTaskWall], P.gas, P.visc);
   if (c->taskTotal[kTaskEdgeVisc] > 0)
     hipLaunchKernelGGL(ath::KEdgeGhosts, dim3(Grid(c->taskTotal[kTaskEdgeVisc], 64)), dim3(64), 0, c->stream,
                        c->dblocks, c->Tasks(kTaskEdgeVisc), nb, c->taskTotal[kTaskEdgeVisc], P.gas);
   hipLaunchKernelGGL(ath::KAux, dim3(c->mapTotal[kMapPad256]), dim3(kBlk), 0, c->stream, mp, P);
   if (P.visc.recon4)
-    hipLaunchKernelGGL((ath::KViscFlux<true>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
+    hipLaunchKernelGGL((ath::KViscFlux<true>), tiles, dim3(256), 0, c->stream, mt, P);
   else
-    hipLaunchKernelGGL((ath::KViscFlux<false>), tiles, dim3(256), 0, c->stream, mt, P, c->dcfl);
+    hipLaunchKernelGGL((ath::KViscFlux<false>), tiles, dim3(256), 0, c->stream, mt, P);
   // the block diagonal needs dt, which the viscous pass completes
   if (P.block) {
     if (P.visc.recon4)
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), cells, dim3(128), 0, c->stream, mc, P);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), cells, dim3(128), 0, c->stream, mc, P, c->dflag);
+      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), cells, dim3(128), 0, c->stream, mc, P);
   }
 }
 
@@ -936,24 +1105,23 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     const int order = EffectiveOrder(c);
     if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_PENCIL) {
       if (sw == 0)
-        for (auto &br : c->blk)
-          hipLaunchKernelGGL(ath::KLusgsRhs, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P);
+        hipLaunchKernelGGL(ath::KLusgsRhs, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                           c->Map(kMapCell256), c->P);
       for (int dir = 0; dir < 2; ++dir) {
         LaunchConnTerms<T>(c);
         mark(true);
-        for (auto &br : c->blk) {
-          hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, br.pencil);
-          const dim3 grid(br.pencil.npen), blk(ath::kPenWaves * ath::kPenLanes);
-          if (dir == 0 && full)
-            hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
-          else if (dir == 0)
-            hipLaunchKernelGGL((ath::KLusgsPencil<T, true, false>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
-          else if (full)
-            hipLaunchKernelGGL((ath::KLusgsPencil<T, false, true>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
-          else
-            hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), grid, blk, 0, c->stream, br.d, c->P, br.pencil);
-          ++launches;
-        }
+        // every block's pencils in one launch
+        hipLaunchKernelGGL(ath::KPencilPrologue, dim3(1), dim3(256), 0, c->stream, c->pen);
+        const dim3 grid(c->pen.nent), blk(ath::kPenWaves * ath::kPenLanes);
+        if (dir == 0 && full)
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, true>), grid, blk, 0, c->stream, c->pen, c->P);
+        else if (dir == 0)
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, true, false>), grid, blk, 0, c->stream, c->pen, c->P);
+        else if (full)
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, true>), grid, blk, 0, c->stream, c->pen, c->P);
+        else
+          hipLaunchKernelGGL((ath::KLusgsPencil<T, false, false>), grid, blk, 0, c->stream, c->pen, c->P);
+        ++launches;
         mark(false);
       }
     } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED && c->P.block) {
@@ -1033,9 +1201,9 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
 template <typename T>
 void LaunchUpdateT(ath_ctx *c) {
   hipLaunchKernelGGL(ath::KUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(ath::kRedThreads), 0, c->stream,
-                     c->Map(kMapCell256), c->P, c->dpart, c->dflag);
-  hipLaunchKernelGGL(ath::KReduceNorms, dim3(1), dim3(ath::kRedThreads), 0, c->stream, c->dpart, c->nparts,
-                     c->dhist, c->dcounter, (c->P.implicit && c->matres) ? 1 : 0);
+                     c->Map(kMapCell256), c->P, c->dpart);
+  hipLaunchKernelGGL(ath::KReduceNorms, dim3(c->active.size()), dim3(ath::kRedThreads), 0, c->stream, c->dpart,
+                     c->dnorm, (c->P.implicit && c->matres) ? 1 : 0);
 }
 
 // everything after the residual: relax (implicit) + update + reduction
@@ -1068,12 +1236,12 @@ int CheckLaunch() {
   return ATH_OK;
 }
 
-// read norms slot `slot` and fill `out` (+ PrintResiduals normalisation)
-int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
-  double h[8];
-  HIP_OK(hipMemcpy(h, c->dhist + 8 * (size_t)slot, sizeof h, hipMemcpyDeviceToHost));
-  int flag = 0;
-  HIP_OK(hipMemcpy(&flag, c->dflag, sizeof flag, hipMemcpyDeviceToHost));
+// the norms of one iteration of case `cr` from its history slot values h
+// (sums of R^2, L-inf value and index, matrix-residual sum), with the
+// PrintResiduals normalisation (src/output.cpp:1028-1063) advanced for outer
+// step nn, nonlinear iteration mm
+void NormsFrom(CaseRes &cr, const ath::Params &P, const double h[8], int nn, int mm, double cfl,
+               ath_norms *out) {
   ath_norms n{};
   n.num_equations = 5;
   for (int e = 0; e < 5; ++e) {
@@ -1087,7 +1255,7 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
     const long long cell = idx / 5;
     n.linf_eqn = (int)(idx % 5) + 1;
     long long base = 0;
-    for (const auto &br : c->blk) {
+    for (const auto &br : cr.blocks) {
       if (cell < base + br.d.ncell) {
         const long long p = cell - base;
         n.linf_block = br.d.blockId;
@@ -1103,30 +1271,52 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
   // (src/mgSolution.cpp:197-206); main divides by cells * neq
   // (src/main.cpp:269-270)
   double padded = 0;
-  for (const auto &br : c->blk) padded += (double)br.d.npad * 5;
-  const double cyc = c->P.implicit ? h[7] / padded : 0.0;
-  n.matrix_resid = std::sqrt(cyc / (c->hc.totalCells * 5));
-  n.cfl = c->cfl;
+  for (const auto &br : cr.blocks) padded += (double)br.d.npad * 5;
+  const double cyc = P.implicit ? h[7] / padded : 0.0;
+  n.matrix_resid = std::sqrt(cyc / (cr.hc->totalCells * 5));
+  n.cfl = cfl;
   // PrintResiduals (src/output.cpp:1028-1063)
   constexpr double eps = 1.0e-30;
   if (nn == 0 && mm == 0) {
-    for (int e = 0; e < 5; ++e) c->l2first[e] = n.l2[e];
+    for (int e = 0; e < 5; ++e) cr.l2first[e] = n.l2[e];
   } else if (nn < 5 && mm == 0) {
-    if (n.l2[0] > c->l2first[0]) c->l2first[0] = n.l2[0];
-    for (int e = 1; e < 5; ++e) c->l2first[e] = std::max(c->l2first[e], n.l2[e]);
+    if (n.l2[0] > cr.l2first[0]) cr.l2first[0] = n.l2[0];
+    for (int e = 1; e < 5; ++e) cr.l2first[e] = std::max(cr.l2first[e], n.l2[e]);
   }
-  for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (c->l2first[e] + eps);
+  for (int e = 0; e < 5; ++e) n.normalized[e] = (n.l2[e] + eps) / (cr.l2first[e] + eps);
   if (out) *out = n;
-  // a device error is reported once: the flag is cleared so that the
-  // context can go on (for example after ath_set_state restores a state)
-  if (flag) {
-    const int zero = 0;
-    HIP_OK(hipMemcpy(c->dflag, &zero, sizeof zero, hipMemcpyHostToDevice));
-  }
+}
+
+// the device error bits of case `cr` and of the context (spin limits),
+// read and cleared: a device error is reported once, so that the context
+// can go on (for example after ath_set_state restores a state)
+int TakeFlags(ath_ctx *c, CaseRes &cr, int *flag) {
+  int fc = 0, fg = 0;
+  HIP_OK(hipMemcpy(&fc, cr.dflag, sizeof fc, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&fg, c->dflag, sizeof fg, hipMemcpyDeviceToHost));
+  const int zero = 0;
+  if (fc) HIP_OK(hipMemcpy(cr.dflag, &zero, sizeof zero, hipMemcpyHostToDevice));
+  if (fg) HIP_OK(hipMemcpy(c->dflag, &zero, sizeof zero, hipMemcpyHostToDevice));
+  *flag = fc | fg;
+  return ATH_OK;
+}
+
+int FlagStatus(int flag) {
   if (flag & 2) return Fail(ATH_ERR_DEVICE, "a sweep dependency wait exceeded its spin limit");
   if (flag & 4) return Fail(ATH_ERR_DEVICE, "singular block main diagonal in Gauss-Jordan elimination");
   if (flag & 1) return Fail(ATH_ERR_NONPHYSICAL, "nonphysical state (rho <= 0, p <= 0 or NaN) after update");
   return ATH_OK;
+}
+
+// single-case context: read norms slot `slot` and fill `out`
+int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
+  CaseRes &cr = *c->cases[0];
+  double h[8];
+  HIP_OK(hipMemcpy(h, cr.dhist + 8 * (size_t)slot, sizeof h, hipMemcpyDeviceToHost));
+  NormsFrom(cr, c->P, h, nn, mm, cr.cfl, out);
+  int flag = 0;
+  if (int r = TakeFlags(c, cr, &flag)) return r;
+  return FlagStatus(flag);
 }
 
 int BuildGraph(ath_ctx *c) {
@@ -1167,8 +1357,6 @@ int ath_open(const char *inp_path, int device, ath_ctx **out) {
   HIP_OK(hipSetDevice(device));
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   if (int r = Upload(c.get())) return r;
-  c->cfl = c->hc.Cfl(0);
-  HIP_OK(hipMemcpy(c->dcfl, &c->cfl, sizeof(double), hipMemcpyHostToDevice));
   HIP_OK(hipDeviceSynchronize());
   *out = c.release();
   return ATH_OK;
@@ -1261,15 +1449,15 @@ int ath_set_state(ath_ctx *c, int b, const double *prim) {
 int ath_begin_step(ath_ctx *c, int nn) {
   if (!c) return Fail(ATH_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
-  c->cfl = c->hc.Cfl(nn);
-  hipLaunchKernelGGL(ath::KSetScalar, dim3(1), dim3(1), 0, c->stream, c->dcfl, c->cfl);
+  CaseRes &cr = *c->cases[0];
+  cr.cfl = c->hc.Cfl(nn);
+  hipLaunchKernelGGL(ath::KSetScalar, dim3(1), dim3(1), 0, c->stream, cr.dcfl, cr.cfl);
   // StoreOldSolution only when the scheme needs U^n
   // (input::NeedToStoreTimeN, include/input.hpp:174-176)
   if (c->P.implicit) {
     const int copyNm1 = (c->P.multilevel && nn == 0) ? 1 : 0;
-    for (auto &br : c->blk)
-      hipLaunchKernelGGL(ath::KBeginStep, dim3(Grid(br.d.ncell)), dim3(kBlk), 0, c->stream, br.d, c->P,
-                         copyNm1);
+    hipLaunchKernelGGL(ath::KBeginStep, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                       c->Map(kMapCell256), c->P, copyNm1);
   }
   return CheckLaunch();
 }
@@ -1314,10 +1502,11 @@ int ath_relax(ath_ctx *c, double *cycle_value) {
 
 static int ResetCounterIfFull(ath_ctx *c, int need) {
   int slot = 0;
-  HIP_OK(hipMemcpyAsync(&slot, c->dcounter, sizeof slot, hipMemcpyDeviceToHost, c->stream));
+  CaseRes &cr = *c->cases[0];
+  HIP_OK(hipMemcpyAsync(&slot, cr.dcounter, sizeof slot, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  if (slot + need > c->histCap) {
-    hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+  if (slot + need > cr.histCap) {
+    hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, cr.dcounter, 0);
     slot = 0;
   }
   return slot;
@@ -1336,9 +1525,10 @@ int ath_update(ath_ctx *c, int mm, ath_norms *norms) {
   // raw norms; the first-iteration normalisation state is only advanced by
   // ath_iterate / ath_run, which know the outer step
   ath_norms n{};
-  const double keep[5] = {c->l2first[0], c->l2first[1], c->l2first[2], c->l2first[3], c->l2first[4]};
+  double *l2first = c->cases[0]->l2first;
+  const double keep[5] = {l2first[0], l2first[1], l2first[2], l2first[3], l2first[4]};
   const int r = ReadNorms(c, slot, 1 << 30, mm, &n);
-  std::memcpy(c->l2first, keep, sizeof keep);
+  std::memcpy(l2first, keep, sizeof keep);
   if (norms) *norms = n;
   return r;
 }
@@ -1365,6 +1555,7 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   if (!c || nsteps < 0) return Fail(ATH_ERR_ARG, "bad argument");
   HIP_OK(hipSetDevice(c->device));
   const int nl = c->hc.nonlinear;
+  CaseRes &cr = *c->cases[0];
   const long iters = (long)nsteps * nl;
   // nsteps == 0 only captures and instantiates the iteration's graph
   if (int r = BuildGraph(c)) return r;
@@ -1383,7 +1574,7 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
       c->evs.push_back(e);
     }
   }
-  hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+  hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, cr.dcounter, 0);
   if (c->timing) HIP_OK(hipEventRecord(c->evs[0], c->stream));
   long it = 0;
   for (int s = 0; s < nsteps; ++s) {
@@ -1401,13 +1592,13 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
       if (c->P.multilevel && mm == nl - 1) LaunchCopyNm1(c);
       if (int r = CheckLaunch()) return r;
       // the norm history holds histCap iterations; drain it when full
-      if ((it + 1) % c->histCap == 0 && it + 1 < iters) {
+      if ((it + 1) % cr.histCap == 0 && it + 1 < iters) {
         HIP_OK(hipStreamSynchronize(c->stream));
-        for (long q = it + 1 - c->histCap; q <= it; ++q) {
+        for (long q = it + 1 - cr.histCap; q <= it; ++q) {
           const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
-          if (qnn < 5 && qmm == 0) ReadNorms(c, (int)(q % c->histCap), qnn, qmm, nullptr);
+          if (qnn < 5 && qmm == 0) ReadNorms(c, (int)(q % cr.histCap), qnn, qmm, nullptr);
         }
-        hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, c->dcounter, 0);
+        hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, cr.dcounter, 0);
       }
     }
   }
@@ -1415,14 +1606,14 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   HIP_OK(hipStreamSynchronize(c->stream));
   // advance the PrintResiduals normalisation through the iterations it
   // depends on, then report the last one
-  const long first = (iters - 1) / c->histCap * c->histCap;
+  const long first = (iters - 1) / cr.histCap * cr.histCap;
   int r = ATH_OK;
   for (long q = first; q < iters; ++q) {
     const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
     ath_norms n{};
     const bool want = (q == iters - 1) || (qnn < 5 && qmm == 0);
     if (!want) continue;
-    r = ReadNorms(c, (int)(q % c->histCap), qnn, qmm, &n);
+    r = ReadNorms(c, (int)(q % cr.histCap), qnn, qmm, &n);
     if (q == iters - 1 && last) *last = n;
     if (r != ATH_OK) break;
   }
@@ -1604,6 +1795,354 @@ int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n, const d
   hipFree(buf);
   if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("eval_face_flux: ") + hipGetErrorString(e));
   return ATH_OK;
+}
+
+
+}  // extern "C"
+
+// ---- serving --------------------------------------------------------------
+
+struct ath_server {
+  std::unique_ptr<ath_ctx> c;
+  int precision = ATH_IMPLICIT_FP64;
+  int maxSlots = 0;
+  double budget = 0;  // bytes the resident cases may hold (0: no limit but free HBM)
+  bool haveNumerics = false;
+  bool dirty = true;  // the set of running cases changed since the group tables were built
+  int nextId = 0;
+  double used = 0;    // device bytes held by the resident cases
+};
+
+namespace {
+
+// device bytes a case will hold (an upper bound of what UploadCase
+// allocates: state, geometry, residual and operator arrays, the block
+// operator and viscous arrays when selected, sweep tables, norm history)
+double EstimateBytes(const ath::HostCase &hc, bool fp32) {
+  double t = 4096.0 * 64 + 4096;  // norm history, scalars, states
+  for (const auto &h : hc.blocks) {
+    const double np = (double)h.NumPadded(), nc = (double)h.NumCells();
+    double nf = 0, nfp = 0;
+    for (int d = 0; d < 3; ++d) {
+      nf += (double)(h.ni + (d == 0)) * (h.nj + (d == 1)) * (h.nk + (d == 2));
+      nfp += (double)h.NumFaces(d);
+    }
+    const double tiles = (double)((h.ni + 7) / 8) * ((h.nj + 7) / 8) * ((h.nk + 3) / 4);
+    const double nbf = 2.0 * ((double)h.nj * h.nk + (double)h.nk * h.ni + (double)h.ni * h.nj);
+    t += np * 8 * 8 + nf * 32 + nc * 8 * 35 + nc * 4 + nbf * 80 + tiles * 32 + 8192;
+    if (fp32) t += nc * 4 * 11 + nbf * 20;
+    if (hc.block) t += nc * 25 * 8 + tiles * 256 * 25 * 8;
+    if (hc.viscous) t += np * 8 * 3 + nfp * 32 + nf * 8 + nbf + (double)h.g * h.g * 4 * (h.ni + h.nj + h.nk) * 2 * 64 +
+                         nbf * 40.0 * h.g;
+    if (hc.block && hc.viscous) t += nf * 9 * 8 + np * 10 * 8;
+  }
+  return t * 1.05 + (1 << 20);
+}
+
+// the numerics every case of a server shares: the kernel parameters plus
+// the choices the launches dispatch on
+bool SameNumerics(const ath::HostCase &a, const ath::HostCase &b) {
+  ath::Params pa, pb;
+  FillParams(a, &pa);
+  FillParams(b, &pb);
+  return std::memcmp(&pa, &pb, sizeof pa) == 0 && a.recon == b.recon && a.limiter == b.limiter &&
+         a.flux == b.flux && a.timeint == b.timeint && a.solver == b.solver && a.sweeps == b.sweeps &&
+         a.nonlinear == b.nonlinear && a.block == b.block && a.viscous == b.viscous && a.roeJac == b.roeJac;
+}
+
+// keep what a resident case still needs on the host (dimensions, patches,
+// boundary states); drop the geometry and the initial state
+void DropGeometry(ath::HostCase &hc) {
+  for (auto &h : hc.blocks) {
+    std::vector<double>().swap(h.vol);
+    std::vector<ath::Vec3>().swap(h.ctr);
+    for (int d = 0; d < 3; ++d) {
+      std::vector<std::array<double, 4>>().swap(h.fa[d]);
+      std::vector<ath::Vec3>().swap(h.fc[d]);
+      std::vector<double>().swap(h.cw[d]);
+    }
+    std::vector<double>().swap(h.W0);
+  }
+}
+
+CaseRes *FindCase(ath_server *s, int id) {
+  for (auto &cp : s->c->cases)
+    if (cp->id == id) return cp.get();
+  return nullptr;
+}
+
+// the residual drop of a case: the largest normalised L2 over the equations
+// whose first residual carries a signal (an equation whose first L2 is below
+// 1e-10 of the largest, such as the spanwise momentum of a flow without
+// spanwise gradients, holds round-off only and is left out)
+double Drop(const CaseRes &cr, const ath_norms &n) {
+  double big = 0;
+  for (int e = 0; e < 5; ++e) big = std::max(big, cr.l2first[e]);
+  double d = 0;
+  for (int e = 0; e < 5; ++e)
+    if (cr.l2first[e] >= 1e-10 * big) d = std::max(d, n.normalized[e]);
+  return d;
+}
+
+// one nonlinear iteration of every running case, each at its own outer
+// step (KCaseCfl advances it): StoreOldSolution, ghost cells, residual,
+// relax, update and the per-case norm reductions
+void LaunchServeIteration(ath_ctx *c) {
+  const int na = (int)c->active.size();
+  hipLaunchKernelGGL(ath::KCaseCfl, dim3(Grid(na, 64)), dim3(64), 0, c->stream, c->dcflc, na);
+  if (c->P.implicit)
+    hipLaunchKernelGGL(ath::KBeginStep, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                       c->Map(kMapCell256), c->P, 0);
+  LaunchBoundary(c);
+  LaunchResidual(c);
+  LaunchAfterResiduals(c);
+  LaunchTail(c, false, 0);
+}
+
+int BuildServeGraph(ath_ctx *c) {
+  if (c->graphExec) return ATH_OK;
+  HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  LaunchServeIteration(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIP_OK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+  return ATH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ath_server_open(int device, double hbm_budget, int max_slots, int precision, ath_server **out) {
+  if (!out || max_slots <= 0 || hbm_budget < 0 ||
+      (precision != ATH_IMPLICIT_FP64 && precision != ATH_IMPLICIT_FP32))
+    return Fail(ATH_ERR_ARG, "bad argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return Fail(ATH_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return Fail(ATH_ERR_ARG, "device index out of range");
+  auto s = std::make_unique<ath_server>();
+  s->c = std::make_unique<ath_ctx>();
+  ath_ctx *c = s->c.get();
+  c->device = device;
+  HIP_OK(hipSetDevice(device));
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (int r = c->Alloc(&c->dflag, 1)) return r;
+  HIP_OK(hipMemset(c->dflag, 0, sizeof(int)));
+  c->fp32 = precision == ATH_IMPLICIT_FP32 ? 1 : 0;
+  s->precision = precision;
+  s->maxSlots = max_slots;
+  s->budget = hbm_budget;
+  *out = s.release();
+  return ATH_OK;
+}
+
+void ath_server_close(ath_server *s) {
+  if (!s) return;
+  hipSetDevice(s->c->device);
+  if (s->c->stream) hipStreamSynchronize(s->c->stream);
+  delete s;
+}
+
+int ath_server_memory(const ath_server *s, double *used, double *free_hbm) {
+  if (!s) return Fail(ATH_ERR_ARG, "null server");
+  HIP_OK(hipSetDevice(s->c->device));
+  size_t fr = 0, tot = 0;
+  HIP_OK(hipMemGetInfo(&fr, &tot));
+  if (used) *used = s->used;
+  if (free_hbm) *free_hbm = (double)fr;
+  return ATH_OK;
+}
+
+int ath_admit(ath_server *s, const char *inp_path, double residual_drop, int max_iterations, int *slot) {
+  if (!s || !inp_path || !slot || residual_drop <= 0 || max_iterations <= 0)
+    return Fail(ATH_ERR_ARG, "bad argument");
+  *slot = -1;
+  ath_ctx *c = s->c.get();
+  HIP_OK(hipSetDevice(c->device));
+  auto cr = std::make_unique<CaseRes>();
+  std::string err = ath::LoadCase(inp_path, &cr->own);
+  if (!err.empty()) return Fail(err.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, err);
+  const ath::HostCase &hc = cr->own;
+  if (!hc.Implicit() || hc.nonlinear != 1 || hc.MultiLevel() || hc.dtIn > 0.0)
+    return Fail(ATH_ERR_CONFIG, "a served case must be a steady solve: implicit local time stepping, one "
+                                "nonlinear iteration per step");
+  if (s->haveNumerics && !SameNumerics(c->hc, hc))
+    return Fail(ATH_ERR_CONFIG, "the case's numerics differ from those of the resident cases");
+  int running = 0;
+  for (auto &cp : c->cases) running += cp->state == ATH_SLOT_RUNNING;
+  if (running >= s->maxSlots) return Fail(ATH_ERR_NOMEM, "no free slot");
+  const bool fp32 = s->precision == ATH_IMPLICIT_FP32;
+  const double need = EstimateBytes(hc, fp32);
+  size_t fr = 0, tot = 0;
+  HIP_OK(hipMemGetInfo(&fr, &tot));
+  const double reserve = 2.0 * (1 << 30);
+  if (need + reserve > (double)fr) return Fail(ATH_ERR_NOMEM, "not enough free device memory for the case");
+  if (s->budget > 0 && s->used + need > s->budget)
+    return Fail(ATH_ERR_NOMEM, "the case does not fit in the remaining memory budget");
+  if (!s->haveNumerics) {
+    // the shared numerics (no geometry)
+    c->hc = hc;
+    DropGeometry(c->hc);
+    c->hc.blocks.clear();
+    c->hc.conns.clear();
+    FillParams(c->hc, &c->P);
+    c->P.fp32 = fp32 ? 1 : 0;
+    s->haveNumerics = true;
+  }
+  cr->hc = &cr->own;
+  if (int r = UploadCase(c, *cr, fp32)) return r;
+  HIP_OK(hipDeviceSynchronize());
+  DropGeometry(cr->own);
+  cr->id = s->nextId++;
+  cr->state = ATH_SLOT_RUNNING;
+  cr->drop = residual_drop;
+  cr->maxIters = max_iterations;
+  s->used += (double)cr->bytes;
+  *slot = cr->id;
+  c->cases.push_back(std::move(cr));
+  s->dirty = true;
+  return ATH_OK;
+}
+
+int ath_step_batch(ath_server *s, int iterations, int *running) {
+  if (!s || iterations <= 0) return Fail(ATH_ERR_ARG, "bad argument");
+  ath_ctx *c = s->c.get();
+  HIP_OK(hipSetDevice(c->device));
+  if (s->dirty) {
+    c->active.clear();
+    for (auto &cp : c->cases)
+      if (cp->state == ATH_SLOT_RUNNING) c->active.push_back(cp.get());
+    if (int r = RebuildGroup(c)) return r;
+    s->dirty = false;
+  }
+  if (running) *running = (int)c->active.size();
+  if (c->active.empty()) return ATH_OK;
+  int cap = 1 << 30;
+  for (CaseRes *cr : c->active) cap = std::min(cap, cr->histCap);
+  if (iterations > cap) return Fail(ATH_ERR_ARG, "more iterations per batch step than the norm history holds");
+  if (int r = BuildServeGraph(c)) return r;
+  const int na = (int)c->active.size();
+  hipLaunchKernelGGL(ath::KZeroCounters, dim3(Grid(na, 64)), dim3(64), 0, c->stream, c->dnorm, na);
+  for (int it = 0; it < iterations; ++it) HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
+  if (int r = CheckLaunch()) return r;
+  HIP_OK(hipStreamSynchronize(c->stream));
+  int left = 0;
+  int status = ATH_OK;
+  std::vector<double> h(8 * (size_t)iterations);
+  for (CaseRes *cr : c->active) {
+    HIP_OK(hipMemcpy(h.data(), cr->dhist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int flag = 0;
+    if (int r = TakeFlags(c, *cr, &flag)) return r;
+    for (int q = 0; q < iterations; ++q) {
+      const int nn = cr->iters + q;
+      NormsFrom(*cr, c->P, &h[8 * (size_t)q], nn, 0, cr->hc->Cfl(nn), &cr->last);
+    }
+    cr->iters += iterations;
+    const double d = Drop(*cr, cr->last);
+    if (flag & 2) status = FlagStatus(flag);  // a spin limit: the launch itself failed
+    if (flag & 5) {
+      cr->state = ATH_SLOT_FAILED;
+      cr->error = (flag & 1) ? ATH_ERR_NONPHYSICAL : ATH_ERR_DEVICE;
+    } else if (!(d == d)) {
+      cr->state = ATH_SLOT_FAILED;
+      cr->error = ATH_ERR_NONPHYSICAL;
+    } else if (d <= cr->drop) {
+      cr->state = ATH_SLOT_CONVERGED;
+    } else if (cr->iters >= cr->maxIters) {
+      cr->state = ATH_SLOT_LIMIT;
+    }
+    if (cr->state != ATH_SLOT_RUNNING) s->dirty = true;
+    else ++left;
+  }
+  if (running) *running = left;
+  return status;
+}
+
+int ath_get_slot_info(ath_server *s, int slot, ath_slot_info *out) {
+  if (!s || !out) return Fail(ATH_ERR_ARG, "null argument");
+  CaseRes *cr = FindCase(s, slot);
+  if (!cr) return Fail(ATH_ERR_ARG, "no such slot");
+  ath_slot_info o{};
+  o.state = cr->state;
+  o.error = cr->error;
+  o.iterations = cr->iters;
+  o.num_blocks = (int)cr->blocks.size();
+  o.cells = cr->hc->totalCells;
+  o.device_bytes = (double)cr->bytes;
+  o.drop = cr->iters > 0 ? Drop(*cr, cr->last) : 1.0;
+  o.last = cr->last;
+  *out = o;
+  return ATH_OK;
+}
+
+static int SlotState(ath_server *s, int slot, int b, double *prim, bool toHost) {
+  if (!s || !prim) return Fail(ATH_ERR_ARG, "null argument");
+  CaseRes *cr = FindCase(s, slot);
+  if (!cr) return Fail(ATH_ERR_ARG, "no such slot");
+  if (b < 0 || b >= (int)cr->blocks.size()) return Fail(ATH_ERR_ARG, "bad block index");
+  ath_ctx *c = s->c.get();
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const auto &d = cr->blocks[b].d;
+  std::vector<double> W(5 * (size_t)d.npad);
+  HIP_OK(hipMemcpy(W.data(), d.W, W.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < d.nk; ++k)
+    for (int j = 0; j < d.nj; ++j)
+      for (int i = 0; i < d.ni; ++i) {
+        const long p = i + (long)j * d.ni + (long)k * d.ni * d.nj;
+        const long q = (i + d.g) + (long)(j + d.g) * d.NI + (long)(k + d.g) * d.NI * d.NJ;
+        for (int e = 0; e < 5; ++e) {
+          if (toHost) prim[5 * p + e] = W[e * d.npad + q];
+          else W[e * d.npad + q] = prim[5 * p + e];
+        }
+      }
+  if (!toHost) HIP_OK(hipMemcpy(d.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+  return ATH_OK;
+}
+
+int ath_slot_block_dims(ath_server *s, int slot, int block, int *ni, int *nj, int *nk) {
+  if (!s || !ni || !nj || !nk) return Fail(ATH_ERR_ARG, "null argument");
+  CaseRes *cr = FindCase(s, slot);
+  if (!cr) return Fail(ATH_ERR_ARG, "no such slot");
+  if (block < 0 || block >= (int)cr->blocks.size()) return Fail(ATH_ERR_ARG, "bad block index");
+  *ni = cr->blocks[block].d.ni;
+  *nj = cr->blocks[block].d.nj;
+  *nk = cr->blocks[block].d.nk;
+  return ATH_OK;
+}
+
+int ath_get_slot_state(ath_server *s, int slot, int block, double *prim) {
+  return SlotState(s, slot, block, prim, true);
+}
+
+int ath_set_slot_state(ath_server *s, int slot, int block, const double *prim) {
+  return SlotState(s, slot, block, const_cast<double *>(prim), false);
+}
+
+int ath_retire(ath_server *s, int slot) {
+  if (!s) return Fail(ATH_ERR_ARG, "null server");
+  ath_ctx *c = s->c.get();
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (size_t q = 0; q < c->cases.size(); ++q)
+    if (c->cases[q]->id == slot) {
+      const bool wasActive = std::find(c->active.begin(), c->active.end(), c->cases[q].get()) != c->active.end();
+      s->used -= (double)c->cases[q]->bytes;
+      if (wasActive) {
+        // the group tables point at the case's memory: rebuild them before
+        // anything launches again
+        c->active.erase(std::find(c->active.begin(), c->active.end(), c->cases[q].get()));
+        s->dirty = true;
+        c->DropGraph();
+      }
+      c->cases.erase(c->cases.begin() + q);
+      if (c->cases.empty()) s->haveNumerics = false;
+      return ATH_OK;
+    }
+  return Fail(ATH_ERR_ARG, "no such slot");
 }
 
 }  // extern "C"

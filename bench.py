@@ -30,6 +30,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -72,7 +74,7 @@ CHANNEL_FLUX_TRAFFIC = 3372.3e6                       # residual stage over all 
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["channel", "vortex"], default="channel")
+    p.add_argument("--workload", choices=["channel", "vortex", "serve"], default="channel")
     p.add_argument("--steps", type=int, default=None,
                    help="timed nonlinear iterations (channel: a multiple of 20; default 1000, vortex 200)")
     p.add_argument("--warmup", type=int, default=None, help="untimed nonlinear iterations (default 20 / 10)")
@@ -86,6 +88,18 @@ def parse():
     p.add_argument("--no-fp32-leg", action="store_true")
     p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
                    help="scalar LU-SGS execution order (all give identical results)")
+    p.add_argument("--requests", type=int, default=16, help="serve: requests (the full stream is 256)")
+    p.add_argument("--nmin", type=int, default=24, help="serve: smallest cells per direction")
+    p.add_argument("--nmax", type=int, default=48, help="serve: largest cells per direction (full: 96)")
+    p.add_argument("--seed", type=int, default=1, help="serve: request and arrival seed")
+    p.add_argument("--serve-cfl", type=float, default=20.0)
+    p.add_argument("--serve-step", type=int, default=20, help="serve: iterations per batch step / chunk")
+    p.add_argument("--serve-load", type=float, default=0.8,
+                   help="serve: offered load as a fraction of the per-case path's capacity")
+    p.add_argument("--max-iterations", type=int, default=20000, help="serve: iteration cap per request")
+    p.add_argument("--max-slots", type=int, default=256, help="serve: cases running at once")
+    p.add_argument("--no-serve", action="store_true", help="channel: leave out the serve block")
+    p.add_argument("--serve-json", default=None, help="serve: write per-request records to this file")
     p.add_argument("--hbm-calibration", action="store_true",
                    help="only run a device-to-device copy of known size (to scale HBM counters)")
     return p.parse_args()
@@ -293,6 +307,96 @@ def bench_channel(args, Solver, cases):
     return out
 
 
+def serve_requests(cases, work, n, nmin, nmax, seed, cfl):
+    """n seeded bump-channel requests (sizes uniform in nmin^3..nmax^3, bump
+    height/length uniform in [0.02, 0.10], Mach uniform in [0.2, 0.8]; Roe,
+    MUSCL kappa = 1/3 with the minmod limiter, LU-SGS, constant CFL)"""
+    reqs = []
+    rng = np.random.default_rng(seed)
+    for q in range(n):
+        size = int(rng.integers(nmin, nmax + 1))
+        path, meta = cases.bump_case(os.path.join(work, f"r{q}"), seed=seed * 100000 + q, n=size,
+                                     iterations=1, cfl=cfl, limiter="minmod")
+        reqs.append({"path": path, "cells": size ** 3, "n": size, "mach": meta["mach"],
+                     "h_over_l": meta["h_over_l"]})
+    return reqs
+
+
+def bench_serve(args, Solver, Server, cases_mod):
+    """C4: steady Euler bump channels as a Poisson request stream, served by
+    continuous batching, with the plain per-case GPU path beside it."""
+    from aither_amd import serve
+    work = tempfile.mkdtemp(prefix="aither_serve_")
+
+    def progress(msg):
+        print(f"[serve] {msg}", file=sys.stderr, flush=True)
+
+    reqs = serve_requests(cases_mod, work, args.requests, args.nmin, args.nmax, args.seed, args.serve_cfl)
+    drop = 1e-8
+    step = args.serve_step
+    # plain per-case path, back to back: service times
+    t0 = time.perf_counter()
+    service = serve.per_case_service(Solver, reqs, step_iters=step, drop=drop, max_iterations=args.max_iterations,
+                                     precision=Solver.FP32, progress=progress)
+    plain_wall = time.perf_counter() - t0
+    total_service = sum(r["service_s"] for r in service)
+    rate = args.serve_load * len(reqs) / total_service
+    arrivals = serve.poisson_arrivals(len(reqs), rate, args.seed + 1)
+    plain = serve.per_case_queue(service, arrivals)
+    srv = Server(max_slots=args.max_slots, precision=Solver.FP32)
+    batched = serve.serve(srv, reqs, arrivals, step_iters=step, drop=drop, max_iterations=args.max_iterations,
+                          progress=progress)
+    srv.close()
+    # the same result alone and in the mix: a served case's iteration count
+    # equals its per-case count (same arithmetic, same stopping rule)
+    same_iters = sum(a["iterations"] == b["iterations"] for a, b in zip(batched["records"], service))
+
+    def summary(o):
+        return {k: o[k] for k in ("requests", "converged", "failed", "limit", "p50_s", "p99_s", "max_s",
+                                  "cell_updates", "makespan_s", "busy_s", "cell_updates_per_s",
+                                  "busy_cell_updates_per_s")}
+
+    out = {
+        "workload": f"{len(reqs)} steady Euler bump channels, {args.nmin}^3..{args.nmax}^3 cells, Mach 0.2-0.8, "
+                    f"bump h/l 0.02-0.10, Roe + MUSCL (minmod), LU-SGS FP32 operator, CFL {args.serve_cfl:g}, "
+                    f"to a 1e-8 residual drop (cap {args.max_iterations} iterations), Poisson arrivals at "
+                    f"{rate:.3g}/s (offered load {args.serve_load:g} of the per-case path)",
+        "step_iterations": step,
+        "serve": summary(batched),
+        "per_case": summary(plain),
+        "per_case_back_to_back_s": plain_wall,
+        "iterations_equal_per_case": same_iters,
+        "batch_steps": batched["batch_steps"], "peak_running": batched["peak_running"],
+    }
+    return out, batched, plain
+
+
+def serve_line(args, Solver, Server, cases):
+    """the serve workload as the headline: aggregate cell-updates/s of the
+    batched server over the makespan of the request stream"""
+    block, batched, plain = bench_serve(args, Solver, Server, cases)
+    out = {
+        "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle; serve p50/p99 time-to-converge",
+        "value": batched["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": batched["batch_steps"] * args.serve_step,
+        "warmup": 0,
+        "ms_per_step": batched["makespan_s"] / max(1, batched["batch_steps"] * args.serve_step) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64-residual/f32-implicit",
+        "data": "synthetic (seeded bump channels and Poisson arrivals)",
+        "config": {"workload": block["workload"], "parallelism": "single GPU"},
+        "serve": block,
+    }
+    if args.serve_json:
+        with open(args.serve_json, "w") as f:
+            json.dump({"serve": batched["records"], "per_case": plain["records"]}, f)
+    return out
+
+
 def bench_vortex(args, Solver, cases):
     steps = args.steps if args.steps is not None else 200
     warm = args.warmup if args.warmup is not None else 10
@@ -384,8 +488,15 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
-    from aither_amd import Solver, cases
-    out = bench_channel(args, Solver, cases) if args.workload == "channel" else bench_vortex(args, Solver, cases)
+    from aither_amd import Server, Solver, cases
+    if args.workload == "serve":
+        out = serve_line(args, Solver, Server, cases)
+    elif args.workload == "channel":
+        out = bench_channel(args, Solver, cases)
+        if not args.no_serve:
+            out["serve"], _, _ = bench_serve(args, Solver, Server, cases)
+    else:
+        out = bench_vortex(args, Solver, cases)
     print(json.dumps(out), flush=True)
 
 

@@ -192,6 +192,68 @@ int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n,
                        const double *WL, const double *WR, const double *n,
                        int count, double *F);
 
+/* ---- serving: many steady cases resident on one device ------------------
+ * No reference counterpart: the reference runs one case per process
+ * (src/main.cpp:231-302).  A server keeps every admitted case's state,
+ * geometry and operator on the device and advances all running cases
+ * together: one launch of each kernel covers the blocks of every running
+ * case (the per-block unit grids of the single-case launches, concatenated
+ * across cases), each case at its own outer step and CFL, each reduced to
+ * its own norms by a tree that depends only on that case.  A case's results
+ * are the same alone and in any mix of cases.  Every case of a server has
+ * the same numerics (schemes, solver, gas and the implicit-operator
+ * precision); its grid, boundary states and initial state are its own.
+ * Cases are steady solves: implicit local time stepping, one nonlinear
+ * iteration per outer step. */
+typedef struct ath_server ath_server;
+
+#define ATH_SLOT_RUNNING 0
+#define ATH_SLOT_CONVERGED 1  /* the residual drop was reached */
+#define ATH_SLOT_FAILED 2     /* `error` holds the ATH_ERR_* status */
+#define ATH_SLOT_LIMIT 3      /* max_iterations done without converging */
+
+typedef struct {
+  int state;            /* ATH_SLOT_* */
+  int error;            /* ATH_OK, or why the case failed */
+  int iterations;       /* nonlinear iterations done */
+  int num_blocks;
+  double cells;         /* physical cells */
+  double device_bytes;  /* device memory the case holds */
+  double drop;          /* residual drop reached: the largest normalised L2
+                           (PrintResiduals, src/output.cpp:1028-1063) over the
+                           equations whose first residual carries a signal
+                           (first L2 >= 1e-10 of the largest) */
+  ath_norms last;       /* norms of the last iteration */
+} ath_slot_info;
+
+/* `hbm_budget`: device bytes the resident cases may hold together (0: no
+ * limit beyond the free device memory less a 2 GiB reserve); `max_slots`:
+ * cases that may run at once; `precision`: ATH_IMPLICIT_FP64/FP32 for every
+ * case. */
+int ath_server_open(int device, double hbm_budget, int max_slots, int precision, ath_server **out);
+void ath_server_close(ath_server *srv);
+/* device bytes the resident cases hold, and the device's free memory */
+int ath_server_memory(const ath_server *srv, double *used, double *free_hbm);
+/* Read the case (as ath_open does) and upload it if its estimated device
+ * memory fits the free memory and the budget and a slot is free; else
+ * ATH_ERR_NOMEM (nothing changed: try again after a retire).  The case runs
+ * until its residual drop is <= `residual_drop` (checked after every batch
+ * step) or `max_iterations` are done.  *slot identifies it from then on. */
+int ath_admit(ath_server *srv, const char *inp_path, double residual_drop, int max_iterations, int *slot);
+/* `iterations` nonlinear iterations of every running case, launched
+ * together and replayed from a captured HIP graph; then each case's norms
+ * are read and its state updated (converged, failed, limit).  A case that
+ * fails (nonphysical state, singular block diagonal) is stopped alone; the
+ * others go on.  *running: cases still running. */
+int ath_step_batch(ath_server *srv, int iterations, int *running);
+int ath_get_slot_info(ath_server *srv, int slot, ath_slot_info *out);
+int ath_slot_block_dims(ath_server *srv, int slot, int block, int *ni, int *nj, int *nk);
+/* physical-cell primitive state of block `block` of a case (as ath_get_state) */
+int ath_get_slot_state(ath_server *srv, int slot, int block, double *prim);
+int ath_set_slot_state(ath_server *srv, int slot, int block, const double *prim);
+/* free the case's device memory and its slot */
+int ath_retire(ath_server *srv, int slot);
+
 #ifdef __cplusplus
 }
 #endif
