@@ -121,9 +121,15 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_long)],
     "ath_eval_face_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                            ctypes.c_int, _D],
+    "ath_write_restart": [_P, ctypes.c_char_p, ctypes.c_int],
+    "ath_read_restart": [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
+    "ath_write_output": [_P, ctypes.c_char_p, ctypes.c_int],
+    "ath_open_logs": [_P, ctypes.c_char_p, ctypes.c_int],
+    "ath_close_logs": [_P],
     "ath_server_open": [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)],
     "ath_server_close": [_P],
     "ath_server_memory": [_P, _D, _D],
+    "ath_estimate_case_bytes": [ctypes.c_char_p, ctypes.c_int, _D],
     "ath_admit": [_P, ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     "ath_step_batch": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     "ath_get_slot_info": [_P, ctypes.c_int, ctypes.POINTER(SlotInfo)],
@@ -271,6 +277,29 @@ class Solver:
         W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1)
         _check(self._lib.ath_set_state(self._h, b, _ptr(W)))
 
+    # --- files
+    def write_restart(self, path, iteration):
+        """the reference's binary restart file (WriteRestart)"""
+        _check(self._lib.ath_write_restart(self._h, os.fsencode(path), int(iteration)))
+
+    def read_restart(self, path):
+        """state and residual normalisation from a restart file; returns the
+        file's iteration"""
+        it = ctypes.c_int()
+        _check(self._lib.ath_read_restart(self._h, os.fsencode(path), ctypes.byref(it)))
+        return it.value
+
+    def write_output(self, directory, iteration):
+        """cell-centre grid, function file and meta file (WriteOutput)"""
+        _check(self._lib.ath_write_output(self._h, os.fsencode(directory), int(iteration)))
+
+    def open_logs(self, directory, append=False):
+        """<simName>.resid and <simName>.tme, a line per iteration"""
+        _check(self._lib.ath_open_logs(self._h, os.fsencode(directory), 1 if append else 0))
+
+    def close_logs(self):
+        _check(self._lib.ath_close_logs(self._h))
+
     # --- hot path
     def begin_step(self, nn):
         _check(self._lib.ath_begin_step(self._h, nn))
@@ -367,6 +396,14 @@ class Server:
         used, free = ctypes.c_double(), ctypes.c_double()
         _check(self._lib.ath_server_memory(self._h, ctypes.byref(used), ctypes.byref(free)))
         return used.value, free.value
+
+    @staticmethod
+    def estimate(inp_path, precision=0):
+        """device bytes admission reserves for the case"""
+        lib = load_library()
+        v = ctypes.c_double()
+        _check(lib.ath_estimate_case_bytes(os.fsencode(inp_path), int(precision), ctypes.byref(v)))
+        return v.value
 
     def admit(self, inp_path, residual_drop=1e-8, max_iterations=10000):
         """slot id, or None when the case does not fit now (ATH_ERR_NOMEM)"""

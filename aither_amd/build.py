@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaither_amd.so")
-SOURCES = ["host_case.cpp", "solver.hip"]
+SOURCES = ["host_case.cpp", "io.cpp", "solver.hip"]
 ARCH = os.environ.get("AITHER_AMD_ARCH", "gfx950")
 
 # packed FP32 instructions are disabled for all device code.  Multiply-add
@@ -42,7 +42,7 @@ def build_library(verbose=True):
 
 def build_oracle(verbose=True):
     odir = os.path.join(ROOT, "oracle")
-    cmd = ["make", "-s", "-C", odir, "liboracle.so"]
+    cmd = ["make", "-s", "-C", odir, "liboracle.so", "liboracle_omp.so"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -643,6 +643,11 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (!f) return "cannot open input file " + inp_path;
   const auto sl = inp_path.rfind('/');
   c.dir = (sl == std::string::npos) ? "." : inp_path.substr(0, sl);
+  {
+    // input::SimNameRoot (src/input.cpp:872-874): the name up to its first '.'
+    const std::string base = sl == std::string::npos ? inp_path : inp_path.substr(sl + 1);
+    c.simName = base.substr(0, base.find('.'));
+  }
   std::string fluid = "air";
   struct Ic { int tag = -1; bool file = false; std::string fname; double rho = 0, p = 0; std::array<double, 3> vel{0, 0, 0}; };
   std::vector<Ic> ics;
@@ -688,6 +693,20 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
     }
     try {
       if (key == "gridName") c.grid = val;
+      else if (key == "outputVariables") {
+        // ReadStringList: <a, b, c>
+        c.outputVars.clear();
+        std::string body = val;
+        if (!body.empty() && body.front() == '<') body = body.substr(1);
+        const auto close = body.find('>');
+        if (close != std::string::npos) body = body.substr(0, close);
+        std::istringstream ls(body);
+        std::string item;
+        while (std::getline(ls, item, ',')) {
+          item = Trim(item);
+          if (!item.empty()) c.outputVars.insert(item);
+        }
+      }
       else if (key == "timeStep") c.dtIn = std::stod(val);
       else if (key == "iterations") c.iterations = std::stoi(val);
       else if (key == "referenceDensity") c.rRef = std::stod(val);
@@ -820,6 +839,7 @@ std::string LoadCase(const std::string &inp_path, HostCase *out) {
   if (c.roeJac && c.viscous) return "approximateRoe with the viscous equations is not supported";
   double fn = 0, fm = 0;
   if (!FluidProps(c.dir, fluid, &fn, &fm, &c.suth)) return "unknown fluid " + fluid;
+  c.fluidName = fluid;
 
   // reference speed of sound (src/input.cpp:616-621) and nondimensional gas
   // constant (src/fluid.cpp:89-103)

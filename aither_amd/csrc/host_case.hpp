@@ -8,6 +8,7 @@
 #pragma once
 
 #include <array>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,11 @@ struct Connection {
 
 struct HostCase {
   std::string dir, grid;
+  std::string simName;      // input file name without directory and extension (input::SimNameRoot)
+  std::string fluidName = "air";
+  // outputVariables (a sorted set, as the reference's std::set), with its
+  // default (src/input.cpp:106)
+  std::set<std::string> outputVars{"density", "vel_x", "vel_y", "vel_z", "pressure"};
   int iterations = 1, nonlinear = 1, sweeps = 1;
   double rRef = -1, tRef = -1, lRef = 1, aRef = 0;
   double dtIn = -1;

@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <fstream>
 #include <limits>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +14,7 @@
 
 #include "../../include/aither_amd.h"
 #include "host_case.hpp"
+#include "io.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -66,7 +69,9 @@ struct CaseRes {
   int *dcounter = nullptr;
   int histCap = 0;
   double l2first[ATH_MAX_EQ]{};
+  bool restart = false;  // started from a restart file: the normalisation is the file's
   double cfl = 0;
+  double reserved = 0;  // served cases: the estimate admission reserved
   std::vector<ath::HaloTask> halo;  // interblock connections, block indices within the case
   std::vector<ath::ConnTask> conn;
   // served cases
@@ -134,6 +139,9 @@ struct ath_ctx {
   int timing = 0;
   std::vector<hipEvent_t> evs;
   ath_timing tm{};
+  // residual and timing logs (<simName>.resid, .tme)
+  std::unique_ptr<std::ofstream> logResid, logTime;
+  std::chrono::steady_clock::time_point logStart;
 
   ~ath_ctx() {
     DropGraph();
@@ -1277,7 +1285,9 @@ void NormsFrom(CaseRes &cr, const ath::Params &P, const double h[8], int nn, int
   n.cfl = cfl;
   // PrintResiduals (src/output.cpp:1028-1063)
   constexpr double eps = 1.0e-30;
-  if (nn == 0 && mm == 0) {
+  if (cr.restart) {
+    // a restart keeps the normalisation read from the file
+  } else if (nn == 0 && mm == 0) {
     for (int e = 0; e < 5; ++e) cr.l2first[e] = n.l2[e];
   } else if (nn < 5 && mm == 0) {
     if (n.l2[0] > cr.l2first[0]) cr.l2first[0] = n.l2[0];
@@ -1317,6 +1327,18 @@ int ReadNorms(ath_ctx *c, int slot, int nn, int mm, ath_norms *out) {
   int flag = 0;
   if (int r = TakeFlags(c, cr, &flag)) return r;
   return FlagStatus(flag);
+}
+
+// one line of the residual log for outer step nn, iteration mm
+void LogLine(ath_ctx *c, const ath_norms &n, int nn, int mm, double iterSeconds) {
+  if (!c->logResid) return;
+  const double v = c->hc.dtIn > 0.0 ? c->hc.dtIn : n.cfl;
+  ath::PrintResidualLine(c->hc, nn, mm, v, n.normalized, n.linf_eqn, n.linf_block, n.linf_i, n.linf_j, n.linf_k,
+                         n.linf, n.matrix_resid, *c->logResid);
+  if (mm == c->hc.nonlinear - 1 && c->logTime) {
+    const double sim = std::chrono::duration<double>(std::chrono::steady_clock::now() - c->logStart).count();
+    ath::PrintTimeLine(nn, iterSeconds, sim, *c->logTime);
+  }
 }
 
 int BuildGraph(ath_ctx *c) {
@@ -1536,6 +1558,7 @@ int ath_update(ath_ctx *c, int mm, ath_norms *norms) {
 int ath_iterate(ath_ctx *c, int nn, int mm, ath_norms *norms) {
   if (!c) return Fail(ATH_ERR_ARG, "null context");
   HIP_OK(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
   const int slot = ResetCounterIfFull(c, 1);
   if (slot < 0) return slot;
   LaunchBoundary(c);
@@ -1548,6 +1571,7 @@ int ath_iterate(ath_ctx *c, int nn, int mm, ath_norms *norms) {
   ath_norms n{};
   const int r = ReadNorms(c, slot, nn, mm, &n);
   if (norms) *norms = n;
+  LogLine(c, n, nn, mm, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   return r;
 }
 
@@ -1576,6 +1600,24 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
   }
   hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, cr.dcounter, 0);
   if (c->timing) HIP_OK(hipEventRecord(c->evs[0], c->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  // iterations [q0, q1) of this run, from ring slots q0 % histCap on, in
+  // order: the PrintResiduals normalisation advances through every one of
+  // them, the log gets a line for each, `last` the last one
+  auto consume = [&](long q0, long q1) -> int {
+    std::vector<double> h(8 * (size_t)(q1 - q0));
+    HIP_OK(hipMemcpy(h.data(), cr.dhist + 8 * (size_t)(q0 % cr.histCap), h.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    const double per = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / (double)q1;
+    for (long q = q0; q < q1; ++q) {
+      const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
+      ath_norms n{};
+      NormsFrom(cr, c->P, &h[8 * (size_t)(q - q0)], qnn, qmm, c->hc.Cfl(qnn), &n);
+      LogLine(c, n, qnn, qmm, per);
+      if (q == iters - 1 && last) *last = n;
+    }
+    return ATH_OK;
+  };
   long it = 0;
   for (int s = 0; s < nsteps; ++s) {
     const int nn = nn0 + s;
@@ -1594,29 +1636,18 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
       // the norm history holds histCap iterations; drain it when full
       if ((it + 1) % cr.histCap == 0 && it + 1 < iters) {
         HIP_OK(hipStreamSynchronize(c->stream));
-        for (long q = it + 1 - cr.histCap; q <= it; ++q) {
-          const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
-          if (qnn < 5 && qmm == 0) ReadNorms(c, (int)(q % cr.histCap), qnn, qmm, nullptr);
-        }
+        if (int r = consume(it + 1 - cr.histCap, it + 1)) return r;
         hipLaunchKernelGGL(ath::KSetInt, dim3(1), dim3(1), 0, c->stream, cr.dcounter, 0);
       }
     }
   }
   if (c->timing) HIP_OK(hipEventRecord(c->evs[1], c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  // advance the PrintResiduals normalisation through the iterations it
-  // depends on, then report the last one
-  const long first = (iters - 1) / cr.histCap * cr.histCap;
-  int r = ATH_OK;
-  for (long q = first; q < iters; ++q) {
-    const int qnn = nn0 + (int)(q / nl), qmm = (int)(q % nl);
-    ath_norms n{};
-    const bool want = (q == iters - 1) || (qnn < 5 && qmm == 0);
-    if (!want) continue;
-    r = ReadNorms(c, (int)(q % cr.histCap), qnn, qmm, &n);
-    if (q == iters - 1 && last) *last = n;
-    if (r != ATH_OK) break;
-  }
+  if (int r = consume((iters - 1) / cr.histCap * cr.histCap, iters)) return r;
+  cr.cfl = c->hc.Cfl(nn0 + nsteps - 1);
+  int flag = 0;
+  if (int r2 = TakeFlags(c, cr, &flag)) return r2;
+  const int r = FlagStatus(flag);
   if (c->timing) {
     ath_timing t{};
     float ms = 0;
@@ -1716,6 +1747,150 @@ int ath_get_block_diagonal(ath_ctx *c, int b, int inverse, double *a) {
 }
 int ath_get_update(ath_ctx *c, int b, double *du) { return GetPhys(c, b, du, 4); }
 
+// the cell-centre fields of every block of a single-case context, physical
+// cells, AoS (what the output files are written from)
+static int GatherIo(ath_ctx *c, std::vector<ath::IoBlock> *out) {
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const auto &hc = c->hc;
+  out->assign(c->blk.size(), ath::IoBlock());
+  for (size_t b = 0; b < c->blk.size(); ++b) {
+    const auto &d = c->blk[b].d;
+    ath::IoBlock &o = (*out)[b];
+    o.ni = d.ni;
+    o.nj = d.nj;
+    o.nk = d.nk;
+    const long nc = d.ncell;
+    o.W.resize(5 * (size_t)nc);
+    if (int r = CopyState(c, (int)b, o.W.data(), true)) return r;
+    std::vector<double> soa(5 * (size_t)nc);
+    auto aos = [&](const double *dev, std::vector<double> *dst) -> int {
+      HIP_OK(hipMemcpy(soa.data(), dev, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+      dst->resize(soa.size());
+      for (long p = 0; p < nc; ++p)
+        for (int e = 0; e < 5; ++e) (*dst)[5 * p + e] = soa[e * nc + p];
+      return ATH_OK;
+    };
+    if (int r = aos(d.R, &o.R)) return r;
+    if (c->P.multilevel)
+      if (int r = aos(d.consNm1, &o.consNm1)) return r;
+    o.dt.resize(nc);
+    HIP_OK(hipMemcpy(o.dt.data(), d.dt, nc * sizeof(double), hipMemcpyDeviceToHost));
+    if (d.vgrad) {
+      std::vector<double> vg(9 * (size_t)d.npad);
+      HIP_OK(hipMemcpy(vg.data(), d.vgrad, vg.size() * sizeof(double), hipMemcpyDeviceToHost));
+      o.vgrad.resize(9 * (size_t)nc);
+      for (int k = 0; k < d.nk; ++k)
+        for (int j = 0; j < d.nj; ++j)
+          for (int i = 0; i < d.ni; ++i) {
+            const long p = i + (long)j * d.ni + (long)k * d.ni * d.nj;
+            const long q = (i + d.g) + (long)(j + d.g) * d.NI + (long)(k + d.g) * d.NI * d.NJ;
+            for (int e = 0; e < 9; ++e) o.vgrad[9 * p + e] = vg[e * d.npad + q];
+          }
+    }
+    const ath::HostBlock &h = hc.blocks[b];
+    o.ctr.resize(3 * (size_t)nc);
+    for (int k = 0; k < d.nk; ++k)
+      for (int j = 0; j < d.nj; ++j)
+        for (int i = 0; i < d.ni; ++i) {
+          const long p = i + (long)j * d.ni + (long)k * d.ni * d.nj;
+          const ath::Vec3 &v = h.ctr[h.Cell(i, j, k)];
+          o.ctr[3 * p] = v.x;
+          o.ctr[3 * p + 1] = v.y;
+          o.ctr[3 * p + 2] = v.z;
+        }
+  }
+  return ATH_OK;
+}
+
+static std::string Join(const char *dir, const std::string &name) {
+  std::string d = dir && *dir ? dir : ".";
+  return d.back() == '/' ? d + name : d + "/" + name;
+}
+
+int ath_write_restart(ath_ctx *c, const char *path, int iteration) {
+  if (!c || !path) return Fail(ATH_ERR_ARG, "null argument");
+  std::vector<ath::IoBlock> io;
+  if (int r = GatherIo(c, &io)) return r;
+  const std::string e = ath::WriteRestart(path, c->hc, io, iteration, c->cases[0]->l2first);
+  return e.empty() ? ATH_OK : Fail(ATH_ERR_IO, e);
+}
+
+int ath_read_restart(ath_ctx *c, const char *path, int *iteration) {
+  if (!c || !path) return Fail(ATH_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  std::vector<ath::IoBlock> io(c->blk.size());
+  for (size_t b = 0; b < io.size(); ++b) {
+    io[b].ni = c->blk[b].d.ni;
+    io[b].nj = c->blk[b].d.nj;
+    io[b].nk = c->blk[b].d.nk;
+  }
+  int iter = 0;
+  bool haveNm1 = false;
+  double l2[5];
+  const std::string e = ath::ReadRestart(path, c->hc, &io, &iter, l2, &haveNm1);
+  if (!e.empty()) return Fail(e.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, e);
+  for (size_t b = 0; b < io.size(); ++b) {
+    if (int r = CopyState(c, (int)b, io[b].W.data(), false)) return r;
+    if (c->P.multilevel) {
+      // U^{n-1} from the file; with one time level the reference assigns
+      // the solution to both n and n-1 (src/output.cpp:887-893)
+      const auto &d = c->blk[b].d;
+      const long nc = d.ncell;
+      std::vector<double> soa(5 * (size_t)nc);
+      if (haveNm1) {
+        for (long p = 0; p < nc; ++p)
+          for (int q = 0; q < 5; ++q) soa[q * nc + p] = io[b].consNm1[5 * p + q];
+        HIP_OK(hipMemcpy(d.consNm1, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+      } else {
+        hipLaunchKernelGGL(ath::KBeginStep, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                           c->Map(kMapCell256), c->P, 1);
+        HIP_OK(hipStreamSynchronize(c->stream));
+      }
+    }
+  }
+  CaseRes &cr = *c->cases[0];
+  for (int q = 0; q < 5; ++q) cr.l2first[q] = l2[q];
+  cr.restart = true;
+  if (iteration) *iteration = iter;
+  return ATH_OK;
+}
+
+int ath_write_output(ath_ctx *c, const char *dir, int iteration) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  std::vector<ath::IoBlock> io;
+  if (int r = GatherIo(c, &io)) return r;
+  const std::string grid = c->hc.grid + "_center.xyz";
+  const std::string fun = c->hc.simName + "_" + std::to_string(iteration) + "_center.fun";
+  std::string e = ath::WriteCenters(Join(dir, grid), c->hc, io);
+  if (e.empty()) e = ath::WriteFunction(Join(dir, fun), c->hc, io);
+  if (e.empty()) e = ath::WriteMeta(Join(dir, c->hc.simName + "_center.p3d"), c->hc, grid, fun, iteration);
+  if (e.empty()) return ATH_OK;
+  return Fail(e.find("not available") != std::string::npos || e.find("needs") != std::string::npos
+                  ? ATH_ERR_CONFIG : ATH_ERR_IO, e);
+}
+
+int ath_open_logs(ath_ctx *c, const char *dir, int append) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  const auto mode = append ? std::ios::app : std::ios::out;
+  auto r = std::make_unique<std::ofstream>(Join(dir, c->hc.simName + ".resid"), mode);
+  auto t = std::make_unique<std::ofstream>(Join(dir, c->hc.simName + ".tme"), std::ios::out);
+  if (!*r || !*t) return Fail(ATH_ERR_IO, "cannot open the residual or timing log in " + std::string(dir ? dir : "."));
+  ath::PrintHeaders(c->hc, *r);
+  ath::PrintTimeHeader(*t);
+  c->logResid = std::move(r);
+  c->logTime = std::move(t);
+  c->logStart = std::chrono::steady_clock::now();
+  return ATH_OK;
+}
+
+int ath_close_logs(ath_ctx *c) {
+  if (!c) return Fail(ATH_ERR_ARG, "null context");
+  c->logResid.reset();
+  c->logTime.reset();
+  return ATH_OK;
+}
+
 static void FillInfo(const ath::HostCase &hc, ath_info *o) {
   o->num_blocks = (int)hc.blocks.size();
   o->num_equations = 5;
@@ -1810,7 +1985,7 @@ struct ath_server {
   bool haveNumerics = false;
   bool dirty = true;  // the set of running cases changed since the group tables were built
   int nextId = 0;
-  double used = 0;    // device bytes held by the resident cases
+  double used = 0;    // device bytes reserved by the resident cases (their estimates)
 };
 
 namespace {
@@ -1946,6 +2121,15 @@ void ath_server_close(ath_server *s) {
   delete s;
 }
 
+int ath_estimate_case_bytes(const char *inp_path, int precision, double *bytes) {
+  if (!inp_path || !bytes) return Fail(ATH_ERR_ARG, "null argument");
+  ath::HostCase hc;
+  std::string err = ath::LoadCase(inp_path, &hc);
+  if (!err.empty()) return Fail(err.find("cannot open") == 0 ? ATH_ERR_IO : ATH_ERR_CONFIG, err);
+  *bytes = EstimateBytes(hc, precision == ATH_IMPLICIT_FP32);
+  return ATH_OK;
+}
+
 int ath_server_memory(const ath_server *s, double *used, double *free_hbm) {
   if (!s) return Fail(ATH_ERR_ARG, "null server");
   HIP_OK(hipSetDevice(s->c->device));
@@ -2000,7 +2184,8 @@ int ath_admit(ath_server *s, const char *inp_path, double residual_drop, int max
   cr->state = ATH_SLOT_RUNNING;
   cr->drop = residual_drop;
   cr->maxIters = max_iterations;
-  s->used += (double)cr->bytes;
+  cr->reserved = need;
+  s->used += need;
   *slot = cr->id;
   c->cases.push_back(std::move(cr));
   s->dirty = true;
@@ -2130,7 +2315,7 @@ int ath_retire(ath_server *s, int slot) {
   for (size_t q = 0; q < c->cases.size(); ++q)
     if (c->cases[q]->id == slot) {
       const bool wasActive = std::find(c->active.begin(), c->active.end(), c->cases[q].get()) != c->active.end();
-      s->used -= (double)c->cases[q]->bytes;
+      s->used -= c->cases[q]->reserved;
       if (wasActive) {
         // the group tables point at the case's memory: rebuild them before
         // anything launches again

@@ -84,7 +84,7 @@ def parse():
                    help="vortex: implicit-operator precision (residual is always FP64)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-iters", type=int, default=None,
-                   help="oracle iterations timed for cpu_baseline (default 2 channel, 12 vortex)")
+                   help="oracle iterations timed for cpu_baseline (default 3 channel, 20 vortex)")
     p.add_argument("--no-fp32-leg", action="store_true")
     p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
                    help="scalar LU-SGS execution order (all give identical results)")
@@ -105,15 +105,26 @@ def parse():
     return p.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(inp, iters, what):
-    """The oracle (single-threaded FP64 C restatement) on the first `iters`
-    nonlinear iterations of the same case.  Returns the baseline record and
-    the oracle's residual after each iteration and final state, which the
-    parity check compares with the GPU."""
-    import numpy as np
+    """The oracle (FP64 C restatement) on the first `iters` nonlinear
+    iterations of the same case, on every host core: its OpenMP build, whose
+    results equal the single-threaded build's bit for bit.  Returns the
+    baseline record and the oracle's residual after each iteration and final
+    state, which the parity check compares with the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    c = pyoracle.OracleCase(inp)
+    c = pyoracle.OracleCase(inp, parallel=True)
     ncell = sum(c.ncells(b) for b in range(c.nb))
     t0 = time.perf_counter()
     done = 0
@@ -133,9 +144,10 @@ def cpu_baseline(inp, iters, what):
         nn += 1
     dt = time.perf_counter() - t0 - tres
     state = np.concatenate([c.state(b).reshape(-1) for b in range(c.nb)])
-    rec = {"value": ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-           "sample": f"first {iters} nonlinear iterations of the same {what} ({ncell} cells), "
-                     f"oracle C restatement, 1 thread ({dt:.1f} s)"}
+    rec = {"value": ncell * iters / dt, "unit": "cell-updates/s", "cores": c.threads, "kind": "port",
+           "cpu": cpu_model(),
+           "sample": f"first {iters} nonlinear iterations of the same {what} ({ncell} cells), oracle C "
+                     f"restatement, OpenMP on {c.threads} threads ({dt:.1f} s)"}
     return rec, resid, state
 
 
@@ -301,7 +313,7 @@ def bench_channel(args, Solver, cases):
         "final_normalized_residual": list(last.normalized[:5]),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"], resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 2, "multiblock channel")
+        out["cpu_baseline"], resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 3, "multiblock channel")
         out["parity"] = gpu_parity(s, states0, resid_ref, state_ref)
     s.close()
     return out
@@ -413,7 +425,7 @@ def bench_vortex(args, Solver, cases):
     last, wall, tm = timed_run(s, [W0], steps, warm, prof_phys=min(steps, 50))
     cpu = None
     if not args.no_cpu_baseline:
-        cpu, resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 12, f"{args.n}^3 vortex")
+        cpu, resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 20, f"{args.n}^3 vortex")
         parity = gpu_parity(s, [W0], resid_ref, state_ref)
     value = ncell * steps / wall
     flux_avg_s = tm.flux_ms / max(tm.flux_launches, 1) * 1e-3

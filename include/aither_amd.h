@@ -155,6 +155,30 @@ int ath_get_update(ath_ctx *ctx, int block, double *du);      /* neq/cell */
  * cases */
 int ath_get_block_diagonal(ath_ctx *ctx, int block, int inverse, double *a);
 
+/* ---- files of the output layer ----------------------------------------- */
+/* Restart file in the reference's binary format (WriteRestart,
+ * src/output.cpp:591-755): dimensional state of every block (and U^{n-1}
+ * with BDF2), the species name and the residual normalisation. */
+int ath_write_restart(ath_ctx *ctx, const char *path, int iteration);
+/* ReadRestart (src/output.cpp:756-899): state (and U^{n-1}) and the
+ * residual normalisation from the file, which the norms keep from then on
+ * (the reference's restart does not re-normalise); *iteration = the file's
+ * iteration, where the caller's outer-step count resumes. */
+int ath_read_restart(ath_ctx *ctx, const char *path, int *iteration);
+/* WriteOutput's cell-centre files (src/output.cpp:573-578) into directory
+ * `dir`: <gridName>_center.xyz (cell centres, WriteCellCenter :58-98),
+ * <simName>_<iteration>_center.fun (the input's outputVariables, WriteFunFile
+ * :206-439) and the ParaView meta file <simName>_center.p3d (:903-956).  A
+ * requested variable this library does not hold (turbulence, gradients other
+ * than the block solvers' velocity gradients) fails with ATH_ERR_CONFIG. */
+int ath_write_output(ath_ctx *ctx, const char *dir, int iteration);
+/* residual and timing logs <simName>.resid and <simName>.tme in `dir`
+ * (logFileManager, src/logFileManager.cpp:34-113; PrintResiduals lines,
+ * src/output.cpp:1007-1085): every ath_iterate / ath_run iteration appends a
+ * line; ath_run's Iter-Time is the run's mean per iteration */
+int ath_open_logs(ath_ctx *ctx, const char *dir, int append);
+int ath_close_logs(ath_ctx *ctx);
+
 /* ---- timing of the last ath_run (HIP events on the context's stream) ---- */
 typedef struct {
   double run_ms;            /* whole ath_run */
@@ -226,14 +250,18 @@ typedef struct {
   ath_norms last;       /* norms of the last iteration */
 } ath_slot_info;
 
-/* `hbm_budget`: device bytes the resident cases may hold together (0: no
+/* `hbm_budget`: device bytes the resident cases may reserve together (0: no
  * limit beyond the free device memory less a 2 GiB reserve); `max_slots`:
  * cases that may run at once; `precision`: ATH_IMPLICIT_FP64/FP32 for every
  * case. */
 int ath_server_open(int device, double hbm_budget, int max_slots, int precision, ath_server **out);
 void ath_server_close(ath_server *srv);
-/* device bytes the resident cases hold, and the device's free memory */
+/* device bytes the resident cases have reserved (the sum of their
+ * estimates), and the device's free memory */
 int ath_server_memory(const ath_server *srv, double *used, double *free_hbm);
+/* the device bytes admission reserves for a case: an upper bound of what it
+ * allocates (no device needed) */
+int ath_estimate_case_bytes(const char *inp_path, int precision, double *bytes);
 /* Read the case (as ath_open does) and upload it if its estimated device
  * memory fits the free memory and the budget and a slot is free; else
  * ATH_ERR_NOMEM (nothing changed: try again after a retire).  The case runs

@@ -21,6 +21,29 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Parallel loops (the OpenMP build, liboracle_omp.so): every loop marked
+ * ORC_PARALLEL writes disjoint cells in each iteration of its outer loop
+ * and keeps, for each cell, the order in which the serial build accumulates
+ * into it; sums over cells are taken serially in the serial order.  The
+ * OpenMP build therefore gives the serial build's results bit for bit. */
+#ifdef _OPENMP
+#define ORC_PARALLEL _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PARALLEL
+#endif
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 #define EPS 1.0e-30 /* include/macros.hpp.in:21 */
 #define NEQ 5       /* rho, u, v, w, p (single species, no turbulence) */
 #define MAXBC 64
@@ -2327,6 +2350,7 @@ static int at_corner(const blk_t *b, int i, int j, int k) {
 /* procBlock::UpdateAuxillaryVariables (src/procBlock.cpp:6171-6189): every
  * cell but the block corners, ghost cells included */
 static void update_aux(const orc_case *c, blk_t *b) {
+  ORC_PARALLEL
   for (int k = -b->g; k < b->nk + b->g; ++k)
     for (int j = -b->g; j < b->nj + b->g; ++j)
       for (int i = -b->g; i < b->ni + b->g; ++i) {
@@ -2701,9 +2725,16 @@ void orc_tsl_jacobian(const orc_case *c, const double *W, double muLam, const do
 static void visc_flux_dir(orc_case *c, blk_t *b, int d) {
   int ei = (d == 0), ej = (d == 1), ek = (d == 2);
   int n3 = ndir(b, d);
-  for (int k = 0; k < b->nk + ek; ++k)
-    for (int j = 0; j < b->nj + ej; ++j)
+  /* faces of direction d touch the cells on their two sides: the k-faces
+   * of one (i, j) column, or the i- and j-faces of one k plane, touch no
+   * other column's (plane's) cells, and each cell sees its faces in the
+   * serial order */
+  const int outerN = d == 2 ? b->nj : b->nk + ek, midN = d == 2 ? b->nk + ek : b->nj + ej;
+  ORC_PARALLEL
+  for (int o = 0; o < outerN; ++o)
+    for (int m = 0; m < midN; ++m)
       for (int i = 0; i < b->ni + ei; ++i) {
+        const int k = d == 2 ? m : o, j = d == 2 ? o : m;
         int f3 = d == 0 ? i : (d == 1 ? j : k);
         double vg[3][3], tg[3];
         face_grads(c, b, d, i, j, k, vg, tg);
@@ -2802,9 +2833,13 @@ static double proj_c2c(const blk_t *b, int d, int i, int j, int k) {
 static void inv_flux_dir(orc_case *c, blk_t *b, int d) {
   int ei = (d == 0), ej = (d == 1), ek = (d == 2);
   int n3 = ndir(b, d);
-  for (int k = 0; k < b->nk + ek; ++k)
-    for (int j = 0; j < b->nj + ej; ++j)
+  /* as in visc_flux_dir: columns (k-faces) or planes (i-, j-faces) */
+  const int outerN = d == 2 ? b->nj : b->nk + ek, midN = d == 2 ? b->nk + ek : b->nj + ej;
+  ORC_PARALLEL
+  for (int o = 0; o < outerN; ++o)
+    for (int m = 0; m < midN; ++m)
       for (int i = 0; i < b->ni + ei; ++i) {
+        const int k = d == 2 ? m : o, j = d == 2 ? o : m;
         int f3 = d == 0 ? i : (d == 1 ? j : k);
         const double *Wm1 = &b->W[cidx(b, i - ei, j - ej, k - ek) * NEQ];
         const double *W0 = &b->W[cidx(b, i, j, k) * NEQ];
@@ -2896,6 +2931,7 @@ void orc_residual(orc_case *c) {
 void orc_time_step(orc_case *c) {
   for (int bq = 0; bq < c->nb; ++bq) {
     blk_t *b = &c->blk[bq];
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i) {
@@ -2910,6 +2946,7 @@ void orc_time_step(orc_case *c) {
 void orc_invert_diagonal(orc_case *c) {
   for (int bq = 0; bq < c->nb; ++bq) {
     blk_t *b = &c->blk[bq];
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i) {
@@ -3024,6 +3061,7 @@ void orc_init_update(orc_case *c) {
     blk_t *b = &c->blk[bq];
     memset(b->x, 0, sizeof(double) * NEQ * (size_t)ncells_padded(b));
     if (!c->initMatrix) continue;
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i) {
@@ -3038,9 +3076,21 @@ void orc_init_update(orc_case *c) {
 }
 
 /* lusgs::LUSGS_Forward / LUSGS_Backward: src/linearSolver.cpp:341-428 */
+/* the cells of one i+j+k hyperplane depend only on earlier planes: each
+ * plane's cells run in parallel, the planes in order */
+static long plane_end(const blk_t *b, long q0, long n) {
+  const int s = b->order[3 * q0] + b->order[3 * q0 + 1] + b->order[3 * q0 + 2];
+  long q = q0;
+  while (q < n && b->order[3 * q] + b->order[3 * q + 1] + b->order[3 * q + 2] == s) ++q;
+  return q;
+}
+
 static void lusgs_forward(orc_case *c, blk_t *b, int sweep) {
   long n = ncells(b);
-  for (long q = 0; q < n; ++q) {
+  for (long q0 = 0; q0 < n;) {
+  const long q1 = plane_end(b, q0, n);
+  ORC_PARALLEL
+  for (long q = q0; q < q1; ++q) {
     int i = b->order[3 * q], j = b->order[3 * q + 1], k = b->order[3 * q + 2];
     double L[NEQ], U[NEQ], rb[NEQ];
     implicit_lu(c, b, i, j, k, b->x, 0, L);
@@ -3058,11 +3108,20 @@ static void lusgs_forward(orc_case *c, blk_t *b, int sweep) {
     }
     for (int e = 0; e < NEQ; ++e) b->x[cidx(b, i, j, k) * NEQ + e] = (rb[e] + L[e]) * b->Ainv[p];
   }
+  q0 = q1;
+  }
 }
 
 static void lusgs_backward(orc_case *c, blk_t *b, int sweep) {
   long n = ncells(b);
-  for (long q = n - 1; q >= 0; --q) {
+  /* planes from the last to the first */
+  long *ends = (long *)malloc(sizeof(long) * (size_t)(b->ni + b->nj + b->nk));
+  int np = 0;
+  for (long q0 = 0; q0 < n; q0 = ends[np++]) ends[np] = plane_end(b, q0, n);
+  for (int pl = np - 1; pl >= 0; --pl) {
+  const long q0 = pl > 0 ? ends[pl - 1] : 0, q1 = ends[pl];
+  ORC_PARALLEL
+  for (long q = q1 - 1; q >= q0; --q) {
     int i = b->order[3 * q], j = b->order[3 * q + 1], k = b->order[3 * q + 2];
     double U[NEQ];
     implicit_lu(c, b, i, j, k, b->x, 1, U);
@@ -3087,6 +3146,8 @@ static void lusgs_backward(orc_case *c, blk_t *b, int sweep) {
       for (int e = 0; e < NEQ; ++e) xc[e] = xc[e] - U[e] * b->Ainv[p];
     }
   }
+  }
+  free(ends);
 }
 
 /* dplur::DPLUR: src/linearSolver.cpp:473-507 */
@@ -3094,6 +3155,7 @@ static void dplur_sweep(orc_case *c, blk_t *b) {
   long npad = ncells_padded(b);
   double *xold = (double *)malloc(sizeof(double) * NEQ * (size_t)npad);
   memcpy(xold, b->x, sizeof(double) * NEQ * (size_t)npad);
+  ORC_PARALLEL
   for (int k = 0; k < b->nk; ++k)
     for (int j = 0; j < b->nj; ++j)
       for (int i = 0; i < b->ni; ++i) {
@@ -3123,6 +3185,8 @@ static double matrix_residual(orc_case *c) {
   for (int bq = 0; bq < c->nb; ++bq) {
     blk_t *b = &c->blk[bq];
     double part = 0.0;
+    double *sq = (double *)malloc(sizeof(double) * NEQ * (size_t)ncells(b));
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i) {
@@ -3138,9 +3202,11 @@ static double matrix_residual(orc_case *c) {
             double ax = c->block ? axb[e] : b->x[cidx(b, i, j, k) * NEQ + e] * b->A[p];
             double axmb = (ax - L[e]) - rb[e];
             double r = 0.0 - axmb;
-            part += r * r;
+            sq[p * NEQ + e] = r * r;
           }
         }
+    for (long q = 0; q < NEQ * ncells(b); ++q) part += sq[q];
+    free(sq);
     l2 += part;
     total += ncells_padded(b) * NEQ;
   }
@@ -3174,6 +3240,7 @@ void orc_update(orc_case *c, int mm, double *l2, double *linf, int *linfloc) {
   for (int q = 0; q < 5; ++q) linfloc[q] = 0;
   for (int bq = 0; bq < c->nb; ++bq) {
     blk_t *b = &c->blk[bq];
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i) {
@@ -3191,6 +3258,12 @@ void orc_update(orc_case *c, int mm, double *l2, double *linf, int *linfloc) {
             update_prim(&c->gas, W, &b->x[cidx(b, i, j, k) * NEQ], Wn);
           }
           memcpy(W, Wn, sizeof Wn);
+        }
+    /* norms in the serial order */
+    for (int k = 0; k < b->nk; ++k)
+      for (int j = 0; j < b->nj; ++j)
+        for (int i = 0; i < b->ni; ++i) {
+          const double *R = &b->R[pidx(b, i, j, k) * NEQ];
           for (int e = 0; e < NEQ; ++e) l2[e] += R[e] * R[e];
           for (int e = 0; e < NEQ; ++e)
             if (R[e] > *linf) {
@@ -3214,6 +3287,7 @@ void orc_begin_step(orc_case *c, int nn) {
   if (!c->storeN) return;
   for (int bq = 0; bq < c->nb; ++bq) {
     blk_t *b = &c->blk[bq];
+    ORC_PARALLEL
     for (int k = 0; k < b->nk; ++k)
       for (int j = 0; j < b->nj; ++j)
         for (int i = 0; i < b->ni; ++i)

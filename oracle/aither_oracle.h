@@ -142,6 +142,8 @@ void orc_muscl(const double *u2, const double *u1, const double *d1,
 double orc_pyramid_volume(const double *p, const double *a, const double *b,
                           const double *c, const double *d);
 void orc_hyperplane_order(int ni, int nj, int nk, int *out /* 3 per cell */);
+/* threads of the parallel loops (1 in the serial build) */
+int orc_num_threads(void);
 double orc_inv_cell_specrad(const orc_gas *g, const double *W,
                             const double *aL /*ux,uy,uz,mag*/,
                             const double *aR);

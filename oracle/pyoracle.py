@@ -11,8 +11,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")  # same code, parallel loops (OpenMP)
 
-_lib = None
+_libs = {}
 _V = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 
@@ -21,12 +22,16 @@ class Gas(ctypes.Structure):
     _fields_ = [("R", ctypes.c_double), ("n", ctypes.c_double)]
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
-        L = ctypes.CDLL(LIB)
+def lib(parallel=False):
+    """the serial oracle, or (parallel=True) its OpenMP build, which gives
+    the same results bit for bit"""
+    path = LIB_OMP if parallel else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", HERE, os.path.basename(path)], check=True)
+        L = ctypes.CDLL(path)
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_num_threads.argtypes = []
         L.orc_load.restype = _V
         L.orc_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_free.argtypes = [_V]
@@ -97,8 +102,8 @@ def lib():
         L.orc_get_cell_grad.argtypes = [_V, ctypes.c_int, _D]
         L.orc_is_block.argtypes = [_V]
         L.orc_is_block.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def P(a):
@@ -112,8 +117,9 @@ def arr(x):
 class OracleCase:
     """One case in the oracle, with the same call sequence as the reference."""
 
-    def __init__(self, inp_path):
-        L = lib()
+    def __init__(self, inp_path, parallel=False):
+        L = lib(parallel)
+        self.threads = L.orc_num_threads()
         err = ctypes.create_string_buffer(1024)
         h = L.orc_load(os.fsencode(inp_path), err, 1024)
         if not h:

@@ -120,18 +120,17 @@ def test_serve_quarantines_a_failing_case(tmp_path):
 
 def test_serve_admission_by_memory_budget(tmp_path):
     paths = [bump(tmp_path / f"c{q}", 400 + q, 10)[0] for q in range(3)]
-    probe = Server(max_slots=8, precision=1)
-    s0 = probe.admit(paths[0], 1e-30, 5)
-    one = probe.info(s0).device_bytes
-    probe.close()
+    est = Server.estimate(paths[0], 1)
     # room for two such cases, not three
-    srv = Server(hbm_budget=2.5 * one * 1.1, max_slots=8, precision=1)
+    srv = Server(hbm_budget=2.5 * est, max_slots=8, precision=1)
     a = srv.admit(paths[0], 1e-30, 5)
     b = srv.admit(paths[1], 1e-30, 5)
     assert a is not None and b is not None
     assert srv.admit(paths[2], 1e-30, 5) is None
     used, free = srv.memory()
-    assert used >= 2 * one and free > 0
+    assert used == 2 * est and free > 0
+    # the estimate bounds what a case holds
+    assert 0 < srv.info(a).device_bytes <= est
     srv.step(5)
     srv.retire(a)
     c = srv.admit(paths[2], 1e-30, 5)
