@@ -99,6 +99,7 @@ _SIGS = {
     "ath_set_matrix_residual": [_P, ctypes.c_int],
     "ath_set_sweep_ordering": [_P, ctypes.c_int],
     "ath_get_sweep_ordering": [_P, ctypes.POINTER(ctypes.c_int)],
+    "ath_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D],
     "ath_get_state": [_P, ctypes.c_int, _D],
     "ath_set_state": [_P, ctypes.c_int, _D],
     "ath_begin_step": [_P, ctypes.c_int],
@@ -255,6 +256,13 @@ class Solver:
         v = ctypes.c_int()
         _check(self._lib.ath_get_sweep_ordering(self._h, ctypes.byref(v)))
         return v.value
+
+    def autotune(self, iterations=2):
+        """time every sweep order the case admits and keep the fastest;
+        returns (order, mean ms per sweep pass)"""
+        order, ms = ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.ath_autotune(self._h, int(iterations), ctypes.byref(order), ctypes.byref(ms)))
+        return order.value, ms.value
 
     def set_matrix_residual(self, on):
         _check(self._lib.ath_set_matrix_residual(self._h, 1 if on else 0))

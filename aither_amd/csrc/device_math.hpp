@@ -534,6 +534,40 @@ __device__ __forceinline__ void RoeOffDiag(const GasT<double> &g, const double W
   }
 }
 
+// F(prim(cons(W) + du)) - F(W), the flux change the off-diagonal products
+// take, with every term a product of the increments and the base state, so
+// that nothing cancels: rho' = rho + drho, v' = v + dv with
+// dv = (d(rho v) - v drho) / rho', dp = (gamma - 1)(d(rho E) - (drho |v'|^2 +
+// rho dv.(2 v + dv)) / 2), and
+//   dF_rho   = rho dv_n + drho v_n'
+//   dF_rhov  = dF_rho v' + rho v_n dv + dp n
+//   dF_rhoE  = (rho E + p) dv_n + (d(rho E) + dp) v_n'
+// The FP32 operator takes this form: its du falls far below the precision
+// of the state once the iteration converges, and F(U + du) - F(U) formed in
+// FP32 would round it away.  (The FP64 operator forms both fluxes, as the
+// reference's RusanovScalarOffDiagonal does.)
+template <typename T>
+__device__ __forceinline__ void DeltaFlux(const GasT<T> &g, const T W[5], const T du[5], T nx, T ny, T nz,
+                                          T dF[5]) {
+  const T rho = W[0], u = W[1], v = W[2], w = W[3], p = W[4];
+  const T gm1 = g.gamma - T(1);
+  const T dr = du[0], r1 = rho + dr;
+  const T dux = (du[1] - u * dr) / r1, duy = (du[2] - v * dr) / r1, duz = (du[3] - w * dr) / r1;
+  const T u1 = u + dux, v1 = v + duy, w1 = w + duz;
+  const T q1 = (u1 * u1 + v1 * v1) + w1 * w1;
+  const T dq = (dux * (T(2) * u + dux) + duy * (T(2) * v + duy)) + duz * (T(2) * w + duz);
+  const T dp = gm1 * (du[4] - T(0.5) * (dr * q1 + rho * dq));
+  const T vn = (u * nx + v * ny) + w * nz, dvn = (dux * nx + duy * ny) + duz * nz, vn1 = vn + dvn;
+  const T dm = rho * dvn + dr * vn1;
+  const T rE = p / gm1 + T(0.5) * rho * ((u * u + v * v) + w * w);
+  const T rvn = rho * vn;
+  dF[0] = dm;
+  dF[1] = dm * u1 + rvn * dux + dp * nx;
+  dF[2] = dm * v1 + rvn * duy + dp * ny;
+  dF[3] = dm * w1 + rvn * duz + dp * nz;
+  dF[4] = (rE + p) * dvn + (du[4] + dp) * vn1;
+}
+
 // RusanovScalarOffDiagonal for the inviscid equations
 // (src/fluxJacobian.cpp:122-162): 0.5|A| (F(U_nb + du) - F(U_nb)) +/- lambda du
 // with lambda = InvFaceSpectralRadius (include/spectralRadius.hpp:66-80)
@@ -543,16 +577,23 @@ template <typename T>
 __device__ __forceinline__ void ScalarOffDiag(const GasT<T> &g, const T Wnb[5], const T du[5],
                                               T nx, T ny, T nz, T mag, bool positive, T out[5],
                                               bool viscous = false, T lamv = T(0)) {
-  T Wu[5], Fo[5], Fn[5];
-  UpdatePrim(g, Wnb, du, Wu);
-  PhysFlux(g, Wnb, nx, ny, nz, Fo);
-  PhysFlux(g, Wu, nx, ny, nz, Fn);
+  T dF[5];
+  if constexpr (sizeof(T) == 4) {
+    DeltaFlux(g, Wnb, du, nx, ny, nz, dF);
+  } else {
+    T Wu[5], Fo[5], Fn[5];
+    UpdatePrim(g, Wnb, du, Wu);
+    PhysFlux(g, Wnb, nx, ny, nz, Fo);
+    PhysFlux(g, Wu, nx, ny, nz, Fn);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) dF[e] = Fn[e] - Fo[e];
+  }
   const T half = T(0.5) * mag;
   T lam = T(0.5) * mag * (fabs((Wnb[1] * nx + Wnb[2] * ny) + Wnb[3] * nz) + SoS(g, Wnb[0], Wnb[4]));
   if (viscous) lam += lamv;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
-    const T fc = half * (Fn[e] - Fo[e]);
+    const T fc = half * dF[e];
     out[e] = positive ? fc + du[e] * lam : fc - du[e] * lam;
   }
 }

@@ -37,6 +37,7 @@ struct DevBlock {
   // whole cache lines no other tile shares).  The off-diagonal blocks are
   // formed where they are used (NeighbourBlock).
   double *Ab, *Abi;
+  float *Abi32;       // FP32 operator: Abi rounded to float, same layout
   int tni, tnj, tnk;  // tile grid of the layout
   // block solvers with the viscous terms: the velocity gradient the viscous
   // pass formed at every physical face of direction d, 9 * faces(d), entry q
@@ -1080,6 +1081,10 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P) {
   if (!GJInverse5<double>(D, I)) atomicOr(b.flag, 4);
 #pragma unroll
   for (int q = 0; q < 25; ++q) b.Abi[TileBase(b, i, j, k) + 256 * q] = I[q];
+  if (P.fp32 && b.Abi32) {
+#pragma unroll
+    for (int q = 0; q < 25; ++q) b.Abi32[TileBase(b, i, j, k) + 256 * q] = float(I[q]);
+  }
 }
 
 // RusanovBlockOffDiagonal (src/fluxJacobian.cpp:164-194): the neighbour's
@@ -1139,8 +1144,8 @@ __device__ __forceinline__ void NeighbourBlock(const DevBlock &b, const GasT<dou
 
 // BlockMult with the matrix read entry by entry from global memory (entry q
 // at M[q * stride + idx]), in the same order of operations
-template <typename T>
-__device__ __forceinline__ void BlockMultG(const double *__restrict__ M, long stride, long idx, const T v[5],
+template <typename T, typename MT>
+__device__ __forceinline__ void BlockMultG(const MT *__restrict__ M, long stride, long idx, const T v[5],
                                            T out[5]) {
   // all 25 loads issued before the arithmetic
   T m[25];
@@ -1152,11 +1157,11 @@ __device__ __forceinline__ void BlockMultG(const double *__restrict__ M, long st
 // block product with the cell's main diagonal Ab (cell-major layout)
 template <typename T>
 __device__ __forceinline__ void CellBlockMult(const DevBlock &b, const double *M, long p, const T v[5], T out[5]) {
-  BlockMultG<T>(M, b.ncell, p, v, out);
+  BlockMultG<T, double>(M, b.ncell, p, v, out);
 }
 // block product with a tile-blocked per-cell block (Abi)
-template <typename T>
-__device__ __forceinline__ void TileBlockMult(const DevBlock &b, const double *M, int i, int j, int k, const T v[5],
+template <typename T, typename MT>
+__device__ __forceinline__ void TileBlockMult(const DevBlock &b, const MT *M, int i, int j, int k, const T v[5],
                                               T out[5]) {
   BlockMultG<T>(M, 256, TileBase(b, i, j, k), v, out);
 }
@@ -1169,6 +1174,7 @@ struct OpBuf<double> {
   static __device__ __forceinline__ double *X(const DevBlock &b) { return b.x; }
   static __device__ __forceinline__ double *Xold(const DevBlock &b) { return b.xold; }
   static __device__ __forceinline__ const double *Ainv(const DevBlock &b) { return b.Ainv; }
+  static __device__ __forceinline__ const double *Abi(const DevBlock &b) { return b.Abi; }
 };
 template <>
 struct OpBuf<float> {
@@ -1176,6 +1182,7 @@ struct OpBuf<float> {
   static __device__ __forceinline__ float *X(const DevBlock &b) { return b.x32; }
   static __device__ __forceinline__ float *Xold(const DevBlock &b) { return b.xold32; }
   static __device__ __forceinline__ const float *Ainv(const DevBlock &b) { return b.Ainv32; }
+  static __device__ __forceinline__ const float *Abi(const DevBlock &b) { return b.Abi32; }
 };
 
 // sum over the lower (upper == false) or upper neighbours of the Rusanov
@@ -1187,7 +1194,10 @@ struct OpBuf<float> {
 template <typename T>
 __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, const ViscParams &vp,
                                            const T *x, int i, int j, int k, bool upper, T out[5],
-                                           int block = 0, int roe = 0) {
+                                           int block = 0, int roe = 0, const Gas *gas64 = nullptr) {
+  // the block and Roe products are formed in FP64, with the FP64 gas
+  const GasT<double> gd = gas64 ? Cast<double>(*gas64)
+                                : GasT<double>{double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
 #pragma unroll
   for (int e = 0; e < 5; ++e) out[e] = T(0);
   const T *bt = OpBuf<T>::Bt(b);
@@ -1214,7 +1224,6 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       du[e] = x[e * b.ncell + q];
     }
     if (block) {
-      const GasT<double> gd = {double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
       double J[25];
       NeighbourBlock(b, gd, vp, d, upper, i, j, k, J);
       T m[25];
@@ -1223,7 +1232,6 @@ __device__ __forceinline__ void OffDiagSum(const DevBlock &b, const GasT<T> &g, 
       BlockMult<T>(m, du, o);
     } else if (roe) {
       // RoeOffDiagonal: the cell's own state is the other side of the face
-      const GasT<double> gd = {double(g.R), double(g.gamma), double(g.cps), double(g.cvs)};
       const long cc = PadIdx(b, i, j, k);
       double Wd[5], Wn[5], dd[5], od[5];
 #pragma unroll
@@ -1264,7 +1272,7 @@ __global__ void __launch_bounds__(256) KInitUpdate(BlockMap M, Params P) {
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = T(rb[e]);
-    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
+    TileBlockMult<T>(b, OpBuf<T>::Abi(b), i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1286,10 +1294,10 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T L[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac, &P.gas);
   if (useUpper) {
     T U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac, &P.gas);
 #pragma unroll
     for (int e = 0; e < 5; ++e) L[e] -= U[e];
   }
@@ -1299,7 +1307,7 @@ __global__ void __launch_bounds__(64) KLusgsForward(DevBlock b, Params P,
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = T(rb[e]) + L[e];
-    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
+    TileBlockMult<T>(b, OpBuf<T>::Abi(b), i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1321,21 +1329,21 @@ __global__ void __launch_bounds__(64) KLusgsBackward(DevBlock b, Params P,
   const GasT<T> g = Cast<T>(P.gas);
   T *x = OpBuf<T>::X(b);
   T U[5];
-  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
+  OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac, &P.gas);
   if (P.block) {
     T v[5], o[5];
     if (full) {
       T L[5];
-      OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
+      OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac, &P.gas);
       double rb[5];
       Rhs(b, P, p, PadIdx(b, i, j, k), rb);
 #pragma unroll
       for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + L[e]) - U[e];
-      TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
+      TileBlockMult<T>(b, OpBuf<T>::Abi(b), i, j, k, v, o);
 #pragma unroll
       for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     } else {
-      TileBlockMult<T>(b, b.Abi, i, j, k, U, o);
+      TileBlockMult<T>(b, OpBuf<T>::Abi(b), i, j, k, U, o);
 #pragma unroll
       for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = x[e * b.ncell + p] - o[e];
     }
@@ -1365,8 +1373,8 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
   const GasT<T> g = Cast<T>(P.gas);
   const T *xo = OpBuf<T>::Xold(b);
   T L[5], U[5];
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L, P.block, P.roeJac);
-  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U, P.block, P.roeJac);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, false, L, P.block, P.roeJac, &P.gas);
+  OffDiagSum<T>(b, g, P.visc, xo, i, j, k, true, U, P.block, P.roeJac, &P.gas);
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
   T *x = OpBuf<T>::X(b);
@@ -1374,7 +1382,7 @@ __global__ void __launch_bounds__(256) KDplur(DevBlock b, Params P) {
     T v[5], o[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) v[e] = (T(rb[e]) + T(0)) + (L[e] - U[e]);
-    TileBlockMult<T>(b, b.Abi, i, j, k, v, o);
+    TileBlockMult<T>(b, OpBuf<T>::Abi(b), i, j, k, v, o);
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = o[e];
     return;
@@ -1429,8 +1437,8 @@ __global__ void __launch_bounds__(kRedThreads) KMatrixResidual(BlockMap M, Param
     const GasT<T> g = Cast<T>(P.gas);
     const T *x = OpBuf<T>::X(b);
     T L[5], U[5];
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac);
-    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, false, L, P.block, P.roeJac, &P.gas);
+    OffDiagSum<T>(b, g, P.visc, x, i, j, k, true, U, P.block, P.roeJac, &P.gas);
     double rb[5];
     Rhs(b, P, p, PadIdx(b, i, j, k), rb);
     const double A = P.block ? 0.0 : b.A[p];
@@ -1788,18 +1796,22 @@ __global__ void __launch_bounds__(256) KConnTerms(const DevBlock *__restrict__ b
     dn[e] = xb[e * B.ncell + c.scell];
   }
   if (P.block) {
-    T W[5];
+    // the block is formed in FP64 from the FP64 state (with either operator
+    // precision)
+    double dd[5], od[5];
 #pragma unroll
-    for (int e = 0; e < 5; ++e) W[e] = T(Wn[e]);
+    for (int e = 0; e < 5; ++e) dd[e] = double(dn[e]);
     if (P.visc.on) {
       double vgn[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) vgn[q] = A.vgrad[q * A.npad + c.gpad];
-      BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o, &P.visc, A.mu[c.gpad],
-                      A.kc[c.gpad], A.c2c[c.d][c.face], vgn);
+      BlockOffDiag<double>(Cast<double>(P.gas), Wn, dd, A.fa[c.d][c.face], c.upper == 0, od, &P.visc,
+                           A.mu[c.gpad], A.kc[c.gpad], A.c2c[c.d][c.face], vgn);
     } else {
-      BlockOffDiag<T>(Cast<T>(P.gas), W, dn, A.fa[c.d][c.face], c.upper == 0, o);
+      BlockOffDiag<double>(Cast<double>(P.gas), Wn, dd, A.fa[c.d][c.face], c.upper == 0, od);
     }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) o[e] = T(od[e]);
   } else if (P.roeJac) {
     // RoeOffDiagonal with the interior cell next to the ghost as the other
     // side (a lower ghost lies below it, an upper ghost above)
@@ -2082,7 +2094,7 @@ constexpr int kBHaloI = kBTJ * kBTK, kBHaloJ = kBTI * kBTK, kBHaloK = kBTI * kBT
 constexpr int kBHalo = kBHaloI + kBHaloJ + kBHaloK;
 
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
+__global__ void __launch_bounds__(kBT, sizeof(T) == 4 ? 2 : 1) KBlockSweepTiles(const DevBlock *__restrict__ blocks, Params P,
                                                         MultiSweepCtl s, int full) {
   __shared__ T sA[5][kBT + kBHalo];  // du of the tile's cells, then of the halo columns
   __shared__ int sTicket, sNext;
@@ -2135,11 +2147,19 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
                      : kBT + (d == 0 ? lj + kBTJ * lk
                                      : (d == 1 ? kBHaloI + li + kBTI * lk : kBHaloI + kBHaloJ + li + kBTI * lj));
     }
-    double rb[5] = {0, 0, 0, 0, 0};
-    double mdep[3][25], mai[25];
+    // the right-hand side is formed in FP64 and rounded once to the
+    // operator's precision; the dependency blocks are formed in FP64 and
+    // held in the operator's precision, as is D^-1
+    T rb[5] = {T(0), T(0), T(0), T(0), T(0)};
+    T mdep[3][25], mai[25];
     T Ost[5];  // full sweeps: the other side's sum, from the previous sweep's du
     if (valid) {
-      if (FWD || full) Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+      if (FWD || full) {
+        double r64[5];
+        Rhs(b, P, p, PadIdx(b, i, j, k), r64);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) rb[e] = T(r64[e]);
+      }
       const GasT<double> gd = Cast<double>(P.gas);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -2149,19 +2169,22 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
         // which its LDS column holds; the identity block passes it on
         // unchanged (x + 0 y = x, and the sum never starts from -0)
         if (inside) {
-          NeighbourBlock(b, gd, P.visc, d, !FWD, i, j, k, mdep[d]);
+          double J[25];
+          NeighbourBlock(b, gd, P.visc, d, !FWD, i, j, k, J);
+#pragma unroll
+          for (int q = 0; q < 25; ++q) mdep[d][q] = T(J[q]);
         } else {
 #pragma unroll
-          for (int q = 0; q < 25; ++q) mdep[d][q] = q % 6 == 0 ? 1.0 : 0.0;
+          for (int q = 0; q < 25; ++q) mdep[d][q] = q % 6 == 0 ? T(1) : T(0);
         }
       }
       // the other side's du is not written before this tile publishes: its
       // cells in other tiles wait for this one, those in it are stored after
       // the walk
-      if (full) OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, Ost, 1);
-      const double *Mi = b.Abi + TileBase(b, i, j, k);
+      if (full) OffDiagSum<T>(b, g, P.visc, x, i, j, k, FWD, Ost, 1, 0, &P.gas);
+      const auto *Mi = OpBuf<T>::Abi(b) + TileBase(b, i, j, k);
 #pragma unroll
-      for (int q = 0; q < 25; ++q) mai[q] = Mi[256 * q];
+      for (int q = 0; q < 25; ++q) mai[q] = T(Mi[256 * q]);
       // backward: the forward result, which only this tile overwrites
       if (!FWD) {
 #pragma unroll
@@ -2244,18 +2267,15 @@ __global__ void __launch_bounds__(kBT) KBlockSweepTiles(const DevBlock *__restri
         for (int e = 0; e < 5; ++e) D[e] = T(0);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          T dn[5], o[5], m[25];
+          T dn[5], o[5];
 #pragma unroll
           for (int e = 0; e < 5; ++e) dn[e] = sA[e][nb[d]];
-#pragma unroll
-          for (int q = 0; q < 25; ++q) m[q] = T(mdep[d][q]);
-          BlockMult<T>(m, dn, o);
+          BlockMult<T>(mdep[d], dn, o);
 #pragma unroll
           for (int e = 0; e < 5; ++e) D[e] += o[e];
         }
-        T xn[5], v[5], ai[25];
-#pragma unroll
-        for (int q = 0; q < 25; ++q) ai[q] = T(mai[q]);
+        T xn[5], v[5];
+        const T *ai = mai;
         if (full) {
           if (FWD) {
 #pragma unroll
@@ -2539,8 +2559,18 @@ __device__ __forceinline__ void PenStage(const DevBlock &b, const Params &P, con
 #pragma unroll
     for (int e = 0; e < 5; ++e) W[e] = T(b.W[e * b.npad + c]);
     const T nx = T(a.x), ny = T(a.y), nz = T(a.z), mag = T(a.w);
-    PrimToCons(g, W, U);
-    PhysFlux(g, W, nx, ny, nz, Fo);
+    if constexpr (sizeof(T) == 4) {
+      // the FP32 operator's flux change is formed from the primitive state
+      // (DeltaFlux): the slot of U carries W
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        U[e] = W[e];
+        Fo[e] = T(0);
+      }
+    } else {
+      PrimToCons(g, W, U);
+      PhysFlux(g, W, nx, ny, nz, Fo);
+    }
     const T half = T(0.5) * mag;
     T lam = T(0.5) * mag * (fabs((W[1] * nx + W[2] * ny) + W[3] * nz) + SoS(g, W[0], W[4]));
     if (P.visc.on) lam += ViscFaceSR<T>(g, W, mag, T(b.c2c[d][fq]), T(b.mu[c]), P.visc);
@@ -2690,16 +2720,24 @@ __global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(PencilGrou
         du[e] = cross ? hx : px;
         U[e] = sl.U[d][e][lane] + du[e];
       }
-      // UpdatePrim + PhysFlux of the updated state + the Rusanov terms
-      ConsToPrim(g, U, Wu);
-      PhysFlux(g, Wu, sl.n[d][0][lane], sl.n[d][1][lane], sl.n[d][2][lane], Fn);
+      if constexpr (sizeof(T) == 4) {
+        // flux change from the staged primitive state (DeltaFlux)
+        T Wn[5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) Wn[e] = sl.U[d][e][lane];
+        DeltaFlux(g, Wn, du, sl.n[d][0][lane], sl.n[d][1][lane], sl.n[d][2][lane], Fn);
+      } else {
+        // UpdatePrim + PhysFlux of the updated state + the Rusanov terms
+        ConsToPrim(g, U, Wu);
+        PhysFlux(g, Wu, sl.n[d][0][lane], sl.n[d][1][lane], sl.n[d][2][lane], Fn);
+      }
       const T half = sl.half[d][lane], lam = sl.lam[d][lane];
       // a neighbour outside the block contributes the staged connection
       // term (zero off connections); idle steps contribute zero
       const bool outside = k >= 0 && !dep;
 #pragma unroll
       for (int e = 0; e < 5; ++e) {
-        const T fc = half * (Fn[e] - sl.Fo[d][e][lane]);
+        const T fc = half * (sizeof(T) == 4 ? Fn[e] : Fn[e] - sl.Fo[d][e][lane]);
         const T o = FWD ? fc + du[e] * lam : fc - du[e] * lam;
         sh.sO[d][e][lane] = dep ? o : (outside ? sl.Fo[d][e][lane] : T(0));
       }

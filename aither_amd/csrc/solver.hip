@@ -465,6 +465,12 @@ int AllocFp32(ath_ctx *c, ath::DevBlock &d) {
     if (int r = c->Alloc(&d.bt32, 5 * d.nbf)) return r;
     HIP_OK(hipMemset(d.bt32, 0, 5 * d.nbf * sizeof(float)));
   }
+  if (d.Abi) {
+    // block solvers: D^-1 rounded to float in the tile-blocked layout
+    const long tb = 25L * 256 * d.tni * d.tnj * d.tnk;
+    if (int r = c->Alloc(&d.Abi32, tb)) return r;
+    HIP_OK(hipMemset(d.Abi32, 0, tb * sizeof(float)));
+  }
   return ATH_OK;
 }
 
@@ -620,6 +626,7 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
       }
     }
     d.Ainv32 = nullptr;
+    d.Abi32 = nullptr;
     d.x32 = nullptr;
     d.xold32 = nullptr;
     d.nbf = 2 * ((long)h.nj * h.nk + (long)h.nk * h.ni + (long)h.ni * h.nj);
@@ -1411,8 +1418,6 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
   if (!c || (precision != ATH_IMPLICIT_FP64 && precision != ATH_IMPLICIT_FP32))
     return Fail(ATH_ERR_ARG, "bad precision");
   HIP_OK(hipSetDevice(c->device));
-  if (precision == ATH_IMPLICIT_FP32 && c->P.block)
-    return Fail(ATH_ERR_ARG, "the FP32 operator is not available with a block matrix solver");
   if (precision == ATH_IMPLICIT_FP32 && c->P.roeJac)
     return Fail(ATH_ERR_ARG, "the FP32 operator is not available with the approximateRoe flux Jacobian");
   if (precision == ATH_IMPLICIT_FP32)
@@ -1670,6 +1675,88 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
     c->tm = t;
   }
   return r;
+}
+
+// Autotuning of the sweep execution order: every order the case admits
+// (pencil, tiled and hyperplane for scalar LU-SGS; the tiled all-block sweep
+// with one or two workgroups per CU for the block solvers) runs `iterations`
+// nonlinear iterations from the current state, timed by events around each
+// sweep pass; the fastest is kept.  All orders give the same results bit for
+// bit (topological orders of the same dependencies), so the choice changes
+// speed only.  The state, the old time levels and the norm normalisation are
+// restored afterwards.
+int ath_autotune(ath_ctx *c, int iterations, int *chosen, double *pass_ms) {
+  if (!c || iterations <= 0) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  CaseRes &cr = *c->cases[0];
+  // snapshot
+  struct Snap { std::vector<double> W, N, Nm1; };
+  std::vector<Snap> snap(c->blk.size());
+  for (size_t b = 0; b < c->blk.size(); ++b) {
+    const auto &d = c->blk[b].d;
+    snap[b].W.resize(5 * (size_t)d.npad);
+    snap[b].N.resize(5 * (size_t)d.ncell);
+    snap[b].Nm1.resize(5 * (size_t)d.ncell);
+    HIP_OK(hipMemcpy(snap[b].W.data(), d.W, snap[b].W.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(snap[b].N.data(), d.consN, snap[b].N.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(snap[b].Nm1.data(), d.consNm1, snap[b].Nm1.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  double l2keep[ATH_MAX_EQ];
+  std::memcpy(l2keep, cr.l2first, sizeof l2keep);
+  auto restore = [&]() -> int {
+    for (size_t b = 0; b < c->blk.size(); ++b) {
+      const auto &d = c->blk[b].d;
+      HIP_OK(hipMemcpy(d.W, snap[b].W.data(), snap[b].W.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d.consN, snap[b].N.data(), snap[b].N.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d.consNm1, snap[b].Nm1.data(), snap[b].Nm1.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::memcpy(cr.l2first, l2keep, sizeof l2keep);
+    return ATH_OK;
+  };
+  struct Cand { int order, gridMul; };
+  std::vector<Cand> cands;
+  if (!c->P.implicit) return Fail(ATH_ERR_ARG, "autotuning needs an implicit time integration");
+  if (c->P.roeJac || c->hc.solver != ath::Solver::LUSGS) {
+    cands.push_back({EffectiveOrder(c), 0});
+  } else if (c->P.block) {
+    cands.push_back({ATH_SWEEP_TILED, 1});
+    cands.push_back({ATH_SWEEP_TILED, 2});
+    cands.push_back({ATH_SWEEP_HYPERPLANE, 0});
+  } else {
+    cands.push_back({ATH_SWEEP_PENCIL, 0});
+    cands.push_back({ATH_SWEEP_TILED, 0});
+    cands.push_back({ATH_SWEEP_HYPERPLANE, 0});
+  }
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, c->device));
+  const int keepTiming = c->timing;
+  const int nl = c->hc.nonlinear;
+  const int steps = std::max(1, (iterations + nl - 1) / nl);
+  double best = 1e300;
+  Cand pick = cands[0];
+  for (const Cand &k : cands) {
+    c->sweepOrder = k.order;
+    if (k.gridMul > 0)
+      c->msweepGrid = std::max(1, std::min(c->msweep.ntiles, k.gridMul * prop.multiProcessorCount));
+    c->timing = 1;
+    const int r = ath_run(c, 0, steps, nullptr);
+    c->timing = keepTiming;
+    if (int q = restore()) return q;
+    if (r != ATH_OK && r != ATH_ERR_NONPHYSICAL) return r;
+    const double per = c->tm.sweep_ms / std::max(1, c->tm.sweep_passes);
+    if (per < best) {
+      best = per;
+      pick = k;
+    }
+  }
+  c->sweepOrder = pick.order;
+  if (pick.gridMul > 0)
+    c->msweepGrid = std::max(1, std::min(c->msweep.ntiles, pick.gridMul * prop.multiProcessorCount));
+  c->DropGraph();
+  if (chosen) *chosen = pick.order;
+  if (pass_ms) *pass_ms = best;
+  return ATH_OK;
 }
 
 int ath_enable_timing(ath_ctx *c, int on) {

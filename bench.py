@@ -316,7 +316,71 @@ def bench_channel(args, Solver, cases):
         out["cpu_baseline"], resid_ref, state_ref = cpu_baseline(inp, args.cpu_iters or 3, "multiblock channel")
         out["parity"] = gpu_parity(s, states0, resid_ref, state_ref)
     s.close()
+    if not args.no_fp32_leg:
+        # the same timed run with the FP32 implicit operator (residual, dt
+        # and spectral radius stay FP64), and its converged-state error
+        s32 = Solver(inp)
+        s32.set_implicit_precision(Solver.FP32)
+        _, w32, tm32 = timed_run(s32, states0, steps, warm, prof_phys=min(nphys, 2))
+        W32 = np.concatenate([s32.get_state(b).reshape(-1) for b in range(nb)])
+        s32.close()
+        s64 = Solver(inp)
+        run_iterations(s64, steps)
+        W64 = np.concatenate([s64.get_state(b).reshape(-1) for b in range(nb)])
+        s64.close()
+        out["fp32_operator"] = {
+            "value": ncell * steps / w32, "ms_per_step": w32 / steps * 1e3,
+            "sweep_pass_ms": tm32.sweep_ms / max(tm32.sweep_passes, 1),
+            "state_rel_l2_vs_fp64_after_timed_steps": float(np.linalg.norm(W32 - W64) / np.linalg.norm(W64)),
+            "converged": fp32_converged(Solver, cases, work),
+        }
     return out
+
+
+def fp32_converged(Solver, cases, work, drop=1e-11):
+    """The FP32 implicit operator's converged-state error and extra
+    iterations: a steady 10^3 bump channel with BLU-SGS (5x5 blocks) run to
+    a `drop` residual drop with each operator precision and in the oracle
+    (its OpenMP build)."""
+    from aither_amd.serve import drop_of
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    path, meta = cases.bump_case(os.path.join(work, "fp32conv"), seed=5, n=10, iterations=1, cfl=20.0,
+                                 limiter="minmod")
+    text = open(path).read().replace("matrixSolver: lusgs", "matrixSolver: blusgs")
+    open(path, "w").write(text)
+
+    def gpu(prec):
+        s = Solver(path)
+        s.set_implicit_precision(prec)
+        it = 0
+        while it < 20000:
+            n = s.run(it, 10)
+            it += 10
+            if drop_of(list(n.l2[:5]), list(n.normalized[:5])) <= drop:
+                break
+        W = s.get_state().reshape(-1)
+        s.close()
+        return W, it
+
+    o = pyoracle.OracleCase(path, parallel=True)
+    it = 0
+    while it < 20000:
+        o.begin_step(it)
+        r = o.iterate(it, 0)
+        it += 1
+        if it % 10 == 0 and drop_of(list(r["l2"]), list(r["normalized"])) <= drop:
+            break
+    Wo = o.state().reshape(-1)
+    W64, i64 = gpu(Solver.FP64)
+    W32, i32 = gpu(Solver.FP32)
+    err = lambda W: float(np.linalg.norm(W - Wo) / np.linalg.norm(Wo))
+    return {"case": f"steady 10^3 bump channel (Mach {meta['mach']:.3f}, h/l {meta['h_over_l']:.3f}), Roe + "
+                    f"MUSCL minmod, BLU-SGS, CFL 20, to a {drop:g} residual drop",
+            "iterations_oracle": it, "iterations_fp64": i64, "iterations_fp32": i32,
+            "extra_iterations_fp32": i32 - i64,
+            "converged_state_rel_l2_fp64_vs_oracle": err(W64),
+            "converged_state_rel_l2_fp32_vs_oracle": err(W32)}
 
 
 def serve_requests(cases, work, n, nmin, nmax, seed, cfl):

@@ -103,6 +103,13 @@ int ath_set_sweep_ordering(ath_ctx *ctx, int ordering);
  * ATH_SWEEP_PENCIL to ATH_SWEEP_TILED, and the approximateRoe flux Jacobian
  * always runs ATH_SWEEP_HYPERPLANE, whatever was set */
 int ath_get_sweep_ordering(const ath_ctx *ctx, int *ordering);
+/* Autotune the sweep execution order for this case: time `iterations`
+ * nonlinear iterations in each order the case admits (pencil, tiled and
+ * hyperplane for scalar LU-SGS; the all-block tiled sweep with one or two
+ * workgroups per CU, and the hyperplane order, for the block solvers) and
+ * keep the fastest per sweep pass.  Results do not depend on the order; the
+ * state and the norm normalisation are restored.  No reference counterpart. */
+int ath_autotune(ath_ctx *ctx, int iterations, int *chosen, double *pass_ms);
 /* compute the matrix residual each iteration (default 1), as
  * linearSolver::Residual does (src/linearSolver.cpp:95-109) */
 int ath_set_matrix_residual(ath_ctx *ctx, int on);

@@ -103,3 +103,19 @@ def test_block_matrix_with_viscous_terms(tmp_path):
     open(inp, "w").write(text)
     assert inspect_info(inp).block_matrix == 1
     assert pyoracle.OracleCase(inp).is_block()
+
+
+def test_estimate_case_bytes(tmp_path):
+    """The device memory admission reserves for a case grows with the cell
+    count, with the FP32 operator's buffers and with the block operator."""
+    from aither_amd import Server
+    small = cases.bump_case(str(tmp_path / "s"), seed=1, n=8)[0]
+    big = cases.bump_case(str(tmp_path / "b"), seed=1, n=16)[0]
+    e_small, e_big = Server.estimate(small, 0), Server.estimate(big, 0)
+    assert e_big > 4 * (e_small - 1.1 * (1 << 20))
+    assert Server.estimate(small, 1) > e_small
+    blk = small.replace(".inp", "_blk.inp")
+    open(blk, "w").write(open(small).read().replace("matrixSolver: lusgs", "matrixSolver: blusgs"))
+    assert Server.estimate(blk, 0) > e_small + 8 ** 3 * 25 * 8
+    with pytest.raises(aither_amd.AitherError):
+        Server.estimate(str(tmp_path / "missing.inp"), 0)
