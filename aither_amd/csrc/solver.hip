@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2061,6 +2062,69 @@ int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n, const d
 
 
 }  // extern "C"
+
+// ---- face-flux evaluation without allocation -------------------------------
+
+// Device staging of ath_eval_face_flux_static: 18 doubles per face (left and
+// right states, normal, flux) for at most kStaticFaces faces, 9 MiB on every
+// device the library runs on, held until the process ends.  Host code never
+// uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device (each device holds its own copy).
+static constexpr int kStaticFaces = 1 << 16;
+static __device__ __align__(256) double scratch[(size_t)kStaticFaces * 18];
+
+namespace {
+
+std::mutex g_scratchMu;
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+
+// the device address of `scratch` on `device` (current), looked up on the
+// first call on that device
+int ScratchOn(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (device < 0 || device >= kMaxDevices) return Fail(ATH_ERR_ARG, "device index out of range");
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return Fail(ATH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch[device];
+  return ATH_OK;
+}
+
+}  // namespace
+
+extern "C" int ath_eval_face_flux_static(int device, int flux, double gas_r, double gas_n, const double *WL,
+                                         const double *WR, const double *n, int count, double *F) {
+  if (count <= 0 || !WL || !WR || !n || !F || (flux != 0 && flux != 1)) return Fail(ATH_ERR_ARG, "bad argument");
+  if (count > kStaticFaces)
+    return Fail(ATH_ERR_ARG, "ath_eval_face_flux_static takes at most " + std::to_string(kStaticFaces) +
+                                 " faces per call");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return Fail(ATH_ERR_ARG, "device index out of range");
+  HIP_OK(hipSetDevice(device));
+  double *buf = nullptr;
+  if (int r = ScratchOn(device, &buf)) return r;
+  ath::Gas g{};
+  g.R = gas_r;
+  g.n = gas_n;
+  const double cp = 0.0 + 1.0 * (gas_r * (gas_n + 1.0)), cv = 0.0 + 1.0 * (gas_r * gas_n);
+  g.gamma = cp / cv;
+  g.cps = gas_r * (gas_n + 1.0);
+  g.cvs = gas_r * gas_n;
+  double *dL = buf, *dR = buf + 5 * (size_t)count, *dn = buf + 10 * (size_t)count, *dF = buf + 13 * (size_t)count;
+  HIP_OK(hipMemcpy(dL, WL, 5 * (size_t)count * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dR, WR, 5 * (size_t)count * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dn, n, 3 * (size_t)count * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(ath::KEvalFlux, dim3(Grid(count)), dim3(kBlk), 0, 0, flux, g, dL, dR, dn, count, dF);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(F, dF, 5 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost));
+  return ATH_OK;
+}
 
 // ---- serving --------------------------------------------------------------
 

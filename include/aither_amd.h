@@ -222,6 +222,17 @@ int ath_inspect_field(const char *inp_path, int block, int which, double *out,
 int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n,
                        const double *WL, const double *WR, const double *n,
                        int count, double *F);
+/* ath_eval_face_flux without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The inputs and outputs are staged in a fixed device array of the library
+ * (9 MiB per device, allocated with the code object), so `count` is at most
+ * 65536 (larger counts return ATH_ERR_ARG).  Every call on one device uses
+ * the same array: calls that overlap on one device (from several threads)
+ * race and must be serialised by the caller. */
+int ath_eval_face_flux_static(int device, int flux, double gas_r, double gas_n,
+                              const double *WL, const double *WR, const double *n,
+                              int count, double *F);
 
 /* ---- serving: many steady cases resident on one device ------------------
  * No reference counterpart: the reference runs one case per process
