@@ -401,6 +401,54 @@ __device__ __forceinline__ void FaceStates(const DevBlock &b, double kappa, int 
   }
 }
 
+// one side of FaceStates: the lower state WL (upper == false) or the upper
+// state WR of face (i, j, k) of direction d, with the same arithmetic
+template <int RECON, int LIM>
+__device__ __forceinline__ void FaceSide(const DevBlock &b, double kappa, int d, int i, int j, int k, bool upper,
+                                         double Wout[5]) {
+  const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
+  const long c0 = PadIdx(b, i, j, k);
+  if (RECON == 0) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) Wout[e] = b.W[e * b.npad + c0 - (upper ? 0 : s)];
+  } else if (RECON >= 2) {
+    const double *cw = b.cw + d * b.npad;
+    double wc[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) wc[q] = cw[c0 + (q - 3) * s];
+    const double w5[5] = {upper ? wc[5] : wc[0], upper ? wc[4] : wc[1], upper ? wc[3] : wc[2],
+                          upper ? wc[2] : wc[3], upper ? wc[1] : wc[4]};
+    const WenoGeom G = MakeWenoGeom(w5);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double *We = b.W + e * b.npad + c0;
+      double u[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) u[q] = We[(q - 3) * s];
+      Wout[e] = upper ? Weno<RECON == 3>(G, w5, u[5], u[4], u[3], u[2], u[1])
+                      : Weno<RECON == 3>(G, w5, u[0], u[1], u[2], u[3], u[4]);
+    }
+  } else {
+    const double *cw = b.cw + d * b.npad;
+    const double wm2 = cw[c0 - 2 * s], wm1 = cw[c0 - s], w0 = cw[c0], wp1 = cw[c0 + s];
+    if (!upper) {
+      const double dPL = (wm1 + wm1) / (wm1 + w0), dML = (wm1 + wm1) / (wm1 + wm2);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        const double *We = b.W + e * b.npad;
+        Wout[e] = Muscl<LIM>(We[c0 - 2 * s], We[c0 - s], We[c0], kappa, dPL, dML);
+      }
+    } else {
+      const double dPR = (w0 + w0) / (w0 + wm1), dMR = (w0 + w0) / (w0 + wp1);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        const double *We = b.W + e * b.npad;
+        Wout[e] = Muscl<LIM>(We[c0 + s], We[c0], We[c0 - s], kappa, dPR, dMR);
+      }
+    }
+  }
+}
+
 template <int RECON, int LIM, int FLUX>
 __device__ __forceinline__ void FaceFlux(const DevBlock &b, const GasT<double> &g, double kappa,
                                          int d, int i, int j, int k, double F[5]) {
@@ -1019,16 +1067,18 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P) {
   for (int q = 0; q < 25; ++q) D[q] = MODE >= 2 ? b.Ab[q * b.ncell + p] : 0.0;
 #pragma unroll
   for (int d = 0; d < (MODE >= 2 ? 0 : 3); ++d) {
-    double WL[5], WR[5], J[25];
-    FaceStates<RECON, LIM>(b, P.kappa, d, i, j, k, WL, WR);
+    // the cell is the upper cell of its lower face (upper state WR) and the
+    // lower cell of its upper face (lower state WL): one side of each
+    double Wf[5], J[25];
+    FaceSide<RECON, LIM>(b, P.kappa, d, i, j, k, true, Wf);
     double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
-    RusanovJacobian<double>(g, WR, a.x, a.y, a.z, a.w, false, J);
+    RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, false, J);
 #pragma unroll
     for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
     const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
-    FaceStates<RECON, LIM>(b, P.kappa, d, ui, uj, uk, WL, WR);
+    FaceSide<RECON, LIM>(b, P.kappa, d, ui, uj, uk, false, Wf);
     a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
-    RusanovJacobian<double>(g, WL, a.x, a.y, a.z, a.w, true, J);
+    RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, true, J);
 #pragma unroll
     for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
   }
@@ -2496,8 +2546,14 @@ struct PenShared {
 __device__ __forceinline__ int LdsLoad(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// publish a step counter for the whole wave: the wave barrier is a
+// convergent point the compiler cannot duplicate into a lane-divergent
+// branch, the release fence waits for every lane's earlier LDS and global
+// writes, and then one lane stores the count
 __device__ __forceinline__ void LdsStore(int *p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (__lane_id() == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 // wait until an LDS counter reaches `need`; uniform across the wave
 __device__ __forceinline__ void LdsWait(const int *p, int need, int *err) {
@@ -2518,13 +2574,15 @@ __device__ __forceinline__ bool InBlock(const DevBlock &b, int i, int j, int k) 
 }
 
 // Lane divergence and the step counters.  A counter is stored after data
-// written by every lane of the wave; a lane-divergent branch between the
-// data and the store would let the compiler duplicate the store into both
-// sides of the branch and run the idle lanes' copy first, publishing the
-// counter before the active lanes have written.  So everything from a
-// wave's loads to its counter store is straight-line code: idle lanes and
-// absent neighbours compute on clamped, valid addresses, their results are
-// selected away, and their stores go to a per-pencil sink.
+// written by every lane of the wave.  If the counter store sat in a
+// lane-divergent region, the compiler could duplicate it into both sides of
+// a branch and run the idle lanes' copy first, publishing the counter before
+// the active lanes have written.  Two things prevent that: LdsStore joins
+// the wave at a wave barrier and fences before a single lane stores the
+// counter, and everything from a wave's loads to its counter store is
+// straight-line code (idle lanes and absent neighbours compute on clamped,
+// valid addresses, their results are selected away, and their stores go to
+// a per-pencil sink).
 
 // stager: prepare step st into `sl`
 template <typename T, bool FWD, bool FULL>

@@ -112,7 +112,7 @@ struct ath_ctx {
   int sweepOrder = ATH_SWEEP_PENCIL;  // ATH_SWEEP_*
   double cfl = 0;
   double l2first[ATH_MAX_EQ]{};
-  // graph of one nonlinear iteration (after the boundary pass)
+  // graph of one whole nonlinear iteration (ghost cells to norms)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphFp32 = -1, graphMatres = -1, graphOrder = -1;
@@ -1354,7 +1354,12 @@ int BuildGraph(ath_ctx *c) {
     return ATH_OK;
   if (c->graphExec) { hipGraphExecDestroy(c->graphExec); c->graphExec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  // the whole nonlinear iteration: ghost cells, residual stage, relax,
+  // update and norms
   HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  LaunchBoundary(c);
+  LaunchResidual(c);
+  LaunchAfterResiduals(c);
   LaunchTail(c, false, 0);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -1629,14 +1634,17 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
     const int nn = nn0 + s;
     if (int r = ath_begin_step(c, nn)) return r;
     for (int mm = 0; mm < nl; ++mm, ++it) {
-      LaunchBoundary(c);
-      const size_t eb = 2 + (size_t)it * perIt;
-      if (c->timing) hipEventRecord(c->evs[eb], c->stream);
-      LaunchResidual(c);
-      if (c->timing) hipEventRecord(c->evs[eb + 1], c->stream);
-      LaunchAfterResiduals(c);
-      if (c->timing) sweepLaunches += LaunchTail(c, true, eb + 2 * nflux);
-      else HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
+      if (c->timing) {
+        LaunchBoundary(c);
+        const size_t eb = 2 + (size_t)it * perIt;
+        hipEventRecord(c->evs[eb], c->stream);
+        LaunchResidual(c);
+        hipEventRecord(c->evs[eb + 1], c->stream);
+        LaunchAfterResiduals(c);
+        sweepLaunches += LaunchTail(c, true, eb + 2 * nflux);
+      } else {
+        HIP_OK(hipGraphLaunch(c->graphExec, c->stream));
+      }
       if (c->P.multilevel && mm == nl - 1) LaunchCopyNm1(c);
       if (int r = CheckLaunch()) return r;
       // the norm history holds histCap iterations; drain it when full
@@ -2258,6 +2266,9 @@ int ath_server_open(int device, double hbm_budget, int max_slots, int precision,
   if (int r = c->Alloc(&c->dflag, 1)) return r;
   HIP_OK(hipMemset(c->dflag, 0, sizeof(int)));
   c->fp32 = precision == ATH_IMPLICIT_FP32 ? 1 : 0;
+  // served cases report no matrix residual (the reference computes it for
+  // its log column only); it would cost a fourth off-diagonal pass
+  c->matres = 0;
   s->precision = precision;
   s->maxSlots = max_slots;
   s->budget = hbm_budget;

@@ -88,13 +88,13 @@ def parse():
     p.add_argument("--no-fp32-leg", action="store_true")
     p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
                    help="scalar LU-SGS execution order (all give identical results)")
-    p.add_argument("--requests", type=int, default=16, help="serve: requests (the full stream is 256)")
+    p.add_argument("--requests", type=int, default=8, help="serve: requests (the full stream is 256)")
     p.add_argument("--nmin", type=int, default=24, help="serve: smallest cells per direction")
-    p.add_argument("--nmax", type=int, default=48, help="serve: largest cells per direction (full: 96)")
+    p.add_argument("--nmax", type=int, default=40, help="serve: largest cells per direction (full: 96)")
     p.add_argument("--seed", type=int, default=1, help="serve: request and arrival seed")
     p.add_argument("--serve-cfl", type=float, default=20.0)
     p.add_argument("--serve-step", type=int, default=20, help="serve: iterations per batch step / chunk")
-    p.add_argument("--serve-load", type=float, default=0.8,
+    p.add_argument("--serve-load", type=float, default=1.5,
                    help="serve: offered load as a fraction of the per-case path's capacity")
     p.add_argument("--max-iterations", type=int, default=20000, help="serve: iteration cap per request")
     p.add_argument("--max-slots", type=int, default=256, help="serve: cases running at once")

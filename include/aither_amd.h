@@ -198,8 +198,9 @@ typedef struct {
   int sweep_launches;       /* sweep kernel launches in those passes */
   int sweep_passes;
 } ath_timing;
-/* Timed runs launch the work after the residual eagerly (untimed runs replay
- * a captured HIP graph) so that events can bracket every sweep pass. */
+/* Timed runs launch every kernel eagerly (untimed runs replay each whole
+ * nonlinear iteration from a captured HIP graph) so that events can bracket
+ * every residual stage and sweep pass. */
 int ath_enable_timing(ath_ctx *ctx, int on);
 int ath_get_timing(const ath_ctx *ctx, ath_timing *out);
 
@@ -265,7 +266,8 @@ typedef struct {
                            (PrintResiduals, src/output.cpp:1028-1063) over the
                            equations whose first residual carries a signal
                            (first L2 >= 1e-10 of the largest) */
-  ath_norms last;       /* norms of the last iteration */
+  ath_norms last;       /* norms of the last iteration (matrix_resid is 0:
+                           served cases skip the matrix residual) */
 } ath_slot_info;
 
 /* `hbm_budget`: device bytes the resident cases may reserve together (0: no
