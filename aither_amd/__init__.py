@@ -246,6 +246,7 @@ class Solver:
     TILED = 0
     HYPERPLANE = 1
     PENCIL = 2
+    ROWS = 3
 
     def set_sweep_ordering(self, ordering):
         _check(self._lib.ath_set_sweep_ordering(self._h, ordering))

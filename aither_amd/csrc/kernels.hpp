@@ -12,61 +12,77 @@ namespace ath {
 
 struct BcStateDev;
 
+// The device pointers of a block point to global memory.  In the device pass
+// they carry the global address space, so that the compiler emits global
+// loads and stores rather than flat ones: a flat access also counts against
+// the LDS wait counter, so every wait for LDS would wait for outstanding
+// global loads as well.  The host pass sees plain pointers of the same size.
+#ifdef __HIP_DEVICE_COMPILE__
+#define ATH_GLOBAL __attribute__((address_space(1)))
+#else
+#define ATH_GLOBAL
+#endif
+// a device allocation as a block's pointer
+template <typename T>
+__host__ __device__ inline ATH_GLOBAL T *Dev(T *p) {
+  return (ATH_GLOBAL T *)p;
+}
+
 struct DevBlock {
   int ni, nj, nk, g;
   int NI, NJ;
   long npad, ncell;
-  double *W;             // 5 * npad
-  const double *cw;      // 3 * npad
-  const double4 *fa[3];  // physical faces per direction
-  const double *vol;     // ncell
-  double *R;             // 5 * ncell
-  double *sr, *dt, *A;   // ncell
-  double *Ainv;          // ncell (FP64 operator)
-  float *Ainv32;         // ncell (FP32 operator)
-  double *consN, *consNm1;  // 5 * ncell
-  double *x;             // 5 * ncell (FP64 operator)
-  float *x32;            // 5 * ncell (FP32 operator)
-  double *xold;          // 5 * ncell, DP-LUR Jacobi copy
-  float *xold32;
-  double *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
+  double ATH_GLOBAL *W;             // 5 * npad
+  const double ATH_GLOBAL *cw;      // 3 * npad
+  const double4 ATH_GLOBAL *fa[3];  // physical faces per direction
+  const double ATH_GLOBAL *vol;     // ncell
+  double ATH_GLOBAL *R;             // 5 * ncell
+  double ATH_GLOBAL *sr, *dt, *A;   // ncell
+  double ATH_GLOBAL *Ainv;          // ncell (FP64 operator)
+  float ATH_GLOBAL *Ainv32;         // ncell (FP32 operator)
+  double ATH_GLOBAL *consN, *consNm1;  // 5 * ncell
+  double ATH_GLOBAL *x;             // 5 * ncell (FP64 operator)
+  float ATH_GLOBAL *x32;            // 5 * ncell (FP32 operator)
+  double ATH_GLOBAL *xold;          // 5 * ncell, DP-LUR Jacobi copy
+  float ATH_GLOBAL *xold32;
+  double ATH_GLOBAL *rhs;           // 5 * ncell, LU-SGS right-hand side (pencil sweep)
   // block matrix solvers: main diagonal after AddDiagonalTerms, 25 * ncell,
   // entry (r, c) of cell p at (5 r + c) ncell + p; its inverse in the
   // tile-blocked layout of TileBase (the sweep's 8 x 8 x 4 tiles: one tile's
   // values of one entry are 256 consecutive doubles, so a wavefront reads
   // whole cache lines no other tile shares).  The off-diagonal blocks are
   // formed where they are used (NeighbourBlock).
-  double *Ab, *Abi;
-  float *Abi32;       // FP32 operator: Abi rounded to float, same layout
+  double ATH_GLOBAL *Ab, *Abi;
+  float ATH_GLOBAL *Abi32;       // FP32 operator: Abi rounded to float, same layout
   int tni, tnj, tnk;  // tile grid of the layout
   // block solvers with the viscous terms: the velocity gradient the viscous
   // pass formed at every physical face of direction d, 9 * faces(d), entry q
   // of face f at q * faces(d) + f; the thin-shear-layer part of the block
   // diagonal reads it back (and forms the face state, viscosity and
   // conductivity again from the cells, with the same functions)
-  double *fvis[3];
+  double ATH_GLOBAL *fvis[3];
   // off-diagonal product of the ghost neighbour across each boundary face
   // that lies on an interblock connection (zero on other faces), per sweep
   // pass; null when the block has no connection.  5 * nbf, SoA.
-  double *bt;            // FP64 operator
-  float *bt32;           // FP32 operator
+  double ATH_GLOBAL *bt;            // FP64 operator
+  float ATH_GLOBAL *bt32;           // FP32 operator
   long nbf;              // boundary faces: 2 (nj nk + nk ni + ni nj)
   // viscous cases only
-  double *T, *mu;               // padded: temperature, viscosity
-  double *kc;                   // padded: conductivity before scaling (block solvers only)
-  const double4 *faP[3];        // padded faces (ghost faces included)
-  const double *volP;           // padded volumes
-  const double *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
-  const unsigned char *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
-  double *vgrad;                // block solvers: padded cell velocity gradient, 9 * npad, d v_c / d x_r at (3 r + c) npad
+  double ATH_GLOBAL *T, *mu;               // padded: temperature, viscosity
+  double ATH_GLOBAL *kc;                   // padded: conductivity before scaling (block solvers only)
+  const double4 ATH_GLOBAL *faP[3];        // padded faces (ghost faces included)
+  const double ATH_GLOBAL *volP;           // padded volumes
+  const double ATH_GLOBAL *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
+  const unsigned char ATH_GLOBAL *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
+  double ATH_GLOBAL *vgrad;                // block solvers: padded cell velocity gradient, 9 * npad, d v_c / d x_r at (3 r + c) npad
   long cellBase;         // cell offset of this block within its case (L-inf index)
   int blockId;           // block index within its case
   // per case: the CFL number of the current iteration, the device error bits
   // (1 nonphysical state, 4 singular block diagonal) and the boundary states
   // by tag
-  const double *cfl;
-  int *flag;
-  const BcStateDev *states;
+  const double ATH_GLOBAL *cfl;
+  int ATH_GLOBAL *flag;
+  const BcStateDev ATH_GLOBAL *states;
 };
 
 // laminar viscous terms: Sutherland transport (src/transport.cpp:29-176),
@@ -88,6 +104,7 @@ struct Params {
   int block;  // matrixSolver blusgs / bdplur: 5x5 main diagonal and off-diagonals
   int fp32;   // the scalar implicit operator runs in FP32: the residual stage writes Ainv32, else Ainv
   int roeJac; // inviscidFluxJacobian approximateRoe: the off-diagonals are Roe flux changes (FP64)
+  int rhsOnce; // the relax forms the right-hand side once, into rhs (the row-parallel block sweep reads it)
 };
 
 // ---------------------------------------------------------------------------
@@ -1310,7 +1327,7 @@ __global__ void __launch_bounds__(256) KInitUpdate(BlockMap M, Params P) {
   const long p = (long)U.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
   T *x = OpBuf<T>::X(b);
-  if (!P.initMatrix) {
+  if (!P.initMatrix && !P.rhsOnce) {
 #pragma unroll
     for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = T(0);
     return;
@@ -1318,6 +1335,17 @@ __global__ void __launch_bounds__(256) KInitUpdate(BlockMap M, Params P) {
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   double rb[5];
   Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+  if (P.rhsOnce) {
+    // the right-hand side is fixed for the relax: formed once here, read by
+    // every sweep (b.rhs is FP64; the operator rounds it to its precision)
+#pragma unroll
+    for (int e = 0; e < 5; ++e) b.rhs[e * b.ncell + p] = rb[e];
+  }
+  if (!P.initMatrix) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) x[e * b.ncell + p] = T(0);
+    return;
+  }
   if (P.block) {
     T v[5], o[5];
 #pragma unroll
@@ -2366,6 +2394,376 @@ __global__ void __launch_bounds__(kBT, sizeof(T) == 4 ? 2 : 1) KBlockSweepTiles(
       sTicket = sNext;
     }
     __syncthreads();
+  }
+}
+
+// A full block LU-SGS sweep's other-side sums (OffDiagSum over the upper
+// neighbours in a forward pass, the lower ones in a backward pass, from the
+// previous sweep's du and this pass's connection terms), formed for every
+// cell before the pass into xold: none of the du they read is written by the
+// pass before the cell's own tile publishes, so the values are those the
+// sweep would form in place.
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) KOtherSide(BlockMap M, Params P) {
+  const Unit U = MapUnit(M);
+  const DevBlock &b = *U.b;
+  const long p = (long)U.local * blockDim.x + threadIdx.x;
+  if (p >= b.ncell) return;
+  const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
+  const GasT<T> g = Cast<T>(P.gas);
+  T o[5];
+  OffDiagSum<T>(b, g, P.visc, OpBuf<T>::X(b), i, j, k, FWD, o, 1, 0, &P.gas);
+  T *xo = OpBuf<T>::Xold(b);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) xo[e * b.ncell + p] = o[e];
+}
+
+// ---------------------------------------------------------------------------
+// Row-parallel block LU-SGS sweep over all blocks (blusgs).
+//
+// The same dependency scheme as KBlockSweepTiles (tiles queued by tile
+// hyperplane, a tile starts once the tiles of its block that it depends on
+// have published), with the work of a tile spread over eleven wavefronts and
+// the loads of the next tile overlapped with the current one:
+//
+// * Forming the operator: roles of two wavefronts each, one lane per cell of
+//   the 8 x 4 x 4 tile.  Roles 0-2 form the dependency-side block of
+//   direction i, j, k into LDS; role 3 fills the halo columns that lie
+//   outside the block (connection terms) and, backward, the positions of a
+//   partial tile just past the block.  A full sweep's other-side sums come
+//   from KOtherSide, launched before the pass.
+// * The walk: five lanes per cell, twelve cells per wavefront.  Per cell
+//   hyperplane, lane r forms row r of the dependency sum (the blocks' rows
+//   from LDS) and of v, the cell's five lanes exchange v, and lane r forms
+//   row r of D^-1 v (its row of D^-1 in registers).  Cells are given to lane
+//   groups in hyperplane order, so a hyperplane occupies few wavefronts.
+// * Pipelining: the next ticket is taken when a tile starts, and once the
+//   tile's operator is formed the forming lanes issue the loads of the next
+//   tile's operator into registers; they arrive while this tile waits, walks
+//   and publishes.  The right-hand side is formed once per relax by
+//   KInitUpdate.
+//
+// Each row is summed in the order BlockMult sums it and the dependency sum
+// adds the directions in i, j, k order from +0, so the result is that of
+// KLusgsForward / KLusgsBackward bit for bit.  Eleven wavefronts leave 168
+// registers per lane.
+constexpr int kRTI = 8, kRTJ = 4, kRTK = 4;
+constexpr int kRT = kRTI * kRTJ * kRTK;
+constexpr int kRTP = kRT + 1;  // padded row of a block entry: the five rows of a cell in distinct banks
+constexpr int kRCellsPerWave = 12;  // five lanes each, four lanes idle
+constexpr int kRWaves = (kRT + kRCellsPerWave - 1) / kRCellsPerWave;
+constexpr int kRThreads = kRWaves * 64;
+constexpr int kRPlanes = kRTI + kRTJ + kRTK - 2;
+constexpr int kRHaloI = kRTJ * kRTK, kRHaloJ = kRTI * kRTK, kRHaloK = kRTI * kRTJ;
+constexpr int kRHalo = kRHaloI + kRHaloJ + kRHaloK;
+static_assert(kRThreads == 704 && kRHalo <= kRT && 5 * kRT <= kRThreads, "row sweep layout");
+
+// the loads of NeighbourBlock (the neighbour's state, velocity gradient,
+// viscosity, conductivity, the shared face and its centre distance)
+struct NbIn {
+  double W[5];
+  double vg[9];
+  double mu, kc, c2c;
+  double4 a;
+};
+__device__ __forceinline__ void NbLoad(const DevBlock &b, const ViscParams &vp, int d, bool upper, int i, int j,
+                                       int k, NbIn &in) {
+  const int s = upper ? 1 : -1;
+  const int ni = i + (d == 0 ? s : 0), nj = j + (d == 1 ? s : 0), nk = k + (d == 2 ? s : 0);
+  const long f = upper ? FaceIdx(b, d, ni, nj, nk) : FaceIdx(b, d, i, j, k);
+  in.a = b.fa[d][f];
+  const long c = PadIdx(b, ni, nj, nk);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) in.W[e] = b.W[e * b.npad + c];
+  if (vp.on) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) in.vg[q] = b.vgrad[q * b.npad + c];
+    in.mu = b.mu[c];
+    in.kc = b.kc[c];
+    in.c2c = b.c2c[d][f];
+  }
+}
+// NeighbourBlock's arithmetic on loaded inputs, written to LDS entry by entry
+// (entry q at dst[q * kRTP]).  The thin-shear-layer matrix is formed first
+// and parked in `stage` (entry q at stage[q * kRTP]; `stage` may be `dst`
+// when T is double), then the Rusanov Jacobian is combined with it entry by
+// entry, so that the two 5x5 matrices are never in registers together.
+template <typename T>
+__device__ __forceinline__ void NbFormLds(const GasT<double> &g, const ViscParams &vp, bool upper, const NbIn &in,
+                                          double *stage, T *dst) {
+#pragma clang fp contract(off)
+  const bool positive = !upper;
+  if (vp.on) {
+    double V[25];
+    TSLJacobian(g, vp, in.W, in.mu, in.kc, in.a, in.c2c, positive, in.vg, V);
+#pragma unroll
+    for (int q = 0; q < 25; ++q) stage[q * kRTP] = V[q];
+  }
+  double J[25];
+  RusanovJacobian<double>(g, in.W, in.a.x, in.a.y, in.a.z, in.a.w, positive, J);
+  if (vp.on) {
+#pragma unroll
+    for (int q = 0; q < 25; ++q) {
+      const double v = stage[q * kRTP];
+      dst[q * kRTP] = T(positive ? J[q] - v : J[q] + v);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 25; ++q) dst[q * kRTP] = T(J[q]);
+  }
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(kRThreads) KBlockSweepRows(const DevBlock *__restrict__ blocks, Params P,
+                                                             MultiSweepCtl s, int full) {
+  __shared__ T sA[5][kRT + kRHalo];  // du of the tile's cells, then of the halo columns
+  __shared__ T sM[3][25][kRTP];      // dependency-side blocks, entry-major
+  // FP32 operator: the thin-shear-layer matrices in FP64 while the blocks
+  // are formed (the FP64 operator parks them in sM itself)
+  __shared__ double sV[sizeof(T) == sizeof(double) ? 1 : 3][25][kRTP];
+  __shared__ unsigned char sCell[kRT];  // walk group -> cell, in hyperplane order
+  __shared__ int sTicket, sNext, sEpoch;
+
+  const int tid = threadIdx.x;
+  const int sgn = FWD ? -1 : 1;  // direction of the dependency neighbours
+  // the forming roles: one lane per cell, two wavefronts per role
+  const int pc = tid % kRT, role = tid / kRT;
+  const int pli = pc % kRTI, plj = (pc / kRTI) % kRTJ, plk = pc / (kRTI * kRTJ);
+  // the walk: group g of five lanes, row r; the group's first lane in its
+  // wavefront is gbase
+  const int lane = tid & 63;
+  const int g = (tid >> 6) * kRCellsPerWave + lane / 5, r = lane % 5, gbase = lane - r;
+  const bool glane = lane < 5 * kRCellsPerWave && g < kRT;
+  const bool connZero = FWD && !full;
+
+  if (tid < kRT) {
+    // rank of cell tid among the tile's cells by (hyperplane, index)
+    const int li = tid % kRTI, lj = (tid / kRTI) % kRTJ, lk = tid / (kRTI * kRTJ);
+    const int pl = li + lj + lk;
+    int rank = 0;
+    for (int q = 0; q < kRT; ++q) {
+      const int qp = q % kRTI + (q / kRTI) % kRTJ + q / (kRTI * kRTJ);
+      rank += (qp < pl || (qp == pl && q < tid)) ? 1 : 0;
+    }
+    sCell[rank] = (unsigned char)tid;
+  }
+  if (tid == 0) {
+    sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sEpoch = __hip_atomic_load(s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const int epoch = sEpoch;
+  const GasT<double> gd = Cast<double>(P.gas);
+  const int wc = glane ? sCell[g] : 0;
+  const int wli = wc % kRTI, wlj = (wc / kRTI) % kRTJ, wlk = wc / (kRTI * kRTJ);
+
+  // the forming role's loads of a tile, issued a tile ahead
+  NbIn pin;  // roles 0-2: the dependency-side block's inputs; role 3: pin.W holds a halo column's connection term
+  auto prefetch = [&](int t) {
+    const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
+    const DevBlock &b = blocks[tl.w];
+    const int ni = b.ni, nj = b.nj, nk = b.nk;
+    const int i = tl.x * kRTI + pli, j = tl.y * kRTJ + plj, k = tl.z * kRTK + plk;
+    if (role < 3) {
+      const int d = role;
+      const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+      const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < ni && gj < nj && gk < nk;
+      if (i < ni && j < nj && k < nk && inside) NbLoad(b, P.visc, d, !FWD, i, j, k, pin);
+    } else if (role == 3 && pc < kRHalo) {
+      const T *bt = OpBuf<T>::Bt(b);
+      const int h = pc;
+      const int di = FWD ? -1 : kRTI, dj = FWD ? -1 : kRTJ, dk = FWD ? -1 : kRTK;
+      int hd, hi, hj, hk;
+      if (h < kRHaloI) { hd = 0; hi = di; hj = h % kRTJ; hk = h / kRTJ; }
+      else if (h < kRHaloI + kRHaloJ) { const int q = h - kRHaloI; hd = 1; hi = q % kRTI; hj = dj; hk = q / kRTI; }
+      else { const int q = h - kRHaloI - kRHaloJ; hd = 2; hi = q % kRTI; hj = q / kRTI; hk = dk; }
+      hi += tl.x * kRTI; hj += tl.y * kRTJ; hk += tl.z * kRTK;
+      const bool hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
+      // the tile cell next to it along hd
+      const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
+      const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
+      if (!hin && face && bt && !connZero) {
+        const long q = BtSlot(b, hd, FWD ? 0 : 1, ci, cj, ck);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) pin.W[e] = double(bt[e * b.nbf + q]);
+      }
+    }
+  };
+
+  int t = sTicket;
+  if (t < s.ntiles) prefetch(t);
+  for (;;) {
+    if (t >= s.ntiles) break;
+    int nxt = 0;
+    if (tid == 0) nxt = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
+    const DevBlock &b = blocks[tl.w];
+    const int4 gr = s.grid[tl.w];
+    T *x = OpBuf<T>::X(b);
+    const int ni = b.ni, nj = b.nj, nk = b.nk;
+    const long ncell = b.ncell, nbf = b.nbf;
+    const T *bt = OpBuf<T>::Bt(b);
+
+    // --- forming, one lane per cell (wave-uniform roles)
+    const int i = tl.x * kRTI + pli, j = tl.y * kRTJ + plj, k = tl.z * kRTK + plk;
+    const bool valid = i < ni && j < nj && k < nk;
+    int hi = 0, hj = 0, hk = 0;
+    bool hin = false;
+    if (role < 3) {
+      const int d = role;
+      double *stage = sizeof(T) == sizeof(double) ? reinterpret_cast<double *>(&sM[d][0][pc])
+                                                  : &sV[0][0][pc] + d * 25 * kRTP;
+      if (valid) {
+        const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
+        const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < ni && gj < nj && gk < nk;
+        // a neighbour outside the block enters through its connection term,
+        // which its LDS column holds; the identity block passes it on
+        // unchanged (x + 0 y = x, and the sum never starts from -0)
+        if (inside) {
+          NbFormLds<T>(gd, P.visc, !FWD, pin, stage, &sM[d][0][pc]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 25; ++q) sM[d][q][pc] = q % 6 == 0 ? T(1) : T(0);
+        }
+      }
+    } else if (role == 3) {
+      if (pc < kRHalo) {
+        // halo columns: positions outside the block hold connection terms
+        // now, those inside the du of the dependency tiles after the wait
+        const int h = pc;
+        const int di = FWD ? -1 : kRTI, dj = FWD ? -1 : kRTJ, dk = FWD ? -1 : kRTK;
+        int hd;
+        if (h < kRHaloI) { hd = 0; hi = di; hj = h % kRTJ; hk = h / kRTJ; }
+        else if (h < kRHaloI + kRHaloJ) { const int q = h - kRHaloI; hd = 1; hi = q % kRTI; hj = dj; hk = q / kRTI; }
+        else { const int q = h - kRHaloI - kRHaloJ; hd = 2; hi = q % kRTI; hj = q / kRTI; hk = dk; }
+        hi += tl.x * kRTI; hj += tl.y * kRTJ; hk += tl.z * kRTK;
+        hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
+        if (!hin) {
+          const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
+          const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
+          const bool use = face && bt && !connZero;
+#pragma unroll
+          for (int e = 0; e < 5; ++e) sA[e][kRT + h] = use ? T(pin.W[e]) : T(0);
+        }
+      }
+      if (!FWD && !valid) {
+        // a partial tile's position just past the block's upper face along
+        // one axis is the dependency neighbour of the face cell below it
+        const int oi = i >= ni, oj = j >= nj, ok = k >= nk;
+        const int d = oi ? 0 : (oj ? 1 : 2);
+        const bool edge = oi + oj + ok == 1 && (oi ? i == ni : (oj ? j == nj : k == nk));
+        const bool use = edge && bt && !connZero;
+        const long q = use ? BtSlot(b, d, 1, i - (d == 0), j - (d == 1), k - (d == 2)) : 0;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) sA[e][pc] = use ? bt[e * nbf + q] : T(0);
+      }
+    }
+    if (tid == 0) sNext = nxt;
+    __syncthreads();
+
+    // --- the walk lane's cell, its dependency neighbours' LDS columns (a
+    // cell of the tile or a halo column) and its row of D^-1, right-hand
+    // side and forward result; then the next tile's loads
+    const int tn = sNext;
+    const int wplane = wli + wlj + wlk;
+    int nb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int l3 = d == 0 ? wli : (d == 1 ? wlj : wlk);
+      const int L3 = d == 0 ? kRTI : (d == 1 ? kRTJ : kRTK);
+      const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
+      nb[d] = inTile ? wc + sgn * (d == 0 ? 1 : (d == 1 ? kRTI : kRTI * kRTJ))
+                     : kRT + (d == 0 ? wlj + kRTJ * wlk
+                                     : (d == 1 ? kRHaloI + wli + kRTI * wlk : kRHaloI + kRHaloJ + wli + kRTI * wlj));
+    }
+    const int wi = tl.x * kRTI + wli, wj = tl.y * kRTJ + wlj, wk = tl.z * kRTK + wlk;
+    const bool wvalid = glane && wi < ni && wj < nj && wk < nk;
+    T ai[5], rb = T(0), xf = T(0), ost = T(0);
+    if (wvalid) {
+      const auto *Mi = OpBuf<T>::Abi(b) + TileBase(b, wi, wj, wk);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) ai[q] = T(Mi[256 * (5 * r + q)]);
+      const long p = (long)wi + (long)wj * ni + (long)wk * ni * nj;
+      if (FWD || full) rb = T(b.rhs[r * ncell + p]);
+      if (!FWD && !full) xf = x[r * ncell + p];
+    }
+
+    if (full && wvalid) ost = OpBuf<T>::Xold(b)[r * ncell + (long)wi + (long)wj * ni + (long)wk * ni * nj];
+    // the next tile's forming loads, in flight while this tile waits, walks
+    // and publishes
+    if (tn < s.ntiles) prefetch(tn);
+
+    // --- wait for the tiles of the same block this one depends on
+    if (tid == 0) {
+      const int deps[3][3] = {{tl.x + sgn, tl.y, tl.z}, {tl.x, tl.y + sgn, tl.z}, {tl.x, tl.y, tl.z + sgn}};
+      for (int q = 0; q < 3; ++q) {
+        const int a = deps[q][0], bb = deps[q][1], cc = deps[q][2];
+        if (a < 0 || bb < 0 || cc < 0 || a >= gr.x || bb >= gr.y || cc >= gr.z) continue;
+        const int id = gr.w + a + gr.x * (bb + gr.y * cc);
+        long spins = 0;
+        while (__hip_atomic_load(&s.flags[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+          if ((++spins & 255) == 0 &&
+              (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2))
+            break;
+          if (spins > kSpinLimit) {
+            __hip_atomic_fetch_or(s.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      // one acquire for the workgroup; the barrier below orders it before
+      // every wave's halo loads
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (hin) {
+      const long q = (long)hi + (long)hj * ni + (long)hk * ni * nj;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sA[e][kRT + pc] = x[e * ncell + q];
+    }
+    __syncthreads();
+
+    // --- intra-tile hyperplanes
+    for (int st = 0; st < kRPlanes; ++st) {
+      const int pl = FWD ? st : kRPlanes - 1 - st;
+      if (wvalid && wplane == pl) {
+        // row r of the dependency-side sum: L (forward) or U (backward)
+        T D = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          T o = T(0);
+#pragma unroll
+          for (int q = 0; q < 5; ++q) o += sM[d][5 * r + q][wc] * sA[q][nb[d]];
+          D += o;
+        }
+        T v;
+        if (full) v = FWD ? rb + (D - ost) : (rb + ost) - D;
+        else v = FWD ? rb + D : D;
+        // the cell's five rows of v (or of D, backward), from its five lanes
+        T vv[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) vv[q] = __shfl(v, gbase + q, 64);
+        T o = T(0);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) o += ai[q] * vv[q];
+        sA[r][wc] = (!full && !FWD) ? xf - o : o;
+      }
+      __syncthreads();
+    }
+    if (tid < 5 * kRT) {
+      const int e = tid / kRT, c = tid % kRT;
+      const int ci = tl.x * kRTI + c % kRTI, cj = tl.y * kRTJ + (c / kRTI) % kRTJ, ck = tl.z * kRTK + c / (kRTI * kRTJ);
+      if (ci < ni && cj < nj && ck < nk) x[e * ncell + (long)ci + (long)cj * ni + (long)ck * ni * nj] = sA[e][c];
+    }
+
+    // --- publish: the barrier orders every wave's du stores before thread
+    // 0's release store
+    __syncthreads();
+    if (tid == 0)
+      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    t = tn;
   }
 }
 

@@ -123,9 +123,13 @@ struct ath_ctx {
   int nhalo = 0;
   ath::ConnTask *dconn = nullptr;
   int nconn = 0;
-  // block solvers: one tiled sweep over all blocks
+  // block solvers: one tiled sweep over all blocks, 8 x 8 x 4 tiles with a
+  // lane per cell (msweep, ATH_SWEEP_TILED) or 8 x 4 x 4 tiles with the rows
+  // of each cell on their own lanes (rsweep, ATH_SWEEP_ROWS)
   ath::MultiSweepCtl msweep{};
   int msweepGrid = 0;
+  ath::MultiSweepCtl rsweep{};
+  int rsweepGrid = 0;
   // unit grids of the all-block launches (ath::BlockMap), kNumMaps of them,
   // nb entries each, and each one's total unit count
   int4 *dmaps = nullptr;
@@ -166,7 +170,7 @@ struct ath_ctx {
       return Fail(ATH_ERR_NOMEM, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
     (allocTo ? allocTo : &allocs)->push_back(q);
     if (bytesTo) *bytesTo += bytes;
-    *p = static_cast<T *>(q);
+    *p = (T *)q;
     return ATH_OK;
   }
 };
@@ -253,7 +257,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     double *v = nullptr;
     if (int r = c->Alloc(&v, np)) return r;
     HIP_OK(hipMemcpy(v, h.vol.data(), np * sizeof(double), hipMemcpyHostToDevice));
-    d.volP = v;
+    d.volP = ath::Dev(v);
   }
   for (int dd = 0; dd < 3; ++dd) {
     std::vector<double4> fa(h.NumFaces(dd));
@@ -264,7 +268,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     double4 *dfa = nullptr;
     if (int r = c->Alloc(&dfa, fa.size())) return r;
     HIP_OK(hipMemcpy(dfa, fa.data(), fa.size() * sizeof(double4), hipMemcpyHostToDevice));
-    d.faP[dd] = dfa;
+    d.faP[dd] = ath::Dev(dfa);
     // ProjC2CDist (src/procBlock.cpp:6316-6341) at every physical face
     const int FI = h.ni + (dd == 0), FJ = h.nj + (dd == 1), FK = h.nk + (dd == 2);
     std::vector<double> cc((size_t)FI * FJ * FK);
@@ -280,7 +284,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     double *dc = nullptr;
     if (int r = c->Alloc(&dc, cc.size())) return r;
     HIP_OK(hipMemcpy(dc, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
-    d.c2c[dd] = dc;
+    d.c2c[dd] = ath::Dev(dc);
   }
   // viscousWall flag of every boundary face (BtSlot order)
   {
@@ -301,7 +305,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     unsigned char *dw = nullptr;
     if (int r = c->Alloc(&dw, wf.size())) return r;
     HIP_OK(hipMemcpy(dw, wf.data(), wf.size(), hipMemcpyHostToDevice));
-    d.wallf = dw;
+    d.wallf = ath::Dev(dw);
   }
   // edge ghost cells, per line in the reference's layer order
   // (src/procBlock.cpp:2565-2714, 2874-3030)
@@ -535,9 +539,9 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
     const HostBlock &h = hc.blocks[bq];
     BlockRes &br = cr.blocks[bq];
     DevBlock &d = br.d;
-    d.cfl = cr.dcfl;
-    d.flag = cr.dflag;
-    d.states = cr.dstates;
+    d.cfl = ath::Dev(cr.dcfl);
+    d.flag = ath::Dev(cr.dflag);
+    d.states = ath::Dev(cr.dstates);
     d.ni = h.ni; d.nj = h.nj; d.nk = h.nk; d.g = h.g;
     d.NI = h.NI; d.NJ = h.NJ;
     d.npad = h.NumPadded();
@@ -564,7 +568,7 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
       double *dcw = nullptr;
       if (int r = c->Alloc(&dcw, 3 * np)) return r;
       HIP_OK(hipMemcpy(dcw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice));
-      d.cw = dcw;
+      d.cw = ath::Dev(dcw);
     }
     for (int dd = 0; dd < 3; ++dd) {
       const int FI = h.ni + (dd == 0), FJ = h.nj + (dd == 1), FK = h.nk + (dd == 2);
@@ -578,7 +582,7 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
       double4 *dfa = nullptr;
       if (int r = c->Alloc(&dfa, fa.size())) return r;
       HIP_OK(hipMemcpy(dfa, fa.data(), fa.size() * sizeof(double4), hipMemcpyHostToDevice));
-      d.fa[dd] = dfa;
+      d.fa[dd] = ath::Dev(dfa);
     }
     {
       std::vector<double> vol((size_t)nc);
@@ -588,7 +592,7 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
       double *dv = nullptr;
       if (int r = c->Alloc(&dv, nc)) return r;
       HIP_OK(hipMemcpy(dv, vol.data(), vol.size() * sizeof(double), hipMemcpyHostToDevice));
-      d.vol = dv;
+      d.vol = ath::Dev(dv);
     }
     if (int r = c->Alloc(&d.R, 5 * nc)) return r;
     if (int r = c->Alloc(&d.sr, nc)) return r;
@@ -775,6 +779,7 @@ int RebuildGroup(ath_ctx *c) {
   c->nparts = 0;
   c->pen = PencilGroup{};
   c->msweep = MultiSweepCtl{};
+  c->rsweep = MultiSweepCtl{};
   for (int q = 0; q < kNumMaps; ++q) c->mapTotal[q] = 0;
   for (int q = 0; q < kNumTaskMaps; ++q) c->taskTotal[q] = 0;
   if (nb == 0) return ATH_OK;
@@ -915,39 +920,46 @@ int RebuildGroup(ath_ctx *c) {
   // block solvers: one tiled sweep over all blocks, tiles by tile hyperplane,
   // then block, then k, j, i; per block its tile grid and first flag
   if (c->hc.block) {
-    std::vector<int4> grid, tiles;
-    int base = 0, maxPlane = 0;
-    for (auto &br : c->blk) {
-      const int ta = (br.d.ni + kBTI - 1) / kBTI, tb = (br.d.nj + kBTJ - 1) / kBTJ,
-                tc = (br.d.nk + kBTK - 1) / kBTK;
-      grid.push_back(make_int4(ta, tb, tc, base));
-      base += ta * tb * tc;
-      maxPlane = std::max(maxPlane, ta + tb + tc - 2);
-    }
-    for (int pl = 0; pl < maxPlane; ++pl)
-      for (size_t q = 0; q < grid.size(); ++q)
-        for (int cc = 0; cc < grid[q].z; ++cc)
-          for (int bb = 0; bb < grid[q].y; ++bb)
-            for (int aa = 0; aa < grid[q].x; ++aa)
-              if (aa + bb + cc == pl) tiles.push_back(make_int4(aa, bb, cc, (int)q));
-    MultiSweepCtl &m = c->msweep;
-    m.ntiles = (int)tiles.size();
-    int4 *dt = nullptr, *dg = nullptr;
-    if (int r = c->Alloc(&dt, tiles.size())) return r;
-    if (int r = c->Alloc(&dg, grid.size())) return r;
-    HIP_OK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dg, grid.data(), grid.size() * sizeof(int4), hipMemcpyHostToDevice));
-    m.tiles = dt;
-    m.grid = dg;
-    if (int r = c->Alloc(&m.flags, base)) return r;
-    if (int r = c->Alloc(&m.ticket, 2)) return r;
-    HIP_OK(hipMemset(m.flags, 0, base * sizeof(int)));
-    HIP_OK(hipMemset(m.ticket, 0, 2 * sizeof(int)));
-    m.epoch = m.ticket + 1;
-    m.err = c->dflag;
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, c->device));
-    c->msweepGrid = std::max(1, std::min(m.ntiles, 2 * prop.multiProcessorCount));
+    // the tiles of every block by tile hyperplane, then block, then k, j, i;
+    // per block its tile grid and first flag
+    auto queue = [&](int TI, int TJ, int TK, MultiSweepCtl &m) -> int {
+      std::vector<int4> grid, tiles;
+      int base = 0, maxPlane = 0;
+      for (auto &br : c->blk) {
+        const int ta = (br.d.ni + TI - 1) / TI, tb = (br.d.nj + TJ - 1) / TJ, tc = (br.d.nk + TK - 1) / TK;
+        grid.push_back(make_int4(ta, tb, tc, base));
+        base += ta * tb * tc;
+        maxPlane = std::max(maxPlane, ta + tb + tc - 2);
+      }
+      for (int pl = 0; pl < maxPlane; ++pl)
+        for (size_t q = 0; q < grid.size(); ++q)
+          for (int cc = 0; cc < grid[q].z; ++cc)
+            for (int bb = 0; bb < grid[q].y; ++bb)
+              for (int aa = 0; aa < grid[q].x; ++aa)
+                if (aa + bb + cc == pl) tiles.push_back(make_int4(aa, bb, cc, (int)q));
+      m.ntiles = (int)tiles.size();
+      int4 *dt = nullptr, *dg = nullptr;
+      if (int r = c->Alloc(&dt, tiles.size())) return r;
+      if (int r = c->Alloc(&dg, grid.size())) return r;
+      HIP_OK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(dg, grid.data(), grid.size() * sizeof(int4), hipMemcpyHostToDevice));
+      m.tiles = dt;
+      m.grid = dg;
+      if (int r = c->Alloc(&m.flags, base)) return r;
+      if (int r = c->Alloc(&m.ticket, 2)) return r;
+      HIP_OK(hipMemset(m.flags, 0, base * sizeof(int)));
+      HIP_OK(hipMemset(m.ticket, 0, 2 * sizeof(int)));
+      m.epoch = m.ticket + 1;
+      m.err = c->dflag;
+      return ATH_OK;
+    };
+    if (int r = queue(kBTI, kBTJ, kBTK, c->msweep)) return r;
+    if (int r = queue(kRTI, kRTJ, kRTK, c->rsweep)) return r;
+    c->msweepGrid = std::max(1, std::min(c->msweep.ntiles, 2 * prop.multiProcessorCount));
+    // one 1024-thread workgroup fits a CU
+    c->rsweepGrid = std::max(1, std::min(c->rsweep.ntiles, prop.multiProcessorCount));
   }
   return ATH_OK;
 }
@@ -1094,12 +1106,15 @@ void LaunchResidual(ath_ctx *c) {
 #undef ATH_RES
 }
 
-// the order the sweeps run in: the block solvers have the tiled and
-// hyperplane orders (the pencil order maps to the tiled one); the Roe
-// flux-change off-diagonals (approximateRoe) run in the hyperplane order only
+// the order the sweeps run in: the block solvers have the tiled,
+// row-parallel and hyperplane orders (the pencil order maps to the tiled
+// one), scalar LU-SGS the pencil, tiled and hyperplane orders (the
+// row-parallel order maps to the tiled one); the Roe flux-change
+// off-diagonals (approximateRoe) run in the hyperplane order only
 int EffectiveOrder(const ath_ctx *c) {
   if (c->P.roeJac) return ATH_SWEEP_HYPERPLANE;
   if (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL) return ATH_SWEEP_TILED;
+  if (!c->P.block && c->sweepOrder == ATH_SWEEP_ROWS) return ATH_SWEEP_TILED;
   return c->sweepOrder;
 }
 
@@ -1114,8 +1129,12 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
     if (timed) hipEventRecord(c->evs[evBase + 2 * pass + (begin ? 0 : 1)], c->stream);
     if (!begin) ++pass;
   };
+  // the row-parallel block sweep reads the right-hand side that the
+  // initialisation forms once per relax
+  ath::Params P0 = c->P;
+  P0.rhsOnce = (hc.solver == ath::Solver::LUSGS && c->P.block && EffectiveOrder(c) == ATH_SWEEP_ROWS) ? 1 : 0;
   hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
-                     c->Map(kMapCell256), c->P);
+                     c->Map(kMapCell256), P0);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     const int order = EffectiveOrder(c);
@@ -1140,19 +1159,38 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
         ++launches;
         mark(false);
       }
-    } else if (hc.solver == ath::Solver::LUSGS && order == ATH_SWEEP_TILED && c->P.block) {
+    } else if (hc.solver == ath::Solver::LUSGS && c->P.block &&
+               (order == ATH_SWEEP_TILED || order == ATH_SWEEP_ROWS)) {
       for (int dir = 0; dir < 2; ++dir) {
         // the first forward pass starts from du = 0: its connection terms
         // are zero and the sweep does not read them
         if (dir > 0 || full) LaunchConnTerms<T>(c);
         mark(true);
-        hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
-        if (dir == 0)
-          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, true>), dim3(c->msweepGrid), dim3(ath::kBT), 0, c->stream,
-                             c->dblocks, c->P, c->msweep, full);
-        else
-          hipLaunchKernelGGL((ath::KBlockSweepTiles<T, false>), dim3(c->msweepGrid), dim3(ath::kBT), 0, c->stream,
-                             c->dblocks, c->P, c->msweep, full);
+        if (order == ATH_SWEEP_ROWS) {
+          if (full) {
+            if (dir == 0)
+              hipLaunchKernelGGL((ath::KOtherSide<T, true>), dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0,
+                                 c->stream, c->Map(kMapCell256), c->P);
+            else
+              hipLaunchKernelGGL((ath::KOtherSide<T, false>), dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0,
+                                 c->stream, c->Map(kMapCell256), c->P);
+          }
+          hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->rsweep);
+          if (dir == 0)
+            hipLaunchKernelGGL((ath::KBlockSweepRows<T, true>), dim3(c->rsweepGrid), dim3(ath::kRThreads), 0,
+                               c->stream, c->dblocks, c->P, c->rsweep, full);
+          else
+            hipLaunchKernelGGL((ath::KBlockSweepRows<T, false>), dim3(c->rsweepGrid), dim3(ath::kRThreads), 0,
+                               c->stream, c->dblocks, c->P, c->rsweep, full);
+        } else {
+          hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
+          if (dir == 0)
+            hipLaunchKernelGGL((ath::KBlockSweepTiles<T, true>), dim3(c->msweepGrid), dim3(ath::kBT), 0,
+                               c->stream, c->dblocks, c->P, c->msweep, full);
+          else
+            hipLaunchKernelGGL((ath::KBlockSweepTiles<T, false>), dim3(c->msweepGrid), dim3(ath::kBT), 0,
+                               c->stream, c->dblocks, c->P, c->msweep, full);
+        }
         ++launches;
         mark(false);
       }
@@ -1435,7 +1473,7 @@ int ath_set_implicit_precision(ath_ctx *c, int precision) {
 
 int ath_set_sweep_ordering(ath_ctx *c, int ordering) {
   if (!c || (ordering != ATH_SWEEP_TILED && ordering != ATH_SWEEP_HYPERPLANE &&
-             ordering != ATH_SWEEP_PENCIL))
+             ordering != ATH_SWEEP_PENCIL && ordering != ATH_SWEEP_ROWS))
     return Fail(ATH_ERR_ARG, "bad sweep ordering");
   c->sweepOrder = ordering;
   return ATH_OK;
@@ -1687,8 +1725,9 @@ int ath_run(ath_ctx *c, int nn0, int nsteps, ath_norms *last) {
 }
 
 // Autotuning of the sweep execution order: every order the case admits
-// (pencil, tiled and hyperplane for scalar LU-SGS; the tiled all-block sweep
-// with one or two workgroups per CU for the block solvers) runs `iterations`
+// (pencil, tiled and hyperplane for scalar LU-SGS; the row-parallel sweep,
+// the tiled sweep with one or two workgroups per CU and the hyperplane order
+// for the block solvers) runs `iterations`
 // nonlinear iterations from the current state, timed by events around each
 // sweep pass; the fastest is kept.  All orders give the same results bit for
 // bit (topological orders of the same dependencies), so the choice changes
@@ -1729,6 +1768,7 @@ int ath_autotune(ath_ctx *c, int iterations, int *chosen, double *pass_ms) {
   if (c->P.roeJac || c->hc.solver != ath::Solver::LUSGS) {
     cands.push_back({EffectiveOrder(c), 0});
   } else if (c->P.block) {
+    cands.push_back({ATH_SWEEP_ROWS, 0});
     cands.push_back({ATH_SWEEP_TILED, 1});
     cands.push_back({ATH_SWEEP_TILED, 2});
     cands.push_back({ATH_SWEEP_HYPERPLANE, 0});

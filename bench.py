@@ -67,8 +67,16 @@ CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
-CHANNEL_SWEEP_TRAFFIC = 0.5 * (1336.7e6 + 1118.7e6)   # one pass, forward and backward averaged
-CHANNEL_FLUX_TRAFFIC = 3372.3e6                       # residual stage over all blocks
+CHANNEL_SWEEP_TRAFFIC = 0.5 * (1338.3e6 + 1119.8e6)   # one pass, forward and backward averaged
+CHANNEL_FLUX_TRAFFIC = 3253.5e6                       # residual stage over all blocks
+# what the occupancy and VALU counters show binds each (rocprofv3 SQ_WAVES,
+# SQ_INSTS_VALU, VALUBusy, OccupancyPercent on the same case; see
+# profiles/channel-multiblock-1M_sq_counters.md)
+SWEEP_LIMITER = ("dependency latency: one workgroup per CU (10.6 % occupancy), VALU 17 % busy, HBM about 20 %; "
+                 "the pass is the chain of tile hyperplanes of the largest block")
+FLUX_LIMITER = ("mixed: KResidual and the inviscid block diagonal FP64-VALU issue (VALUBusy 78 % and 69 %), the "
+                "viscous block diagonal HBM (61 %), the viscous flux the latency of its gathers (occupancy 23 %, "
+                "VALU 20 %, HBM 33 %)")
 
 
 def parse():
@@ -86,7 +94,7 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=None,
                    help="oracle iterations timed for cpu_baseline (default 3 channel, 20 vortex)")
     p.add_argument("--no-fp32-leg", action="store_true")
-    p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil"], default="default",
+    p.add_argument("--sweep", choices=["default", "hyperplane", "tiled", "pencil", "rows"], default="default",
                    help="scalar LU-SGS execution order (all give identical results)")
     p.add_argument("--requests", type=int, default=8, help="serve: requests (the full stream is 256)")
     p.add_argument("--nmin", type=int, default=24, help="serve: smallest cells per direction")
@@ -265,7 +273,10 @@ def bench_channel(args, Solver, cases):
     inp, meta = cases.channel_case(work, refine=args.refine, steps=max(1, -(-steps // 20)), subits=20)
     sub = meta["subits"]
     nphys = -(-steps // sub)
+    order = None if args.sweep == "default" else getattr(Solver, args.sweep.upper())
     s = Solver(inp)
+    if order is not None:
+        s.set_sweep_ordering(order)
     nb = s.info.num_blocks
     states0 = [s.get_state(b) for b in range(nb)]
     ncell = sum(s.ncells(b) for b in range(nb))
@@ -282,10 +293,11 @@ def bench_channel(args, Solver, cases):
     sweep = roofline("block LU-SGS sweep pass over all eight blocks (KBlockSweepTiles<double, FWD>, one "
                      "launch), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
                      traffic=CHANNEL_SWEEP_TRAFFIC if full else None,
-                     launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1))
+                     launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1),
+                     limiter=SWEEP_LIMITER)
     flux = roofline("residual stage over all eight blocks (inviscid + viscous fluxes, spectral radii, dt, block "
                     "diagonal and inverse; one launch per kernel)", CHANNEL_FLUX_BYTES * ncell, flux_avg_s,
-                    flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full else None)
+                    flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full else None, limiter=FLUX_LIMITER)
     dominant, other = (sweep, flux) if sweep_share >= flux_share else (flux, sweep)
     out = {
         "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
@@ -321,6 +333,8 @@ def bench_channel(args, Solver, cases):
         # and spectral radius stay FP64), and its converged-state error
         s32 = Solver(inp)
         s32.set_implicit_precision(Solver.FP32)
+        if order is not None:
+            s32.set_sweep_ordering(order)
         _, w32, tm32 = timed_run(s32, states0, steps, warm, prof_phys=min(nphys, 2))
         W32 = np.concatenate([s32.get_state(b).reshape(-1) for b in range(nb)])
         s32.close()

@@ -93,15 +93,19 @@ int ath_set_implicit_precision(ath_ctx *ctx, int precision);
  * on a skewed wavefront, one launch per sweep pass.  ATH_SWEEP_TILED: 8x8x4
  * tiles in tile-hyperplane order, dataflow-synchronised, one launch per pass.
  * ATH_SWEEP_HYPERPLANE: one launch per i+j+k hyperplane in the order of
- * HyperplaneReorder (src/utility.cpp:377-398).  All are topological orders of
- * the same Gauss-Seidel dependencies and give identical results. */
+ * HyperplaneReorder (src/utility.cpp:377-398).  ATH_SWEEP_ROWS (block
+ * solvers): 8x4x4 tiles in tile-hyperplane order with each cell's five rows
+ * on their own lanes, one launch per pass.  All are topological orders of the
+ * same Gauss-Seidel dependencies and give identical results. */
 #define ATH_SWEEP_TILED 0
 #define ATH_SWEEP_HYPERPLANE 1
 #define ATH_SWEEP_PENCIL 2
+#define ATH_SWEEP_ROWS 3
 int ath_set_sweep_ordering(ath_ctx *ctx, int ordering);
 /* the order the sweeps actually run in: block solvers (blusgs, bdplur) map
- * ATH_SWEEP_PENCIL to ATH_SWEEP_TILED, and the approximateRoe flux Jacobian
- * always runs ATH_SWEEP_HYPERPLANE, whatever was set */
+ * ATH_SWEEP_PENCIL to ATH_SWEEP_TILED, scalar LU-SGS maps ATH_SWEEP_ROWS to
+ * ATH_SWEEP_TILED, and the approximateRoe flux Jacobian always runs
+ * ATH_SWEEP_HYPERPLANE, whatever was set */
 int ath_get_sweep_ordering(const ath_ctx *ctx, int *ordering);
 /* Autotune the sweep execution order for this case: time `iterations`
  * nonlinear iterations in each order the case admits (pencil, tiled and

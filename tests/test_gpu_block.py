@@ -105,22 +105,26 @@ def test_block_solvers_step_parity(tmp_path, solver, sweeps, timeint):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_block_sweep_orderings_agree(tmp_path, precision):
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_block_sweep_orderings_agree(tmp_path, precision, sweeps):
     """every sweep order of the block solvers gives the same bits, with the
-    FP64 and with the FP32 operator"""
-    path = _bump(tmp_path)
+    FP64 and with the FP32 operator, in first and in full sweeps (the pencil
+    order runs the tiled sweep)"""
+    path = _bump(tmp_path, sweeps=sweeps)
     out = []
-    for order in ("PENCIL", "TILED", "HYPERPLANE"):
-        say("block ordering", order, "precision", precision)
+    for order in ("ROWS", "PENCIL", "TILED", "HYPERPLANE"):
+        say("block ordering", order, "precision", precision, "sweeps", sweeps)
         s = Solver(path)
         s.set_implicit_precision(precision)
         s.set_sweep_ordering(getattr(Solver, order))
+        assert s.sweep_ordering() == (Solver.TILED if order == "PENCIL" else getattr(Solver, order))
         for nn in range(3):
             s.begin_step(nn)
             s.iterate(nn, 0)
         out.append(s.get_state())
         s.close()
-    assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[2])
+    for q in range(3):
+        assert np.array_equal(out[q], out[3]), q
 
 
 def test_block_graph_run_equals_iterate(tmp_path):
@@ -260,3 +264,38 @@ def test_multiblock_channel_parity(tmp_path, refine, solver):
     Wo = np.concatenate([o.state(b) for b in range(o.nb)])
     assert rel(Ws, Wo) <= 1e-10
     s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", ["viscous_connection", "channel"])
+def test_row_sweep_equals_tiled(tmp_path, case, precision):
+    """The row-parallel block sweep (ATH_SWEEP_ROWS) gives the tiled sweep's
+    bits: viscous blocks joined by a rotated connection with two sweeps per
+    relax (full sweeps, partial tiles), and the multiblock channel (eight
+    blocks, BDF2 subiterations), with the FP64 and the FP32 operator.  The
+    tiled sweep is compared with the oracle above."""
+    if case == "viscous_connection":
+        import conn_cases as cc
+        r0, r1 = cc.orientation_set()[350]
+        path, _ = cc.write_pair(str(tmp_path), "cut", r0, r1, timeint="implicitEuler", eq="navierStokes",
+                                scale=1e-3, solver="blusgs", sweeps=2)
+        steps, subits = 4, 1
+    else:
+        path, meta = cases.channel_case(str(tmp_path), refine=0.25, steps=2, subits=3)
+        steps, subits = 2, 3
+    out = []
+    for order in ("TILED", "ROWS"):
+        say("row sweep", case, precision, order)
+        s = Solver(path)
+        s.set_implicit_precision(precision)
+        s.set_sweep_ordering(getattr(Solver, order))
+        assert s.sweep_ordering() == getattr(Solver, order)
+        norms = []
+        for nn in range(steps):
+            s.begin_step(nn)
+            for mm in range(subits):
+                norms.append(s.iterate(nn, mm).l2_sq[:5])
+        out.append((np.concatenate([s.get_state(b).reshape(-1) for b in range(s.info.num_blocks)]), norms))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
