@@ -713,15 +713,41 @@ __global__ void __launch_bounds__(256) KAux(BlockMap M, Params P) {
   if (b.kc) b.kc[q] = SutherlandK(P.visc, t);  // read by the block Jacobians
 }
 
+// The cell values the viscous face terms read (velocity components and
+// temperature, padded volume, state, viscosity), from the block's padded
+// arrays (CellsG) or staged in LDS for a box of cells (CellsL: eight arrays
+// of n cells, rho, u, v, w, p, T, volume, viscosity, the box's cell (i0, j0,
+// k0) first).  The values are the same doubles either way.
+struct CellsG {
+  const DevBlock &b;
+  __device__ __forceinline__ long At(int i, int j, int k) const { return PadIdx(b, i, j, k); }
+  __device__ __forceinline__ double Var(int v, long c) const { return (v < 3 ? b.W + (v + 1) * b.npad : b.T)[c]; }
+  __device__ __forceinline__ double Vol(long c) const { return b.volP[c]; }
+  __device__ __forceinline__ double State(int e, long c) const { return b.W[e * b.npad + c]; }
+  __device__ __forceinline__ double Mu(long c) const { return b.mu[c]; }
+};
+struct CellsL {
+  const double *s;
+  int i0, j0, k0, sx, sy, n;
+  __device__ __forceinline__ long At(int i, int j, int k) const {
+    return (i - i0) + sx * ((j - j0) + (long)sy * (k - k0));
+  }
+  __device__ __forceinline__ double Var(int v, long c) const { return s[(v < 3 ? v + 1 : 5) * n + c]; }
+  __device__ __forceinline__ double Vol(long c) const { return s[6 * n + c]; }
+  __device__ __forceinline__ double State(int e, long c) const { return s[e * n + c]; }
+  __device__ __forceinline__ double Mu(long c) const { return s[7 * n + c]; }
+};
+
 // CalcGradsI/J/K at a face (src/procBlock.cpp:5173-5788) with VectorGradGG /
 // ScalarGradGG (src/utility.cpp:59-188): Green-Gauss over the control volume
 // spanning the face's lower cell L and upper cell U; velocity gradient
 // vg[r][c] = d v_c / d x_r, temperature gradient tg
-__device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j, int k, double vg[3][3],
-                                          double tg[3]) {
+template <class C>
+__device__ __forceinline__ void FaceGrads(const DevBlock &b, const C &cv, int d, int i, int j, int k,
+                                          double vg[3][3], double tg[3]) {
   const int L[3] = {i - (d == 0), j - (d == 1), k - (d == 2)}, U[3] = {i, j, k};
-  const long cL = PadIdx(b, L[0], L[1], L[2]), cU = PadIdx(b, U[0], U[1], U[2]);
-  const double vol = 0.5 * (b.volP[cL] + b.volP[cU]);
+  const long cL = cv.At(L[0], L[1], L[2]), cU = cv.At(U[0], U[1], U[2]);
+  const double vol = 0.5 * (cv.Vol(cL) + cv.Vol(cU));
   // Green-Gauss over the control volume's three face pairs: each entry is
   // the sum, left to right, of +vu au - vl al for the pairs q = 0, 1, 2, so
   // it is accumulated one pair at a time
@@ -750,21 +776,20 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
     al[1] = 0.5 * (f3.y * f3.w + f4.y * f4.w);
     al[2] = 0.5 * (f3.z * f3.w + f4.z * f4.w);
     // face values of the pair: three velocity components, then temperature
-    const long uP = PadIdx(b, U[0] + eq[0], U[1] + eq[1], U[2] + eq[2]);
-    const long lP = PadIdx(b, L[0] + eq[0], L[1] + eq[1], L[2] + eq[2]);
-    const long uM = PadIdx(b, U[0] - eq[0], U[1] - eq[1], U[2] - eq[2]);
-    const long lM = PadIdx(b, L[0] - eq[0], L[1] - eq[1], L[2] - eq[2]);
+    const long uP = cv.At(U[0] + eq[0], U[1] + eq[1], U[2] + eq[2]);
+    const long lP = cv.At(L[0] + eq[0], L[1] + eq[1], L[2] + eq[2]);
+    const long uM = cv.At(U[0] - eq[0], U[1] - eq[1], U[2] - eq[2]);
+    const long lM = cv.At(L[0] - eq[0], L[1] - eq[1], L[2] - eq[2]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const double *A = v < 3 ? b.W + (v + 1) * b.npad : b.T;
-      const double xl = A[cL], xu = A[cU];
+      const double xl = cv.Var(v, cL), xu = cv.Var(v, cU);
       double vu, vl;
       if (q == d) {
         vu = xu;
         vl = xl;
       } else {
-        vu = 0.25 * (xl + xu + A[uP] + A[lP]);
-        vl = 0.25 * (xl + xu + A[uM] + A[lM]);
+        vu = 0.25 * (xl + xu + cv.Var(v, uP) + cv.Var(v, lP));
+        vl = 0.25 * (xl + xu + cv.Var(v, uM) + cv.Var(v, lM));
       }
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
@@ -786,9 +811,11 @@ __device__ __forceinline__ void FaceGrads(const DevBlock &b, int d, int i, int j
 // fourth-order face state and viscosity (include/reconstruction.hpp:314-349),
 // viscousFlux::CalcFlux / CalcWallFlux (src/viscousFlux.cpp:58-188),
 // TauNormal (src/utility.cpp:426-436); the wall form has no +0 species term
-template <bool R4>
-__device__ __forceinline__ void ViscFaceState(const DevBlock &b, int d, int i, int j, int k, double Wf[5],
-                                              double &muF) {
+// (the fourth-order state reads the cells two away from the face, which a
+// staged box does not hold: it always reads the block's arrays)
+template <bool R4, class C>
+__device__ __forceinline__ void ViscFaceState(const DevBlock &b, const C &cv, int d, int i, int j, int k,
+                                              double Wf[5], double &muF) {
 #pragma clang fp contract(off)
   const long s = d == 0 ? 1 : (d == 1 ? (long)b.NI : (long)b.NI * b.NJ);
   const long cU = PadIdx(b, i, j, k), cL = cU - s;
@@ -796,9 +823,10 @@ __device__ __forceinline__ void ViscFaceState(const DevBlock &b, int d, int i, i
   if (!R4) {
     const double w[2] = {cw[cL], cw[cU]};
     const double c0 = LagrangeCoeffN<2, 1, 0, 0>(w, 0), c1 = LagrangeCoeffN<2, 1, 0, 0>(w, 1);
+    const long vU = cv.At(i, j, k), vL = cv.At(i - (d == 0), j - (d == 1), k - (d == 2));
 #pragma unroll
-    for (int e = 0; e < 5; ++e) Wf[e] = c0 * b.W[e * b.npad + cU] + c1 * b.W[e * b.npad + cL];
-    muF = c0 * b.mu[cU] + c1 * b.mu[cL];
+    for (int e = 0; e < 5; ++e) Wf[e] = c0 * cv.State(e, vU) + c1 * cv.State(e, vL);
+    muF = c0 * cv.Mu(vU) + c1 * cv.Mu(vL);
   } else {
     const long cLL = cL - s, cUU = cU + s;
     const double w[4] = {cw[cLL], cw[cL], cw[cU], cw[cUU]};
@@ -815,17 +843,23 @@ __device__ __forceinline__ void ViscFaceState(const DevBlock &b, int d, int i, i
 }
 
 template <bool R4>
-__device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const Params &P, int d, int i, int j, int k,
-                                             double F[5]) {
+__device__ __forceinline__ void ViscFaceState(const DevBlock &b, int d, int i, int j, int k, double Wf[5],
+                                              double &muF) {
+  ViscFaceState<R4>(b, CellsG{b}, d, i, j, k, Wf, muF);
+}
+
+template <bool R4, class C>
+__device__ __forceinline__ void ViscFaceFlux(const DevBlock &b, const C &cv, const Params &P, int d, int i, int j,
+                                             int k, double F[5]) {
 #pragma clang fp contract(off)
   const GasT<double> g = Cast<double>(P.gas);
   double vg[3][3], tg[3];
-  FaceGrads(b, d, i, j, k, vg, tg);
+  FaceGrads(b, cv, d, i, j, k, vg, tg);
   const int n3 = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
   const int f3 = d == 0 ? i : (d == 1 ? j : k);
   const bool wall = (f3 == 0 || f3 == n3) && b.wallf[BtSlot(b, d, f3 == 0 ? 0 : 1, i, j, k)];
   double Wf[5], muF;
-  ViscFaceState<R4>(b, d, i, j, k, Wf, muF);
+  ViscFaceState<R4>(b, cv, d, i, j, k, Wf, muF);
   const long fq = FaceIdx(b, d, i, j, k);
   const double4 a = b.fa[d][fq];
   const double n[3] = {a.x, a.y, a.z};
@@ -951,12 +985,30 @@ __device__ __forceinline__ void TSLJacobian(const GasT<double> &g, const ViscPar
 // (+Fv_i,lo -Fv_i,hi +Fv_j,lo -Fv_j,hi +Fv_k,lo -Fv_k,hi), adds the viscous
 // spectral radii (i, j, k) to its spectral radius (times the viscous CFL
 // coefficient) and twice them to its diagonal, then forms dt and A.
+constexpr int kVX = kTI + 2, kVY = kTJ + 2, kVZ = kTK + 2, kVN = kVX * kVY * kVZ;
 template <bool R4>
 __global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P) {
   __shared__ double sF[5][kNF];
+  // the tile's cells and one layer around them: every cell a face gradient
+  // or a central face state of the tile reads
+  __shared__ double sC[8][kVN];
   const Unit U = MapUnit(M);
   const DevBlock &b = *U.b;
   const int i0 = U.x * kTI, j0 = U.y * kTJ, k0 = U.z * kTK;
+  for (int q = threadIdx.x; q < kVN; q += blockDim.x) {
+    const int i = i0 - 1 + q % kVX, j = j0 - 1 + (q / kVX) % kVY, k = k0 - 1 + q / (kVX * kVY);
+    // positions past a partial tile's padded extent are never read
+    if (i < b.ni + b.g && j < b.nj + b.g && k < b.nk + b.g) {
+      const long c = PadIdx(b, i, j, k);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sC[e][q] = b.W[e * b.npad + c];
+      sC[5][q] = b.T[c];
+      sC[6][q] = b.volP[c];
+      sC[7][q] = b.mu[c];
+    }
+  }
+  __syncthreads();
+  const CellsL cv{&sC[0][0], i0 - 1, j0 - 1, k0 - 1, kVX, kVY, kVN};
   for (int f = threadIdx.x; f < kNF; f += blockDim.x) {
     int d, li, lj, lk;
     if (f < kNFI) {
@@ -972,7 +1024,7 @@ __global__ void __launch_bounds__(256) KViscFlux(BlockMap M, Params P) {
     const bool ok = (i < b.ni + (d == 0)) && (j < b.nj + (d == 1)) && (k < b.nk + (d == 2));
     if (ok) {
       double F[5];
-      ViscFaceFlux<R4>(b, P, d, i, j, k, F);
+      ViscFaceFlux<R4>(b, cv, P, d, i, j, k, F);
 #pragma unroll
       for (int e = 0; e < 5; ++e) sF[e][f] = F[e];
     }
