@@ -2473,128 +2473,48 @@ __global__ void __launch_bounds__(256) KOtherSide(BlockMap M, Params P) {
 // ---------------------------------------------------------------------------
 // Row-parallel block LU-SGS sweep over all blocks (blusgs).
 //
-// The same dependency scheme as KBlockSweepTiles (tiles queued by tile
-// hyperplane, a tile starts once the tiles of its block that it depends on
-// have published), with the work of a tile spread over eleven wavefronts and
-// the loads of the next tile overlapped with the current one:
+// The tiles, queue and flags of KBlockSweepTiles (8 x 8 x 4 cells), with two
+// lanes per cell: lane h of a cell's pair holds rows 3h .. 3h+2 (rows 3 and
+// 4 for h = 1) of its three dependency-side blocks, of D^-1, of the
+// right-hand side and of the forward result, in registers.
 //
-// * Forming the operator: roles of two wavefronts each, one lane per cell of
-//   the 8 x 4 x 4 tile.  Roles 0-2 form the dependency-side block of
-//   direction i, j, k into LDS; role 3 fills the halo columns that lie
-//   outside the block (connection terms) and, backward, the positions of a
-//   partial tile just past the block.  A full sweep's other-side sums come
-//   from KOtherSide, launched before the pass.
-// * The walk: five lanes per cell, twelve cells per wavefront.  Per cell
-//   hyperplane, lane r forms row r of the dependency sum (the blocks' rows
-//   from LDS) and of v, the cell's five lanes exchange v, and lane r forms
-//   row r of D^-1 v (its row of D^-1 in registers).  Cells are given to lane
-//   groups in hyperplane order, so a hyperplane occupies few wavefronts.
-// * Pipelining: the next ticket is taken when a tile starts, and once the
-//   tile's operator is formed the forming lanes issue the loads of the next
-//   tile's operator into registers; they arrive while this tile waits, walks
-//   and publishes.  The right-hand side is formed once per relax by
-//   KInitUpdate.
+// * Forming: lane h forms the block of direction 2h (i or k) and both lanes
+//   form the j block; a lane keeps its rows and hands the pair's other rows
+//   to its partner with a lane shuffle.  The right-hand side is formed once
+//   per relax by KInitUpdate, a full sweep's other-side sums by KOtherSide
+//   before the pass.
+// * The walk: per cell hyperplane a lane forms its rows of the dependency
+//   sum and of v, the pair exchanges v, and each lane forms its rows of
+//   D^-1 v.  Pairs are given to cells in hyperplane order, so a hyperplane
+//   occupies one or two of the eight wavefronts.
 //
 // Each row is summed in the order BlockMult sums it and the dependency sum
 // adds the directions in i, j, k order from +0, so the result is that of
-// KLusgsForward / KLusgsBackward bit for bit.  Eleven wavefronts leave 168
-// registers per lane.
-constexpr int kRTI = 8, kRTJ = 4, kRTK = 4;
-constexpr int kRT = kRTI * kRTJ * kRTK;
-constexpr int kRTP = kRT + 1;  // padded row of a block entry: the five rows of a cell in distinct banks
-constexpr int kRCellsPerWave = 12;  // five lanes each, four lanes idle
-constexpr int kRWaves = (kRT + kRCellsPerWave - 1) / kRCellsPerWave;
-constexpr int kRThreads = kRWaves * 64;
-constexpr int kRPlanes = kRTI + kRTJ + kRTK - 2;
-constexpr int kRHaloI = kRTJ * kRTK, kRHaloJ = kRTI * kRTK, kRHaloK = kRTI * kRTJ;
-constexpr int kRHalo = kRHaloI + kRHaloJ + kRHaloK;
-static_assert(kRThreads == 704 && kRHalo <= kRT && 5 * kRT <= kRThreads, "row sweep layout");
-
-// the loads of NeighbourBlock (the neighbour's state, velocity gradient,
-// viscosity, conductivity, the shared face and its centre distance)
-struct NbIn {
-  double W[5];
-  double vg[9];
-  double mu, kc, c2c;
-  double4 a;
-};
-__device__ __forceinline__ void NbLoad(const DevBlock &b, const ViscParams &vp, int d, bool upper, int i, int j,
-                                       int k, NbIn &in) {
-  const int s = upper ? 1 : -1;
-  const int ni = i + (d == 0 ? s : 0), nj = j + (d == 1 ? s : 0), nk = k + (d == 2 ? s : 0);
-  const long f = upper ? FaceIdx(b, d, ni, nj, nk) : FaceIdx(b, d, i, j, k);
-  in.a = b.fa[d][f];
-  const long c = PadIdx(b, ni, nj, nk);
-#pragma unroll
-  for (int e = 0; e < 5; ++e) in.W[e] = b.W[e * b.npad + c];
-  if (vp.on) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) in.vg[q] = b.vgrad[q * b.npad + c];
-    in.mu = b.mu[c];
-    in.kc = b.kc[c];
-    in.c2c = b.c2c[d][f];
-  }
-}
-// NeighbourBlock's arithmetic on loaded inputs, written to LDS entry by entry
-// (entry q at dst[q * kRTP]).  The thin-shear-layer matrix is formed first
-// and parked in `stage` (entry q at stage[q * kRTP]; `stage` may be `dst`
-// when T is double), then the Rusanov Jacobian is combined with it entry by
-// entry, so that the two 5x5 matrices are never in registers together.
-template <typename T>
-__device__ __forceinline__ void NbFormLds(const GasT<double> &g, const ViscParams &vp, bool upper, const NbIn &in,
-                                          double *stage, T *dst) {
-#pragma clang fp contract(off)
-  const bool positive = !upper;
-  if (vp.on) {
-    double V[25];
-    TSLJacobian(g, vp, in.W, in.mu, in.kc, in.a, in.c2c, positive, in.vg, V);
-#pragma unroll
-    for (int q = 0; q < 25; ++q) stage[q * kRTP] = V[q];
-  }
-  double J[25];
-  RusanovJacobian<double>(g, in.W, in.a.x, in.a.y, in.a.z, in.a.w, positive, J);
-  if (vp.on) {
-#pragma unroll
-    for (int q = 0; q < 25; ++q) {
-      const double v = stage[q * kRTP];
-      dst[q * kRTP] = T(positive ? J[q] - v : J[q] + v);
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 25; ++q) dst[q * kRTP] = T(J[q]);
-  }
-}
+// KLusgsForward / KLusgsBackward bit for bit.
+constexpr int kPThreads = 2 * kBT;
+static_assert(kPThreads == 512 && kBHalo <= kBT, "row sweep layout");
 
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(kRThreads) KBlockSweepRows(const DevBlock *__restrict__ blocks, Params P,
+__global__ void __launch_bounds__(kPThreads) KBlockSweepRows(const DevBlock *__restrict__ blocks, Params P,
                                                              MultiSweepCtl s, int full) {
-  __shared__ T sA[5][kRT + kRHalo];  // du of the tile's cells, then of the halo columns
-  __shared__ T sM[3][25][kRTP];      // dependency-side blocks, entry-major
-  // FP32 operator: the thin-shear-layer matrices in FP64 while the blocks
-  // are formed (the FP64 operator parks them in sM itself)
-  __shared__ double sV[sizeof(T) == sizeof(double) ? 1 : 3][25][kRTP];
-  __shared__ unsigned char sCell[kRT];  // walk group -> cell, in hyperplane order
+  __shared__ T sA[5][kBT + kBHalo];  // du of the tile's cells, then of the halo columns
+  __shared__ unsigned char sCell[kBT];  // pair -> cell, in hyperplane order
   __shared__ int sTicket, sNext, sEpoch;
 
   const int tid = threadIdx.x;
   const int sgn = FWD ? -1 : 1;  // direction of the dependency neighbours
-  // the forming roles: one lane per cell, two wavefronts per role
-  const int pc = tid % kRT, role = tid / kRT;
-  const int pli = pc % kRTI, plj = (pc / kRTI) % kRTJ, plk = pc / (kRTI * kRTJ);
-  // the walk: group g of five lanes, row r; the group's first lane in its
-  // wavefront is gbase
-  const int lane = tid & 63;
-  const int g = (tid >> 6) * kRCellsPerWave + lane / 5, r = lane % 5, gbase = lane - r;
-  const bool glane = lane < 5 * kRCellsPerWave && g < kRT;
+  const int g = tid >> 1, h = tid & 1;
+  const int r0 = 3 * h;           // this lane's first row
+  const int nrow = h ? 2 : 3;     // its rows r0 .. r0 + nrow - 1
   const bool connZero = FWD && !full;
 
-  if (tid < kRT) {
+  if (tid < kBT) {
     // rank of cell tid among the tile's cells by (hyperplane, index)
-    const int li = tid % kRTI, lj = (tid / kRTI) % kRTJ, lk = tid / (kRTI * kRTJ);
+    const int li = tid % kBTI, lj = (tid / kBTI) % kBTJ, lk = tid / (kBTI * kBTJ);
     const int pl = li + lj + lk;
     int rank = 0;
-    for (int q = 0; q < kRT; ++q) {
-      const int qp = q % kRTI + (q / kRTI) % kRTJ + q / (kRTI * kRTJ);
+    for (int q = 0; q < kBT; ++q) {
+      const int qp = q % kBTI + (q / kBTI) % kBTJ + q / (kBTI * kBTJ);
       rank += (qp < pl || (qp == pl && q < tid)) ? 1 : 0;
     }
     sCell[rank] = (unsigned char)tid;
@@ -2606,48 +2526,25 @@ __global__ void __launch_bounds__(kRThreads) KBlockSweepRows(const DevBlock *__r
   __syncthreads();
   const int epoch = sEpoch;
   const GasT<double> gd = Cast<double>(P.gas);
-  const int wc = glane ? sCell[g] : 0;
-  const int wli = wc % kRTI, wlj = (wc / kRTI) % kRTJ, wlk = wc / (kRTI * kRTJ);
-
-  // the forming role's loads of a tile, issued a tile ahead
-  NbIn pin;  // roles 0-2: the dependency-side block's inputs; role 3: pin.W holds a halo column's connection term
-  auto prefetch = [&](int t) {
-    const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
-    const DevBlock &b = blocks[tl.w];
-    const int ni = b.ni, nj = b.nj, nk = b.nk;
-    const int i = tl.x * kRTI + pli, j = tl.y * kRTJ + plj, k = tl.z * kRTK + plk;
-    if (role < 3) {
-      const int d = role;
-      const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
-      const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < ni && gj < nj && gk < nk;
-      if (i < ni && j < nj && k < nk && inside) NbLoad(b, P.visc, d, !FWD, i, j, k, pin);
-    } else if (role == 3 && pc < kRHalo) {
-      const T *bt = OpBuf<T>::Bt(b);
-      const int h = pc;
-      const int di = FWD ? -1 : kRTI, dj = FWD ? -1 : kRTJ, dk = FWD ? -1 : kRTK;
-      int hd, hi, hj, hk;
-      if (h < kRHaloI) { hd = 0; hi = di; hj = h % kRTJ; hk = h / kRTJ; }
-      else if (h < kRHaloI + kRHaloJ) { const int q = h - kRHaloI; hd = 1; hi = q % kRTI; hj = dj; hk = q / kRTI; }
-      else { const int q = h - kRHaloI - kRHaloJ; hd = 2; hi = q % kRTI; hj = q / kRTI; hk = dk; }
-      hi += tl.x * kRTI; hj += tl.y * kRTJ; hk += tl.z * kRTK;
-      const bool hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
-      // the tile cell next to it along hd
-      const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
-      const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
-      if (!hin && face && bt && !connZero) {
-        const long q = BtSlot(b, hd, FWD ? 0 : 1, ci, cj, ck);
+  const int wc = sCell[g];
+  const int li = wc % kBTI, lj = (wc / kBTI) % kBTJ, lk = wc / (kBTI * kBTJ);
+  const int plane = li + lj + lk;
+  // LDS column of the dependency neighbour in each direction: a cell of the
+  // tile or a halo column
+  int nb[3];
 #pragma unroll
-        for (int e = 0; e < 5; ++e) pin.W[e] = double(bt[e * b.nbf + q]);
-      }
-    }
-  };
+  for (int d = 0; d < 3; ++d) {
+    const int l3 = d == 0 ? li : (d == 1 ? lj : lk);
+    const int L3 = d == 0 ? kBTI : (d == 1 ? kBTJ : kBTK);
+    const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
+    nb[d] = inTile ? wc + sgn * (d == 0 ? 1 : (d == 1 ? kBTI : kBTI * kBTJ))
+                   : kBT + (d == 0 ? lj + kBTJ * lk
+                                   : (d == 1 ? kBHaloI + li + kBTI * lk : kBHaloI + kBHaloJ + li + kBTI * lj));
+  }
 
-  int t = sTicket;
-  if (t < s.ntiles) prefetch(t);
   for (;;) {
+    const int t = sTicket;
     if (t >= s.ntiles) break;
-    int nxt = 0;
-    if (tid == 0) nxt = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int4 tl = s.tiles[FWD ? t : s.ntiles - 1 - t];
     const DevBlock &b = blocks[tl.w];
     const int4 gr = s.grid[tl.w];
@@ -2655,95 +2552,125 @@ __global__ void __launch_bounds__(kRThreads) KBlockSweepRows(const DevBlock *__r
     const int ni = b.ni, nj = b.nj, nk = b.nk;
     const long ncell = b.ncell, nbf = b.nbf;
     const T *bt = OpBuf<T>::Bt(b);
+    auto conn = [&](int d, int ci, int cj, int ck, int e) -> T {
+      return (bt && !connZero) ? bt[e * nbf + BtSlot(b, d, FWD ? 0 : 1, ci, cj, ck)] : T(0);
+    };
 
-    // --- forming, one lane per cell (wave-uniform roles)
-    const int i = tl.x * kRTI + pli, j = tl.y * kRTJ + plj, k = tl.z * kRTK + plk;
+    // --- static data before the dependency wait
+    const int i = tl.x * kBTI + li, j = tl.y * kBTJ + lj, k = tl.z * kBTK + lk;
     const bool valid = i < ni && j < nj && k < nk;
-    int hi = 0, hj = 0, hk = 0;
-    bool hin = false;
-    if (role < 3) {
-      const int d = role;
-      double *stage = sizeof(T) == sizeof(double) ? reinterpret_cast<double *>(&sM[d][0][pc])
-                                                  : &sV[0][0][pc] + d * 25 * kRTP;
-      if (valid) {
+    const long p = (long)i + (long)j * ni + (long)k * ni * nj;
+    T m[3][3][5];  // [direction][slot][column]: rows r0 + slot of the dependency blocks
+    T ai[3][5], rb[3], ost[3], xf[3];
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+      rb[sl] = ost[sl] = xf[sl] = T(0);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        ai[sl][q] = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m[d][sl][q] = T(0);
+      }
+    }
+    if (valid) {
+      // one dependency-side block of direction d, formed in FP64 and held in
+      // the operator's precision; a neighbour outside the block enters
+      // through its connection term in the LDS column, the identity block
+      // passing it on unchanged (x + 0 y = x, and the sum never starts from -0)
+      auto form = [&](int d, T Jt[25]) {
         const int gi = i + (d == 0 ? sgn : 0), gj = j + (d == 1 ? sgn : 0), gk = k + (d == 2 ? sgn : 0);
         const bool inside = gi >= 0 && gj >= 0 && gk >= 0 && gi < ni && gj < nj && gk < nk;
-        // a neighbour outside the block enters through its connection term,
-        // which its LDS column holds; the identity block passes it on
-        // unchanged (x + 0 y = x, and the sum never starts from -0)
         if (inside) {
-          NbFormLds<T>(gd, P.visc, !FWD, pin, stage, &sM[d][0][pc]);
+          double J[25];
+          NeighbourBlock(b, gd, P.visc, d, !FWD, i, j, k, J);
+#pragma unroll
+          for (int q = 0; q < 25; ++q) Jt[q] = T(J[q]);
         } else {
 #pragma unroll
-          for (int q = 0; q < 25; ++q) sM[d][q][pc] = q % 6 == 0 ? T(1) : T(0);
+          for (int q = 0; q < 25; ++q) Jt[q] = q % 6 == 0 ? T(1) : T(0);
         }
-      }
-    } else if (role == 3) {
-      if (pc < kRHalo) {
-        // halo columns: positions outside the block hold connection terms
-        // now, those inside the du of the dependency tiles after the wait
-        const int h = pc;
-        const int di = FWD ? -1 : kRTI, dj = FWD ? -1 : kRTJ, dk = FWD ? -1 : kRTK;
-        int hd;
-        if (h < kRHaloI) { hd = 0; hi = di; hj = h % kRTJ; hk = h / kRTJ; }
-        else if (h < kRHaloI + kRHaloJ) { const int q = h - kRHaloI; hd = 1; hi = q % kRTI; hj = dj; hk = q / kRTI; }
-        else { const int q = h - kRHaloI - kRHaloJ; hd = 2; hi = q % kRTI; hj = q / kRTI; hk = dk; }
-        hi += tl.x * kRTI; hj += tl.y * kRTJ; hk += tl.z * kRTK;
-        hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
-        if (!hin) {
-          const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
-          const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
-          const bool use = face && bt && !connZero;
+      };
+      // (every register index below is a compile-time constant; the lane's
+      // half selects between values)
+      {
+        // lane h forms direction 2h; each keeps its rows and sends the
+        // partner's rows (rows 3-4 from lane 0, rows 0-2 from lane 1)
+        T J[25];
+        form(2 * h, J);
+        T send[15], own[3][5];
 #pragma unroll
-          for (int e = 0; e < 5; ++e) sA[e][kRT + h] = use ? T(pin.W[e]) : T(0);
+        for (int q = 0; q < 15; ++q) send[q] = h ? J[q] : (q < 10 ? J[15 + q] : T(0));
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+          for (int q = 0; q < 5; ++q) own[sl][q] = h ? (sl < 2 ? J[15 + 5 * sl + q] : T(0)) : J[5 * sl + q];
+        T recv[15];
+#pragma unroll
+        for (int q = 0; q < 15; ++q) recv[q] = __shfl_xor(send[q], 1, 64);
+        // lane 0 receives direction k, rows 0-2; lane 1 direction i, rows 3-4
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const T got = (sl == 2) ? (h ? T(0) : recv[5 * sl + q]) : recv[5 * sl + q];
+            m[0][sl][q] = h ? got : own[sl][q];
+            m[2][sl][q] = h ? own[sl][q] : got;
+          }
+      }
+      {
+        // both lanes form direction j and keep their rows
+        T J[25];
+        form(1, J);
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+          for (int q = 0; q < 5; ++q) m[1][sl][q] = h ? (sl < 2 ? J[15 + 5 * sl + q] : T(0)) : J[5 * sl + q];
+      }
+      const auto *Mi = OpBuf<T>::Abi(b) + TileBase(b, i, j, k);
+#pragma unroll
+      for (int sl = 0; sl < 3; ++sl) {
+        const int row = r0 + sl;
+        if (row < 5) {
+#pragma unroll
+          for (int q = 0; q < 5; ++q) ai[sl][q] = T(Mi[256 * (5 * row + q)]);
+          if (FWD || full) rb[sl] = T(b.rhs[row * ncell + p]);
+          if (full) ost[sl] = OpBuf<T>::Xold(b)[row * ncell + p];
+          if (!FWD && !full) xf[sl] = x[row * ncell + p];
         }
       }
-      if (!FWD && !valid) {
-        // a partial tile's position just past the block's upper face along
-        // one axis is the dependency neighbour of the face cell below it
-        const int oi = i >= ni, oj = j >= nj, ok = k >= nk;
+    }
+    // backward: a partial tile's position just past the block's upper face
+    // along one axis is the dependency neighbour of the face cell below it
+    if (!FWD && tid < kBT) {
+      const int pi = tl.x * kBTI + tid % kBTI, pj = tl.y * kBTJ + (tid / kBTI) % kBTJ,
+                pk = tl.z * kBTK + tid / (kBTI * kBTJ);
+      if (!(pi < ni && pj < nj && pk < nk)) {
+        const int oi = pi >= ni, oj = pj >= nj, ok = pk >= nk;
         const int d = oi ? 0 : (oj ? 1 : 2);
-        const bool edge = oi + oj + ok == 1 && (oi ? i == ni : (oj ? j == nj : k == nk));
-        const bool use = edge && bt && !connZero;
-        const long q = use ? BtSlot(b, d, 1, i - (d == 0), j - (d == 1), k - (d == 2)) : 0;
+        const bool edge = oi + oj + ok == 1 && (oi ? pi == ni : (oj ? pj == nj : pk == nk));
 #pragma unroll
-        for (int e = 0; e < 5; ++e) sA[e][pc] = use ? bt[e * nbf + q] : T(0);
+        for (int e = 0; e < 5; ++e)
+          sA[e][tid] = edge ? conn(d, pi - (d == 0), pj - (d == 1), pk - (d == 2), e) : T(0);
       }
     }
-    if (tid == 0) sNext = nxt;
-    __syncthreads();
-
-    // --- the walk lane's cell, its dependency neighbours' LDS columns (a
-    // cell of the tile or a halo column) and its row of D^-1, right-hand
-    // side and forward result; then the next tile's loads
-    const int tn = sNext;
-    const int wplane = wli + wlj + wlk;
-    int nb[3];
+    // halo columns: positions outside the block hold connection terms now,
+    // those inside the du of the dependency tiles after the wait
+    int hi = 0, hj = 0, hk = 0, hd = 0;
+    bool hin = false;
+    if (tid < kBHalo) {
+      const int di = FWD ? -1 : kBTI, dj = FWD ? -1 : kBTJ, dk = FWD ? -1 : kBTK;
+      if (tid < kBHaloI) { hd = 0; hi = di; hj = tid % kBTJ; hk = tid / kBTJ; }
+      else if (tid < kBHaloI + kBHaloJ) { const int q = tid - kBHaloI; hd = 1; hi = q % kBTI; hj = dj; hk = q / kBTI; }
+      else { const int q = tid - kBHaloI - kBHaloJ; hd = 2; hi = q % kBTI; hj = q / kBTI; hk = dk; }
+      hi += tl.x * kBTI; hj += tl.y * kBTJ; hk += tl.z * kBTK;
+      hin = hi >= 0 && hj >= 0 && hk >= 0 && hi < ni && hj < nj && hk < nk;
+      if (!hin) {
+        const int ci = hi - (hd == 0 ? sgn : 0), cj = hj - (hd == 1 ? sgn : 0), ck = hk - (hd == 2 ? sgn : 0);
+        const bool face = ci >= 0 && cj >= 0 && ck >= 0 && ci < ni && cj < nj && ck < nk;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int l3 = d == 0 ? wli : (d == 1 ? wlj : wlk);
-      const int L3 = d == 0 ? kRTI : (d == 1 ? kRTJ : kRTK);
-      const bool inTile = FWD ? (l3 > 0) : (l3 < L3 - 1);
-      nb[d] = inTile ? wc + sgn * (d == 0 ? 1 : (d == 1 ? kRTI : kRTI * kRTJ))
-                     : kRT + (d == 0 ? wlj + kRTJ * wlk
-                                     : (d == 1 ? kRHaloI + wli + kRTI * wlk : kRHaloI + kRHaloJ + wli + kRTI * wlj));
+        for (int e = 0; e < 5; ++e) sA[e][kBT + tid] = face ? conn(hd, ci, cj, ck, e) : T(0);
+      }
     }
-    const int wi = tl.x * kRTI + wli, wj = tl.y * kRTJ + wlj, wk = tl.z * kRTK + wlk;
-    const bool wvalid = glane && wi < ni && wj < nj && wk < nk;
-    T ai[5], rb = T(0), xf = T(0), ost = T(0);
-    if (wvalid) {
-      const auto *Mi = OpBuf<T>::Abi(b) + TileBase(b, wi, wj, wk);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) ai[q] = T(Mi[256 * (5 * r + q)]);
-      const long p = (long)wi + (long)wj * ni + (long)wk * ni * nj;
-      if (FWD || full) rb = T(b.rhs[r * ncell + p]);
-      if (!FWD && !full) xf = x[r * ncell + p];
-    }
-
-    if (full && wvalid) ost = OpBuf<T>::Xold(b)[r * ncell + (long)wi + (long)wj * ni + (long)wk * ni * nj];
-    // the next tile's forming loads, in flight while this tile waits, walks
-    // and publishes
-    if (tn < s.ntiles) prefetch(tn);
 
     // --- wait for the tiles of the same block this one depends on
     if (tid == 0) {
@@ -2767,55 +2694,75 @@ __global__ void __launch_bounds__(kRThreads) KBlockSweepRows(const DevBlock *__r
       // one acquire for the workgroup; the barrier below orders it before
       // every wave's halo loads
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // the next ticket now, so that its atomic overlaps this tile's walk
+      sNext = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (hin) {
       const long q = (long)hi + (long)hj * ni + (long)hk * ni * nj;
 #pragma unroll
-      for (int e = 0; e < 5; ++e) sA[e][kRT + pc] = x[e * ncell + q];
+      for (int e = 0; e < 5; ++e) sA[e][kBT + tid] = x[e * ncell + q];
     }
     __syncthreads();
 
     // --- intra-tile hyperplanes
-    for (int st = 0; st < kRPlanes; ++st) {
-      const int pl = FWD ? st : kRPlanes - 1 - st;
-      if (wvalid && wplane == pl) {
-        // row r of the dependency-side sum: L (forward) or U (backward)
-        T D = T(0);
+    for (int st = 0; st < kBTPlanes; ++st) {
+      const int pl = FWD ? st : kBTPlanes - 1 - st;
+      if (valid && plane == pl) {
+        T dn[3][5];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          T o = T(0);
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
-          for (int q = 0; q < 5; ++q) o += sM[d][5 * r + q][wc] * sA[q][nb[d]];
-          D += o;
+          for (int q = 0; q < 5; ++q) dn[d][q] = sA[q][nb[d]];
+        // this lane's rows of the dependency-side sum, L (forward) or U
+        // (backward), and of v (of D, backward)
+        T v[3];
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl) {
+          T D = T(0);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            T o = T(0);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) o += m[d][sl][q] * dn[d][q];
+            D += o;
+          }
+          if (full) v[sl] = FWD ? rb[sl] + (D - ost[sl]) : (rb[sl] + ost[sl]) - D;
+          else v[sl] = FWD ? rb[sl] + D : D;
         }
-        T v;
-        if (full) v = FWD ? rb + (D - ost) : (rb + ost) - D;
-        else v = FWD ? rb + D : D;
-        // the cell's five rows of v (or of D, backward), from its five lanes
-        T vv[5];
+        // the pair's five rows of v
+        T w[3];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) vv[q] = __shfl(v, gbase + q, 64);
-        T o = T(0);
+        for (int sl = 0; sl < 3; ++sl) w[sl] = __shfl_xor(v[sl], 1, 64);
+        const T vv[5] = {h ? w[0] : v[0], h ? w[1] : v[1], h ? w[2] : v[2], h ? v[0] : w[0], h ? v[1] : w[1]};
 #pragma unroll
-        for (int q = 0; q < 5; ++q) o += ai[q] * vv[q];
-        sA[r][wc] = (!full && !FWD) ? xf - o : o;
+        for (int sl = 0; sl < 3; ++sl) {
+          const int row = r0 + sl;
+          if (sl < nrow) {
+            T o = T(0);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) o += ai[sl][q] * vv[q];
+            sA[row < 5 ? row : 4][wc] = (!full && !FWD) ? xf[sl] - o : o;
+          }
+        }
       }
       __syncthreads();
     }
-    if (tid < 5 * kRT) {
-      const int e = tid / kRT, c = tid % kRT;
-      const int ci = tl.x * kRTI + c % kRTI, cj = tl.y * kRTJ + (c / kRTI) % kRTJ, ck = tl.z * kRTK + c / (kRTI * kRTJ);
+    for (int q = tid; q < 5 * kBT; q += kPThreads) {
+      const int e = q / kBT, c = q % kBT;
+      const int ci = tl.x * kBTI + c % kBTI, cj = tl.y * kBTJ + (c / kBTI) % kBTJ, ck = tl.z * kBTK + c / (kBTI * kBTJ);
       if (ci < ni && cj < nj && ck < nk) x[e * ncell + (long)ci + (long)cj * ni + (long)ck * ni * nj] = sA[e][c];
     }
 
     // --- publish: the barrier orders every wave's du stores before thread
-    // 0's release store
+    // 0's release store, then the next ticket
     __syncthreads();
-    if (tid == 0)
+    if (tid == 0) {
       __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_AGENT);
-    t = tn;
+      sTicket = sNext;
+    }
+    __syncthreads();
   }
 }
 

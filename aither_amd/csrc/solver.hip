@@ -124,8 +124,8 @@ struct ath_ctx {
   ath::ConnTask *dconn = nullptr;
   int nconn = 0;
   // block solvers: one tiled sweep over all blocks, 8 x 8 x 4 tiles with a
-  // lane per cell (msweep, ATH_SWEEP_TILED) or 8 x 4 x 4 tiles with the rows
-  // of each cell on their own lanes (rsweep, ATH_SWEEP_ROWS)
+  // lane per cell (msweep, ATH_SWEEP_TILED) or two lanes per cell, each
+  // holding some of the cell's rows (rsweep, ATH_SWEEP_ROWS)
   ath::MultiSweepCtl msweep{};
   int msweepGrid = 0;
   ath::MultiSweepCtl rsweep{};
@@ -956,9 +956,9 @@ int RebuildGroup(ath_ctx *c) {
       return ATH_OK;
     };
     if (int r = queue(kBTI, kBTJ, kBTK, c->msweep)) return r;
-    if (int r = queue(kRTI, kRTJ, kRTK, c->rsweep)) return r;
+    if (int r = queue(kBTI, kBTJ, kBTK, c->rsweep)) return r;
     c->msweepGrid = std::max(1, std::min(c->msweep.ntiles, 2 * prop.multiProcessorCount));
-    // one 1024-thread workgroup fits a CU
+    // one 512-thread workgroup fits a CU (registers)
     c->rsweepGrid = std::max(1, std::min(c->rsweep.ntiles, prop.multiProcessorCount));
   }
   return ATH_OK;
@@ -1177,10 +1177,10 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
           }
           hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->rsweep);
           if (dir == 0)
-            hipLaunchKernelGGL((ath::KBlockSweepRows<T, true>), dim3(c->rsweepGrid), dim3(ath::kRThreads), 0,
+            hipLaunchKernelGGL((ath::KBlockSweepRows<T, true>), dim3(c->rsweepGrid), dim3(ath::kPThreads), 0,
                                c->stream, c->dblocks, c->P, c->rsweep, full);
           else
-            hipLaunchKernelGGL((ath::KBlockSweepRows<T, false>), dim3(c->rsweepGrid), dim3(ath::kRThreads), 0,
+            hipLaunchKernelGGL((ath::KBlockSweepRows<T, false>), dim3(c->rsweepGrid), dim3(ath::kPThreads), 0,
                                c->stream, c->dblocks, c->P, c->rsweep, full);
         } else {
           hipLaunchKernelGGL(ath::KMultiSweepPrologue, dim3(1), dim3(1), 0, c->stream, c->msweep);
