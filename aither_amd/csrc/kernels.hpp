@@ -1811,6 +1811,26 @@ __global__ void __launch_bounds__(256) KEvalFlux(int flux, Gas gas, const double
 namespace ath {
 
 // ---------------------------------------------------------------------------
+// Publishing a sweep's du to other workgroups.
+//
+// A workgroup publishes a tile's (or a pencil's) du by storing it and then a
+// flag or progress count that other workgroups poll; they read the du after
+// an agent-scope acquire.  An agent-scope release on this target writes back
+// the XCD's L2 and then waits for the wave's stores: the write-back is there
+// for stores that are not coherent at agent scope, and it cost about 100 us
+// of each 760 us block-sweep pass.  The sweeps therefore store du with
+// agent-scope atomic stores (written through to the agent's point of
+// coherence), wait for each wave's own stores to complete, and only then
+// store the flag; no write-back is needed for stores that are already
+// coherent.
+template <typename T>
+__device__ __forceinline__ void StoreCoherent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every store the wave issued has completed
+__device__ __forceinline__ void WaitStores() { __builtin_amdgcn_s_waitcnt(0); }
+
+// ---------------------------------------------------------------------------
 // tiled, dataflow-synchronised LU-SGS sweep (lusgs::LUSGS_Forward/Backward,
 // src/linearSolver.cpp:341-428).
 //
@@ -2167,17 +2187,19 @@ __global__ void __launch_bounds__(kLT) KLusgsTile(DevBlock b, Params P, SweepCtl
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
           sX[e][tid] = xn[e];
-          x[e * b.ncell + p] = xn[e];
+          StoreCoherent(&x[e * b.ncell + p], xn[e]);
         }
       }
       __syncthreads();
     }
 
-    // --- publish: the barrier orders every wave's du stores before thread
-    // 0's release store; the same thread takes the next ticket
+    // --- publish: every wave's du stores have completed before the barrier,
+    // and thread 0 stores the flag after it; the same thread takes the next
+    // ticket
+    WaitStores();
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_store(&s.flags[TileId(s, tl.x, tl.y, tl.z)], epoch, __ATOMIC_RELEASE,
+      __hip_atomic_store(&s.flags[TileId(s, tl.x, tl.y, tl.z)], epoch, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       sTicket = __hip_atomic_fetch_add(s.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -2434,14 +2456,15 @@ __global__ void __launch_bounds__(kBT, sizeof(T) == 4 ? 2 : 1) KBlockSweepTiles(
     }
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) x[e * ncell + p] = sA[e][tid];
+      for (int e = 0; e < 5; ++e) StoreCoherent(&x[e * ncell + p], sA[e][tid]);
     }
 
-    // --- publish: the barrier orders every wave's du stores before thread
-    // 0's release store (one L2 write-back per tile), then the next ticket
+    // --- publish: every wave's du stores have completed before the barrier,
+    // and thread 0 stores the flag after it, then takes the next ticket
+    WaitStores();
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
+      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       sTicket = sNext;
     }
@@ -2751,14 +2774,16 @@ __global__ void __launch_bounds__(kPThreads) KBlockSweepRows(const DevBlock *__r
     for (int q = tid; q < 5 * kBT; q += kPThreads) {
       const int e = q / kBT, c = q % kBT;
       const int ci = tl.x * kBTI + c % kBTI, cj = tl.y * kBTJ + (c / kBTI) % kBTJ, ck = tl.z * kBTK + c / (kBTI * kBTJ);
-      if (ci < ni && cj < nj && ck < nk) x[e * ncell + (long)ci + (long)cj * ni + (long)ck * ni * nj] = sA[e][c];
+      if (ci < ni && cj < nj && ck < nk)
+        StoreCoherent(&x[e * ncell + (long)ci + (long)cj * ni + (long)ck * ni * nj], sA[e][c]);
     }
 
-    // --- publish: the barrier orders every wave's du stores before thread
-    // 0's release store, then the next ticket
+    // --- publish: every wave's du stores have completed before the barrier,
+    // and thread 0 stores the flag after it, then takes the next ticket
+    WaitStores();
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELEASE,
+      __hip_atomic_store(&s.flags[gr.w + tl.x + gr.x * (tl.y + gr.y * tl.z)], epoch, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       sTicket = sNext;
     }
@@ -3234,9 +3259,10 @@ __global__ void __launch_bounds__(kPenWaves * kPenLanes) KLusgsPencil(PencilGrou
       LdsStore(&sh.xReady, st + 1);
       const long p = (long)L.i + (long)L.j * b.ni + (long)(k < 0 ? 0 : k) * b.ni * b.nj;
 #pragma unroll
-      for (int e = 0; e < 5; ++e) *(k >= 0 ? &x[e * b.ncell + p] : &sink[e * kPenLanes + lane]) = xn[e];
+      for (int e = 0; e < 5; ++e) StoreCoherent(k >= 0 ? &x[e * b.ncell + p] : &sink[e * kPenLanes + lane], xn[e]);
       if ((st + 1) % kPubEvery == 0 || st + 1 == ns) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // the owner wave's du stores have completed before the count
+        WaitStores();
         if (lane == 0)
           __hip_atomic_store(&s.prog[pid], st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
