@@ -1106,14 +1106,16 @@ void LaunchResidual(ath_ctx *c) {
 #undef ATH_RES
 }
 
-// the order the sweeps run in: the block solvers have the tiled,
-// row-parallel and hyperplane orders (the pencil order maps to the tiled
-// one), scalar LU-SGS the pencil, tiled and hyperplane orders (the
-// row-parallel order maps to the tiled one); the Roe flux-change
-// off-diagonals (approximateRoe) run in the hyperplane order only
+// the order the sweeps run in: the block solvers have the row-parallel,
+// tiled and hyperplane orders (the pencil order maps to the row-parallel
+// one with the FP64 operator and to the tiled one with the FP32 operator,
+// the faster of the two for each on the 1 M-cell channel), scalar LU-SGS
+// the pencil, tiled and hyperplane orders (the row-parallel order maps to
+// the tiled one); the Roe flux-change off-diagonals (approximateRoe) run in
+// the hyperplane order only
 int EffectiveOrder(const ath_ctx *c) {
   if (c->P.roeJac) return ATH_SWEEP_HYPERPLANE;
-  if (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL) return ATH_SWEEP_TILED;
+  if (c->P.block && c->sweepOrder == ATH_SWEEP_PENCIL) return c->fp32 ? ATH_SWEEP_TILED : ATH_SWEEP_ROWS;
   if (!c->P.block && c->sweepOrder == ATH_SWEEP_ROWS) return ATH_SWEEP_TILED;
   return c->sweepOrder;
 }

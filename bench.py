@@ -62,16 +62,23 @@ VORTEX_SWEEP_TRAFFIC = 0.5 * (146.3e6 + 126.5e6)
 #   backward pass (first sweep, no initialisation: x - D^-1 U): W 40 +
 #   geometry 96 + D^-1 200 + mu 8 + k 8 + velocity gradient 72 + centre
 #   distances 24 + x read 40 + x write 40 = 528
+#   the row-parallel sweep reads the right-hand side formed once per relax
+#   (rhs 40 in place of R, U^n, U^n-1, V and dt): 528 forward and backward
 CHANNEL_FLUX_BYTES = 712
 CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
+CHANNEL_ROWS_SWEEP_BYTES = {"fwd": 528, "bwd": 528}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
 CHANNEL_SWEEP_TRAFFIC = 0.5 * (1338.3e6 + 1119.8e6)   # one pass, forward and backward averaged
-CHANNEL_FLUX_TRAFFIC = 3253.5e6                       # residual stage over all blocks
+CHANNEL_ROWS_SWEEP_TRAFFIC = 0.5 * (1183.3e6 + 1128.8e6)  # the row-parallel sweep
+CHANNEL_FLUX_TRAFFIC = 3055.2e6                       # residual stage over all blocks
 # what the occupancy and VALU counters show binds each (rocprofv3 SQ_WAVES,
 # SQ_INSTS_VALU, VALUBusy, OccupancyPercent on the same case; see
 # profiles/channel-multiblock-1M_sq_counters.md)
+ROWS_SWEEP_LIMITER = ("dependency latency: one workgroup of 8 waves per CU (22 % occupancy), VALU 20 % busy, HBM about "
+                      "28 %; a tile's forming is a chain of memory round trips, and the tile hyperplanes of the largest "
+                      "block set the ramps")
 SWEEP_LIMITER = ("dependency latency: one workgroup per CU (10.6 % occupancy), VALU 17 % busy, HBM about 20 %; "
                  "the pass is the chain of tile hyperplanes of the largest block")
 FLUX_LIMITER = ("mixed: KResidual and the inviscid block diagonal FP64-VALU issue (VALUBusy 78 % and 69 %), the "
@@ -288,13 +295,16 @@ def bench_channel(args, Solver, cases):
     pass_avg_s = tm.sweep_ms / max(tm.sweep_passes, 1) * 1e-3
     flux_share = tm.flux_ms / max(tm.run_ms, 1e-30)
     sweep_share = tm.sweep_ms / max(tm.run_ms, 1e-30)
-    pass_bytes = 0.5 * (CHANNEL_SWEEP_BYTES["fwd"] + CHANNEL_SWEEP_BYTES["bwd"]) * ncell
+    rows = s.sweep_ordering() == Solver.ROWS
+    per_cell = CHANNEL_ROWS_SWEEP_BYTES if rows else CHANNEL_SWEEP_BYTES
+    pass_bytes = 0.5 * (per_cell["fwd"] + per_cell["bwd"]) * ncell
     full = args.refine == 1.0
-    sweep = roofline("block LU-SGS sweep pass over all eight blocks (KBlockSweepTiles<double, FWD>, one "
-                     "launch), average of forward and backward", pass_bytes, pass_avg_s, sweep_share,
-                     traffic=CHANNEL_SWEEP_TRAFFIC if full else None,
+    kname = "KBlockSweepRows<double, FWD>, two lanes per cell" if rows else "KBlockSweepTiles<double, FWD>"
+    sweep = roofline(f"block LU-SGS sweep pass over all eight blocks ({kname}, one launch), average of forward "
+                     "and backward", pass_bytes, pass_avg_s, sweep_share,
+                     traffic=(CHANNEL_ROWS_SWEEP_TRAFFIC if rows else CHANNEL_SWEEP_TRAFFIC) if full else None,
                      launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1),
-                     limiter=SWEEP_LIMITER)
+                     limiter=ROWS_SWEEP_LIMITER if rows else SWEEP_LIMITER)
     flux = roofline("residual stage over all eight blocks (inviscid + viscous fluxes, spectral radii, dt, block "
                     "diagonal and inverse; one launch per kernel)", CHANNEL_FLUX_BYTES * ncell, flux_avg_s,
                     flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full else None, limiter=FLUX_LIMITER)

@@ -103,8 +103,9 @@ int ath_set_implicit_precision(ath_ctx *ctx, int precision);
 #define ATH_SWEEP_ROWS 3
 int ath_set_sweep_ordering(ath_ctx *ctx, int ordering);
 /* the order the sweeps actually run in: block solvers (blusgs, bdplur) map
- * ATH_SWEEP_PENCIL to ATH_SWEEP_TILED, scalar LU-SGS maps ATH_SWEEP_ROWS to
- * ATH_SWEEP_TILED, and the approximateRoe flux Jacobian always runs
+ * ATH_SWEEP_PENCIL to ATH_SWEEP_ROWS with the FP64 operator and to
+ * ATH_SWEEP_TILED with the FP32 operator, scalar LU-SGS maps ATH_SWEEP_ROWS
+ * to ATH_SWEEP_TILED, and the approximateRoe flux Jacobian always runs
  * ATH_SWEEP_HYPERPLANE, whatever was set */
 int ath_get_sweep_ordering(const ath_ctx *ctx, int *ordering);
 /* Autotune the sweep execution order for this case: time `iterations`
