@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -380,6 +381,9 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
   // viscousWall ghost cells, all layers (src/procBlock.cpp:2760-2836)
   {
     std::vector<WallTask> wt;
+    // heat-flux walls: the wall distance of the cell next to the wall, the
+    // same for every ghost layer, found once per cell
+    std::unordered_map<long, double> wallDist;
     for (int layer = 1; layer <= h.g; ++layer)
       for (const auto &sf : h.surf) {
         if (sf.type != "viscousWall") continue;
@@ -400,7 +404,13 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
             HostMap3(dir, iC, q1, q2, &ii, &ij, &ik);
             HostMap3(dir, gC, q1, q2, &gi, &gj, &gk);
             HostMap3(dir, aC, q1, q2, &ai, &aj, &ak);
-            const double wd = flux ? WallDistance(hc, h.ctr[h.Cell(ai, aj, ak)]) : 0.0;
+            double wd = 0.0;
+            if (flux) {
+              const long ac = h.Cell(ai, aj, ak);
+              auto it = wallDist.find(ac);
+              if (it == wallDist.end()) it = wallDist.emplace(ac, WallDistance(hc, h.ctr[ac])).first;
+              wd = it->second;
+            }
             wt.push_back({h.Cell(ii, ij, ik), h.Cell(gi, gj, gk), state, 0, wd});
           }
       }
