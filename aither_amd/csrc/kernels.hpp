@@ -1843,9 +1843,10 @@ __device__ __forceinline__ void WaitStores() { __builtin_amdgcn_s_waitcnt(0); }
 // persistent workgroups, and a tile starts once the (at most three) tiles it
 // depends on have published their du.  Inside a tile the workgroup walks the
 // 18 intra-tile hyperplanes with LDS-resident state, du, right-hand side and
-// face vectors.  Cross-tile visibility uses agent-scope release/acquire on a
-// per-tile flag carrying the sweep's epoch; every wait is bounded and reports
-// through `err` instead of hanging.
+// face vectors.  Cross-tile visibility: du is stored agent-coherent and the
+// per-tile flag, carrying the sweep's epoch, is stored once those stores have
+// completed (StoreCoherent / WaitStores); readers acquire at agent scope.
+// Every wait is bounded and reports through `err` instead of hanging.
 constexpr int kLTI = 8, kLTJ = 8, kLTK = 4;
 constexpr int kLT = kLTI * kLTJ * kLTK;  // 256 cells, one per thread
 constexpr int kLTPlanes = kLTI + kLTJ + kLTK - 2;
@@ -2816,8 +2817,8 @@ __global__ void __launch_bounds__(kPThreads) KBlockSweepRows(const DevBlock *__r
 // least kPW steps ahead, so a 64^3 block's dependency chain stays at the
 // 3*64-2 hyperplanes of HyperplaneReorder plus 14 pencil handoffs.
 //
-// Progress: wave 3 stores du with plain stores, then an agent-scope release
-// fence, then the step count with an agent-scope store.  A consumer reads the
+// Progress: wave 3 stores du with agent-coherent stores, waits for them to
+// complete, then stores the step count with an agent-scope store.  A consumer reads the
 // count with agent-scope loads and issues its loads of that du only after it
 // has seen a count that covers them; those loads are agent-scope (coherent)
 // loads, so no acquire fence (which would drain the prefetch loads in flight)
@@ -2930,8 +2931,8 @@ __global__ void __launch_bounds__(256) KLusgsRhs(BlockMap M, Params P) {
 //    (the operations of RusanovScalarOffDiagonal, in the same order).
 //  * wave 3 (owner): sums the products in the order of ImplicitLower /
 //    ImplicitUpper, forms du, passes it to the neighbour waves through LDS,
-//    stores it, and every kPubEvery steps releases the stores and publishes
-//    the pencil's progress.
+//    stores it (agent-coherent), and every kPubEvery steps waits for its
+//    stores and publishes the pencil's progress.
 //
 // Counters are monotone; each has one writer and is written with a
 // workgroup-scope release after the data it covers, read with an acquire.
