@@ -49,8 +49,8 @@ VORTEX_SWEEP_BYTES = {"fwd": 280, "bwd": 224}
 # HBM bytes per launch on the 64^3 vortex (FP64), from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE scaled by the same counters on a 256 MiB device copy (factor 2.0
 # for FETCH_SIZE, 1.0 for WRITE_SIZE); see profiles/euler-vortex-64_hbm_counters.md
-VORTEX_FLUX_TRAFFIC = 87.6e6
-VORTEX_SWEEP_TRAFFIC = 0.5 * (146.3e6 + 126.5e6)
+VORTEX_FLUX_TRAFFIC = 85.4e6
+VORTEX_SWEEP_TRAFFIC = 0.5 * (263.6e6 + 257.8e6)
 # channel (laminar, block LU-SGS, BDF2):
 #   residual stage (inviscid and viscous face fluxes, spectral radii, dt,
 #   block diagonal and inverse): W 40 + geometry 128 + R 40 + lambda 8 + dt 8
