@@ -75,6 +75,9 @@ struct DevBlock {
   const double ATH_GLOBAL *c2c[3];         // physical faces: centre-to-centre distance projected on the normal
   const unsigned char ATH_GLOBAL *wallf;   // per boundary face (BtSlot order): 1 on a viscousWall patch
   double ATH_GLOBAL *vgrad;                // block solvers: padded cell velocity gradient, 9 * npad, d v_c / d x_r at (3 r + c) npad
+  // viscous block solvers: per cell, 1 where the inviscid face states reach
+  // a viscousWall ghost cell (null when the block has none)
+  const unsigned char ATH_GLOBAL *invPark;
   long cellBase;         // cell offset of this block within its case (L-inf index)
   int blockId;           // block index within its case
   // per case: the CFL number of the current iteration, the device error bits
@@ -1115,10 +1118,13 @@ __device__ __forceinline__ void Rhs(const DevBlock &b, const Params &P, long p, 
 //
 // MODE 0: the inviscid sum, then AddDiagonalTerms and the inverse.
 // With the viscous terms the inviscid sum must see the inviscid ghost states
-// (a viscousWall is a slip wall there), so it runs right after the inviscid
-// residual kernel and parks the sum in Ab (MODE 1); after the viscous pass
-// MODE 2 (central) or 3 (fourth-order face states) adds the viscous faces'
-// thin-shear-layer Jacobians, the cell as upper cell adding
+// (a viscousWall is a slip wall there).  The cells whose face states reach a
+// viscousWall ghost (`invPark`) form it right after the inviscid residual
+// kernel and park it in Ab (MODE 1); after the viscous pass MODE 2 (central)
+// or 3 (fourth-order face states) takes the parked sum or forms the sum
+// itself (every other cell: its face states read no ghost cell that the
+// viscous pass changes, so the sum is the same bits), adds the viscous
+// faces' thin-shear-layer Jacobians, the cell as upper cell adding
 // (src/procBlock.cpp:1467-1474) and as lower cell subtracting (:1416-1424),
 // stores the cell velocity gradient, a sixth of each face's in the same order
 // (:1396-1414, 1427-1436), and finishes.
@@ -1129,27 +1135,31 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P) {
   const DevBlock &b = *U.b;
   const long p = (long)U.local * blockDim.x + threadIdx.x;
   if (p >= b.ncell) return;
+  const bool parked = MODE >= 1 && b.invPark && b.invPark[p];
+  if (MODE == 1 && !parked) return;
   const int i = (int)(p % b.ni), j = (int)((p / b.ni) % b.nj), k = (int)(p / ((long)b.ni * b.nj));
   const GasT<double> g = Cast<double>(P.gas);
   double D[25];
 #pragma unroll
-  for (int q = 0; q < 25; ++q) D[q] = MODE >= 2 ? b.Ab[q * b.ncell + p] : 0.0;
+  for (int q = 0; q < 25; ++q) D[q] = MODE >= 2 && parked ? b.Ab[q * b.ncell + p] : 0.0;
+  if (!(MODE >= 2 && parked)) {
 #pragma unroll
-  for (int d = 0; d < (MODE >= 2 ? 0 : 3); ++d) {
-    // the cell is the upper cell of its lower face (upper state WR) and the
-    // lower cell of its upper face (lower state WL): one side of each
-    double Wf[5], J[25];
-    FaceSide<RECON, LIM>(b, P.kappa, d, i, j, k, true, Wf);
-    double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
-    RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, false, J);
+    for (int d = 0; d < 3; ++d) {
+      // the cell is the upper cell of its lower face (upper state WR) and the
+      // lower cell of its upper face (lower state WL): one side of each
+      double Wf[5], J[25];
+      FaceSide<RECON, LIM>(b, P.kappa, d, i, j, k, true, Wf);
+      double4 a = b.fa[d][FaceIdx(b, d, i, j, k)];
+      RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, false, J);
 #pragma unroll
-    for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
-    const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
-    FaceSide<RECON, LIM>(b, P.kappa, d, ui, uj, uk, false, Wf);
-    a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
-    RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, true, J);
+      for (int q = 0; q < 25; ++q) D[q] = D[q] - J[q];
+      const int ui = i + (d == 0), uj = j + (d == 1), uk = k + (d == 2);
+      FaceSide<RECON, LIM>(b, P.kappa, d, ui, uj, uk, false, Wf);
+      a = b.fa[d][FaceIdx(b, d, ui, uj, uk)];
+      RusanovJacobian<double>(g, Wf, a.x, a.y, a.z, a.w, true, J);
 #pragma unroll
-    for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
+      for (int q = 0; q < 25; ++q) D[q] = D[q] + J[q];
+    }
   }
   if (MODE == 1) {
 #pragma unroll

@@ -90,6 +90,7 @@ struct CaseRes {
 struct ath_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  int anyPark = 0;  // some block parks its wall cells' inviscid block sums (DevBlock::invPark)
   ath::HostCase hc;  // the case (a single-case context) or the numerics every case shares (a server)
   ath::Params P{};
   // the resident cases; blk concatenates the blocks of the cases that
@@ -307,6 +308,38 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     if (int r = c->Alloc(&dw, wf.size())) return r;
     HIP_OK(hipMemcpy(dw, wf.data(), wf.size(), hipMemcpyHostToDevice));
     d.wallf = ath::Dev(dw);
+  }
+  // block solvers: the cells within the ghost depth of a viscousWall face
+  // along its normal, whose inviscid face states read the wall's ghost cells
+  // (which the viscous pass replaces); their inviscid block sum is formed
+  // before the viscous pass
+  if (hc.block) {
+    std::vector<unsigned char> pk((size_t)h.ni * h.nj * h.nk, 0);
+    bool any = false;
+    for (int dd = 0; dd < 3; ++dd)
+      for (int up = 0; up < 2; ++up) {
+        const int d1 = (dd + 1) % 3, d2 = (dd + 2) % 3;
+        for (int c2 = 0; c2 < h.N(d2); ++c2)
+          for (int c1 = 0; c1 < h.N(d1); ++c1) {
+            int i, j, k;
+            HostMap3(dd, up ? h.N(dd) : 0, c1, c2, &i, &j, &k);
+            const Surface *sf = SurfaceAt(h, i, j, k, 2 * dd + 1 + up);
+            if (!sf || sf->type != "viscousWall") continue;
+            for (int layer = 0; layer < h.g && layer < h.N(dd); ++layer) {
+              int ci, cj, ck;
+              HostMap3(dd, up ? h.N(dd) - 1 - layer : layer, c1, c2, &ci, &cj, &ck);
+              pk[(size_t)ci + (size_t)cj * h.ni + (size_t)ck * h.ni * h.nj] = 1;
+              any = true;
+            }
+          }
+      }
+    if (any) {
+      unsigned char *dp = nullptr;
+      if (int r = c->Alloc(&dp, pk.size())) return r;
+      HIP_OK(hipMemcpy(dp, pk.data(), pk.size(), hipMemcpyHostToDevice));
+      d.invPark = ath::Dev(dp);
+      c->anyPark = 1;
+    }
   }
   // edge ghost cells, per line in the reference's layer order
   // (src/procBlock.cpp:2565-2714, 2874-3030)
@@ -618,6 +651,7 @@ int UploadCase(ath_ctx *c, CaseRes &cr, bool fp32) {
     HIP_OK(hipMemset(d.A, 0, nc * sizeof(double)));
     d.Ab = d.Abi = nullptr;
     d.vgrad = nullptr;
+    d.invPark = nullptr;
     d.kc = nullptr;
     for (int dd = 0; dd < 3; ++dd) d.fvis[dd] = nullptr;
     d.tni = (h.ni + 7) / 8;
@@ -1058,12 +1092,13 @@ void LaunchResidualT(ath_ctx *c) {
   const dim3 tiles(c->mapTotal[kMapTile]), cells(c->mapTotal[kMapCell128]);
   hipLaunchKernelGGL((ath::KResidual<RECON, LIM, FLUX>), tiles, dim3(256), 0, c->stream, mt, P);
   if (P.block) {
-    // inviscid block sum from the inviscid ghost states; finished here, or
-    // parked in Ab until the viscous pass has formed dt
-    if (P.visc.on)
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), cells, dim3(128), 0, c->stream, mc, P);
-    else
+    // inviscid block sum from the inviscid ghost states: finished here, or,
+    // for the cells next to a viscous wall, parked in Ab until the viscous
+    // pass has formed dt (the other cells form it then)
+    if (!P.visc.on)
       hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 0>), cells, dim3(128), 0, c->stream, mc, P);
+    else if (c->anyPark)
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 1>), cells, dim3(128), 0, c->stream, mc, P);
   }
   if (!P.visc.on) return;
   // viscous ghost cells, temperature and viscosity, viscous fluxes
@@ -1084,9 +1119,9 @@ void LaunchResidualT(ath_ctx *c) {
   // the block diagonal needs dt, which the viscous pass completes
   if (P.block) {
     if (P.visc.recon4)
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 3>), cells, dim3(128), 0, c->stream, mc, P);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 3>), cells, dim3(128), 0, c->stream, mc, P);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<0, 0, 2>), cells, dim3(128), 0, c->stream, mc, P);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 2>), cells, dim3(128), 0, c->stream, mc, P);
   }
 }
 
