@@ -71,8 +71,8 @@ CHANNEL_ROWS_SWEEP_BYTES = {"fwd": 528, "bwd": 528}
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
 CHANNEL_SWEEP_TRAFFIC = 0.5 * (1338.3e6 + 1119.8e6)   # one pass, forward and backward averaged
-CHANNEL_ROWS_SWEEP_TRAFFIC = 0.5 * (1183.3e6 + 1128.8e6)  # the row-parallel sweep
-CHANNEL_FLUX_TRAFFIC = 3055.2e6                       # residual stage over all blocks
+CHANNEL_ROWS_SWEEP_TRAFFIC = 0.5 * (1190.1e6 + 1123.9e6)  # the row-parallel sweep
+CHANNEL_FLUX_TRAFFIC = 2571.1e6                       # residual stage over all blocks
 # what the occupancy and VALU counters show binds each (rocprofv3 SQ_WAVES,
 # SQ_INSTS_VALU, VALUBusy, OccupancyPercent on the same case; see
 # profiles/channel-multiblock-1M_sq_counters.md)
@@ -81,9 +81,9 @@ ROWS_SWEEP_LIMITER = ("dependency latency: one workgroup of 8 waves per CU (22 %
                       "block set the ramps")
 SWEEP_LIMITER = ("dependency latency: one workgroup per CU (10.6 % occupancy), VALU 17 % busy, HBM about 20 %; "
                  "the pass is the chain of tile hyperplanes of the largest block")
-FLUX_LIMITER = ("mixed: KResidual and the inviscid block diagonal FP64-VALU issue (VALUBusy 78 % and 69 %), the "
-                "viscous block diagonal HBM (61 %), the viscous flux the latency of its gathers (occupancy 23 %, "
-                "VALU 20 %, HBM 33 %)")
+FLUX_LIMITER = ("mixed: KResidual FP64-VALU issue (VALUBusy 80 %), the viscous block diagonal with the inviscid "
+                "block sum VALU (56 %) and HBM (42 %), the viscous flux the latency of its gathers (occupancy 22 %, "
+                "VALU 27 %, HBM 40 %)")
 
 
 def parse():
