@@ -184,11 +184,16 @@ def _common_state_parity(s, o, iterations, tol=1e-12):
                 assert abs(ns.matrix_resid - no["matrix_resid"]) <= 1e-10 * no["matrix_resid"], (nn, mm)
 
 
-@pytest.mark.parametrize("visc_recon", ["central", "centralFourth"])
-def test_viscous_block_diagonal_matches_oracle(tmp_path, visc_recon):
-    p = cases.bump_case(str(tmp_path), seed=4, n=8, iterations=3, viscous=True, visc_recon=visc_recon)[0]
+@pytest.mark.parametrize("visc_recon,recon", [("central", "thirdOrder"), ("centralFourth", "thirdOrder"),
+                                              ("central", "weno")])
+def test_viscous_block_diagonal_matches_oracle(tmp_path, visc_recon, recon):
+    """The wall cells' inviscid sums are formed before the viscous ghost
+    cells, every other cell's after them (three ghost layers deep with
+    WENO); both against the oracle."""
+    p = cases.bump_case(str(tmp_path), seed=4, n=8, iterations=3, viscous=True, visc_recon=visc_recon,
+                        recon=recon)[0]
     path = _variant(p, "blk", **{"matrixSolver: lusgs": "matrixSolver: blusgs"})
-    say("viscous block diagonal", visc_recon)
+    say("viscous block diagonal", visc_recon, recon)
     s = Solver(path)
     o = pyoracle.OracleCase(path)
     # from a non-uniform state: at the uniform initial state the limiter
