@@ -108,6 +108,7 @@ struct Params {
   int fp32;   // the scalar implicit operator runs in FP32: the residual stage writes Ainv32, else Ainv
   int roeJac; // inviscidFluxJacobian approximateRoe: the off-diagonals are Roe flux changes (FP64)
   int rhsOnce; // the relax forms the right-hand side once, into rhs (the row-parallel block sweep reads it)
+  int rhsInDiag; // the viscous block diagonal forms that right-hand side and du = 0 (the relax does not)
 };
 
 // ---------------------------------------------------------------------------
@@ -1213,6 +1214,17 @@ __global__ void __launch_bounds__(128) KBlockDiag(BlockMap M, Params P) {
   if (P.fp32 && b.Abi32) {
 #pragma unroll
     for (int q = 0; q < 25; ++q) b.Abi32[TileBase(b, i, j, k) + 256 * q] = float(I[q]);
+  }
+  if (MODE >= 2 && P.rhsInDiag) {
+    // the initialisation of a one-sweep relax (KInitUpdate with rhsOnce,
+    // FP64 operator) from the values at hand: the right-hand side and du = 0
+    double rb[5];
+    Rhs(b, P, p, PadIdx(b, i, j, k), rb);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      b.rhs[e * b.ncell + p] = rb[e];
+      b.x[e * b.ncell + p] = 0.0;
+    }
   }
 }
 

@@ -1134,6 +1134,18 @@ void LaunchConnTerms(ath_ctx *c) {
                        c->nconn, c->P);
 }
 
+int EffectiveOrder(const ath_ctx *c);
+
+// the relax's initialisation (right-hand side once per relax, du = 0) is
+// formed by the viscous block diagonal: one-sweep block LU-SGS with the FP64
+// operator in the row-parallel order, viscous terms on.  A one-sweep relax
+// never reads its initial du, so a relax repeated without a new residual
+// gives the same result.
+bool RhsInDiag(const ath_ctx *c) {
+  return c->P.block && c->P.visc.on && !c->fp32 && c->hc.solver == ath::Solver::LUSGS &&
+         !c->hc.InitMatrix() && EffectiveOrder(c) == ATH_SWEEP_ROWS;
+}
+
 template <int RECON, int LIM, int FLUX>
 void LaunchResidualT(ath_ctx *c) {
   const ath::Params &P = c->P;
@@ -1167,10 +1179,12 @@ void LaunchResidualT(ath_ctx *c) {
     hipLaunchKernelGGL((ath::KViscFlux<false>), tiles, dim3(256), 0, c->stream, mt, P);
   // the block diagonal needs dt, which the viscous pass completes
   if (P.block) {
+    ath::Params PD = P;
+    PD.rhsInDiag = RhsInDiag(c) ? 1 : 0;
     if (P.visc.recon4)
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 3>), cells, dim3(128), 0, c->stream, mc, P);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 3>), cells, dim3(128), 0, c->stream, mc, PD);
     else
-      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 2>), cells, dim3(128), 0, c->stream, mc, P);
+      hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 2>), cells, dim3(128), 0, c->stream, mc, PD);
   }
 }
 
@@ -1229,8 +1243,9 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
   // initialisation forms once per relax
   ath::Params P0 = c->P;
   P0.rhsOnce = (hc.solver == ath::Solver::LUSGS && c->P.block && EffectiveOrder(c) == ATH_SWEEP_ROWS) ? 1 : 0;
-  hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
-                     c->Map(kMapCell256), P0);
+  if (!RhsInDiag(c))
+    hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
+                       c->Map(kMapCell256), P0);
   for (int sw = 0; sw < hc.sweeps; ++sw) {
     const int full = (sw > 0 || hc.InitMatrix()) ? 1 : 0;
     const int order = EffectiveOrder(c);

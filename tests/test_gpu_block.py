@@ -306,3 +306,31 @@ def test_row_sweep_equals_tiled(tmp_path, case, precision):
         s.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+def test_split_calls_equal_iterate_on_the_channel(tmp_path):
+    """The C ABI's separate calls (ghost cells, residual, relax, update) give
+    the bits of ath_iterate on the multiblock channel, whose viscous block
+    diagonal also forms the one-sweep relax's right-hand side for the
+    row-parallel sweep."""
+    path, meta = cases.channel_case(str(tmp_path), refine=0.25, steps=2, subits=3)
+    say("split calls", "channel")
+    out = []
+    for split in (False, True):
+        s = Solver(path)
+        assert s.sweep_ordering() == Solver.ROWS or s.sweep_ordering() == Solver.PENCIL
+        norms = []
+        for nn in range(2):
+            s.begin_step(nn)
+            for mm in range(3):
+                if split:
+                    s.assign_ghost_cells()
+                    s.calc_residual()
+                    s.relax()
+                    norms.append(s.update(mm).l2_sq[:5])
+                else:
+                    norms.append(s.iterate(nn, mm).l2_sq[:5])
+        out.append((np.concatenate([s.get_state(b).reshape(-1) for b in range(s.info.num_blocks)]), norms))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
