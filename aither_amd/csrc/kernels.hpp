@@ -2527,7 +2527,8 @@ __global__ void __launch_bounds__(256) KOtherSide(BlockMap M, Params P) {
 // * Forming: lane h forms the block of direction 2h (i or k) and both lanes
 //   form the j block; a lane keeps its rows and hands the pair's other rows
 //   to its partner with a lane shuffle.  The right-hand side is formed once
-//   per relax by KInitUpdate, a full sweep's other-side sums by KOtherSide
+//   per relax (by the viscous block diagonal for a one-sweep FP64 relax,
+//   else by KInitUpdate), a full sweep's other-side sums by KOtherSide
 //   before the pass.
 // * The walk: per cell hyperplane a lane forms its rows of the dependency
 //   sum and of v, the pair exchanges v, and each lane forms its rows of

@@ -65,14 +65,18 @@ VORTEX_SWEEP_TRAFFIC = 0.5 * (263.6e6 + 257.8e6)
 #   the row-parallel sweep reads the right-hand side formed once per relax
 #   (rhs 40 in place of R, U^n, U^n-1, V and dt): 528 forward and backward
 CHANNEL_FLUX_BYTES = 712
+# with the row-parallel sweep (FP64 operator, one sweep) the viscous block
+# diagonal also forms the relax's right-hand side and du = 0: U^n 40 +
+# U^n-1 40 read, rhs 40 + x 40 written (W, R, V and dt are counted above)
+CHANNEL_FLUX_RHS_BYTES = 160
 CHANNEL_SWEEP_BYTES = {"fwd": 624, "bwd": 528}
 CHANNEL_ROWS_SWEEP_BYTES = {"fwd": 528, "bwd": 528}
 # HBM bytes per launch on the full-size channel (refine 1), from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by the device-copy calibration (2.0, 1.0);
 # see profiles/channel-multiblock-1M_hbm_counters.md
 CHANNEL_SWEEP_TRAFFIC = 0.5 * (1338.3e6 + 1119.8e6)   # one pass, forward and backward averaged
-CHANNEL_ROWS_SWEEP_TRAFFIC = 0.5 * (1190.1e6 + 1123.9e6)  # the row-parallel sweep
-CHANNEL_FLUX_TRAFFIC = 2571.1e6                       # residual stage over all blocks
+CHANNEL_ROWS_SWEEP_TRAFFIC = 0.5 * (1177.5e6 + 1123.9e6)  # the row-parallel sweep
+CHANNEL_FLUX_TRAFFIC = 2827.7e6                       # residual stage over all blocks
 # what the occupancy and VALU counters show binds each (rocprofv3 SQ_WAVES,
 # SQ_INSTS_VALU, VALUBusy, OccupancyPercent on the same case; see
 # profiles/channel-multiblock-1M_sq_counters.md)
@@ -82,8 +86,8 @@ ROWS_SWEEP_LIMITER = ("dependency latency: one workgroup of 8 waves per CU (22 %
 SWEEP_LIMITER = ("dependency latency: one workgroup per CU (10.6 % occupancy), VALU 17 % busy, HBM about 20 %; "
                  "the pass is the chain of tile hyperplanes of the largest block")
 FLUX_LIMITER = ("mixed: KResidual FP64-VALU issue (VALUBusy 80 %), the viscous block diagonal with the inviscid "
-                "block sum VALU (56 %) and HBM (42 %), the viscous flux the latency of its gathers (occupancy 22 %, "
-                "VALU 27 %, HBM 40 %)")
+                "block sum and the relax's right-hand side VALU (52 %) and HBM (45 %), the viscous flux the latency "
+                "of its gathers (occupancy 22 %, VALU 27 %, HBM 40 %)")
 
 
 def parse():
@@ -305,9 +309,13 @@ def bench_channel(args, Solver, cases):
                      traffic=(CHANNEL_ROWS_SWEEP_TRAFFIC if rows else CHANNEL_SWEEP_TRAFFIC) if full else None,
                      launches_per_pass=tm.sweep_launches / max(tm.sweep_passes, 1),
                      limiter=ROWS_SWEEP_LIMITER if rows else SWEEP_LIMITER)
+    rhs_in_stage = rows  # the FP64 operator (this run) with the row-parallel sweep
     flux = roofline("residual stage over all eight blocks (inviscid + viscous fluxes, spectral radii, dt, block "
-                    "diagonal and inverse; one launch per kernel)", CHANNEL_FLUX_BYTES * ncell, flux_avg_s,
-                    flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full else None, limiter=FLUX_LIMITER)
+                    "diagonal and inverse" + (", the relax's right-hand side" if rhs_in_stage else "") +
+                    "; one launch per kernel)",
+                    (CHANNEL_FLUX_BYTES + (CHANNEL_FLUX_RHS_BYTES if rhs_in_stage else 0)) * ncell, flux_avg_s,
+                    flux_share, traffic=CHANNEL_FLUX_TRAFFIC if full and rhs_in_stage else None,
+                    limiter=FLUX_LIMITER)
     dominant, other = (sweep, flux) if sweep_share >= flux_share else (flux, sweep)
     out = {
         "metric": "cell-updates/s at <=1e-10 err vs FP64 oracle",
