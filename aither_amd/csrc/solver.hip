@@ -18,6 +18,7 @@
 #include "host_case.hpp"
 #include "io.hpp"
 #include "kernels.hpp"
+#include "wall_tree.hpp"
 
 namespace {
 
@@ -223,72 +224,24 @@ void HostMap3(int dir, int d3, int d1, int d2, int *i, int *j, int *k) {
 // procBlock::CalcWallDistance through kdtree::NearestNeighbor
 // (src/procBlock.cpp:6030-6041, src/kdtree.cpp:123-224): the distance from a
 // point to the nearest centre of a viscousWall face of any block
-// (GetViscousFaceCenters, src/utility.cpp:310-366).  The centres are held in
-// a k-d tree (median split, axis by depth, small leaves searched in full).
-// The search is exact: a subtree beyond a splitting plane is skipped only
-// when the squared distance to the plane exceeds the best so far, and the
-// rounded squared distance of any point beyond the plane is at least that
-// (rounding is monotonic), so the minimum of the squared distances is the
-// one a full scan finds, bit for bit.
-class WallTree {
- public:
-  explicit WallTree(const ath::HostCase &hc) {
-    for (const auto &h : hc.blocks)
-      for (const auto &sf : h.surf) {
-        if (sf.type != "viscousWall") continue;
-        const int st = sf.SurfType(), dir = (st - 1) / 2;
-        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
-        for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
-          for (int q1 = sf.d[2 * d1]; q1 < sf.d[2 * d1 + 1]; ++q1) {
-            int i, j, k;
-            HostMap3(dir, sf.d[2 * dir], q1, q2, &i, &j, &k);
-            pts_.push_back(h.fc[dir][h.Face(dir, i, j, k)]);
-          }
-      }
-    Build(0, pts_.size(), 0);
-  }
-  double Distance(const ath::Vec3 &q) const {
-    double best = std::numeric_limits<double>::max();
-    Search(q, 0, pts_.size(), 0, best);
-    return std::sqrt(best);
-  }
-
- private:
-  static constexpr size_t kLeaf = 8;
-  std::vector<ath::Vec3> pts_;
-  static double Coord(const ath::Vec3 &v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
-  void Build(size_t lo, size_t hi, int depth) {
-    if (hi - lo <= kLeaf) return;
-    const size_t mid = lo + (hi - lo) / 2;
-    const int a = depth % 3;
-    std::nth_element(pts_.begin() + lo, pts_.begin() + mid, pts_.begin() + hi,
-                     [a](const ath::Vec3 &u, const ath::Vec3 &v) { return Coord(u, a) < Coord(v, a); });
-    Build(lo, mid, depth + 1);
-    Build(mid + 1, hi, depth + 1);
-  }
-  static double Dist2(const ath::Vec3 &q, const ath::Vec3 &f) {
-    const double dx = q.x - f.x, dy = q.y - f.y, dz = q.z - f.z;
-    return ((0.0 + dx * dx) + dy * dy) + dz * dz;
-  }
-  void Search(const ath::Vec3 &q, size_t lo, size_t hi, int depth, double &best) const {
-    if (hi - lo <= kLeaf) {
-      for (size_t n = lo; n < hi; ++n) best = std::min(best, Dist2(q, pts_[n]));
-      return;
+// (GetViscousFaceCenters, src/utility.cpp:310-366), from a k-d tree whose
+// search gives the full scan's minimum bit for bit (wall_tree.hpp)
+std::vector<ath::Vec3> WallFaceCentres(const ath::HostCase &hc) {
+  std::vector<ath::Vec3> pts;
+  for (const auto &h : hc.blocks)
+    for (const auto &sf : h.surf) {
+      if (sf.type != "viscousWall") continue;
+      const int st = sf.SurfType(), dir = (st - 1) / 2;
+      const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+      for (int q2 = sf.d[2 * d2]; q2 < sf.d[2 * d2 + 1]; ++q2)
+        for (int q1 = sf.d[2 * d1]; q1 < sf.d[2 * d1 + 1]; ++q1) {
+          int i, j, k;
+          HostMap3(dir, sf.d[2 * dir], q1, q2, &i, &j, &k);
+          pts.push_back(h.fc[dir][h.Face(dir, i, j, k)]);
+        }
     }
-    const size_t mid = lo + (hi - lo) / 2;
-    const int a = depth % 3;
-    best = std::min(best, Dist2(q, pts_[mid]));
-    // points below mid have coordinate <= the split, points above >= it
-    const double diff = Coord(q, a) - Coord(pts_[mid], a);
-    const bool below = diff < 0.0;
-    if (below) Search(q, lo, mid, depth + 1, best);
-    else Search(q, mid + 1, hi, depth + 1, best);
-    if (!(diff * diff > best)) {
-      if (below) Search(q, mid + 1, hi, depth + 1, best);
-      else Search(q, lo, mid, depth + 1, best);
-    }
-  }
-};
+  return pts;
+}
 
 // padded geometry, projected centre distances, wall flags and the ghost-cell
 // task lists of a viscous case
@@ -462,7 +415,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
     // heat-flux walls: the wall distance of the cell next to the wall, the
     // same for every ghost layer, found once per cell
     std::unordered_map<long, double> wallDist;
-    std::unique_ptr<WallTree> tree;
+    std::unique_ptr<ath::NearestPoints> tree;
     for (int layer = 1; layer <= h.g; ++layer)
       for (const auto &sf : h.surf) {
         if (sf.type != "viscousWall") continue;
@@ -488,7 +441,7 @@ int UploadViscous(ath_ctx *c, const ath::HostCase &hc, const ath::HostBlock &h, 
               const long ac = h.Cell(ai, aj, ak);
               auto it = wallDist.find(ac);
               if (it == wallDist.end()) {
-                if (!tree) tree = std::make_unique<WallTree>(hc);
+                if (!tree) tree = std::make_unique<ath::NearestPoints>(WallFaceCentres(hc));
                 it = wallDist.emplace(ac, tree->Distance(h.ctr[ac])).first;
               }
               wd = it->second;
