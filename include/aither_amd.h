@@ -149,7 +149,10 @@ int ath_assign_ghost_cells(ath_ctx *ctx);
  * Invert (src/gridLevel.cpp:372-406): one fused launch per block */
 int ath_calc_residual(ath_ctx *ctx);
 /* linearSolver::InitializeMatrixUpdate + Relax (src/linearSolver.cpp:111-144,
- * 430-470, 509-535); returns the matrix residual (CycleAtLevel value) */
+ * 430-470, 509-535); returns the matrix residual (CycleAtLevel value).  Uses
+ * the last ath_calc_residual's residual, time step and diagonal (for a
+ * one-sweep FP64 block LU-SGS relax in the row-parallel order also the
+ * right-hand side that call formed). */
 int ath_relax(ath_ctx *ctx, double *cycle_value);
 /* gridLevel::UpdateBlocks + ResetDiagonal (src/gridLevel.cpp:418-431) */
 int ath_update(ath_ctx *ctx, int mm, ath_norms *norms);
