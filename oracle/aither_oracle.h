@@ -29,8 +29,10 @@
  *     run on the reference's own grids and input files copied as fixtures
  *     under tests/golden/;
  *   - closed forms and invariants for the kernel-level functions.
+ * crankNicholson has no reference vector; it is pinned by one relax being
+ * exactly twice an implicitEuler relax at half the CFL (block solvers,
+ * tests/test_oracle_block.py).
  * Parity unpinned (no reference vector; checked only as stated):
- *   - crankNicholson time integration (agrees with the GPU);
  *   - the temperature columns of the thin-shear-layer Jacobian (form only);
  *   - approximateRoe off-diagonals (linearisation and du = 0 terms pinned by
  *     mathematics; no reference case uses them);

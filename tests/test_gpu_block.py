@@ -334,3 +334,29 @@ def test_split_calls_equal_iterate_on_the_channel(tmp_path):
         s.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("solver,sweeps,viscous", [("blusgs", 1, True), ("bdplur", 2, False)])
+def test_crank_nicolson_step_is_twice_implicit_euler_on_gpu(tmp_path, solver, sweeps, viscous):
+    """The relation that pins Crank-Nicolson in the oracle, on the GPU: one
+    Crank-Nicolson relax is twice the implicit-Euler relax at half the CFL,
+    bit for bit (linear block operators, theta in -R/theta and in the time
+    term of the diagonal)."""
+    out = {}
+    for tag, timeint, cfl in (("cn", "crankNicholson", 10.0), ("ie", "implicitEuler", 5.0)):
+        d = tmp_path / tag
+        d.mkdir()
+        base = cases.bump_case(str(d), seed=7, n=8, iterations=1, cfl=cfl, viscous=viscous)[0]
+        path = _variant(base, "v", **{"timeIntegration: implicitEuler": f"timeIntegration: {timeint}",
+                                      "matrixSolver: lusgs": f"matrixSolver: {solver}",
+                                      "matrixSweeps: 1": f"matrixSweeps: {sweeps}"})
+        say("crank-nicolson relation", solver, sweeps, viscous, tag)
+        s = Solver(path)
+        s.begin_step(0)
+        s.assign_ghost_cells()
+        s.calc_residual()
+        s.relax()
+        out[tag] = np.concatenate([s.update_vector(b).reshape(-1) for b in range(s.info.num_blocks)])
+        s.close()
+    assert np.abs(out["ie"]).max() > 0
+    assert np.array_equal(out["cn"], 2.0 * out["ie"])

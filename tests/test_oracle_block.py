@@ -177,3 +177,35 @@ def test_block_and_scalar_solvers_reach_the_same_steady_state(tmp_path):
         W, R = states[solver]
         assert R < 1e-7, (solver, R)
         assert (np.abs(W - ref).max(axis=0) / scale).max() < 2e-5, solver
+
+
+@pytest.mark.parametrize("solver,sweeps,viscous", [("blusgs", 1, False), ("blusgs", 2, False),
+                                                   ("bdplur", 2, False), ("blusgs", 1, True),
+                                                   ("bdplur", 3, True)])
+def test_crank_nicolson_step_is_twice_implicit_euler_at_half_cfl(tmp_path, solver, sweeps, viscous):
+    """Crank-Nicolson (theta 1/2, zeta 0) against implicit Euler (theta 1)
+    at half the time step, from the same state, one nonlinear iteration:
+    the right-hand side -R/theta is -2R against -R, and the time term
+    V(1 + zeta)/(dt theta) of the diagonal is the same.  The block operators
+    are linear, and scaling by 2 is exact in every operation, so the update
+    of the first is twice the second's bit for bit, and its matrix residual
+    (a sum of squares) four times.  A theta on the wrong side of either term
+    breaks this."""
+    out = {}
+    for tag, timeint, cfl in (("cn", "crankNicholson", 10.0), ("ie", "implicitEuler", 5.0)):
+        d = tmp_path / tag
+        d.mkdir()
+        base = cases.bump_case(str(d), seed=7, n=8, iterations=1, cfl=cfl, viscous=viscous)[0]
+        text = open(base).read()
+        for a, b in (("timeIntegration: implicitEuler", f"timeIntegration: {timeint}"),
+                     ("matrixSolver: lusgs", f"matrixSolver: {solver}"),
+                     ("matrixSweeps: 1", f"matrixSweeps: {sweeps}")):
+            assert a in text
+            text = text.replace(a, b)
+        open(base, "w").write(text)
+        out[tag] = _solve_once(base)
+    x_cn, mr_cn = out["cn"]
+    x_ie, mr_ie = out["ie"]
+    assert np.abs(x_ie).max() > 0
+    assert np.array_equal(x_cn, 2.0 * x_ie)
+    assert mr_cn == 4.0 * mr_ie
