@@ -92,6 +92,7 @@ struct ath_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int anyPark = 0;  // some block parks its wall cells' inviscid block sums (DevBlock::invPark)
+  int rhsFormed = 0;  // the last residual stage formed the relax's right-hand side (RhsInDiag)
   ath::HostCase hc;  // the case (a single-case context) or the numerics every case shares (a server)
   ath::Params P{};
   // the resident cases; blk concatenates the blocks of the cases that
@@ -1134,6 +1135,7 @@ void LaunchResidualT(ath_ctx *c) {
   if (P.block) {
     ath::Params PD = P;
     PD.rhsInDiag = RhsInDiag(c) ? 1 : 0;
+    c->rhsFormed = PD.rhsInDiag;
     if (P.visc.recon4)
       hipLaunchKernelGGL((ath::KBlockDiag<RECON, LIM, 3>), cells, dim3(128), 0, c->stream, mc, PD);
     else
@@ -1151,6 +1153,7 @@ void LaunchAfterResiduals(ath_ctx *c) {
 
 // the residual stage of every block: one launch per kernel over all blocks
 void LaunchResidual(ath_ctx *c) {
+  c->rhsFormed = 0;
   const auto &hc = c->hc;
   const int recon = (int)hc.recon;
   const int lim = (int)hc.limiter;
@@ -1196,7 +1199,8 @@ int LaunchRelaxT(ath_ctx *c, bool timed, size_t evBase) {
   // initialisation forms once per relax
   ath::Params P0 = c->P;
   P0.rhsOnce = (hc.solver == ath::Solver::LUSGS && c->P.block && EffectiveOrder(c) == ATH_SWEEP_ROWS) ? 1 : 0;
-  if (!RhsInDiag(c))
+  // (formed by the residual stage unless the sweep order changed since)
+  if (!(RhsInDiag(c) && c->rhsFormed))
     hipLaunchKernelGGL(ath::KInitUpdate<T>, dim3(c->mapTotal[kMapCell256]), dim3(kBlk), 0, c->stream,
                        c->Map(kMapCell256), P0);
   for (int sw = 0; sw < hc.sweeps; ++sw) {

@@ -360,3 +360,25 @@ def test_crank_nicolson_step_is_twice_implicit_euler_on_gpu(tmp_path, solver, sw
         s.close()
     assert np.abs(out["ie"]).max() > 0
     assert np.array_equal(out["cn"], 2.0 * out["ie"])
+
+
+def test_sweep_order_changed_between_residual_and_relax(tmp_path):
+    """A residual computed with one sweep order and a relax run with another
+    (the row-parallel order's relax reads a right-hand side that its
+    residual stage forms) give the same update as either order alone."""
+    path, meta = cases.channel_case(str(tmp_path), refine=0.125, steps=1, subits=1)
+    say("order change between residual and relax")
+    ups = []
+    for first, second in (("TILED", "TILED"), ("ROWS", "ROWS"), ("TILED", "ROWS"), ("ROWS", "TILED")):
+        s = Solver(path)
+        s.begin_step(0)
+        s.set_sweep_ordering(getattr(Solver, first))
+        s.assign_ghost_cells()
+        s.calc_residual()
+        s.set_sweep_ordering(getattr(Solver, second))
+        s.relax()
+        ups.append(np.concatenate([s.update_vector(b).reshape(-1) for b in range(s.info.num_blocks)]))
+        s.close()
+    assert np.abs(ups[0]).max() > 0
+    for u in ups[1:]:
+        assert np.array_equal(u, ups[0])
