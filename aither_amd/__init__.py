@@ -19,7 +19,7 @@ import os
 import numpy as np
 
 __all__ = ["Solver", "Server", "SlotInfo", "Info", "Norms", "Timing", "AitherError", "load_library",
-           "eval_face_flux", "inspect_info", "inspect_field", "LIB_PATH"]
+           "eval_face_flux", "eval_sst_cells", "inspect_info", "inspect_field", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libaither_amd.so")
 MAX_EQ = 8
@@ -122,6 +122,7 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_long)],
     "ath_eval_face_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D,
                            ctypes.c_int, _D],
+    "ath_eval_sst_cells": [ctypes.c_int, _D, ctypes.c_int, _D],
     "ath_write_restart": [_P, ctypes.c_char_p, ctypes.c_int],
     "ath_read_restart": [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
     "ath_write_output": [_P, ctypes.c_char_p, ctypes.c_int],
@@ -197,6 +198,20 @@ def eval_face_flux(flux, gas_r, gas_n, WL, WR, n, device=0):
     F = np.empty((count, 5), dtype=np.float64)
     _check(lib.ath_eval_face_flux(device, flux, gas_r, gas_n, _ptr(WL), _ptr(WR), _ptr(n), count, _ptr(F)))
     return F
+
+
+def eval_sst_cells(cells, device=0):
+    """Device SST k-omega 2003 cell model: ``cells`` is (n, 27) (primitive
+    state with k and omega, velocity gradient, k and omega gradients, laminar
+    viscosity, wall distance, volume, scaling, 1/scaling); returns (n, 7):
+    F1, F2, eddy viscosity, k and omega sources, source Jacobian (0,0), (1,1)."""
+    lib = load_library()
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    if cells.ndim != 2 or cells.shape[1] != 27:
+        raise ValueError("cells must be (n, 27)")
+    out = np.empty((cells.shape[0], 7), dtype=np.float64)
+    _check(lib.ath_eval_sst_cells(device, _ptr(cells), cells.shape[0], _ptr(out)))
+    return out
 
 
 class Solver:

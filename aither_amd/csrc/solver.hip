@@ -18,6 +18,7 @@
 #include "host_case.hpp"
 #include "io.hpp"
 #include "kernels.hpp"
+#include "sst.hpp"
 #include "wall_tree.hpp"
 
 namespace {
@@ -2176,6 +2177,22 @@ int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n, const d
   return ATH_OK;
 }
 
+int ath_eval_sst_cells(int device, const double *in, int count, double *out) {
+  if (count <= 0 || !in || !out) return Fail(ATH_ERR_ARG, "bad argument");
+  HIP_OK(hipSetDevice(device));
+  double *buf = nullptr;
+  const size_t ni = (size_t)count * ath::sst::kIn, no = (size_t)count * ath::sst::kOut;
+  HIP_OK(hipMalloc(&buf, (ni + no) * sizeof(double)));
+  hipError_t e = hipMemcpy(buf, in, ni * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(ath::sst::KCells, dim3(Grid(count)), dim3(kBlk), 0, 0, buf, count, buf + ni);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, buf + ni, no * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(buf);
+  if (e != hipSuccess) return Fail(ATH_ERR_HIP, std::string("eval_sst_cells: ") + hipGetErrorString(e));
+  return ATH_OK;
+}
 
 }  // extern "C"
 

@@ -231,6 +231,14 @@ int ath_inspect_field(const char *inp_path, int block, int which, double *out,
 int ath_eval_face_flux(int device, int flux, double gas_r, double gas_n,
                        const double *WL, const double *WR, const double *n,
                        int count, double *F);
+/* The SST k-omega 2003 cell model (turbKWSst::EddyViscAndBlending, CalcTurbSrc
+ * and TurbSrcJac, src/turbulence.cpp:573-760): per cell in[27] = primitive
+ * state with k and omega (7), velocity gradient (9, the reference's tensor
+ * order), k gradient (3), omega gradient (3), laminar viscosity, wall
+ * distance, volume, viscous scaling and its inverse; out[7] = F1, F2, eddy
+ * viscosity, k source, omega source, source Jacobian (0,0) and (1,1).  The
+ * model alone: the RANS equations are not part of the iteration. */
+int ath_eval_sst_cells(int device, const double *in, int count, double *out);
 /* ath_eval_face_flux without any device allocation: no counterpart in the
  * reference.  For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -3440,3 +3440,87 @@ void orc_get_cell_width(const orc_case *c, int bq, int d, double *out) {
   const blk_t *b = &c->blk[bq];
   memcpy(out, b->cw[d], sizeof(double) * (size_t)ncells_padded(b));
 }
+
+/* ---- SST k-omega 2003 cell model (turbKWSst, src/turbulence.cpp:573-760,
+ * include/turbulence.hpp:489-540; turbModel base src/turbulence.cpp:36-105,
+ * sutherland::Lambda src/transport.cpp:172-175 with zero bulk viscosity).
+ * Pointwise: the blending functions, the limited eddy viscosity, the k and
+ * omega source terms with the 2003 production limiter, and the diagonal of
+ * the source Jacobian, in the reference's operation order.  Tensors are in
+ * the reference's data order (row-major 3 x 3). */
+static const double SST_BETA_STAR = 0.09, SST_SIGMA_W2 = 0.856, SST_BETA1 = 0.075, SST_BETA2 = 0.0828,
+                    SST_GAMMA1 = 5.0 / 9.0, SST_GAMMA2 = 0.44, SST_A1 = 0.31, SST_PROD2DEST = 10.0;
+
+static double sst_ddot(const double *A, const double *B) {
+  /* tensor::DoubleDotTrans: elementwise product, std::accumulate from 0 */
+  double s = 0.0;
+  for (int q = 0; q < 9; ++q) s = s + A[q] * B[q];
+  return s;
+}
+
+static double sst_dot3(const double *a, const double *b) {
+  /* vector3d::DotProd: std::inner_product from 0 */
+  return ((0.0 + a[0] * b[0]) + a[1] * b[1]) + a[2] * b[2];
+}
+
+static double sst_cdkw(const double *W, const double *kg, const double *wg) {
+  double v = (((2.0 * W[0]) * SST_SIGMA_W2) / W[6]) * sst_dot3(kg, wg);
+  return v > 1.0e-10 ? v : 1.0e-10;
+}
+
+/* in: W[7] (rho, u, v, w, p, k, omega), G[9] velocity gradient, kg[3], wg[3],
+ *     mu (laminar), wallDist, vol, scaling, invScaling (27 values)
+ * out: f1, f2, mut, source k, source omega, jacobian (0,0), jacobian (1,1) */
+void orc_sst_cell(const double *in, double *out) {
+  const double *W = in, *G = in + 7, *kg = in + 16, *wg = in + 19;
+  const double mu = in[22], d = in[23], vol = in[24], sc = in[25], isc = in[26];
+  const double rho = W[0], k = W[5], om = W[6];
+  /* EddyViscAndBlending (:697-727) */
+  const double alpha1 = (sc * sqrt(k)) / ((SST_BETA_STAR * om) * (d + EPS));
+  const double alpha2 = (((sc * sc) * 500.0) * mu) / ((((d + EPS) * (d + EPS)) * rho) * om);
+  const double cdkw = sst_cdkw(W, kg, wg);
+  const double alpha3 = (((4.0 * rho) * SST_SIGMA_W2) * k) / ((cdkw * (d + EPS)) * (d + EPS));
+  double arg1 = alpha1 > alpha2 ? alpha1 : alpha2;
+  arg1 = arg1 < alpha3 ? arg1 : alpha3;
+  const double f1 = tanh(pow(arg1, 4.0));
+  const double arg2 = (2.0 * alpha1) > alpha2 ? (2.0 * alpha1) : alpha2;
+  const double f2 = tanh(arg2 * arg2);
+  /* EddyVisc (:573-589): mean strain rate sqrt(2 S:S), S = (G + G^T) / 2 */
+  double S[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) S[3 * r + c] = 0.5 * (G[3 * r + c] + G[3 * c + r]);
+  const double msr = sqrt(2.0 * sst_ddot(S, S));
+  const double lim = (sc * msr) * f2, aw = SST_A1 * om;
+  const double mut = ((rho * SST_A1) * k) / (aw > lim ? aw : lim);
+  /* CalcTurbSrc (:641-695) */
+  const double gamma = f1 * SST_GAMMA1 + (1.0 - f1) * SST_GAMMA2;
+  const double beta = f1 * SST_BETA1 + (1.0 - f1) * SST_BETA2;
+  const double tkeDest = (isc * SST_BETA_STAR) * (((rho * k) * om) * 1.0);
+  const double omgDest = (isc * beta) * ((rho * om) * om);
+  /* Boussinesq Reynolds stress (:53-68): lambda tr(G) I + mut (G + G^T) -
+   * 2/3 rho k I, lambda = 0 - 2/3 mut; production tau : G */
+  const double lambda = 0.0 - (2.0 / 3.0) * mut;
+  const double lt = lambda * (G[0] + G[4] + G[8]);
+  const double c2 = ((2.0 / 3.0) * rho) * k;
+  double tau[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const double I = r == c ? 1.0 : 0.0;
+      tau[3 * r + c] = (lt * I + mut * (G[3 * r + c] + G[3 * c + r])) - c2 * I;
+    }
+  double tkeProd = sc * sst_ddot(tau, G);
+  const double pl = SST_PROD2DEST * tkeDest;
+  tkeProd = tkeProd < pl ? tkeProd : pl;
+  tkeProd = tkeProd > 0.0 ? tkeProd : 0.0;
+  double omgProd = ((gamma * rho) / mut) * tkeProd;
+  omgProd = omgProd > 0.0 ? omgProd : 0.0;
+  const double omgCd = (sc * (1.0 - f1)) * cdkw;
+  out[0] = f1;
+  out[1] = f2;
+  out[2] = mut;
+  out[3] = tkeProd - tkeDest;
+  out[4] = (omgProd - omgDest) + omgCd;
+  /* TurbSrcJac (:745-760), phi = 1 */
+  out[5] = ((((-2.0 * SST_BETA_STAR) * om) * 1.0) * vol) * isc;
+  out[6] = (((-2.0 * beta) * om) * vol) * isc;
+}

@@ -172,6 +172,12 @@ void orc_get_cell_grad(const orc_case *c, int b, double *out);
 /* Gauss-Jordan inverse of a 5x5 row-major matrix in place; -1 if singular */
 int orc_block_inverse(double *m);
 
+/* SST k-omega 2003 cell model (src/turbulence.cpp:573-760): in[27] = W[7]
+ * (rho, u, v, w, p, k, omega), velocity gradient[9], k gradient[3], omega
+ * gradient[3], mu, wall distance, volume, scaling, 1/scaling; out[7] = f1,
+ * f2, mut, source k, source omega, source Jacobian (0,0) and (1,1) */
+void orc_sst_cell(const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

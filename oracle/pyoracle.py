@@ -102,6 +102,7 @@ def lib(parallel=False):
         L.orc_get_cell_grad.argtypes = [_V, ctypes.c_int, _D]
         L.orc_is_block.argtypes = [_V]
         L.orc_is_block.restype = ctypes.c_int
+        L.orc_sst_cell.argtypes = [_D, _D]
         _libs[path] = L
     return _libs[path]
 
@@ -112,6 +113,16 @@ def P(a):
 
 def arr(x):
     return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def sst_cells(cells):
+    """the oracle's SST 2003 cell model over rows of 27 inputs -> rows of 7"""
+    L = lib()
+    cells = arr(np.atleast_2d(cells))
+    out = np.empty((cells.shape[0], 7))
+    for r in range(cells.shape[0]):
+        L.orc_sst_cell(P(cells[r]), P(out[r]))
+    return out
 
 
 class OracleCase:
