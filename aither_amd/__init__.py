@@ -115,6 +115,7 @@ _SIGS = {
     "ath_get_diagonal": [_P, ctypes.c_int, _D],
     "ath_get_update": [_P, ctypes.c_int, _D],
     "ath_get_block_diagonal": [_P, ctypes.c_int, ctypes.c_int, _D],
+    "ath_get_wall_distance": [_P, ctypes.c_int, _D],
     "ath_enable_timing": [_P, ctypes.c_int],
     "ath_get_timing": [_P, ctypes.POINTER(Timing)],
     "ath_inspect_info": [ctypes.c_char_p, ctypes.POINTER(Info)],
@@ -371,6 +372,11 @@ class Solver:
 
     def diagonal(self, b=0):
         return self._field(self._lib.ath_get_diagonal, b, 1)
+
+    def wall_distance(self, b=0):
+        """Distance from each cell centre to the nearest viscous-wall face
+        centre (procBlock::CalcWallDistance), i fastest."""
+        return self._field(self._lib.ath_get_wall_distance, b, 1)
 
     def update_vector(self, b=0):
         return self._field(self._lib.ath_get_update, b, 5)

@@ -1923,6 +1923,17 @@ int ath_get_diagonal(ath_ctx *c, int b, double *a) {
   return GetPhys(c, b, a, 3);
 }
 
+int ath_get_wall_distance(ath_ctx *c, int b, double *a) {
+  if (!c || !a || b < 0 || b >= (int)c->hc.blocks.size()) return Fail(ATH_ERR_ARG, "bad argument");
+  const ath::NearestPoints tree(WallFaceCentres(c->hc));
+  const auto &h = c->hc.blocks[b];
+  long n = 0;
+  for (int k = 0; k < h.nk; ++k)
+    for (int j = 0; j < h.nj; ++j)
+      for (int i = 0; i < h.ni; ++i) a[n++] = tree.Distance(h.ctr[h.Cell(i, j, k)]);
+  return ATH_OK;
+}
+
 int ath_get_block_diagonal(ath_ctx *c, int b, int inverse, double *a) {
   if (!c || !a || b < 0 || b >= (int)c->blk.size()) return Fail(ATH_ERR_ARG, "bad argument");
   if (!c->P.block) return Fail(ATH_ERR_ARG, "the case does not use a block matrix");

@@ -169,6 +169,13 @@ int ath_get_update(ath_ctx *ctx, int block, double *du);      /* neq/cell */
  * row-major (src/linearSolver.cpp:146-184); ath_get_diagonal refuses such
  * cases */
 int ath_get_block_diagonal(ath_ctx *ctx, int block, int inverse, double *a);
+/* procBlock::CalcWallDistance over the physical cells (src/procBlock.cpp:
+ * 6030-6041, kdtree::NearestNeighbor, src/kdtree.cpp:123-224): the distance
+ * from each cell centre to the nearest viscousWall face centre of any block,
+ * one per cell, i fastest; sqrt of the largest double where the case has no
+ * viscous wall, as the reference's empty tree gives.  Computed on the host
+ * from the case geometry (a setup step in the reference). */
+int ath_get_wall_distance(ath_ctx *ctx, int block, double *d);
 
 /* ---- files of the output layer ----------------------------------------- */
 /* Restart file in the reference's binary format (WriteRestart,

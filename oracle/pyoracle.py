@@ -226,6 +226,9 @@ class OracleCase:
     def volume(self, b=0):
         return self._get(self.L.orc_get_volume, b, 1)
 
+    def centre(self, b=0):
+        return self._get(self.L.orc_get_center, b, 3)
+
     def cell_width(self, d, b=0):
         ni, nj, nk = self.dims[b]
         g = self.L.orc_num_ghosts(self.h)
