@@ -54,6 +54,17 @@ def test_sst_cells_match_oracle(n, seed):
     assert near.sum() == 0 or np.all(got[near & sat, 0] == 1.0)
 
 
+def test_sst_cells_two_million_cells_sampled():
+    # a 2,097,152-cell launch (8,192 workgroups) checked on a seeded sample
+    c = _cells(1 << 21, 5)
+    got = aither_amd.eval_sst_cells(c)
+    assert np.all(np.isfinite(got))
+    rows = np.random.default_rng(6).choice(len(c), 4000, replace=False)
+    ref = pyoracle.sst_cells(c[rows])
+    den = np.maximum(np.abs(ref), 1e-300)
+    assert float(np.max(np.abs(got[rows] - ref) / den)) < 1e-13
+
+
 def test_sst_cells_bad_count_is_refused():
     with pytest.raises(aither_amd.AitherError):
         aither_amd.eval_sst_cells(np.zeros((0, 27)))
