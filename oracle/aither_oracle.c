@@ -144,6 +144,7 @@ typedef struct {
   double *T, *mu; /* padded: temperature, viscosity (viscous cases) */
   double *vg;     /* padded: cell velocity gradient, 9 per cell, row-major d v_c / d x_r */
   int *order;     /* 3 per physical cell */
+  double *wd;     /* padded: wall distance of heat-flux wall cells, found once (NAN until then) */
   int nsurf;
   surf_t *surf;
   int ictag;
@@ -2094,7 +2095,7 @@ static void free_case(orc_case *c) {
     for (int d = 0; d < 3; ++d) { free(b->fa[d]); free(b->fc[d]); free(b->cw[d]); }
     free(b->W); free(b->x); free(b->T); free(b->vg); free(b->mu); free(b->R); free(b->sr); free(b->dt);
     free(b->A); free(b->Ainv); free(b->Ab); free(b->Abi); free(b->consN); free(b->consNm1);
-    free(b->order); free(b->surf); free(b->conn);
+    free(b->order); free(b->surf); free(b->conn); free(b->wd);
     for (int q = 0; q < 6; ++q) free(b->isconn[q]);
   }
   free(c->blk);
@@ -2529,7 +2530,17 @@ static void viscous_ghosts(orc_case *c, blk_t *b) {
           map3(dir, iC, q1, q2, &ii, &ij, &ik);
           map3(dir, gC, q1, q2, &gi, &gj, &gk);
           map3(dir, aC, q1, q2, &ai, &aj, &ak);
-          double wd = (bs && bs->heatflux) ? wall_distance(c, b->ctr[cidx(b, ai, aj, ak)]) : 0.0;
+          double wd = 0.0;
+          if (bs && bs->heatflux) {
+            const long ac = cidx(b, ai, aj, ak);
+            if (!b->wd) {
+              const long np = (long)b->NI * b->NJ * b->NK;
+              b->wd = (double *)malloc(sizeof(double) * np);
+              for (long q = 0; q < np; ++q) b->wd[q] = NAN;
+            }
+            if (isnan(b->wd[ac])) b->wd[ac] = wall_distance(c, b->ctr[ac]);
+            wd = b->wd[ac];
+          }
           viscous_wall_ghost(c, &b->W[cidx(b, ii, ij, ik) * NEQ], bs, wd, &b->W[cidx(b, gi, gj, gk) * NEQ]);
         }
     }
